@@ -1,0 +1,394 @@
+"""Python binding of libafx.so (include/afx.h) and the host-side mirror of the reference's
+solver call surface.
+
+The reference entry points this module mirrors (same argument meaning and error codes):
+  COCG13_single        src/solver/COCG13_single.f90:9-341        -> cocg13_single()
+  solveCG              src/solver_mpi/forward_bxyz_mpi.f90:70-94 -> solve_cg()
+  forward_bxyz         src/solver_mpi/forward_bxyz_mpi.f90:3-67  -> forward_bxyz()
+  PREPOBSCOEFF         src/solver_mpi/n_ebfem_bxyz_mpi.f90:653-726 -> System.prepobscoeff()
+  CALOBSEBCOMP/RESP    src/solver_mpi/n_ebfem_bxyz_mpi.f90:397-423,570-609 -> calobsebcomp()
+  genjacobian1         src/src_inv_ap/m_jacobian_ap.f90:174-386  -> System.jacobian_device()
+There is no CPU fallback: every call goes through the HIP library, and loading fails loudly
+when libafx.so is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libafx.so")
+
+AFX_OK = 0
+AFX_ERR_MAXIT = -100
+AFX_ERR_BADOPT = 101
+AFX_ERR_BREAKDOWN = -200
+AFX_ERR_HIP = -300
+AFX_ERR_ARG = -400
+AFX_ERR_GEOM = -500
+AFX_ERR_PATTERN = -600
+AFX_ERR_OUTSIDE = -700
+
+PC = {"NONE": 0, "IC": 1, "ILU": 1, "SSOR": 2, "DIAG": 3}
+FP64, MIXED = 0, 1
+
+
+class AfxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"afx error {code}: {msg}")
+        self.code = code
+
+
+class afx_opts(C.Structure):
+    _fields_ = [("precond", C.c_int), ("sigma_diag", C.c_double), ("sigma", C.c_double), ("tol", C.c_double),
+                ("maxit", C.c_int), ("nset", C.c_int), ("precision", C.c_int), ("inner_tol", C.c_double),
+                ("max_outer", C.c_int), ("chunk", C.c_int), ("check_true", C.c_int)]
+
+
+class afx_stats(C.Structure):
+    _fields_ = [("iter", C.c_int), ("error", C.c_int), ("resid", C.c_double), ("true_resid", C.c_double),
+                ("outer", C.c_int)]
+
+
+# exported symbols and their prototypes (argtypes, restype)
+_VP, _I, _D = C.c_void_p, C.c_int, C.c_double
+SIGNATURES = {
+    "afx_default_opts": ([C.POINTER(afx_opts)], None),
+    "afx_last_error": ([], C.c_char_p),
+    "afx_create": ([_I, _VP, _VP, _VP, _VP, C.POINTER(_VP)], _I),
+    "afx_create_from_n6line": ([_I, _I, _VP, C.POINTER(_VP)], _I),
+    "afx_destroy": ([_VP], _I),
+    "afx_get_sizes": ([_VP, _VP, _VP, _VP, _VP, _VP], _I),
+    "afx_get_pattern": ([_VP, _VP, _VP, _VP, _VP], _I),
+    "afx_set_values": ([_VP, _VP, _VP, _VP], _I),
+    "afx_get_values": ([_VP, _VP, _VP, _VP], _I),
+    "afx_set_mesh": ([_VP, _I, _I, _VP, _VP, _VP, _VP, _VP], _I),
+    "afx_assemble": ([_VP, _VP, _D, _VP], _I),
+    "afx_set_omega": ([_VP, _D], _I),
+    "afx_genbc": ([_VP, _VP, _D, _D, _D, _D, _VP], _I),
+    "afx_set_bc": ([_VP, _VP, _I, _VP, _I], _I),
+    "afx_precond_setup": ([_VP, _I, _D, _D], _I),
+    "afx_solve": ([_VP, _I, _VP, _I, _VP, _I, C.POINTER(afx_opts), _VP], _I),
+    "afx_solve_device": ([_VP, _I, _VP, _VP, _VP, C.POINTER(afx_opts), _VP], _I),
+    "afx_true_residual_device": ([_VP, _I, _VP, _VP, _VP, _VP], _I),
+    "afx_autotune": ([_VP, _I, _I], _I),
+    "afx_spmv": ([_VP, _VP, _VP], _I),
+    "afx_precond_apply": ([_VP, _VP, _VP], _I),
+    "afx_get_precond_diag": ([_VP, _VP], _I),
+    "afx_cocg13_single": ([_I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.c_char_p, _D, _D,
+                           C.POINTER(_D), C.POINTER(_I), C.POINTER(_I), _I], _I),
+    "afx_solve_cg": ([_I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP], _I),
+    "afx_prepobscoeff": ([_VP, _I, _VP, _VP, _VP, _VP, _VP], _I),
+    "afx_obs_apply_device": ([_VP, _I, _VP, _VP, _I, _VP, _VP], _I),
+    "afx_jacobian_device": ([_VP, _I, _VP, _VP, _I, _VP, _VP, _VP, _D, _D, _VP], _I),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libafx.so (raises when it is missing: there is no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m actfemtide_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _err(code):
+    if code not in (AFX_OK,):
+        raise AfxError(code, lib().afx_last_error().decode())
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def default_opts(**kw) -> afx_opts:
+    o = afx_opts()
+    lib().afx_default_opts(C.byref(o))
+    for k, v in kw.items():
+        if k == "precond" and isinstance(v, str):
+            v = PC[v.strip().upper()]
+        setattr(o, k, v)
+    return o
+
+
+@dataclasses.dataclass
+class Stats:
+    iter: int
+    error: int
+    resid: float
+    true_resid: float
+    outer: int
+
+
+def _stats(arr, k):
+    return [Stats(arr[c].iter, arr[c].error, arr[c].resid, arr[c].true_resid, arr[c].outer) for c in range(k)]
+
+
+class System:
+    """One assembled linear system on the device (an afx_system handle)."""
+
+    def __init__(self, handle):
+        self.h = C.c_void_p(handle)
+        n, npu, npl, ll, lu = (C.c_int() for _ in range(5))
+        _err(lib().afx_get_sizes(self.h, C.byref(n), C.byref(npu), C.byref(npl), C.byref(ll), C.byref(lu)))
+        self.n, self.npu, self.npl, self.nlev_l, self.nlev_u = n.value, npu.value, npl.value, ll.value, lu.value
+
+    @classmethod
+    def from_n6line(cls, nline: int, n6line: np.ndarray) -> "System":
+        h = C.c_void_p()
+        n6 = _i32(n6line)
+        _err(lib().afx_create_from_n6line(nline, n6.shape[0], _p(n6), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_pattern(cls, INU, IAU, INL, IAL) -> "System":
+        h = C.c_void_p()
+        INU, IAU, INL, IAL = _i32(INU), _i32(IAU), _i32(INL), _i32(IAL)
+        _err(lib().afx_create(len(INU) - 1, _p(INU), _p(IAU), _p(INL), _p(IAL), C.byref(h)))
+        return cls(h.value)
+
+    def close(self):
+        if self.h:
+            lib().afx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pattern(self):
+        INU = np.zeros(self.n + 1, np.int32)
+        INL = np.zeros(self.n + 1, np.int32)
+        IAU = np.zeros(self.npu, np.int32)
+        IAL = np.zeros(self.npl, np.int32)
+        _err(lib().afx_get_pattern(self.h, _p(INU), _p(IAU), _p(INL), _p(IAL)))
+        return INU, IAU, INL, IAL
+
+    def set_values(self, D, AU, AL):
+        D, AU, AL = _c128(D), _c128(AU), _c128(AL)
+        _err(lib().afx_set_values(self.h, _p(D), _p(AU), _p(AL)))
+
+    def get_values(self):
+        D = np.zeros(self.n, np.complex128)
+        AU = np.zeros(self.npu, np.complex128)
+        AL = np.zeros(self.npl, np.complex128)
+        _err(lib().afx_get_values(self.h, _p(D), _p(AU), _p(AL)))
+        return D, AU, AL
+
+    def set_mesh(self, case):
+        self._xyz = _f64(case.xyz)
+        self._n4 = _i32(case.n4)
+        self._group = _i32(case.group)
+        v = _c128(case.vxyz)
+        f = _f64(case.fxyz)
+        _err(lib().afx_set_mesh(self.h, case.node, case.ntet, _p(self._xyz), _p(self._n4), _p(self._group),
+                                _p(v), _p(f)))
+
+    def assemble(self, sigma_tet, omega: float) -> np.ndarray:
+        """GENMAT + COPY_UL on the device; returns the tidal right-hand side b."""
+        b = np.zeros(self.n, np.complex128)
+        sig = _f64(sigma_tet)
+        _err(lib().afx_assemble(self.h, _p(sig), omega, _p(b)))
+        return b
+
+    def set_omega(self, omega: float):
+        _err(lib().afx_set_omega(self.h, omega))
+
+    def genbc(self, line, zmin, zmax, xout, yout) -> np.ndarray:
+        line = _i32(line)
+        bc = np.zeros(self.n, np.uint8)
+        _err(lib().afx_genbc(self.h, _p(line), zmin, zmax, xout, yout, _p(bc)))
+        return bc.astype(bool)
+
+    def set_bc(self, dirichlet, rhs=None):
+        d = np.ascontiguousarray(dirichlet, dtype=np.uint8)
+        if rhs is None:
+            _err(lib().afx_set_bc(self.h, _p(d), 0, None, self.n))
+            return None
+        r = np.array(rhs, dtype=np.complex128, order="F", copy=True).reshape(self.n, -1, order="F")
+        _err(lib().afx_set_bc(self.h, _p(d), r.shape[1], C.c_void_p(r.ctypes.data), self.n))
+        return r[:, 0] if np.ndim(rhs) == 1 else r
+
+    def precond_setup(self, kind="SSOR", sigma_diag=1.0, sigma=0.0):
+        _err(lib().afx_precond_setup(self.h, PC[kind] if isinstance(kind, str) else kind, sigma_diag, sigma))
+
+    def precond_diag(self):
+        DD = np.zeros(self.n, np.complex128)
+        _err(lib().afx_get_precond_diag(self.h, _p(DD)))
+        return DD
+
+    def precond_apply(self, r):
+        r = _c128(r)
+        z = np.zeros(self.n, np.complex128)
+        _err(lib().afx_precond_apply(self.h, _p(r), _p(z)))
+        return z
+
+    def spmv(self, x):
+        x = _c128(x)
+        y = np.zeros(self.n, np.complex128)
+        _err(lib().afx_spmv(self.h, _p(x), _p(y)))
+        return y
+
+    def autotune(self, nrhs=1, verbose=False):
+        _err(lib().afx_autotune(self.h, nrhs, 1 if verbose else 0))
+
+    def solve(self, B, X0=None, **opts):
+        """Solve A X = B (host arrays, columns = right-hand sides); X0 is the initial guess."""
+        B2 = np.asfortranarray(np.asarray(B, np.complex128).reshape(self.n, -1, order="F"))
+        k = B2.shape[1]
+        X = (np.zeros((self.n, k), np.complex128, order="F") if X0 is None
+             else np.array(np.asarray(X0, np.complex128).reshape(self.n, -1, order="F"), order="F", copy=True))
+        o = default_opts(**opts)
+        st = (afx_stats * k)()
+        rc = lib().afx_solve(self.h, k, C.c_void_p(B2.ctypes.data), self.n, C.c_void_p(X.ctypes.data), self.n,
+                             C.byref(o), st)
+        if rc not in (AFX_OK, AFX_ERR_MAXIT, AFX_ERR_BREAKDOWN):
+            _err(rc)
+        out = X[:, 0] if np.ndim(B) == 1 else X
+        return out, _stats(st, k)
+
+    def solve_device(self, dB, dX, omega_scale=None, **opts):
+        """dB, dX: torch complex128 tensors on the GPU, shape (n, k) row-major (element (i, c) at i*k + c)."""
+        k = 1 if dB.dim() == 1 else dB.shape[1]
+        o = default_opts(**opts)
+        st = (afx_stats * k)()
+        sc = None if omega_scale is None else _f64(omega_scale)
+        rc = lib().afx_solve_device(self.h, k, C.c_void_p(dB.data_ptr()), C.c_void_p(dX.data_ptr()), _p(sc),
+                                    C.byref(o), st)
+        if rc not in (AFX_OK, AFX_ERR_MAXIT, AFX_ERR_BREAKDOWN):
+            _err(rc)
+        return _stats(st, k), rc
+
+    def true_residual_device(self, dB, dX, omega_scale=None):
+        k = 1 if dB.dim() == 1 else dB.shape[1]
+        res = np.zeros(k)
+        sc = None if omega_scale is None else _f64(omega_scale)
+        _err(lib().afx_true_residual_device(self.h, k, C.c_void_p(dB.data_ptr()), C.c_void_p(dX.data_ptr()),
+                                            _p(sc), _p(res)))
+        return res
+
+    def prepobscoeff(self, xyzobs):
+        """PREPOBSCOEFF: (item (nobs,6) 1-based, val_e (3,nobs,6), val_b (3,nobs,6), iele (nobs,))."""
+        xo = _f64(xyzobs).reshape(-1, 3)
+        nobs = xo.shape[0]
+        item = np.zeros((nobs, 6), np.int32)
+        ve = np.zeros((3, nobs, 6))
+        vb = np.zeros((3, nobs, 6))
+        iele = np.zeros(nobs, np.int32)
+        _err(lib().afx_prepobscoeff(self.h, nobs, _p(xo), _p(item), _p(ve), _p(vb), _p(iele)))
+        return item, ve, vb, iele
+
+    def obs_apply_device(self, item, val, dX):
+        """f[o, c] = sum_j val[o, j] * X[item[o, j] - 1, c] for device X (n, ncol) complex128."""
+        item, val = _i32(item), _f64(val)
+        nobs = item.shape[0]
+        ncol = 1 if dX.dim() == 1 else dX.shape[1]
+        out = np.zeros((nobs, ncol), np.complex128)
+        _err(lib().afx_obs_apply_device(self.h, nobs, _p(item), _p(val), ncol, C.c_void_p(dX.data_ptr()), _p(out)))
+        return out
+
+    def jacobian_device(self, dU, dx, par_ptr, par_tet, sigma_tet, omega, dlnsig_dm, dJ):
+        """genjacobian1 accumulation: dJ (npar, nobs) complex128 device tensor is overwritten."""
+        nobs = dU.shape[1]
+        pp, pt, sg = _i32(par_ptr), _i32(par_tet), _f64(sigma_tet)
+        _err(lib().afx_jacobian_device(self.h, nobs, C.c_void_p(dU.data_ptr()), C.c_void_p(dx.data_ptr()),
+                                       len(pp) - 1, _p(pp), _p(pt), _p(sg), omega, dlnsig_dm,
+                                       C.c_void_p(dJ.data_ptr())))
+
+
+# ---------------------------------------------------------------------------------------
+# mirrors of the reference's solver call surface
+
+def cocg13_single(D, AL, INL, IAL, AU, INU, IAU, B, X, precond="IC", sigma_diag=1.0, sigma=0.0,
+                  resid=1e-8, iter=3000, nset=1):
+    """COCG13_single (src/solver/COCG13_single.f90:9-341) through afx_cocg13_single.
+
+    D, AL, AU, B, X are complex128 numpy arrays updated in place exactly as the reference does
+    (D/AU/AL scaled for nset=2, B scaled and restored, X = solution). Returns (resid, iter, error)."""
+    for a in (D, AL, AU, B, X):
+        if a.dtype != np.complex128 or not a.flags.c_contiguous:
+            raise TypeError("D, AL, AU, B, X must be contiguous complex128 arrays")
+    INL, IAL, INU, IAU = _i32(INL), _i32(IAL), _i32(INU), _i32(IAU)
+    r = C.c_double(resid)
+    it = C.c_int(iter)
+    err = C.c_int(0)
+    pc = precond.ljust(20).encode()[:20]
+    rc = lib().afx_cocg13_single(len(D), len(AL), len(AU), _p(D), _p(AL), _p(INL), _p(IAL), _p(AU), _p(INU),
+                                 _p(IAU), _p(B), _p(X), pc, sigma_diag, sigma, C.byref(r), C.byref(it),
+                                 C.byref(err), nset)
+    if rc not in (AFX_OK, AFX_ERR_MAXIT, AFX_ERR_BREAKDOWN, AFX_ERR_BADOPT):
+        _err(rc)
+    return r.value, it.value, err.value
+
+
+def solve_cg(D, AL, INL, IAL, AU, INU, IAU, rhs):
+    """solveCG (src/solver_mpi/forward_bxyz_mpi.f90:70-94): 'SSOR', RESID 1e-4, ITER 3000, X0 = 0."""
+    X = np.zeros(len(D), np.complex128)
+    D, AL, AU, rhs = _c128(D), _c128(AL), _c128(AU), _c128(rhs)
+    INL, IAL, INU, IAU = _i32(INL), _i32(IAL), _i32(INU), _i32(IAU)
+    rc = lib().afx_solve_cg(len(D), len(AL), len(AU), _p(D), _p(AL), _p(INL), _p(IAL), _p(AU), _p(INU), _p(IAU),
+                            _p(rhs), _p(X))
+    if rc not in (AFX_OK, AFX_ERR_MAXIT):
+        _err(rc)
+    return X
+
+
+def forward_system(case, omega: float, sigma_tet=None) -> tuple:
+    """forward_bxyz steps INITIALIZE .. SET_BC_ICCG (forward_bxyz_mpi.f90:43-58) on the device:
+    pattern from n6line, GENMAT + COPY_UL, GENBCCSEM, SET_BC_ICCG. Returns (System, b, dirichlet)."""
+    s = System.from_n6line(case.nline, case.n6line)
+    s.set_mesh(case)
+    b = s.assemble(case.sigma_tet() if sigma_tet is None else sigma_tet, omega)
+    xout, yout, zmin, zmax = case.bounds
+    bc = s.genbc(case.line, zmin, zmax, xout, yout)
+    b = s.set_bc(bc, b)
+    return s, b, bc
+
+
+def forward_bxyz(case, omega: float, precond="SSOR", tol=1e-8, maxit=200000, **opts):
+    """forward_bxyz (forward_bxyz_mpi.f90:3-67) with the COCG solve on the device."""
+    s, b, bc = forward_system(case, omega)
+    x, st = s.solve(b, precond=precond, tol=tol, maxit=maxit, **opts)
+    return s, b, x, st[0]
+
+
+def amp_phase(c):
+    """amp / phase [deg] of CALOBSRESP (n_ebfem_bxyz_mpi.f90:612-626)."""
+    return np.sqrt(c.real ** 2 + c.imag ** 2), np.degrees(np.arctan2(c.imag, c.real))
+
+
+def calobsebcomp(s: System, item, val_e, val_b, dX, omega):
+    """CALOBSEBCOMP (n_ebfem_bxyz_mpi.f90:397-423): fields [Bx, By, Bz, Ex, Ey] (5, nobs, ncol),
+    B = face rows . A, E = -i*omega * edge rows . A."""
+    out = []
+    for c in range(3):
+        out.append(s.obs_apply_device(item, val_b[c], dX))
+    for c in range(2):
+        out.append(-1j * omega * s.obs_apply_device(item, val_e[c], dX))
+    return np.stack(out)

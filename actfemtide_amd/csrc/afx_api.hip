@@ -1,0 +1,1537 @@
+// Host side of the C ABI declared in include/afx.h: pattern analysis (level sets, CSR
+// layouts, assembly slot maps), device residency, and the solve drivers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <vector>
+
+#include "afx_internal.h"
+
+using afx::cx;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const char *msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                    \
+  do {                                                                  \
+    hipError_t e_ = (expr);                                             \
+    if (e_ != hipSuccess) {                                             \
+      g_err = std::string(#expr) + ": " + hipGetErrorString(e_);        \
+      return AFX_ERR_HIP;                                               \
+    }                                                                   \
+  } while (0)
+
+template <typename T>
+int dalloc(T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    g_err = std::string("hipMalloc: ") + hipGetErrorString(e);
+    return AFX_ERR_HIP;
+  }
+  return AFX_OK;
+}
+template <typename T>
+void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+template <typename T>
+int upload(T **p, const T *h, size_t count) {
+  int rc = dalloc(p, count);
+  if (rc) return rc;
+  if (count) HIPCHK(hipMemcpy(*p, h, count * sizeof(T), hipMemcpyHostToDevice));
+  return AFX_OK;
+}
+
+int next_pow2(int k) {
+  int p = 1;
+  while (p < k) p <<= 1;
+  return p;
+}
+}  // namespace
+
+struct afx_system_s {
+  int n = 0;
+  long long nU = 0, nL = 0, nnz = 0;
+  std::vector<int> INU, IAU, INL, IAL;  // reference layout (IAU/IAL 1-based)
+  int nlevL = 0, nlevU = 0;
+  hipStream_t stream = nullptr;
+  // device pattern
+  int *d_INU = nullptr, *d_IAU = nullptr, *d_INL = nullptr, *d_IAL = nullptr;
+  int *d_rowptr = nullptr, *d_col = nullptr, *d_map_full = nullptr;
+  int *d_lvlL = nullptr, *d_rowL = nullptr, *d_Lptr = nullptr, *d_Lcol = nullptr, *d_mapL = nullptr;
+  int *d_lvlU = nullptr, *d_rowU = nullptr, *d_Uptr = nullptr, *d_Ucol = nullptr, *d_mapU = nullptr;
+  // canonical values (K, M): [0,n) D, [n, n+nU) AU, [n+nU, n+nU+nL) AL
+  cx<double> *d_canon = nullptr;
+  bool have_values = false;
+  double omega = 1.0;  // effective entry K + i*omega*M
+  // derived layouts
+  cx<double> *d_val64 = nullptr, *d_Lval64 = nullptr, *d_Uval64 = nullptr;
+  cx<float> *d_val32 = nullptr, *d_Lval32 = nullptr, *d_Uval32 = nullptr;
+  bool derived_valid = false;
+  // preconditioner
+  int pc_kind = -1;
+  double pc_sd = 1.0, pc_sig = 0.0;
+  std::vector<double> pc_w;
+  cx<double> *d_DD = nullptr;
+  cx<float> *d_DD32 = nullptr;
+  bool pc_valid = false;
+  // mesh
+  int node = 0, ntet = 0;
+  std::vector<int> n6line;
+  double *d_xyz = nullptr, *d_fxyz = nullptr;
+  int *d_n4 = nullptr, *d_n6 = nullptr, *d_group = nullptr;
+  cx<double> *d_vxyz = nullptr;
+  std::vector<double> h_xyz;
+  std::vector<int> h_n4;
+  bool have_mesh = false;
+  int *d_cptr = nullptr, *d_cidx = nullptr, *d_rptr = nullptr, *d_ridx = nullptr, *d_mirror = nullptr;
+  bool have_slots = false;
+  cx<double> *d_rhs_unit = nullptr;  // tidal RHS of the last assembly
+  // launch configuration (autotuned)
+  int lpr64 = 8, lpr32 = 8;
+  int grid_single64 = 0, grid_single32 = 0, grid_multi64 = 0, grid_multi32 = 0;
+  int ncu = 0;
+};
+
+namespace {
+
+// Level sets of a strict triangle given as per-row dependency lists.
+// forward: level(i) = 1 + max level(j), j in L(i) (j < i); backward over U with j > i.
+void level_sets(int n, const std::vector<int> &ptr, const std::vector<int> &colm1, bool forward,
+                std::vector<int> &lvl_ptr, std::vector<int> &rows) {
+  std::vector<int> level(n, 0);
+  int maxl = 0;
+  if (forward) {
+    for (int i = 0; i < n; i++) {
+      int l = 0;
+      for (int q = ptr[i]; q < ptr[i + 1]; q++) l = std::max(l, level[colm1[q]] + 1);
+      level[i] = l;
+      maxl = std::max(maxl, l);
+    }
+  } else {
+    for (int i = n - 1; i >= 0; i--) {
+      int l = 0;
+      for (int q = ptr[i]; q < ptr[i + 1]; q++) l = std::max(l, level[colm1[q]] + 1);
+      level[i] = l;
+      maxl = std::max(maxl, l);
+    }
+  }
+  const int nlev = (n > 0) ? maxl + 1 : 0;
+  lvl_ptr.assign(nlev + 1, 0);
+  for (int i = 0; i < n; i++) lvl_ptr[level[i] + 1]++;
+  for (int l = 0; l < nlev; l++) lvl_ptr[l + 1] += lvl_ptr[l];
+  rows.assign(n, 0);
+  std::vector<int> fill(lvl_ptr.begin(), lvl_ptr.end() - (nlev > 0 ? 1 : 0));
+  for (int i = 0; i < n; i++) rows[fill[level[i]]++] = i;  // ascending row id inside a level
+}
+
+int validate_pattern(int n, const int *INU, const int *IAU, const int *INL, const int *IAL) {
+  if (n <= 0) return fail(AFX_ERR_ARG, "n must be positive");
+  if (INU[0] != 0 || INL[0] != 0) return fail(AFX_ERR_ARG, "INU(0)/INL(0) must be 0");
+  for (int i = 0; i < n; i++) {
+    if (INU[i + 1] < INU[i] || INL[i + 1] < INL[i]) return fail(AFX_ERR_ARG, "INU/INL not monotone");
+    for (int q = INU[i]; q < INU[i + 1]; q++)
+      if (IAU[q] <= i + 1 || IAU[q] > n) return fail(AFX_ERR_ARG, "IAU entry not strictly upper");
+    for (int q = INL[i]; q < INL[i + 1]; q++)
+      if (IAL[q] < 1 || IAL[q] >= i + 1) return fail(AFX_ERR_ARG, "IAL entry not strictly lower");
+  }
+  return AFX_OK;
+}
+
+int build_device_pattern(afx_system s) {
+  const int n = s->n;
+  const long long nU = s->nU, nL = s->nL;
+  // full CSR rows: diagonal, upper, lower (the order of COCG13_single's products, :273-288)
+  std::vector<int> rowptr(n + 1), col(s->nnz), map_full(s->nnz);
+  long long pos = 0;
+  for (int i = 0; i < n; i++) {
+    rowptr[i] = (int)pos;
+    col[pos] = i;
+    map_full[pos++] = i;
+    for (int q = s->INU[i]; q < s->INU[i + 1]; q++) {
+      col[pos] = s->IAU[q] - 1;
+      map_full[pos++] = n + q;
+    }
+    for (int q = s->INL[i]; q < s->INL[i + 1]; q++) {
+      col[pos] = s->IAL[q] - 1;
+      map_full[pos++] = (int)(n + nU + q);
+    }
+  }
+  rowptr[n] = (int)pos;
+  // sweeps
+  std::vector<int> Lc(nL), Uc(nU);
+  for (long long q = 0; q < nL; q++) Lc[q] = s->IAL[q] - 1;
+  for (long long q = 0; q < nU; q++) Uc[q] = s->IAU[q] - 1;
+  std::vector<int> lvlL, rowL, lvlU, rowU;
+  level_sets(n, s->INL, Lc, true, lvlL, rowL);
+  level_sets(n, s->INU, Uc, false, lvlU, rowU);
+  s->nlevL = (int)lvlL.size() - 1;
+  s->nlevU = (int)lvlU.size() - 1;
+  std::vector<int> Lptr(n + 1), Lcol(nL), mapL(nL), Uptr(n + 1), Ucol(nU), mapU(nU);
+  pos = 0;
+  for (int p = 0; p < n; p++) {
+    const int i = rowL[p];
+    Lptr[p] = (int)pos;
+    for (int q = s->INL[i]; q < s->INL[i + 1]; q++) {
+      Lcol[pos] = Lc[q];
+      mapL[pos++] = (int)(n + nU + q);
+    }
+  }
+  Lptr[n] = (int)pos;
+  pos = 0;
+  for (int p = 0; p < n; p++) {
+    const int i = rowU[p];
+    Uptr[p] = (int)pos;
+    for (int q = s->INU[i]; q < s->INU[i + 1]; q++) {
+      Ucol[pos] = Uc[q];
+      mapU[pos++] = n + q;
+    }
+  }
+  Uptr[n] = (int)pos;
+  int rc;
+  if ((rc = upload(&s->d_INU, s->INU.data(), n + 1))) return rc;
+  if ((rc = upload(&s->d_INL, s->INL.data(), n + 1))) return rc;
+  if ((rc = upload(&s->d_IAU, s->IAU.data(), nU))) return rc;
+  if ((rc = upload(&s->d_IAL, s->IAL.data(), nL))) return rc;
+  if ((rc = upload(&s->d_rowptr, rowptr.data(), n + 1))) return rc;
+  if ((rc = upload(&s->d_col, col.data(), s->nnz))) return rc;
+  if ((rc = upload(&s->d_map_full, map_full.data(), s->nnz))) return rc;
+  if ((rc = upload(&s->d_lvlL, lvlL.data(), lvlL.size()))) return rc;
+  if ((rc = upload(&s->d_rowL, rowL.data(), n))) return rc;
+  if ((rc = upload(&s->d_Lptr, Lptr.data(), n + 1))) return rc;
+  if ((rc = upload(&s->d_Lcol, Lcol.data(), nL))) return rc;
+  if ((rc = upload(&s->d_mapL, mapL.data(), nL))) return rc;
+  if ((rc = upload(&s->d_lvlU, lvlU.data(), lvlU.size()))) return rc;
+  if ((rc = upload(&s->d_rowU, rowU.data(), n))) return rc;
+  if ((rc = upload(&s->d_Uptr, Uptr.data(), n + 1))) return rc;
+  if ((rc = upload(&s->d_Ucol, Ucol.data(), nU))) return rc;
+  if ((rc = upload(&s->d_mapU, mapU.data(), nU))) return rc;
+  const size_t ncanon = (size_t)n + nU + nL;
+  if ((rc = dalloc(&s->d_canon, ncanon))) return rc;
+  HIPCHK(hipMemset(s->d_canon, 0, ncanon * sizeof(cx<double>)));
+  if ((rc = dalloc(&s->d_val64, s->nnz))) return rc;
+  if ((rc = dalloc(&s->d_val32, s->nnz))) return rc;
+  if ((rc = dalloc(&s->d_Lval64, nL))) return rc;
+  if ((rc = dalloc(&s->d_Lval32, nL))) return rc;
+  if ((rc = dalloc(&s->d_Uval64, nU))) return rc;
+  if ((rc = dalloc(&s->d_Uval32, nU))) return rc;
+  // mirror map for COPY_UL: lower (j, i) <- upper (i, j)
+  std::vector<int> mirror(nL, -1);
+  for (int i = 0; i < n; i++)
+    for (int q = s->INU[i]; q < s->INU[i + 1]; q++) {
+      const int j = s->IAU[q] - 1;
+      const int *b = &s->IAL[s->INL[j]], *e = &s->IAL[s->INL[j + 1]];
+      const int *f = std::lower_bound(b, e, i + 1);
+      if (f == e || *f != i + 1) return fail(AFX_ERR_PATTERN, "pattern is not structurally symmetric");
+      mirror[s->INL[j] + (f - b)] = q;
+    }
+  for (long long q = 0; q < nL; q++)
+    if (mirror[q] < 0) return fail(AFX_ERR_PATTERN, "lower entry without an upper mirror");
+  if ((rc = upload(&s->d_mirror, mirror.data(), nL))) return rc;
+  // launch configuration
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&s->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  return AFX_OK;
+}
+
+int ensure_derived(afx_system s) {
+  if (s->derived_valid) return AFX_OK;
+  HIPCHK(afx::launch_gather_vals((int)s->nnz, s->d_map_full, s->d_canon, s->d_val64, s->d_val32, s->stream));
+  HIPCHK(afx::launch_gather_vals((int)s->nL, s->d_mapL, s->d_canon, s->d_Lval64, s->d_Lval32, s->stream));
+  HIPCHK(afx::launch_gather_vals((int)s->nU, s->d_mapU, s->d_canon, s->d_Uval64, s->d_Uval32, s->stream));
+  s->derived_valid = true;
+  return AFX_OK;
+}
+
+int grid_for(afx_system s, bool multi, bool fp64) {
+  int &g = fp64 ? (multi ? s->grid_multi64 : s->grid_single64) : (multi ? s->grid_multi32 : s->grid_single32);
+  const int lpr = fp64 ? s->lpr64 : s->lpr32;
+  const int gmax = afx::max_coop_grid(lpr, multi, fp64);
+  if (g <= 0 || g > gmax) g = std::min(gmax, 2 * s->ncu);
+  if (g <= 0) g = gmax;
+  return g;
+}
+
+// Preconditioner diagonal for the distinct column frequencies `wf`.
+int ensure_precond(afx_system s, int kind, double sd, double sig, const std::vector<double> &wf) {
+  if (s->pc_valid && s->pc_kind == kind && s->pc_sd == sd && s->pc_sig == sig && s->pc_w == wf) return AFX_OK;
+  int rc = ensure_derived(s);
+  if (rc) return rc;
+  const int nf = (int)wf.size();
+  dfree(s->d_DD);
+  dfree(s->d_DD32);
+  if ((rc = dalloc(&s->d_DD, (size_t)s->n * nf))) return rc;
+  if ((rc = dalloc(&s->d_DD32, (size_t)s->n * nf))) return rc;
+  HIPCHK(hipMemsetAsync(s->d_DD, 0, (size_t)s->n * nf * sizeof(cx<double>), s->stream));
+  double *d_wf = nullptr;
+  if ((rc = upload(&d_wf, wf.data(), nf))) return rc;
+  int grid = 0;
+  if (kind == AFX_PC_IC) {
+    grid = std::min(afx::max_coop_grid_ic(), 2 * s->ncu);
+    if (grid <= 0) return fail(AFX_ERR_HIP, "no cooperative grid for the D-ILU setup");
+  }
+  hipError_t e = afx::launch_precond_setup(s->n, nf, kind, sd, sig, s->d_canon, nullptr, nullptr, nullptr, s->nlevL,
+                                           s->d_lvlL, s->d_rowL, s->d_Lptr, s->d_Lcol, s->d_Lval64, s->d_INU,
+                                           s->d_canon + s->n, d_wf, s->d_DD, grid, s->stream);
+  if (e == hipSuccess) e = afx::launch_cast_vec((long long)s->n * nf, s->d_DD, s->d_DD32, s->stream);
+  hipError_t e2 = hipStreamSynchronize(s->stream);
+  dfree(d_wf);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    g_err = std::string("preconditioner setup: ") + hipGetErrorString(e != hipSuccess ? e : e2);
+    return AFX_ERR_HIP;
+  }
+  s->pc_kind = kind;
+  s->pc_sd = sd;
+  s->pc_sig = sig;
+  s->pc_w = wf;
+  s->pc_valid = true;
+  return AFX_OK;
+}
+
+void invalidate_values(afx_system s) {
+  s->derived_valid = false;
+  s->pc_valid = false;
+}
+
+// column frequencies -> distinct list + map
+void column_freqs(afx_system s, int k, const double *scale, std::vector<double> &wcol, std::vector<double> &wf,
+                  std::vector<int> &col2f) {
+  wcol.resize(k);
+  col2f.resize(k);
+  wf.clear();
+  for (int c = 0; c < k; c++) {
+    const double w = s->omega * (scale ? scale[c] : 1.0);
+    wcol[c] = w;
+    int f = -1;
+    for (size_t q = 0; q < wf.size(); q++)
+      if (wf[q] == w) f = (int)q;
+    if (f < 0) {
+      f = (int)wf.size();
+      wf.push_back(w);
+    }
+    col2f[c] = f;
+  }
+}
+
+// Workspace of one Krylov solve (vectors of type T).
+template <typename T>
+struct Work {
+  cx<T> *R = nullptr, *Z = nullptr, *P = nullptr, *Q = nullptr;
+  cx<double> *part_rho = nullptr, *part_c1 = nullptr, *fin_rho = nullptr, *fin_c1 = nullptr, *rho1 = nullptr;
+  double *part_rr = nullptr, *fin_rr = nullptr, *bnrm2 = nullptr, *resid = nullptr, *norms = nullptr;
+  double *part_a = nullptr, *part_b = nullptr;
+  int *conv = nullptr, *iters = nullptr, *status = nullptr, *col2f = nullptr;
+  T *wcol = nullptr;
+  double *wcol64 = nullptr;
+  int *h_status = nullptr;
+  ~Work() {
+    dfree(R); dfree(Z); dfree(P); dfree(Q);
+    dfree(part_rho); dfree(part_c1); dfree(fin_rho); dfree(fin_c1); dfree(rho1);
+    dfree(part_rr); dfree(fin_rr); dfree(bnrm2); dfree(resid); dfree(norms); dfree(part_a); dfree(part_b);
+    dfree(conv); dfree(iters); dfree(status); dfree(col2f); dfree(wcol); dfree(wcol64);
+    if (h_status) (void)hipHostFree(h_status);
+  }
+  int alloc(long long n, int k, int G) {
+    int rc;
+    const size_t nk = (size_t)n * k;
+    if ((rc = dalloc(&R, nk)) || (rc = dalloc(&Z, nk)) || (rc = dalloc(&P, nk)) || (rc = dalloc(&Q, nk))) return rc;
+    const size_t gk = (size_t)G * k;
+    if ((rc = dalloc(&part_rho, gk)) || (rc = dalloc(&part_c1, gk)) || (rc = dalloc(&part_rr, gk))) return rc;
+    if ((rc = dalloc(&part_a, gk)) || (rc = dalloc(&part_b, gk))) return rc;
+    if ((rc = dalloc(&fin_rho, k)) || (rc = dalloc(&fin_c1, k)) || (rc = dalloc(&fin_rr, k))) return rc;
+    if ((rc = dalloc(&rho1, k)) || (rc = dalloc(&bnrm2, k)) || (rc = dalloc(&resid, k))) return rc;
+    if ((rc = dalloc(&norms, 2 * (size_t)k)) || (rc = dalloc(&conv, k)) || (rc = dalloc(&iters, k))) return rc;
+    if ((rc = dalloc(&status, 2)) || (rc = dalloc(&col2f, k)) || (rc = dalloc(&wcol, k)) || (rc = dalloc(&wcol64, k)))
+      return rc;
+    if (hipHostMalloc((void **)&h_status, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+      return fail(AFX_ERR_HIP, "hipHostMalloc");
+    return AFX_OK;
+  }
+};
+
+__global__ void k_init_state(int k, const double *norms, double *bnrm2, int *conv, int *iters, double *resid,
+                             cx<double> *rho1, int *status) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c == 0) {
+    status[0] = 0;
+    status[1] = 0;
+  }
+  if (c >= k) return;
+  const double bb = norms[2 * c + 1];
+  bnrm2[c] = bb;
+  conv[c] = (bb == 0.0) ? 2 : 0;  // COCG13_single.f90:171-177: zero rhs -> X = 0, RESID = 0, ITER = 0
+  iters[c] = 0;
+  resid[c] = 0.0;
+  rho1[c] = cx<double>{0.0, 0.0};
+}
+
+template <typename T>
+__global__ void k_zero_cols(long long n, int k, const int *conv, cx<T> *X) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n * k) return;
+  if (conv[i % k] == 2) X[i] = cx<T>{T(0), T(0)};
+}
+
+template <typename T>
+__global__ void k_cast_w(int k, const double *w, T *out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < k) out[c] = (T)w[c];
+}
+
+// R = B (FP32 inner solve with x = 0) and per-block partial ||b||^2 in FP64
+template <typename T>
+__global__ void __launch_bounds__(256) k_copy_norm(long long n, int k, int kp, const cx<T> *B, cx<T> *R,
+                                                   double *part_a, double *part_b) {
+  __shared__ double lds[256];
+  const int t = threadIdx.x;
+  const int c = t % kp;
+  const long long slot = ((long long)blockIdx.x * 256 + t) / kp;
+  const long long nslot = ((long long)gridDim.x * 256) / kp;
+  double bb = 0.0;
+  if (c < k)
+    for (long long row = slot; row < n; row += nslot) {
+      const size_t ix = (size_t)row * k + c;
+      const cx<T> b = B[ix];
+      R[ix] = b;
+      bb += (double)b.r * (double)b.r + (double)b.i * (double)b.i;
+    }
+  lds[t] = bb;
+  __syncthreads();
+  for (int s = (256 / kp) >> 1; s >= 1; s >>= 1) {
+    if (t / kp < s) lds[t] += lds[t + s * kp];
+    __syncthreads();
+  }
+  if (t < kp && t < k) {
+    part_a[(size_t)blockIdx.x * k + t] = lds[t];
+    part_b[(size_t)blockIdx.x * k + t] = lds[t];
+  }
+}
+
+__global__ void k_norm_finish2(int G, int k, const double *pa, const double *pb, double *norms) {
+  __shared__ double la[256], lb[256];
+  const int c = blockIdx.x;
+  double a = 0.0, b = 0.0;
+  for (int g = threadIdx.x; g < G; g += 256) {
+    a += pa[(size_t)g * k + c];
+    b += pb[(size_t)g * k + c];
+  }
+  la[threadIdx.x] = a;
+  lb[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s >= 1; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      la[threadIdx.x] += la[threadIdx.x + s];
+      lb[threadIdx.x] += lb[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    norms[2 * c] = la[0];
+    norms[2 * c + 1] = lb[0];
+  }
+}
+
+}  // namespace
+
+namespace {
+
+// One Krylov solve in precision T on device vectors B (const) and X (in/out), [n][k] layout.
+// For T = double the initial residual is b - A x in FP64; for T = float the caller passes
+// x = 0 so the initial residual is b.
+template <typename T>
+int krylov(afx_system s, int k, const cx<T> *B, cx<T> *X, const std::vector<double> &wcol,
+           const std::vector<int> &col2f, int nf, int pc, double tol, int maxit, int chunk, afx_stats *stats,
+           int *host_conv) {
+  const bool fp64 = std::is_same<T, double>::value;
+  const bool multi = k > 1;
+  const int kp = multi ? next_pow2(k) : 1;
+  if (kp > 256) return fail(AFX_ERR_ARG, "at most 256 right-hand sides per launch");
+  const int G = grid_for(s, multi, fp64);
+  if (G <= 0) return fail(AFX_ERR_HIP, "no cooperative grid available");
+  Work<T> w;
+  int rc = w.alloc(s->n, k, G);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(w.col2f, col2f.data(), k * sizeof(int), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(w.wcol64, wcol.data(), k * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  hipLaunchKernelGGL(k_cast_w<T>, dim3((k + 255) / 256), dim3(256), 0, s->stream, k, w.wcol64, w.wcol);
+  HIPCHK(hipGetLastError());
+  // initial residual and ||b||^2 (COCG13_single.f90:139-169)
+  if (fp64) {
+    afx::SpmvParams sp{};
+    sp.n = s->n;
+    sp.k = k;
+    sp.kp = kp;
+    sp.rowptr = s->d_rowptr;
+    sp.col = s->d_col;
+    sp.val = s->d_val64;
+    sp.wcol = w.wcol64;
+    sp.X = (const cx<double> *)X;
+    sp.B = (const cx<double> *)B;
+    sp.Y = (cx<double> *)w.R;
+    sp.part_rr = w.part_a;
+    sp.part_bb = w.part_b;
+    HIPCHK(afx::launch_residual_norms(sp, G, w.norms, s->stream));
+  } else {
+    // FP32 inner solve: x = 0 on entry, so r0 = b
+    hipLaunchKernelGGL(k_copy_norm<T>, dim3(G), dim3(256), 0, s->stream, (long long)s->n, k, kp, B, w.R, w.part_a,
+                       w.part_b);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_norm_finish2, dim3(k), dim3(256), 0, s->stream, G, k, w.part_a, w.part_b, w.norms);
+    HIPCHK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_init_state, dim3((k + 255) / 256), dim3(256), 0, s->stream, k, w.norms, w.bnrm2, w.conv,
+                     w.iters, w.resid, w.rho1, w.status);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_zero_cols<T>, dim3((unsigned)(((long long)s->n * k + 255) / 256)), dim3(256), 0, s->stream,
+                     (long long)s->n, k, w.conv, X);
+  HIPCHK(hipGetLastError());
+  afx::CocgParams<T> p{};
+  p.n = s->n;
+  p.k = k;
+  p.kp = kp;
+  p.nf = nf;
+  p.pc = pc;
+  p.dist_reduce = ((long long)G * k > 4096) ? 1 : 0;
+  p.rowptr = s->d_rowptr;
+  p.col = s->d_col;
+  p.val = fp64 ? (const cx<T> *)s->d_val64 : (const cx<T> *)s->d_val32;
+  p.nlevL = s->nlevL;
+  p.nlevU = s->nlevU;
+  p.lvlL = s->d_lvlL;
+  p.rowL = s->d_rowL;
+  p.Lptr = s->d_Lptr;
+  p.Lcol = s->d_Lcol;
+  p.Lval = fp64 ? (const cx<T> *)s->d_Lval64 : (const cx<T> *)s->d_Lval32;
+  p.lvlU = s->d_lvlU;
+  p.rowU = s->d_rowU;
+  p.Uptr = s->d_Uptr;
+  p.Ucol = s->d_Ucol;
+  p.Uval = fp64 ? (const cx<T> *)s->d_Uval64 : (const cx<T> *)s->d_Uval32;
+  p.DD = fp64 ? (const cx<T> *)s->d_DD : (const cx<T> *)s->d_DD32;
+  p.col2f = w.col2f;
+  p.wcol = w.wcol;
+  p.B = B;
+  p.X = X;
+  p.R = w.R;
+  p.Z = w.Z;
+  p.P = w.P;
+  p.Q = w.Q;
+  p.part_rho = w.part_rho;
+  p.part_c1 = w.part_c1;
+  p.part_rr = w.part_rr;
+  p.fin_rho = w.fin_rho;
+  p.fin_c1 = w.fin_c1;
+  p.fin_rr = w.fin_rr;
+  p.rho1 = w.rho1;
+  p.bnrm2 = w.bnrm2;
+  p.conv = w.conv;
+  p.iters = w.iters;
+  p.resid = w.resid;
+  p.status = w.status;
+  p.tol = tol;
+  p.maxit = maxit;
+  p.chunk = chunk > 0 ? chunk : 200;
+  const int lpr = fp64 ? s->lpr64 : s->lpr32;
+  for (;;) {
+    hipError_t e = fp64 ? afx::launch_cocg64(*(afx::CocgParams<double> *)&p, lpr, G, s->stream)
+                        : afx::launch_cocg32(*(afx::CocgParams<float> *)&p, lpr, G, s->stream);
+    if (e != hipSuccess) {
+      g_err = std::string("cooperative COCG launch: ") + hipGetErrorString(e);
+      return AFX_ERR_HIP;
+    }
+    HIPCHK(hipMemcpyAsync(w.h_status, w.status, 2 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (w.h_status[1] || w.h_status[0] >= maxit) break;
+  }
+  std::vector<int> conv(k), iters(k);
+  std::vector<double> resid(k);
+  HIPCHK(hipMemcpy(conv.data(), w.conv, k * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(iters.data(), w.iters, k * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(resid.data(), w.resid, k * sizeof(double), hipMemcpyDeviceToHost));
+  int ret = AFX_OK;
+  for (int c = 0; c < k; c++) {
+    int err = 0;
+    if (conv[c] == -100) err = AFX_ERR_MAXIT;
+    if (conv[c] == -200) err = AFX_ERR_BREAKDOWN;
+    if (conv[c] == 0) {  // still running when maxit was reached (cannot normally happen)
+      err = AFX_ERR_MAXIT;
+      iters[c] = maxit + 1;
+    }
+    if (stats) {
+      stats[c].iter = iters[c];
+      stats[c].error = err;
+      stats[c].resid = resid[c];
+    }
+    if (host_conv) host_conv[c] = conv[c];
+    if (err == AFX_ERR_BREAKDOWN) ret = AFX_ERR_BREAKDOWN;
+    else if (err && ret == AFX_OK) ret = err;
+  }
+  return ret;
+}
+
+}  // namespace
+
+// =========================================================================================
+// C ABI
+// =========================================================================================
+namespace {
+
+int create_common(afx_system s) {
+  s->nU = s->INU[s->n];
+  s->nL = s->INL[s->n];
+  s->nnz = (long long)s->n + s->nU + s->nL;
+  if (s->nnz > 0x7fffffffLL) return fail(AFX_ERR_ARG, "more than 2^31 stored entries");
+  HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  return build_device_pattern(s);
+}
+
+void release(afx_system s) {
+  dfree(s->d_INU); dfree(s->d_IAU); dfree(s->d_INL); dfree(s->d_IAL);
+  dfree(s->d_rowptr); dfree(s->d_col); dfree(s->d_map_full);
+  dfree(s->d_lvlL); dfree(s->d_rowL); dfree(s->d_Lptr); dfree(s->d_Lcol); dfree(s->d_mapL);
+  dfree(s->d_lvlU); dfree(s->d_rowU); dfree(s->d_Uptr); dfree(s->d_Ucol); dfree(s->d_mapU);
+  dfree(s->d_canon); dfree(s->d_val64); dfree(s->d_Lval64); dfree(s->d_Uval64);
+  dfree(s->d_val32); dfree(s->d_Lval32); dfree(s->d_Uval32);
+  dfree(s->d_DD); dfree(s->d_DD32);
+  dfree(s->d_xyz); dfree(s->d_fxyz); dfree(s->d_n4); dfree(s->d_n6); dfree(s->d_group); dfree(s->d_vxyz);
+  dfree(s->d_cptr); dfree(s->d_cidx); dfree(s->d_rptr); dfree(s->d_ridx); dfree(s->d_mirror);
+  dfree(s->d_rhs_unit);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  s->stream = nullptr;
+}
+
+int pc_from_string(const char *pc) {
+  // COCG13_single.f90:107,126,202,236 compare the leading characters
+  if (!pc) return AFX_PC_NONE;
+  if (strncmp(pc, "IC", 2) == 0 || strncmp(pc, "ILU", 3) == 0) return AFX_PC_IC;
+  if (strncmp(pc, "SSOR", 4) == 0) return AFX_PC_SSOR;
+  if (strncmp(pc, "DIAG", 4) == 0) return AFX_PC_DIAG;
+  return AFX_PC_NONE;
+}
+
+// Assembly slot maps from n6line: canonical slot (D or AU) -> (entry, tet) contributions in
+// ascending tet order, and edge -> (local edge, tet) for the right-hand side.
+int build_slot_maps(afx_system s) {
+  const int n = s->n, ntet = s->ntet;
+  const long long nslot = (long long)n + s->nU;
+  std::vector<int> slot_of((size_t)ntet * 21);
+  std::vector<int> cnt(nslot + 1, 0);
+  for (int t = 0; t < ntet; t++) {
+    int g[6];
+    for (int j = 0; j < 6; j++) g[j] = std::abs(s->n6line[(size_t)t * 6 + j]);
+    for (int a = 0; a < 6; a++)
+      for (int b = 0; b <= a; b++) {
+        const int e = (a == b) ? a : 6 + a * (a - 1) / 2 + b;
+        long long slot;
+        if (a == b) {
+          slot = g[a] - 1;
+        } else {
+          const int lo = std::min(g[a], g[b]), hi = std::max(g[a], g[b]);
+          if (lo == hi) return fail(AFX_ERR_PATTERN, "a tetrahedron repeats an edge");
+          const int *bb = &s->IAU[s->INU[lo - 1]], *ee = &s->IAU[s->INU[lo]];
+          const int *f = std::lower_bound(bb, ee, hi);
+          if (f == ee || *f != hi) return fail(AFX_ERR_PATTERN, "sup_iccg: dof not found");
+          slot = n + s->INU[lo - 1] + (f - bb);
+        }
+        slot_of[(size_t)t * 21 + e] = (int)slot;
+        cnt[slot + 1]++;
+      }
+  }
+  for (long long q = 0; q < nslot; q++) cnt[q + 1] += cnt[q];
+  std::vector<int> cidx((size_t)ntet * 21), fill(cnt.begin(), cnt.end() - 1);
+  for (int t = 0; t < ntet; t++)
+    for (int e = 0; e < 21; e++) cidx[fill[slot_of[(size_t)t * 21 + e]]++] = e * ntet + t;
+  std::vector<int> rcnt(n + 1, 0), ridx((size_t)ntet * 6);
+  for (int t = 0; t < ntet; t++)
+    for (int j = 0; j < 6; j++) rcnt[std::abs(s->n6line[(size_t)t * 6 + j])]++;
+  for (int i = 0; i < n; i++) rcnt[i + 1] += rcnt[i];
+  std::vector<int> rfill(rcnt.begin(), rcnt.end() - 1);
+  for (int t = 0; t < ntet; t++)
+    for (int j = 0; j < 6; j++) ridx[rfill[std::abs(s->n6line[(size_t)t * 6 + j]) - 1]++] = j * ntet + t;
+  int rc;
+  dfree(s->d_cptr); dfree(s->d_cidx); dfree(s->d_rptr); dfree(s->d_ridx);
+  if ((rc = upload(&s->d_cptr, cnt.data(), cnt.size()))) return rc;
+  if ((rc = upload(&s->d_cidx, cidx.data(), cidx.size()))) return rc;
+  if ((rc = upload(&s->d_rptr, rcnt.data(), rcnt.size()))) return rc;
+  if ((rc = upload(&s->d_ridx, ridx.data(), ridx.size()))) return rc;
+  s->have_slots = true;
+  return AFX_OK;
+}
+
+__global__ void k_inv_or_zero(int k, const double *norms, double tol, double *scale, double *inv) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k) return;
+  const double rr = norms[2 * c], bb = norms[2 * c + 1];
+  const double nr = sqrt(rr);
+  const bool done = (bb == 0.0) || (nr <= tol * sqrt(bb));
+  scale[c] = done ? 0.0 : 1.0 / nr;
+  inv[c] = done ? 0.0 : nr;
+}
+
+int solve_device_impl(afx_system s, int k, const cx<double> *dB, cx<double> *dX, const double *omega_scale,
+                      const afx_opts *o, afx_stats *stats) {
+  if (!s->have_values) return fail(AFX_ERR_ARG, "no matrix values");
+  if (k > 256) return fail(AFX_ERR_ARG, "at most 256 right-hand sides per call");
+  int rc = ensure_derived(s);
+  if (rc) return rc;
+  std::vector<double> wcol, wf;
+  std::vector<int> col2f;
+  column_freqs(s, k, omega_scale, wcol, wf, col2f);
+  const int pc = o->precond;
+  if (o->nset == 0) {
+    // reuse the existing setup (the SAVEd DD of COCG13_single.f90:39,57-62)
+    if (!s->pc_valid || s->pc_w != wf) return AFX_ERR_BADOPT;
+  } else if (pc != AFX_PC_NONE) {
+    if ((rc = ensure_precond(s, pc, o->sigma_diag, o->sigma, wf))) return rc;
+  }
+  const int kind = (o->nset == 0) ? s->pc_kind : pc;
+  const int nf = (int)wf.size();
+  std::vector<afx_stats> st(k);
+  for (auto &x : st) x = afx_stats{0, 0, 0.0, -1.0, 0};
+  int ret = AFX_OK;
+  if (o->precision == AFX_FP64) {
+    ret = krylov<double>(s, k, dB, dX, wcol, col2f, nf, kind, o->tol, o->maxit, o->chunk, st.data(), nullptr);
+    if (ret != AFX_OK && ret != AFX_ERR_MAXIT && ret != AFX_ERR_BREAKDOWN) return ret;
+  } else {
+    // FP64 iterative refinement around complex-FP32 COCG
+    const int G = 2 * s->ncu;
+    const size_t nk = (size_t)s->n * k;
+    cx<double> *dR = nullptr;
+    cx<float> *dR32 = nullptr, *dD32 = nullptr;
+    double *dw = nullptr, *pa = nullptr, *pb = nullptr, *norms = nullptr, *scale = nullptr, *inv = nullptr;
+    auto cleanup = [&]() {
+      dfree(dR); dfree(dR32); dfree(dD32); dfree(dw); dfree(pa); dfree(pb); dfree(norms); dfree(scale); dfree(inv);
+    };
+    if ((rc = dalloc(&dR, nk)) || (rc = dalloc(&dR32, nk)) || (rc = dalloc(&dD32, nk)) ||
+        (rc = upload(&dw, wcol.data(), k)) || (rc = dalloc(&pa, (size_t)G * k)) || (rc = dalloc(&pb, (size_t)G * k)) ||
+        (rc = dalloc(&norms, 2 * (size_t)k)) || (rc = dalloc(&scale, k)) || (rc = dalloc(&inv, k))) {
+      cleanup();
+      return rc;
+    }
+    std::vector<int> total(k, 0), conv(k, 0);
+    std::vector<double> hn(2 * (size_t)k);
+    int outer = 0;
+    for (;;) {
+      afx::SpmvParams sp{};
+      sp.n = s->n;
+      sp.k = k;
+      sp.kp = k > 1 ? next_pow2(k) : 1;
+      sp.rowptr = s->d_rowptr;
+      sp.col = s->d_col;
+      sp.val = s->d_val64;
+      sp.wcol = dw;
+      sp.X = dX;
+      sp.B = dB;
+      sp.Y = dR;
+      sp.part_rr = pa;
+      sp.part_bb = pb;
+      hipError_t e = afx::launch_residual_norms(sp, G, norms, s->stream);
+      if (e == hipSuccess) e = hipMemcpyAsync(hn.data(), norms, hn.size() * 8, hipMemcpyDeviceToHost, s->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+      if (e != hipSuccess) {
+        cleanup();
+        return fail(AFX_ERR_HIP, hipGetErrorString(e));
+      }
+      bool all = true;
+      int maxdone = 0;
+      for (int c = 0; c < k; c++) {
+        const double res = hn[2 * c + 1] > 0 ? std::sqrt(hn[2 * c] / hn[2 * c + 1]) : 0.0;
+        st[c].resid = res;
+        st[c].true_resid = res;
+        if (!(res <= o->tol)) all = false;
+        maxdone = std::max(maxdone, total[c]);
+      }
+      if (all || outer >= o->max_outer || maxdone >= o->maxit) break;
+      hipLaunchKernelGGL(k_inv_or_zero, dim3((k + 255) / 256), dim3(256), 0, s->stream, k, norms, o->tol, scale, inv);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = afx::launch_scale_cols(s->n, k, dR, scale, dR32, s->stream);
+      if (e == hipSuccess) e = hipMemsetAsync(dD32, 0, nk * sizeof(cx<float>), s->stream);
+      if (e != hipSuccess) {
+        cleanup();
+        return fail(AFX_ERR_HIP, hipGetErrorString(e));
+      }
+      std::vector<afx_stats> ist(k);
+      const int budget = std::max(1, o->maxit - maxdone);
+      rc = krylov<float>(s, k, dR32, dD32, wcol, col2f, nf, kind, o->inner_tol, budget, o->chunk, ist.data(),
+                         conv.data());
+      if (rc != AFX_OK && rc != AFX_ERR_MAXIT && rc != AFX_ERR_BREAKDOWN) {
+        cleanup();
+        return rc;
+      }
+      for (int c = 0; c < k; c++)
+        if (conv[c] != 2) total[c] += std::min(ist[c].iter, budget);
+      e = afx::launch_axpy_scaled(s->n, k, dD32, inv, dX, s->stream);
+      if (e != hipSuccess) {
+        cleanup();
+        return fail(AFX_ERR_HIP, hipGetErrorString(e));
+      }
+      ++outer;
+    }
+    for (int c = 0; c < k; c++) {
+      st[c].iter = total[c];
+      st[c].outer = outer;
+      st[c].error = (st[c].resid <= o->tol) ? 0 : AFX_ERR_MAXIT;
+      if (st[c].error && ret == AFX_OK) ret = st[c].error;
+    }
+    cleanup();
+  }
+  if (o->check_true && o->precision == AFX_FP64) {
+    std::vector<double> tr(k);
+    if ((rc = afx_true_residual_device(s, k, dB, dX, omega_scale, tr.data()))) return rc;
+    for (int c = 0; c < k; c++) st[c].true_resid = tr[c];
+  }
+  if (stats)
+    for (int c = 0; c < k; c++) stats[c] = st[c];
+  return ret;
+}
+
+}  // namespace
+
+extern "C" {
+
+void afx_default_opts(afx_opts *o) {
+  if (!o) return;
+  o->precond = AFX_PC_SSOR;
+  o->sigma_diag = 1.0;
+  o->sigma = 0.0;
+  o->tol = 1e-8;
+  o->maxit = 100000;
+  o->nset = 1;
+  o->precision = AFX_FP64;
+  o->inner_tol = 1e-4;
+  o->max_outer = 20;
+  o->chunk = 0;
+  o->check_true = 1;
+}
+
+const char *afx_last_error(void) { return g_err.c_str(); }
+
+int afx_create(int n, const int *INU, const int *IAU, const int *INL, const int *IAL, afx_system *out) {
+  if (!out || !INU || !INL) return fail(AFX_ERR_ARG, "null argument");
+  *out = nullptr;
+  int rc = validate_pattern(n, INU, IAU, INL, IAL);
+  if (rc) return rc;
+  afx_system s = new afx_system_s();
+  s->n = n;
+  s->INU.assign(INU, INU + n + 1);
+  s->INL.assign(INL, INL + n + 1);
+  s->IAU.assign(IAU, IAU + INU[n]);
+  s->IAL.assign(IAL, IAL + INL[n]);
+  rc = create_common(s);
+  if (rc) {
+    release(s);
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return AFX_OK;
+}
+
+int afx_create_from_n6line(int nline, int ntet, const int *n6line, afx_system *out) {
+  if (!out || !n6line || nline <= 0 || ntet <= 0) return fail(AFX_ERR_ARG, "bad arguments");
+  *out = nullptr;
+  // C_set_dof2 (m_iccg_var_takuto.f90:361-485): per row, the sorted set of edges sharing a tet
+  std::vector<long long> up_keys, lo_keys;
+  up_keys.reserve((size_t)ntet * 15);
+  lo_keys.reserve((size_t)ntet * 15);
+  for (int t = 0; t < ntet; t++) {
+    int g[6];
+    for (int j = 0; j < 6; j++) {
+      g[j] = std::abs(n6line[(size_t)t * 6 + j]);
+      if (g[j] < 1 || g[j] > nline) return fail(AFX_ERR_ARG, "n6line entry out of range");
+    }
+    for (int a = 0; a < 6; a++)
+      for (int b = 0; b < 6; b++) {
+        if (g[a] < g[b]) up_keys.push_back((long long)(g[a] - 1) * nline + (g[b] - 1));
+        else if (g[a] > g[b]) lo_keys.push_back((long long)(g[a] - 1) * nline + (g[b] - 1));
+      }
+  }
+  auto finish = [&](std::vector<long long> &keys, std::vector<int> &ptr, std::vector<int> &idx) {
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    ptr.assign(nline + 1, 0);
+    idx.resize(keys.size());
+    for (size_t q = 0; q < keys.size(); q++) {
+      ptr[keys[q] / nline + 1]++;
+      idx[q] = (int)(keys[q] % nline) + 1;
+    }
+    for (int i = 0; i < nline; i++) ptr[i + 1] += ptr[i];
+  };
+  afx_system s = new afx_system_s();
+  s->n = nline;
+  finish(up_keys, s->INU, s->IAU);
+  finish(lo_keys, s->INL, s->IAL);
+  s->ntet = ntet;
+  s->n6line.assign(n6line, n6line + (size_t)ntet * 6);
+  int rc = create_common(s);
+  if (rc) {
+    release(s);
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return AFX_OK;
+}
+
+int afx_destroy(afx_system s) {
+  if (!s) return AFX_OK;
+  release(s);
+  delete s;
+  return AFX_OK;
+}
+
+int afx_get_sizes(afx_system s, int *n, int *npu, int *npl, int *nlev_l, int *nlev_u) {
+  if (!s) return fail(AFX_ERR_ARG, "null system");
+  if (n) *n = s->n;
+  if (npu) *npu = (int)s->nU;
+  if (npl) *npl = (int)s->nL;
+  if (nlev_l) *nlev_l = s->nlevL;
+  if (nlev_u) *nlev_u = s->nlevU;
+  return AFX_OK;
+}
+
+int afx_get_pattern(afx_system s, int *INU, int *IAU, int *INL, int *IAL) {
+  if (!s) return fail(AFX_ERR_ARG, "null system");
+  if (INU) std::copy(s->INU.begin(), s->INU.end(), INU);
+  if (IAU) std::copy(s->IAU.begin(), s->IAU.end(), IAU);
+  if (INL) std::copy(s->INL.begin(), s->INL.end(), INL);
+  if (IAL) std::copy(s->IAL.begin(), s->IAL.end(), IAL);
+  return AFX_OK;
+}
+
+int afx_set_values(afx_system s, const double *D, const double *AU, const double *AL) {
+  if (!s || !D) return fail(AFX_ERR_ARG, "null argument");
+  const size_t n = s->n, nU = s->nU, nL = s->nL;
+  HIPCHK(hipMemcpy(s->d_canon, D, n * sizeof(cx<double>), hipMemcpyHostToDevice));
+  if (nU) HIPCHK(hipMemcpy(s->d_canon + n, AU, nU * sizeof(cx<double>), hipMemcpyHostToDevice));
+  if (nL) HIPCHK(hipMemcpy(s->d_canon + n + nU, AL, nL * sizeof(cx<double>), hipMemcpyHostToDevice));
+  s->omega = 1.0;  // (Re, Im) stored as (K, M): entry = K + i*1*M
+  s->have_values = true;
+  invalidate_values(s);
+  return AFX_OK;
+}
+
+int afx_get_values(afx_system s, double *D, double *AU, double *AL) {
+  if (!s) return fail(AFX_ERR_ARG, "null system");
+  const size_t n = s->n, nU = s->nU, nL = s->nL;
+  std::vector<cx<double>> h(n + nU + nL);
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpy(h.data(), s->d_canon, h.size() * sizeof(cx<double>), hipMemcpyDeviceToHost));
+  const double w = s->omega;
+  auto put = [&](double *dst, size_t off, size_t m) {
+    if (!dst) return;
+    for (size_t q = 0; q < m; q++) {
+      dst[2 * q] = h[off + q].r;
+      dst[2 * q + 1] = w * h[off + q].i;
+    }
+  };
+  put(D, 0, n);
+  put(AU, n, nU);
+  put(AL, n + nU, nL);
+  return AFX_OK;
+}
+
+int afx_set_mesh(afx_system s, int node, int ntet, const double *xyz, const int *n4, const int *group,
+                 const double *vxyz, const double *fxyz) {
+  if (!s || !xyz || !n4 || !group) return fail(AFX_ERR_ARG, "null argument");
+  if (s->n6line.empty() || ntet != s->ntet) return fail(AFX_ERR_ARG, "system was not created from this n6line");
+  for (long long q = 0; q < (long long)ntet * 4; q++)
+    if (n4[q] < 1 || n4[q] > node) return fail(AFX_ERR_ARG, "n4 entry out of range");
+  dfree(s->d_xyz); dfree(s->d_fxyz); dfree(s->d_n4); dfree(s->d_n6); dfree(s->d_group); dfree(s->d_vxyz);
+  int rc;
+  std::vector<double> zero3((size_t)node * 3, 0.0), zero6((size_t)node * 6, 0.0);
+  if ((rc = upload(&s->d_xyz, xyz, (size_t)node * 3))) return rc;
+  if ((rc = upload(&s->d_n4, n4, (size_t)ntet * 4))) return rc;
+  if ((rc = upload(&s->d_n6, s->n6line.data(), (size_t)ntet * 6))) return rc;
+  if ((rc = upload(&s->d_group, group, (size_t)ntet))) return rc;
+  if ((rc = upload(&s->d_fxyz, fxyz ? fxyz : zero3.data(), (size_t)node * 3))) return rc;
+  if ((rc = upload((double **)&s->d_vxyz, vxyz ? vxyz : zero6.data(), (size_t)node * 6))) return rc;
+  s->node = node;
+  s->h_xyz.assign(xyz, xyz + (size_t)node * 3);
+  s->h_n4.assign(n4, n4 + (size_t)ntet * 4);
+  s->have_mesh = true;
+  if (!s->have_slots && (rc = build_slot_maps(s))) return rc;
+  return AFX_OK;
+}
+
+int afx_assemble(afx_system s, const double *sigma_tet, double omega, double *b_out) {
+  if (!s || !sigma_tet) return fail(AFX_ERR_ARG, "null argument");
+  if (!s->have_mesh) return fail(AFX_ERR_ARG, "afx_set_mesh first");
+  const int ntet = s->ntet, n = s->n;
+  double *d_sig = nullptr, *eK = nullptr, *eM = nullptr;
+  cx<double> *eB = nullptr;
+  int *d_err = nullptr;
+  int rc;
+  if ((rc = upload(&d_sig, sigma_tet, ntet)) || (rc = dalloc(&eK, (size_t)ntet * 21)) ||
+      (rc = dalloc(&eM, (size_t)ntet * 21)) || (rc = dalloc(&eB, (size_t)ntet * 6)) || (rc = dalloc(&d_err, 1))) {
+    dfree(d_sig); dfree(eK); dfree(eM); dfree(eB); dfree(d_err);
+    return rc;
+  }
+  if (!s->d_rhs_unit && (rc = dalloc(&s->d_rhs_unit, n))) return rc;
+  hipError_t e = hipMemsetAsync(d_err, 0, sizeof(int), s->stream);
+  afx::ElemParams ep{};
+  ep.ntet = ntet;
+  ep.xyz = s->d_xyz;
+  ep.n4 = s->d_n4;
+  ep.n6line = s->d_n6;
+  ep.group = s->d_group;
+  ep.sigma = d_sig;
+  ep.vxyz = s->d_vxyz;
+  ep.fxyz = s->d_fxyz;
+  ep.eK = eK;
+  ep.eM = eM;
+  ep.eB = eB;
+  ep.err = d_err;
+  if (e == hipSuccess) e = afx::launch_element(ep, s->stream);
+  if (e == hipSuccess)
+    e = afx::launch_slot_gather((int)(n + s->nU), s->d_cptr, s->d_cidx, eK, eM, s->d_canon, s->stream);
+  if (e == hipSuccess) e = afx::launch_mirror(s->nL, s->d_mirror, s->d_canon + n, s->d_canon + n + s->nU, s->stream);
+  if (e == hipSuccess) e = afx::launch_rhs_gather(n, s->d_rptr, s->d_ridx, eB, 0.0, s->d_rhs_unit, s->stream);
+  int herr = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&herr, d_err, sizeof(int), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  dfree(d_sig); dfree(eK); dfree(eM); dfree(eB); dfree(d_err);
+  if (e != hipSuccess) {
+    g_err = std::string("assembly: ") + hipGetErrorString(e);
+    return AFX_ERR_HIP;
+  }
+  if (herr) return fail(AFX_ERR_GEOM, "GEGEGE! volume is less than 0 (gradnodebasisfun)");
+  s->omega = omega;
+  s->have_values = true;
+  invalidate_values(s);
+  if (b_out) HIPCHK(hipMemcpy(b_out, s->d_rhs_unit, (size_t)n * sizeof(cx<double>), hipMemcpyDeviceToHost));
+  return AFX_OK;
+}
+
+int afx_set_omega(afx_system s, double omega) {
+  if (!s) return fail(AFX_ERR_ARG, "null system");
+  if (s->omega != omega) {
+    s->omega = omega;
+    s->pc_valid = false;
+  }
+  return AFX_OK;
+}
+
+int afx_genbc(afx_system s, const int *line, double zmin, double zmax, double xout, double yout,
+              unsigned char *line_bc) {
+  if (!s || !line || !line_bc) return fail(AFX_ERR_ARG, "null argument");
+  if (!s->have_mesh) return fail(AFX_ERR_ARG, "afx_set_mesh first");
+  int *d_line = nullptr;
+  unsigned char *d_bc = nullptr;
+  int rc;
+  if ((rc = upload(&d_line, line, (size_t)s->n * 2)) || (rc = dalloc(&d_bc, s->n))) {
+    dfree(d_line);
+    return rc;
+  }
+  hipError_t e = afx::launch_genbc(s->n, d_line, s->d_xyz, zmin, zmax, xout, yout, d_bc, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(line_bc, d_bc, s->n, hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  dfree(d_line);
+  dfree(d_bc);
+  if (e != hipSuccess) return fail(AFX_ERR_HIP, hipGetErrorString(e));
+  return AFX_OK;
+}
+
+int afx_set_bc(afx_system s, const unsigned char *dirichlet, int nrhs, double *rhs, int ldr) {
+  if (!s || !dirichlet) return fail(AFX_ERR_ARG, "null argument");
+  unsigned char *d_dir = nullptr;
+  int rc;
+  if ((rc = upload(&d_dir, dirichlet, s->n))) return rc;
+  hipError_t e = afx::launch_bc_values(s->n, s->d_INU, s->d_IAU, s->d_INL, s->d_IAL, d_dir, s->d_canon,
+                                       s->d_canon + s->n, s->d_canon + s->n + s->nU, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  dfree(d_dir);
+  if (e != hipSuccess) return fail(AFX_ERR_HIP, hipGetErrorString(e));
+  // right-hand side: rf(i) = 0 on Dirichlet rows; non-Dirichlet rows subtract K*0 = 0
+  if (rhs)
+    for (int c = 0; c < nrhs; c++)
+      for (int i = 0; i < s->n; i++)
+        if (dirichlet[i]) {
+          rhs[2 * ((size_t)c * ldr + i)] = 0.0;
+          rhs[2 * ((size_t)c * ldr + i) + 1] = 0.0;
+        }
+  invalidate_values(s);
+  return AFX_OK;
+}
+
+int afx_precond_setup(afx_system s, int kind, double sigma_diag, double sigma) {
+  if (!s) return fail(AFX_ERR_ARG, "null system");
+  if (!s->have_values) return fail(AFX_ERR_ARG, "no matrix values");
+  std::vector<double> wf(1, s->omega);
+  return ensure_precond(s, kind, sigma_diag, sigma, wf);
+}
+
+int afx_get_precond_diag(afx_system s, double *DD) {
+  if (!s || !DD) return fail(AFX_ERR_ARG, "null argument");
+  if (!s->pc_valid) return fail(AFX_ERR_ARG, "no preconditioner set up");
+  const int nf = (int)s->pc_w.size();
+  std::vector<cx<double>> h((size_t)s->n * nf);
+  HIPCHK(hipMemcpy(h.data(), s->d_DD, h.size() * sizeof(cx<double>), hipMemcpyDeviceToHost));
+  for (int i = 0; i < s->n; i++) {
+    DD[2 * i] = h[(size_t)i * nf].r;
+    DD[2 * i + 1] = h[(size_t)i * nf].i;
+  }
+  return AFX_OK;
+}
+
+int afx_spmv(afx_system s, const double *x, double *y) {
+  if (!s || !x || !y) return fail(AFX_ERR_ARG, "null argument");
+  int rc = ensure_derived(s);
+  if (rc) return rc;
+  cx<double> *dx = nullptr, *dy = nullptr;
+  double *dw = nullptr;
+  const double w = s->omega;
+  if ((rc = upload(&dx, (const cx<double> *)x, s->n)) || (rc = dalloc(&dy, s->n)) || (rc = upload(&dw, &w, 1))) {
+    dfree(dx); dfree(dy); dfree(dw);
+    return rc;
+  }
+  afx::SpmvParams sp{};
+  sp.n = s->n;
+  sp.k = 1;
+  sp.kp = 1;
+  sp.rowptr = s->d_rowptr;
+  sp.col = s->d_col;
+  sp.val = s->d_val64;
+  sp.wcol = dw;
+  sp.X = dx;
+  sp.Y = dy;
+  hipError_t e = afx::launch_spmv(sp, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(y, dy, (size_t)s->n * 16, hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  dfree(dx); dfree(dy); dfree(dw);
+  if (e != hipSuccess) return fail(AFX_ERR_HIP, hipGetErrorString(e));
+  return AFX_OK;
+}
+
+int afx_precond_apply(afx_system s, const double *r, double *z) {
+  if (!s || !r || !z) return fail(AFX_ERR_ARG, "null argument");
+  if (!s->pc_valid || s->pc_w.size() != 1) return fail(AFX_ERR_ARG, "no single-frequency preconditioner set up");
+  cx<double> *dr = nullptr, *dz = nullptr;
+  double *dw = nullptr;
+  int rc;
+  const double w = s->pc_w[0];
+  if ((rc = upload(&dr, (const cx<double> *)r, s->n)) || (rc = dalloc(&dz, s->n)) || (rc = upload(&dw, &w, 1))) {
+    dfree(dr); dfree(dz); dfree(dw);
+    return rc;
+  }
+  hipError_t e = afx::launch_precond_apply_levels(s->n, s->pc_kind, s->nlevL, s->d_lvlL, s->d_rowL, s->d_Lptr,
+                                                  s->d_Lcol, s->d_Lval64, s->nlevU, s->d_lvlU, s->d_rowU, s->d_Uptr,
+                                                  s->d_Ucol, s->d_Uval64, s->d_DD, dw, dr, dz, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(z, dz, (size_t)s->n * 16, hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  dfree(dr); dfree(dz); dfree(dw);
+  if (e != hipSuccess) return fail(AFX_ERR_HIP, hipGetErrorString(e));
+  return AFX_OK;
+}
+
+int afx_solve_device(afx_system s, int nrhs, const void *dB, void *dX, const double *omega_scale,
+                     const afx_opts *opts, afx_stats *stats) {
+  if (!s || !dB || !dX || nrhs <= 0) return fail(AFX_ERR_ARG, "bad arguments");
+  afx_opts o;
+  if (opts) o = *opts;
+  else afx_default_opts(&o);
+  return solve_device_impl(s, nrhs, (const cx<double> *)dB, (cx<double> *)dX, omega_scale, &o, stats);
+}
+
+int afx_solve(afx_system s, int nrhs, const double *B, int ldb, double *X, int ldx, const afx_opts *opts,
+              afx_stats *stats) {
+  if (!s || !B || !X || nrhs <= 0 || ldb < s->n || ldx < s->n) return fail(AFX_ERR_ARG, "bad arguments");
+  const size_t n = s->n;
+  std::vector<cx<double>> hb(n * nrhs), hx(n * nrhs);
+  for (int c = 0; c < nrhs; c++)
+    for (size_t i = 0; i < n; i++) {
+      hb[i * nrhs + c] = cx<double>{B[2 * ((size_t)c * ldb + i)], B[2 * ((size_t)c * ldb + i) + 1]};
+      hx[i * nrhs + c] = cx<double>{X[2 * ((size_t)c * ldx + i)], X[2 * ((size_t)c * ldx + i) + 1]};
+    }
+  cx<double> *dB = nullptr, *dX = nullptr;
+  int rc;
+  if ((rc = upload(&dB, hb.data(), hb.size())) || (rc = upload(&dX, hx.data(), hx.size()))) {
+    dfree(dB);
+    dfree(dX);
+    return rc;
+  }
+  afx_opts o;
+  if (opts) o = *opts;
+  else afx_default_opts(&o);
+  rc = solve_device_impl(s, nrhs, dB, dX, nullptr, &o, stats);
+  if (rc == AFX_OK || rc == AFX_ERR_MAXIT || rc == AFX_ERR_BREAKDOWN) {
+    if (hipMemcpy(hx.data(), dX, hx.size() * sizeof(cx<double>), hipMemcpyDeviceToHost) != hipSuccess) rc = AFX_ERR_HIP;
+    for (int c = 0; c < nrhs; c++)
+      for (size_t i = 0; i < n; i++) {
+        X[2 * ((size_t)c * ldx + i)] = hx[i * nrhs + c].r;
+        X[2 * ((size_t)c * ldx + i) + 1] = hx[i * nrhs + c].i;
+      }
+  }
+  dfree(dB);
+  dfree(dX);
+  return rc;
+}
+
+int afx_true_residual_device(afx_system s, int nrhs, const void *dB, const void *dX, const double *omega_scale,
+                             double *res) {
+  if (!s || !dB || !dX || !res || nrhs <= 0) return fail(AFX_ERR_ARG, "bad arguments");
+  int rc = ensure_derived(s);
+  if (rc) return rc;
+  std::vector<double> wcol, wf;
+  std::vector<int> col2f;
+  column_freqs(s, nrhs, omega_scale, wcol, wf, col2f);
+  const int G = 2 * s->ncu;
+  cx<double> *dY = nullptr;
+  double *dw = nullptr, *pa = nullptr, *pb = nullptr, *norms = nullptr;
+  if ((rc = dalloc(&dY, (size_t)s->n * nrhs)) || (rc = upload(&dw, wcol.data(), nrhs)) ||
+      (rc = dalloc(&pa, (size_t)G * nrhs)) || (rc = dalloc(&pb, (size_t)G * nrhs)) ||
+      (rc = dalloc(&norms, 2 * (size_t)nrhs))) {
+    dfree(dY); dfree(dw); dfree(pa); dfree(pb); dfree(norms);
+    return rc;
+  }
+  afx::SpmvParams sp{};
+  sp.n = s->n;
+  sp.k = nrhs;
+  sp.kp = nrhs > 1 ? next_pow2(nrhs) : 1;
+  sp.rowptr = s->d_rowptr;
+  sp.col = s->d_col;
+  sp.val = s->d_val64;
+  sp.wcol = dw;
+  sp.X = (const cx<double> *)dX;
+  sp.B = (const cx<double> *)dB;
+  sp.Y = dY;
+  sp.part_rr = pa;
+  sp.part_bb = pb;
+  hipError_t e = (sp.kp > 256) ? hipErrorInvalidValue : afx::launch_residual_norms(sp, G, norms, s->stream);
+  std::vector<double> h(2 * (size_t)nrhs);
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), norms, h.size() * 8, hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  dfree(dY); dfree(dw); dfree(pa); dfree(pb); dfree(norms);
+  if (e != hipSuccess) return fail(AFX_ERR_HIP, hipGetErrorString(e));
+  for (int c = 0; c < nrhs; c++) res[c] = (h[2 * c + 1] > 0) ? std::sqrt(h[2 * c] / h[2 * c + 1]) : 0.0;
+  return AFX_OK;
+}
+
+}  // extern "C"
+
+// -----------------------------------------------------------------------------------------
+namespace {
+const int KLH[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+const int LMNH[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+const int CROTH[4][6] = {{0, 1, 0, 0, -1, 1}, {0, 0, -1, 1, 0, -1}, {1, 0, 0, -1, 1, 0}, {-1, -1, 1, 0, 0, 0}};
+
+inline double in3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline void out3(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+inline double vol3(const double *a, const double *b, const double *c) {
+  double t[3];
+  out3(a, b, t);
+  return 1.0 / 6.0 * in3(t, c);
+}
+// gradnodebasisfun (m_fem_util.f90:12-40)
+bool grad_h(const double x[4][3], double gn[4][3], double &v) {
+  double x34[3], x13[3], x23[3], a[3], b[3], c[3];
+  for (int d = 0; d < 3; d++) {
+    x34[d] = x[3][d] - x[2][d];
+    x13[d] = x[0][d] - x[2][d];
+    x23[d] = -(x[2][d] - x[1][d]);
+  }
+  v = vol3(x34, x13, x23);
+  if (!(v > 0)) return false;
+  const double s = 1.0 / 6.0 / v;
+  const int P[4][4] = {{3, 1, 2, 1}, {2, 0, 3, 0}, {3, 0, 1, 0}, {1, 0, 2, 0}};
+  for (int k = 0; k < 4; k++) {
+    for (int d = 0; d < 3; d++) {
+      a[d] = x[P[k][0]][d] - x[P[k][1]][d];
+      b[d] = x[P[k][2]][d] - x[P[k][3]][d];
+    }
+    out3(a, b, c);
+    for (int d = 0; d < 3; d++) gn[k][d] = s * c[d];
+  }
+  return true;
+}
+// nodebasisfun (m_fem_util.f90:56-83)
+void lambda_h(const double x[4][3], const double *p, double a[4]) {
+  double v0[3], v1[3], x12[3], x13[3], x14[3], x24[3], x23[3];
+  for (int d = 0; d < 3; d++) {
+    v0[d] = -(x[0][d] - p[d]);
+    v1[d] = -(x[1][d] - p[d]);
+    x12[d] = x[1][d] - x[0][d];
+    x13[d] = x[2][d] - x[0][d];
+    x14[d] = x[3][d] - x[0][d];
+    x24[d] = x[3][d] - x[1][d];
+    x23[d] = x[2][d] - x[1][d];
+  }
+  const double a5 = vol3(x12, x13, x14);
+  a[3] = vol3(x12, x13, v0) / a5;
+  a[2] = vol3(x14, x12, v0) / a5;
+  a[1] = vol3(x13, x14, v0) / a5;
+  a[0] = vol3(x24, x23, v1) / a5;
+}
+}  // namespace
+
+extern "C" {
+
+int afx_autotune(afx_system s, int nrhs, int verbose) {
+  if (!s || nrhs <= 0 || nrhs > 256) return fail(AFX_ERR_ARG, "bad arguments");
+  if (!s->have_values) return fail(AFX_ERR_ARG, "no matrix values");
+  const int kind = s->pc_valid ? s->pc_kind : AFX_PC_SSOR;
+  const size_t nk = (size_t)s->n * nrhs;
+  std::vector<cx<double>> hb(nk, cx<double>{1.0, 0.5});
+  cx<double> *dB = nullptr, *dX = nullptr;
+  int rc;
+  if ((rc = upload(&dB, hb.data(), nk)) || (rc = dalloc(&dX, nk))) {
+    dfree(dB);
+    return rc;
+  }
+  afx_opts o;
+  afx_default_opts(&o);
+  o.precond = kind;
+  o.tol = 0.0;
+  o.maxit = 20;
+  o.check_true = 0;
+  o.nset = s->pc_valid ? 0 : 1;
+  const bool multi = nrhs > 1;
+  std::vector<int> lprs = multi ? std::vector<int>{1} : std::vector<int>{2, 4, 8, 16};
+  for (int prec = 0; prec < 2; prec++) {
+    const bool fp64 = prec == 0;
+    double best = 1e300;
+    int best_lpr = 8, best_g = 0;
+    for (int lpr : lprs) {
+      (fp64 ? s->lpr64 : s->lpr32) = lpr;
+      const int gmax = afx::max_coop_grid(lpr, multi, fp64);
+      for (int g : {s->ncu, 2 * s->ncu, 4 * s->ncu, gmax}) {
+        if (g <= 0 || g > gmax) continue;
+        int &gs = fp64 ? (multi ? s->grid_multi64 : s->grid_single64) : (multi ? s->grid_multi32 : s->grid_single32);
+        gs = g;
+        if (hipMemset(dX, 0, nk * sizeof(cx<double>)) != hipSuccess) break;
+        o.precision = fp64 ? AFX_FP64 : AFX_MIXED;
+        o.max_outer = 1;
+        o.inner_tol = 0.0;
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<afx_stats> st(nrhs);
+        int r2 = solve_device_impl(s, nrhs, dB, dX, nullptr, &o, st.data());
+        (void)hipDeviceSynchronize();
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (r2 != AFX_OK && r2 != AFX_ERR_MAXIT) continue;
+        if (verbose) fprintf(stderr, "afx_autotune %s lpr=%d grid=%d: %.3f ms / 20 it\n", fp64 ? "fp64" : "fp32", lpr, g, dt * 1e3);
+        if (dt < best) {
+          best = dt;
+          best_lpr = lpr;
+          best_g = g;
+        }
+      }
+    }
+    (fp64 ? s->lpr64 : s->lpr32) = best_lpr;
+    (fp64 ? (multi ? s->grid_multi64 : s->grid_single64) : (multi ? s->grid_multi32 : s->grid_single32)) = best_g;
+  }
+  dfree(dB);
+  dfree(dX);
+  return AFX_OK;
+}
+
+int afx_cocg13_single(int NP, int NPL, int NPU, double *D, double *AL, const int *INL, const int *IAL, double *AU,
+                      const int *INU, const int *IAU, double *B, double *X, const char *PRECOND, double SIGMA_DIAG,
+                      double SIGMA, double *RESID, int *ITER, int *ERROR, int NSET) {
+  if (!D || !B || !X || !RESID || !ITER || !ERROR) return fail(AFX_ERR_ARG, "null argument");
+  *ERROR = 0;
+  if (NSET == 0) {  // no SAVEd preconditioner to reuse (COCG13_single.f90:49-52)
+    *ERROR = AFX_ERR_BADOPT;
+    return AFX_ERR_BADOPT;
+  }
+  if (INU[NP] != NPU || INL[NP] != NPL) return fail(AFX_ERR_ARG, "NPU/NPL do not match INU/INL");
+  std::vector<double> scale(NP, 1.0);
+  if (NSET == 2) {  // COCG13_single.f90:77-97, in place on the caller's arrays
+    for (int i = 0; i < NP; i++) scale[i] = 1.0 / std::sqrt(std::hypot(D[2 * i], D[2 * i + 1]));
+    for (int j = 0; j < NP; j++) {
+      const double ad = std::hypot(D[2 * j], D[2 * j + 1]);
+      const double dr = D[2 * j], di = D[2 * j + 1], den = dr * dr + di * di;
+      D[2 * j] = ad * dr / den;
+      D[2 * j + 1] = -ad * di / den;
+      for (int q = INU[j]; q < INU[j + 1]; q++) {
+        const double f = scale[IAU[q] - 1] * scale[j];
+        AU[2 * q] *= f;
+        AU[2 * q + 1] *= f;
+      }
+      for (int q = INL[j]; q < INL[j + 1]; q++) {
+        const double f = scale[IAL[q] - 1] * scale[j];
+        AL[2 * q] *= f;
+        AL[2 * q + 1] *= f;
+      }
+    }
+  }
+  for (int i = 0; i < NP; i++) {
+    B[2 * i] *= scale[i];
+    B[2 * i + 1] *= scale[i];
+  }
+  afx_system s = nullptr;
+  int rc = afx_create(NP, INU, IAU, INL, IAL, &s);
+  if (rc) return rc;
+  rc = afx_set_values(s, D, AU, AL);
+  afx_opts o;
+  afx_default_opts(&o);
+  o.precond = pc_from_string(PRECOND);
+  o.sigma_diag = SIGMA_DIAG;
+  o.sigma = SIGMA;
+  o.tol = *RESID;
+  o.maxit = *ITER;
+  o.nset = 1;
+  o.check_true = 0;
+  afx_stats st{};
+  if (rc == AFX_OK) rc = afx_solve(s, 1, B, NP, X, NP, &o, &st);
+  afx_destroy(s);
+  if (rc != AFX_OK && rc != AFX_ERR_MAXIT && rc != AFX_ERR_BREAKDOWN) return rc;
+  for (int i = 0; i < NP; i++) {  // :326-329
+    X[2 * i] *= scale[i];
+    X[2 * i + 1] *= scale[i];
+    B[2 * i] /= scale[i];
+    B[2 * i + 1] /= scale[i];
+  }
+  *RESID = st.resid;
+  *ITER = st.iter;
+  *ERROR = st.error;
+  return rc;
+}
+
+int afx_solve_cg(int doftot, int NPL, int NPU, const double *D, const double *AL, const int *INL, const int *IAL,
+                 const double *AU, const int *INU, const int *IAU, const double *rhs, double *X) {
+  // solveCG (forward_bxyz_mpi.f90:70-94)
+  std::vector<double> d(D, D + 2 * (size_t)doftot), al(AL, AL + 2 * (size_t)NPL), au(AU, AU + 2 * (size_t)NPU),
+      b(rhs, rhs + 2 * (size_t)doftot);
+  for (int i = 0; i < 2 * doftot; i++) X[i] = 0.0;
+  double resid = 1.e-4f;  // RESID=1.e-4 is a default-real literal
+  int iter = 3000, error = 0;
+  char pc[21] = "SSOR                ";
+  int rc = afx_cocg13_single(doftot, NPL, NPU, d.data(), al.data(), INL, IAL, au.data(), INU, IAU, b.data(), X, pc,
+                             1.0, 0.0, &resid, &iter, &error, 1);
+  return rc;
+}
+
+int afx_prepobscoeff(afx_system s, int nobs, const double *xyzobs, int *item, double *val_e, double *val_b,
+                     int *iele) {
+  if (!s || !xyzobs || !item || !val_e || !val_b) return fail(AFX_ERR_ARG, "null argument");
+  if (!s->have_mesh) return fail(AFX_ERR_ARG, "afx_set_mesh first");
+  if (nobs <= 0) return AFX_OK;
+  double *dp = nullptr;
+  int *dfirst = nullptr;
+  int rc;
+  if ((rc = upload(&dp, xyzobs, (size_t)nobs * 3)) || (rc = dalloc(&dfirst, nobs))) {
+    dfree(dp);
+    return rc;
+  }
+  std::vector<int> first(nobs, 0x7fffffff);
+  hipError_t e = hipMemcpy(dfirst, first.data(), nobs * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = afx::launch_findelement(nobs, dp, s->ntet, s->d_xyz, s->d_n4, dfirst, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(first.data(), dfirst, nobs * sizeof(int), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  dfree(dp);
+  dfree(dfirst);
+  if (e != hipSuccess) return fail(AFX_ERR_HIP, hipGetErrorString(e));
+  for (int o = 0; o < nobs; o++) {
+    const int t = first[o];
+    if (t == 0x7fffffff) return fail(AFX_ERR_OUTSIDE, "GEGEGE! No corresponding element was found");
+    if (iele) iele[o] = t;
+    double x[4][3], gn[4][3], v, lam[4], w[6][3], wf[4][3], wfe[6][3];
+    for (int j = 0; j < 4; j++)
+      for (int d = 0; d < 3; d++) x[j][d] = s->h_xyz[(size_t)(s->h_n4[(size_t)t * 4 + j] - 1) * 3 + d];
+    grad_h(x, gn, v);
+    lambda_h(x, &xyzobs[o * 3], lam);
+    for (int ed = 0; ed < 6; ed++) {  // EDGEBASISFUN (m_fem_edge_util.f90:198-218)
+      const int k = KLH[ed][0], l = KLH[ed][1];
+      for (int d = 0; d < 3; d++) w[ed][d] = lam[k] * gn[l][d] - lam[l] * gn[k][d];
+    }
+    for (int f = 0; f < 4; f++) {  // FACEBASISFUN (:163-195)
+      const int l = LMNH[f][0], m = LMNH[f][1], nn = LMNH[f][2];
+      double c1[3], c2[3], c3[3];
+      out3(gn[m], gn[nn], c1);
+      out3(gn[l], gn[m], c2);
+      out3(gn[nn], gn[l], c3);
+      for (int d = 0; d < 3; d++) wf[f][d] = 2.0 * (lam[l] * c1[d] + lam[nn] * c2[d] + lam[m] * c3[d]);
+    }
+    for (int ed = 0; ed < 6; ed++)
+      for (int d = 0; d < 3; d++) {
+        double acc = 0.0;
+        for (int f = 0; f < 4; f++) acc += wf[f][d] * (double)CROTH[f][ed];
+        wfe[ed][d] = acc;
+      }
+    int order[6] = {0, 1, 2, 3, 4, 5}, ids[6];
+    for (int ed = 0; ed < 6; ed++) ids[ed] = std::abs(s->n6line[(size_t)t * 6 + ed]);
+    std::sort(order, order + 6, [&](int a, int b) { return ids[a] < ids[b]; });  // SORT_INDEX by edge id
+    for (int j = 0; j < 6; j++) {
+      const int ed = order[j];
+      const double sg = (s->n6line[(size_t)t * 6 + ed] < 0) ? -1.0 : 1.0;
+      item[o * 6 + j] = ids[ed];
+      for (int c = 0; c < 3; c++) {
+        val_e[(size_t)c * nobs * 6 + o * 6 + j] = w[ed][c] * sg;
+        val_b[(size_t)c * nobs * 6 + o * 6 + j] = wfe[ed][c] * sg;
+      }
+    }
+  }
+  return AFX_OK;
+}
+
+int afx_obs_apply_device(afx_system s, int nobs, const int *item, const double *val, int ncol, const void *dX,
+                         double *out) {
+  if (!s || !item || !val || !dX || !out) return fail(AFX_ERR_ARG, "null argument");
+  for (int q = 0; q < nobs * 6; q++)
+    if (item[q] < 1 || item[q] > s->n) return fail(AFX_ERR_ARG, "item out of range");
+  int *di = nullptr;
+  double *dv = nullptr;
+  cx<double> *dout = nullptr;
+  int rc;
+  if ((rc = upload(&di, item, (size_t)nobs * 6)) || (rc = upload(&dv, val, (size_t)nobs * 6)) ||
+      (rc = dalloc(&dout, (size_t)nobs * ncol))) {
+    dfree(di); dfree(dv);
+    return rc;
+  }
+  hipError_t e = afx::launch_obs_apply(nobs, di, dv, ncol, (const cx<double> *)dX, dout, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, (size_t)nobs * ncol * 16, hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  dfree(di); dfree(dv); dfree(dout);
+  if (e != hipSuccess) return fail(AFX_ERR_HIP, hipGetErrorString(e));
+  return AFX_OK;
+}
+
+int afx_jacobian_device(afx_system s, int nobs, const void *dU, const void *dx, int npar, const int *par_ptr,
+                        const int *par_tet, const double *sigma_tet, double omega, double dlnsig_dm, void *dJ) {
+  if (!s || !dU || !dx || !par_ptr || !par_tet || !sigma_tet || !dJ) return fail(AFX_ERR_ARG, "null argument");
+  if (!s->have_mesh) return fail(AFX_ERR_ARG, "afx_set_mesh first");
+  const int nsel = par_ptr[npar];
+  for (int q = 0; q < nsel; q++)
+    if (par_tet[q] < 0 || par_tet[q] >= s->ntet) return fail(AFX_ERR_ARG, "par_tet out of range");
+  int *dptr = nullptr, *dtet = nullptr, *dedges = nullptr;
+  double *dsig = nullptr;
+  cx<double> *dw = nullptr;
+  int rc;
+  if ((rc = upload(&dptr, par_ptr, (size_t)npar + 1)) || (rc = upload(&dtet, par_tet, (size_t)nsel)) ||
+      (rc = upload(&dsig, sigma_tet, (size_t)s->ntet)) || (rc = dalloc(&dw, (size_t)nsel * 6)) ||
+      (rc = dalloc(&dedges, (size_t)nsel * 6))) {
+    dfree(dptr); dfree(dtet); dfree(dsig); dfree(dw);
+    return rc;
+  }
+  hipError_t e = afx::launch_jac_w(nsel, dtet, s->d_xyz, s->d_n4, s->d_n6, dsig, omega, dlnsig_dm,
+                                   (const cx<double> *)dx, dw, dedges, s->stream);
+  if (e == hipSuccess)
+    e = afx::launch_jac_acc(npar, dptr, nobs, (const cx<double> *)dU, dw, dedges, (cx<double> *)dJ, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  dfree(dptr); dfree(dtet); dfree(dsig); dfree(dw); dfree(dedges);
+  if (e != hipSuccess) return fail(AFX_ERR_HIP, hipGetErrorString(e));
+  return AFX_OK;
+}
+
+}  // extern "C"

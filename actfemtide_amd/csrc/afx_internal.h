@@ -1,0 +1,127 @@
+// Internal declarations shared by the afx HIP sources (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/afx.h"
+
+namespace afx {
+
+// complex number as a (re, im) pair; for matrix values the same pair holds (K, M) with the
+// effective entry K + i*w*M for a column whose angular-frequency factor is w.
+template <typename T>
+struct alignas(2 * sizeof(T)) cx {
+  T r, i;
+};
+
+template <typename T>
+struct CocgParams {
+  int n, k, kp, nf;  // rows, columns, padded columns (lanes per row slot), distinct frequencies
+  int pc;            // AFX_PC_*
+  int dist_reduce;   // 1: block c reduces column c then grid sync; 0: every block reduces
+  // full CSR (row order: diagonal, strict upper, strict lower)
+  const int *rowptr, *col;
+  const cx<T> *val;
+  // sweeps, rows in level order
+  int nlevL, nlevU;
+  const int *lvlL, *rowL, *Lptr, *Lcol;
+  const cx<T> *Lval;
+  const int *lvlU, *rowU, *Uptr, *Ucol;
+  const cx<T> *Uval;
+  const cx<T> *DD;   // [n][nf]
+  const int *col2f;  // [k]
+  const T *wcol;     // [k] omega of each column
+  // vectors [n][k]
+  const cx<T> *B;
+  cx<T> *X, *R, *Z, *P, *Q;
+  // reductions [G][k] and finals [k]
+  cx<double> *part_rho, *part_c1;
+  double *part_rr;
+  cx<double> *fin_rho, *fin_c1;
+  double *fin_rr;
+  // per-column state
+  cx<double> *rho1;
+  double *bnrm2;
+  int *conv;   // 0 running, 1 converged, 2 zero rhs, -100 maxit, -200 breakdown
+  int *iters;
+  double *resid;
+  int *status;  // [0] iterations done so far, [1] all columns finished
+  double tol;
+  int maxit, chunk, init;
+};
+
+struct SpmvParams {
+  int n, k, kp;  // kp: k rounded up to a power of two (lanes per row slot); 1 when k == 1
+  const int *rowptr, *col;
+  const cx<double> *val;
+  const double *wcol;
+  const cx<double> *X;
+  const cx<double> *B;  // optional: computes B - A X when non-null
+  cx<double> *Y;
+  double *part_rr, *part_bb;  // optional [G][k] partial |Y|^2 and |B|^2
+};
+
+// launchers (afx_solver.hip)
+hipError_t launch_cocg64(const CocgParams<double> &p, int lpr, int grid, hipStream_t s);
+hipError_t launch_cocg32(const CocgParams<float> &p, int lpr, int grid, hipStream_t s);
+int max_coop_grid(int lpr, bool multi, bool fp64);
+int max_coop_grid_ic();
+hipError_t launch_spmv(const SpmvParams &p, hipStream_t s);
+hipError_t launch_residual_norms(const SpmvParams &p, int grid, double *res_out_dev, hipStream_t s);
+hipError_t launch_precond_setup(int n, int nf, int kind, double sd, double sig, const cx<double> *Dkm,
+                                const int *Dpos_full, const cx<double> *full_val, const int *rowptr,
+                                int nlevL, const int *lvlL, const int *rowL, const int *Lptr,
+                                const int *Lcol, const cx<double> *Lval, const int *Uptr_rowmajor,
+                                const cx<double> *AUkm, const double *wf, cx<double> *DD, int grid,
+                                hipStream_t s);
+hipError_t launch_precond_apply_levels(int n, int kind, int nlevL, const int *lvlL, const int *rowL,
+                                       const int *Lptr, const int *Lcol, const cx<double> *Lval, int nlevU,
+                                       const int *lvlU, const int *rowU, const int *Uptr, const int *Ucol,
+                                       const cx<double> *Uval, const cx<double> *DD, const double *w,
+                                       const cx<double> *r, cx<double> *z, hipStream_t s);
+hipError_t launch_gather_vals(int m, const int *map, const cx<double> *canon, cx<double> *out,
+                              cx<float> *out32, hipStream_t s);
+hipError_t launch_cast_vec(long long m, const cx<double> *in, cx<float> *out, hipStream_t s);
+hipError_t launch_axpy_f32_to_f64(long long m, const cx<float> *d, cx<double> *x, hipStream_t s);
+hipError_t launch_scale_cols(long long n, int k, const cx<double> *in, const double *scale,
+                             cx<float> *out, hipStream_t s);
+hipError_t launch_axpy_scaled(long long n, int k, const cx<float> *d, const double *scale,
+                              cx<double> *x, hipStream_t s);
+
+// assembly and post-processing (afx_assembly.hip)
+struct ElemParams {
+  int ntet;
+  const double *xyz;
+  const int *n4, *n6line, *group;
+  const double *sigma;
+  const cx<double> *vxyz;
+  const double *fxyz;
+  double *eK, *eM;   // [21][ntet] element entries (6 diagonal, 15 strictly upper pairs)
+  cx<double> *eB;    // [6][ntet] tidal right-hand side per local edge (without omega)
+  int *err;          // set to 1 when a volume is <= 0
+};
+hipError_t launch_element(const ElemParams &p, hipStream_t s);
+hipError_t launch_slot_gather(int nslot, const int *cptr, const int *cidx, const double *eK, const double *eM,
+                              cx<double> *out, hipStream_t s);
+hipError_t launch_rhs_gather(int n, const int *cptr, const int *cidx, const cx<double> *eB, double omega_unused,
+                             cx<double> *b, hipStream_t s);
+hipError_t launch_mirror(long long nL, const int *mirror, const cx<double> *AU, cx<double> *AL, hipStream_t s);
+hipError_t launch_bc_values(int n, const int *INU, const int *IAU, const int *INL, const int *IAL,
+                            const unsigned char *dir, cx<double> *D, cx<double> *AU, cx<double> *AL,
+                            hipStream_t s);
+hipError_t launch_genbc(int nline, const int *line, const double *xyz, double zmin, double zmax, double xout,
+                        double yout, unsigned char *bc, hipStream_t s);
+hipError_t launch_findelement(int nobs, const double *pts, int ntet, const double *xyz, const int *n4,
+                              int *first, hipStream_t s);
+hipError_t launch_obs_apply(int nobs, const int *item, const double *val, int ncol, const cx<double> *X,
+                            cx<double> *out, hipStream_t s);
+hipError_t launch_jac_w(int ntet_sel, const int *tets, const double *xyz, const int *n4, const int *n6line,
+                        const double *sigma, double omega, double dlnsig, const cx<double> *x, cx<double> *w,
+                        int *edges, hipStream_t s);
+hipError_t launch_jac_acc(int npar, const int *par_ptr, int nobs, const cx<double> *U, const cx<double> *w,
+                          const int *edges, cx<double> *J, hipStream_t s);
+
+}  // namespace afx

@@ -1,0 +1,196 @@
+/*
+ * afx.h -- C ABI of the MI355X (gfx950) tidal edge-FEM hot path.
+ *
+ * Conventions shared by every entry point:
+ *   - plain pointers and sizes only; complex numbers are interleaved (re, im) doubles,
+ *     exactly the memory layout of Fortran complex(8) arrays;
+ *   - index arrays use the reference's layout: INU/INL are offsets (0:n) starting at 0,
+ *     IAU/IAL/n4/n6line/line hold 1-based ids;
+ *   - host-pointer entry points copy in and out; *_device entry points take device
+ *     pointers that already live in HBM (same layouts);
+ *   - return codes mirror COCG13_single (0 ok, -100 max iterations, 101 bad options) and
+ *     add AFX_ERR_* for conditions where the reference calls `stop`;
+ *   - one afx_system per host thread; calls on distinct systems are independent.
+ *     There is no hidden global state.
+ */
+#ifndef AFX_H
+#define AFX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AFX_OK 0
+#define AFX_ERR_MAXIT (-100)     /* COCG13_single.f90:320 ERROR = -100 */
+#define AFX_ERR_BADOPT 101       /* COCG13_single.f90:49-52 ERROR = 101 */
+#define AFX_ERR_BREAKDOWN (-200) /* rho or p^T q became 0 / non-finite (no reference code) */
+#define AFX_ERR_HIP (-300)       /* a HIP runtime call failed */
+#define AFX_ERR_ARG (-400)       /* invalid argument or size */
+#define AFX_ERR_GEOM (-500)      /* tet volume <= 0: gradnodebasisfun stops (m_fem_util.f90:23,31-39) */
+#define AFX_ERR_PATTERN (-600)   /* CSR slot missing: sup_iccg stops (m_iccg_var_takuto.f90:85-88) */
+#define AFX_ERR_OUTSIDE (-700)   /* point outside the mesh: FINDELEMENT0 stops (m_mesh_type.f90:731-733) */
+
+/* preconditioner kinds, PRECOND of COCG13_single.f90:107,126,202,236 */
+#define AFX_PC_NONE 0
+#define AFX_PC_IC 1   /* 'IC' / 'ILU': diagonal-modified ILU (D-ILU) */
+#define AFX_PC_SSOR 2 /* 'SSOR' */
+#define AFX_PC_DIAG 3 /* 'DIAG' */
+
+/* arithmetic of the Krylov loop */
+#define AFX_FP64 0  /* all-FP64 COCG */
+#define AFX_MIXED 1 /* complex-FP32 SpMV / sweeps inside FP64 iterative refinement */
+
+typedef struct afx_system_s *afx_system;
+
+typedef struct {
+  int precond;       /* AFX_PC_* */
+  double sigma_diag; /* SIGMA_DIAG */
+  double sigma;      /* SIGMA */
+  double tol;        /* RESID in: relative recurrence residual ||r||/||b|| */
+  int maxit;         /* ITER in */
+  int nset;          /* NSET: 1 preconditioned, 2 scaled + preconditioned, 0 reuse setup */
+  int precision;     /* AFX_FP64 or AFX_MIXED */
+  double inner_tol;  /* AFX_MIXED: relative tolerance of each FP32 inner solve */
+  int max_outer;     /* AFX_MIXED: maximum refinement steps */
+  int chunk;         /* iterations per persistent launch (0: default) */
+  int check_true;    /* 1: recompute the FP64 true residual at the end */
+} afx_opts;
+
+typedef struct {
+  int iter;          /* COCG iterations (reference ITER out; MAXIT + 1 when exhausted) */
+  int error;         /* 0, -100, -200 */
+  double resid;      /* recurrence residual ||r_k||/||b|| (reference RESID out) */
+  double true_resid; /* FP64 ||b - A x||/||b|| (when check_true) */
+  int outer;         /* refinement steps (AFX_MIXED) */
+} afx_stats;
+
+void afx_default_opts(afx_opts *o);
+const char *afx_last_error(void);
+
+/* ---- system lifecycle -------------------------------------------------------------- */
+
+/* Pattern from the split CSR of type(global_matrix) (src/solver/m_iccg_var_takuto.f90:9-18):
+   analyses level sets, builds the SpMV and sweep layouts on the device. */
+int afx_create(int n, const int *INU, const int *IAU, const int *INL, const int *IAL, afx_system *out);
+
+/* Pattern from the tet->edge map, replacing set_iccg_var7_dofn + C_set_dof2
+   (src/solver/m_iccg_var_takuto.f90:277-485, called at src/solver_mpi/n_ebfem_bxyz_mpi.f90:104-107). */
+int afx_create_from_n6line(int nline, int ntet, const int *n6line, afx_system *out);
+
+int afx_destroy(afx_system s);
+int afx_get_sizes(afx_system s, int *n, int *npu, int *npl, int *nlev_l, int *nlev_u);
+int afx_get_pattern(afx_system s, int *INU, int *IAU, int *INL, int *IAL);
+
+/* ---- matrix values ----------------------------------------------------------------- */
+
+/* Set D(n), AU(npu), AL(npl) (complex, host) -- the arrays COCG13_single receives
+   (COCG13_single.f90:25-29). */
+int afx_set_values(afx_system s, const double *D, const double *AU, const double *AL);
+int afx_get_values(afx_system s, double *D, double *AU, double *AL);
+
+/* Mesh for on-device assembly: xyz(node,3) [km], n4(ntet,4), group(ntet) = n4flag(:,2),
+   vxyz(node,3) complex [m/s], fxyz(node,3) [nT], all row-major (C order); n6line is the
+   one given at afx_create_from_n6line. Mirrors the data GENMAT reads
+   (src/solver_mpi/forward_bxyz_mpi.f90:99-155). */
+int afx_set_mesh(afx_system s, int node, int ntet, const double *xyz, const int *n4, const int *group,
+                 const double *vxyz, const double *fxyz);
+
+/* GENMAT + sup_iccg + COPY_UL_ICCG12 (forward_bxyz_mpi.f90:99-311; m_iccg_var_takuto.f90:43-167)
+   on the device: one thread per tet, then a deterministic slot-ordered reduction.
+   sigma_tet(ntet) [S/m] is the per-element conductivity GENMAT selects (:196-208).
+   The assembled matrix is K + i*omega*M; afx_assemble stores K and M separately so that
+   afx_set_omega can switch frequency without re-assembly. b_out (nline complex, host or NULL)
+   receives the tidal right-hand side for this omega. */
+int afx_assemble(afx_system s, const double *sigma_tet, double omega, double *b_out);
+int afx_set_omega(afx_system s, double omega);
+
+/* GENBCCSEM (forward_bxyz_mpi.f90:467-519): Dirichlet mask of the edges lying within 1 km of
+   one outer face; line(nline,2) 1-based. line_bc out (nline bytes). */
+int afx_genbc(afx_system s, const int *line, double zmin, double zmax, double xout, double yout,
+              unsigned char *line_bc);
+
+/* SET_BC_ICCG (forward_bxyz_mpi.f90:567-636) with zero Dirichlet values: zeroes Dirichlet rows
+   and columns in place (pattern kept), D = 1, and zeroes rhs(n, nrhs) (host, may be NULL)
+   at Dirichlet rows. */
+int afx_set_bc(afx_system s, const unsigned char *dirichlet, int nrhs, double *rhs, int ldr);
+
+/* ---- preconditioner and solves ----------------------------------------------------- */
+
+/* Preconditioner setup of COCG13_single.f90:103-130 on the device. */
+int afx_precond_setup(afx_system s, int kind, double sigma_diag, double sigma);
+
+/* Solve A X = B for nrhs right-hand sides with COCG (COCG13_single.f90:181-323 per column,
+   all columns sharing each matrix read). B(n,nrhs), X(n,nrhs) column-major host arrays
+   (leading dimensions ldb, ldx); X holds the initial guess on entry (warm start) and the
+   solution on exit. stats: nrhs entries or NULL. Returns 0, -100 (some column hit maxit),
+   -200, or an AFX_ERR_*. */
+int afx_solve(afx_system s, int nrhs, const double *B, int ldb, double *X, int ldx, const afx_opts *opts,
+              afx_stats *stats);
+
+/* Same as afx_solve with B and X already in device memory, row-interleaved: element (i, c)
+   at index i*nrhs + c (the layout the batched kernels use). omega_scale (nrhs doubles, host,
+   or NULL for all ones) gives each column its own frequency: column c solves
+   (K + i*omega_c*M) x = b with omega_c = omega * omega_scale[c] (needs afx_assemble values). */
+int afx_solve_device(afx_system s, int nrhs, const void *dB, void *dX, const double *omega_scale,
+                     const afx_opts *opts, afx_stats *stats);
+
+/* FP64 true residual ||b - A x|| / ||b|| per column of device, row-interleaved B and X. */
+int afx_true_residual_device(afx_system s, int nrhs, const void *dB, const void *dX,
+                             const double *omega_scale, double *res);
+
+/* Autotune rows-per-wavefront and persistent grid size for nrhs columns on this system. */
+int afx_autotune(afx_system s, int nrhs, int verbose);
+
+/* y = A x (complex, host arrays) with the device SpMV -- parity hook. */
+int afx_spmv(afx_system s, const double *x, double *y);
+
+/* z = M^-1 r with the device preconditioner (after afx_precond_setup) -- parity hook. */
+int afx_precond_apply(afx_system s, const double *r, double *z);
+
+/* Copy out DD (complex n): the preconditioner diagonal of COCG13_single.f90:105-130. */
+int afx_get_precond_diag(afx_system s, double *DD);
+
+/* ---- drop-in mirrors of the reference solver calls ---------------------------------- */
+
+/* Same arguments and meaning as COCG13_single (src/solver/COCG13_single.f90:9-36), by
+   pointer for the Fortran caller (scalars by value where intent(in)): D, AU, AL modified for
+   NSET = 2; B scaled and restored; RESID, ITER in/out; ERROR out. Unlike the reference it
+   keeps no SAVEd state, so NSET = 0 always returns ERROR = 101. PRECOND is 20 characters. */
+int afx_cocg13_single(int NP, int NPL, int NPU, double *D, double *AL, const int *INL, const int *IAL,
+                      double *AU, const int *INU, const int *IAU, double *B, double *X, const char *PRECOND,
+                      double SIGMA_DIAG, double SIGMA, double *RESID, int *ITER, int *ERROR, int NSET);
+
+/* solveCG (src/solver_mpi/forward_bxyz_mpi.f90:70-94): X = 0, 'SSOR', RESID 1e-4, ITER 3000. */
+int afx_solve_cg(int doftot, int NPL, int NPU, const double *D, const double *AL, const int *INL,
+                 const int *IAL, const double *AU, const int *INU, const int *IAU, const double *rhs, double *X);
+
+/* ---- observation operator ---------------------------------------------------------- */
+
+/* PREPOBSCOEFF (src/solver_mpi/n_ebfem_bxyz_mpi.f90:653-726): for each site, FINDELEMENT0's
+   first containing tet (device search), and the 6-entry rows of edge-basis (E) and face-basis
+   (B = curl A) coefficients sorted by edge id. item(nobs,6) 1-based, val_e/val_b (3,nobs,6),
+   iele(nobs) 0-based. Needs afx_set_mesh. */
+int afx_prepobscoeff(afx_system s, int nobs, const double *xyzobs, int *item, double *val_e, double *val_b,
+                     int *iele);
+
+/* f(nobs, ncol) = sum_j val(o,j) * x(item(o,j), c): mul_matcrs_cv (src/common/m_matrix.f90:83-102)
+   over device, row-interleaved X (n, ncol); out is host (nobs, ncol) complex row-major. */
+int afx_obs_apply_device(afx_system s, int nobs, const int *item, const double *val, int ncol, const void *dX,
+                         double *out);
+
+/* ---- adjoint Jacobian -------------------------------------------------------------- */
+
+/* genjacobian1 (src/src_inv_ap/m_jacobian_ap.f90:174-386) without thresholding:
+     J[m, o] = sum_{t in m} sum_j U[e_j(t), o] * w_t[j],
+     w_t = -(dBB_t/dm) * Y_t x_t,  dBB_t/dm = i*omega*mu0*sigma_t*L0^2 * dlnsig_dm
+   U: device, row-interleaved (nline, nobs) adjoint solutions A^-1 P^T (the afx_solve_device
+   layout); x: device forward solution (nline complex); parameters as CSR par_ptr(npar+1),
+   par_tet (0-based tets). J: device (npar, nobs) complex, o fastest. Needs afx_set_mesh and
+   sigma_tet (ntet) as used in assembly. */
+int afx_jacobian_device(afx_system s, int nobs, const void *dU, const void *dx, int npar, const int *par_ptr,
+                        const int *par_tet, const double *sigma_tet, double omega, double dlnsig_dm, void *dJ);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,243 @@
+"""ctypes front-end of the CPU oracle (liborc.so) -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module,
+as the checker; the product path never does. Every function mirrors one reference
+routine; see afx_oracle.c for the file:line each one follows.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build(force: bool = False) -> str:
+    path = os.path.join(_HERE, "liborc.so")
+    if force or not os.path.exists(path):
+        subprocess.run(["make", "-s", "-C", _HERE, "liborc.so"], check=True)
+    return path
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = C.CDLL(build())
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def pattern(nline: int, n6line: np.ndarray):
+    """C_set_dof2 (m_iccg_var_takuto.f90:361-485): returns INU, IAU, INL, IAL (IAU/IAL 1-based)."""
+    n6 = _i32(n6line)
+    INU = np.zeros(nline + 1, np.int32)
+    INL = np.zeros(nline + 1, np.int32)
+    rc = lib().orc_pattern(C.c_int(nline), C.c_int(n6.shape[0]), _p(n6), _p(INU), None, _p(INL), None)
+    if rc:
+        raise RuntimeError(f"orc_pattern rc={rc}")
+    IAU = np.zeros(INU[-1], np.int32)
+    IAL = np.zeros(INL[-1], np.int32)
+    rc = lib().orc_pattern(C.c_int(nline), C.c_int(n6.shape[0]), _p(n6), _p(INU), _p(IAU), _p(INL), _p(IAL))
+    if rc:
+        raise RuntimeError(f"orc_pattern rc={rc}")
+    return INU, IAU, INL, IAL
+
+
+def genmat(case, omega: float, INU, IAU, sigma=None):
+    """GENMAT + sup_iccg (forward_bxyz_mpi.f90:99-311): returns D, AU, b (complex128)."""
+    sig = _f64(case.sigma_tet() if sigma is None else sigma)
+    n = case.nline
+    D = np.zeros(n, np.complex128)
+    AU = np.zeros(int(INU[-1]), np.complex128)
+    b = np.zeros(n, np.complex128)
+    xyz, n4, grp, n6 = _f64(case.xyz), _i32(case.n4), _i32(case.group), _i32(case.n6line)
+    v, f = _c128(case.vxyz), _f64(case.fxyz)
+    INU, IAU = _i32(INU), _i32(IAU)
+    rc = lib().orc_genmat(C.c_int(case.node), C.c_int(case.ntet), _p(xyz), _p(n4), _p(grp), _p(n6),
+                          C.c_int(n), _p(sig), C.c_double(omega), _p(v), _p(f), _p(INU), _p(IAU),
+                          _p(D), _p(AU), _p(b))
+    if rc:
+        raise RuntimeError(f"orc_genmat rc={rc}")
+    return D, AU, b
+
+
+def genmat_split(case, INU, IAU, sigma=None):
+    """Curl-curl K and conductivity mass M (A = K + i*omega*M) with the sup_iccg slot rule."""
+    sig = _f64(case.sigma_tet() if sigma is None else sigma)
+    n = case.nline
+    Kd = np.zeros(n); Ku = np.zeros(int(INU[-1])); Md = np.zeros(n); Mu = np.zeros(int(INU[-1]))
+    xyz, n4, n6 = _f64(case.xyz), _i32(case.n4), _i32(case.n6line)
+    rc = lib().orc_genmat_split(C.c_int(case.ntet), _p(xyz), _p(n4), _p(n6), _p(sig), _p(_i32(INU)),
+                                _p(_i32(IAU)), _p(Kd), _p(Ku), _p(Md), _p(Mu))
+    if rc:
+        raise RuntimeError(f"orc_genmat_split rc={rc}")
+    return Kd, Ku, Md, Mu
+
+
+def copy_ul(INU, IAU, INL, IAL, AU):
+    n = len(INU) - 1
+    AL = np.zeros(int(INL[-1]), np.complex128)
+    rc = lib().orc_copy_ul(C.c_int(n), _p(_i32(INU)), _p(_i32(IAU)), _p(_i32(INL)), _p(_i32(IAL)),
+                           _p(_c128(AU)), _p(AL))
+    if rc:
+        raise RuntimeError("orc_copy_ul: missing mirror slot")
+    return AL
+
+
+def genbc(case):
+    """GENBCCSEM (forward_bxyz_mpi.f90:467-519): (line_bc bool, line_group int)."""
+    xout, yout, zmin, zmax = case.bounds
+    bc = np.zeros(case.nline, np.uint8)
+    grp = np.zeros(case.nline, np.int32)
+    lib().orc_genbc(C.c_double(zmin), C.c_double(zmax), C.c_double(xout), C.c_double(yout),
+                    C.c_int(case.nline), _p(_i32(case.line)), _p(_f64(case.xyz)), _p(bc), _p(grp))
+    return bc.astype(bool), grp
+
+
+def set_bc(INU, IAU, INL, IAL, D, AU, AL, rf, dirichlet, dval=None):
+    """SET_BC_ICCG (forward_bxyz_mpi.f90:567-636), in place on copies; returns D, AU, AL, rf."""
+    n = len(D)
+    D, AU, AL = _c128(D).copy(), _c128(AU).copy(), _c128(AL).copy()
+    rf = np.asfortranarray(np.asarray(rf, np.complex128).reshape(n, -1)).copy(order="F")
+    nsr = rf.shape[1]
+    dv = np.zeros(n, np.complex128) if dval is None else _c128(dval)
+    dm = np.ascontiguousarray(dirichlet, dtype=np.uint8)
+    lib().orc_set_bc(C.c_int(n), C.c_int(nsr), _p(_i32(INU)), _p(_i32(IAU)), _p(_i32(INL)), _p(_i32(IAL)),
+                     _p(D), _p(AU), _p(AL), rf.ctypes.data_as(C.c_void_p), _p(dv), _p(dm))
+    return D, AU, AL, rf
+
+
+def spmv(INU, IAU, INL, IAL, D, AU, AL, x):
+    n = len(D)
+    y = np.zeros(n, np.complex128)
+    lib().orc_spmv(C.c_int(n), _p(_c128(D)), _p(_i32(INU)), _p(_i32(IAU)), _p(_c128(AU)), _p(_i32(INL)),
+                   _p(_i32(IAL)), _p(_c128(AL)), _p(_c128(x)), _p(y))
+    return y
+
+
+PRECOND_KIND = {"IC": 1, "ILU": 1, "SSOR": 2, "DIAG": 3, "NONE": 0}
+
+
+def precond_setup(kind, D, INU, AU, INL, IAL, AL, sigma_diag=1.0, sigma=0.0):
+    n = len(D)
+    DD = np.zeros(n, np.complex128)
+    lib().orc_precond_setup(C.c_int(n), C.c_int(PRECOND_KIND[kind]), C.c_double(sigma_diag),
+                            C.c_double(sigma), _p(_c128(D)), _p(_i32(INU)), _p(_c128(AU)), _p(_i32(INL)),
+                            _p(_i32(IAL)), _p(_c128(AL)), _p(DD))
+    return DD
+
+
+def precond_apply(kind, INU, IAU, AU, INL, IAL, AL, DD, r):
+    n = len(DD)
+    z = np.zeros(n, np.complex128)
+    lib().orc_precond_apply(C.c_int(n), C.c_int(PRECOND_KIND[kind]), _p(_i32(INU)), _p(_i32(IAU)),
+                            _p(_c128(AU)), _p(_i32(INL)), _p(_i32(IAL)), _p(_c128(AL)), _p(_c128(DD)),
+                            _p(_c128(r)), _p(z))
+    return z
+
+
+def cocg13(INU, IAU, INL, IAL, D, AU, AL, B, X=None, precond="IC", resid=1e-8, maxit=3000, nset=1,
+           sigma_diag=1.0, sigma=0.0, history=False):
+    """COCG13_single (COCG13_single.f90:9-341) on copies of the inputs.
+
+    Returns dict(x, resid, iter, error, hist[, D, AU, AL, B]) -- the modified D/AU/AL/B are
+    returned too because the reference modifies them in place for NSET = 2."""
+    n = len(D)
+    D, AU, AL, B = _c128(D).copy(), _c128(AU).copy(), _c128(AL).copy(), _c128(B).copy()
+    X = np.zeros(n, np.complex128) if X is None else _c128(X).copy()
+    DD = np.zeros(n, np.complex128)
+    have = C.c_int(0)
+    r = C.c_double(resid)
+    it = C.c_int(maxit)
+    err = C.c_int(0)
+    hist = np.zeros(max(maxit, 1), np.float64)
+    pc = C.create_string_buffer(precond.ljust(20).encode()[:20], 21)
+    lib().orc_cocg13(C.c_int(n), C.c_int(int(INL[-1])), C.c_int(int(INU[-1])), _p(D), _p(AL), _p(_i32(INL)),
+                     _p(_i32(IAL)), _p(AU), _p(_i32(INU)), _p(_i32(IAU)), _p(B), _p(X), pc,
+                     C.c_double(sigma_diag), C.c_double(sigma), C.byref(r), C.byref(it), C.byref(err),
+                     C.c_int(nset), _p(DD), C.byref(have), _p(hist))
+    out = dict(x=X, resid=r.value, iter=it.value, error=err.value, D=D, AU=AU, AL=AL, B=B)
+    if history:
+        out["hist"] = hist[: max(0, min(it.value, maxit))].copy()
+    return out
+
+
+def prepobscoeff(case, xyzobs):
+    """PREPOBSCOEFF (n_ebfem_bxyz_mpi.f90:653-726): (item (nobs,6), val_e (3,nobs,6), val_b (3,nobs,6), iele)."""
+    xo = _f64(xyzobs).reshape(-1, 3)
+    nobs = xo.shape[0]
+    item = np.zeros((nobs, 6), np.int32)
+    ve = np.zeros((3, nobs, 6))
+    vb = np.zeros((3, nobs, 6))
+    iele = np.zeros(nobs, np.int32)
+    rc = lib().orc_prepobscoeff(C.c_int(nobs), _p(xo), C.c_int(case.ntet), _p(_f64(case.xyz)),
+                                _p(_i32(case.n4)), _p(_i32(case.n6line)), _p(item), _p(ve), _p(vb), _p(iele))
+    if rc:
+        raise RuntimeError(f"observation {-rc - 1} outside the mesh")
+    return item, ve, vb, iele
+
+
+def obs_apply(item, val, x):
+    nobs = item.shape[0]
+    out = np.zeros(nobs, np.complex128)
+    lib().orc_obs_apply(C.c_int(nobs), _p(_i32(item)), _p(_f64(val)), _p(_c128(x)), _p(out))
+    return out
+
+
+def jacobian(u, x, par_ptr, par_tet, case, omega, dsig_factor, sigma=None):
+    """genjacobian1 restated (m_jacobian_ap.f90:174-386): dbe (nobs, npar) complex."""
+    u = _c128(u)
+    nobs, nline = u.shape
+    npar = len(par_ptr) - 1
+    sig = _f64(case.sigma_tet() if sigma is None else sigma)
+    out = np.zeros((nobs, npar), np.complex128)
+    rc = lib().orc_jacobian(C.c_int(nobs), C.c_int(nline), _p(u), _p(_c128(x)), C.c_int(npar),
+                            _p(_i32(par_ptr)), _p(_i32(par_tet)), _p(_f64(case.xyz)), _p(_i32(case.n4)),
+                            _p(_i32(case.n6line)), _p(sig), C.c_double(omega), C.c_double(dsig_factor),
+                            _p(out))
+    if rc:
+        raise RuntimeError("orc_jacobian failed")
+    return out
+
+
+def forward_system(case, omega, sigma=None):
+    """forward_bxyz steps INITIALIZE..SET_BC_ICCG (forward_bxyz_mpi.f90:43-58).
+
+    Returns dict(INU, IAU, INL, IAL, D, AU, AL, b, bc)."""
+    INU, IAU, INL, IAL = pattern(case.nline, case.n6line)
+    D, AU, b = genmat(case, omega, INU, IAU, sigma)
+    AL = copy_ul(INU, IAU, INL, IAL, AU)
+    bc, _ = genbc(case)
+    D, AU, AL, rf = set_bc(INU, IAU, INL, IAL, D, AU, AL, b, bc)
+    return dict(INU=INU, IAU=IAU, INL=INL, IAL=IAL, D=D, AU=AU, AL=AL, b=rf[:, 0].copy(), bc=bc)
+
+
+def to_scipy(INU, IAU, INL, IAL, D, AU, AL):
+    """Full matrix as scipy CSR (for direct solves and dense checks)."""
+    import scipy.sparse as sp
+    n = len(D)
+    ru = np.repeat(np.arange(n), np.diff(INU))
+    rl = np.repeat(np.arange(n), np.diff(INL))
+    rows = np.concatenate([np.arange(n), ru, rl])
+    cols = np.concatenate([np.arange(n), np.asarray(IAU) - 1, np.asarray(IAL) - 1])
+    vals = np.concatenate([D, AU, AL])
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n))

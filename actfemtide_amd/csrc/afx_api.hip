@@ -65,6 +65,15 @@ int next_pow2(int k) {
 }
 }  // namespace
 
+struct DevSide {
+  int *bstep = nullptr, *spos = nullptr, *row = nullptr, *ptr = nullptr, *col = nullptr, *map = nullptr;
+  int *rptr = nullptr, *rblk = nullptr, *rneed = nullptr;
+  unsigned char *pub = nullptr, *exp = nullptr;
+  cx<double> *val64 = nullptr;
+  cx<float> *val32 = nullptr;
+  long long nnz = 0;
+};
+
 struct afx_system_s {
   int n = 0;
   long long nU = 0, nL = 0, nnz = 0;
@@ -103,9 +112,14 @@ struct afx_system_s {
   int *d_cptr = nullptr, *d_cidx = nullptr, *d_rptr = nullptr, *d_ridx = nullptr, *d_mirror = nullptr;
   bool have_slots = false;
   cx<double> *d_rhs_unit = nullptr;  // tidal RHS of the last assembly
+  // pipelined sweeps
+  int Gs = 0, sweep_stride = 0, rows_per_block = 0;
+  int *d_rstart = nullptr;
+  DevSide fw, bw;
+  unsigned *d_progress = nullptr, *d_bar = nullptr;
+  int *d_err = nullptr;
   // launch configuration (autotuned)
   int lpr64 = 8, lpr32 = 8;
-  int grid_single64 = 0, grid_single32 = 0, grid_multi64 = 0, grid_multi32 = 0;
   int ncu = 0;
 };
 
@@ -256,17 +270,10 @@ int ensure_derived(afx_system s) {
   HIPCHK(afx::launch_gather_vals((int)s->nnz, s->d_map_full, s->d_canon, s->d_val64, s->d_val32, s->stream));
   HIPCHK(afx::launch_gather_vals((int)s->nL, s->d_mapL, s->d_canon, s->d_Lval64, s->d_Lval32, s->stream));
   HIPCHK(afx::launch_gather_vals((int)s->nU, s->d_mapU, s->d_canon, s->d_Uval64, s->d_Uval32, s->stream));
+  HIPCHK(afx::launch_gather_vals((int)s->fw.nnz, s->fw.map, s->d_canon, s->fw.val64, s->fw.val32, s->stream));
+  HIPCHK(afx::launch_gather_vals((int)s->bw.nnz, s->bw.map, s->d_canon, s->bw.val64, s->bw.val32, s->stream));
   s->derived_valid = true;
   return AFX_OK;
-}
-
-int grid_for(afx_system s, bool multi, bool fp64) {
-  int &g = fp64 ? (multi ? s->grid_multi64 : s->grid_single64) : (multi ? s->grid_multi32 : s->grid_single32);
-  const int lpr = fp64 ? s->lpr64 : s->lpr32;
-  const int gmax = afx::max_coop_grid(lpr, multi, fp64);
-  if (g <= 0 || g > gmax) g = std::min(gmax, 2 * s->ncu);
-  if (g <= 0) g = gmax;
-  return g;
 }
 
 // Preconditioner diagonal for the distinct column frequencies `wf`.
@@ -284,7 +291,7 @@ int ensure_precond(afx_system s, int kind, double sd, double sig, const std::vec
   if ((rc = upload(&d_wf, wf.data(), nf))) return rc;
   int grid = 0;
   if (kind == AFX_PC_IC) {
-    grid = std::min(afx::max_coop_grid_ic(), 2 * s->ncu);
+    grid = std::min(afx::max_coop_grid_ic(), std::max(8, s->ncu / 8));
     if (grid <= 0) return fail(AFX_ERR_HIP, "no cooperative grid for the D-ILU setup");
   }
   hipError_t e = afx::launch_precond_setup(s->n, nf, kind, sd, sig, s->d_canon, nullptr, nullptr, nullptr, s->nlevL,
@@ -330,22 +337,171 @@ void column_freqs(afx_system s, int k, const double *scale, std::vector<double> 
   }
 }
 
-// Workspace of one Krylov solve (vectors of type T).
-template <typename T>
+// Pipelined-sweep layout of one triangle (see afx_solver.hip): rows of each block sorted by
+// dependency level into steps, the step requirements on other blocks, publish / export flags.
+struct SideHost {
+  std::vector<int> bstep, spos, row, ptr, col, map, rptr, rblk, rneed;
+  std::vector<unsigned char> pub, exp;
+  int maxsteps = 0;
+};
+
+void build_side(int n, int G, const std::vector<int> &rstart, const std::vector<int> &blk,
+                const std::vector<int> &P, const std::vector<int> &C1, long long canon_off, bool fwd, SideHost &h) {
+  std::vector<int> lev(n, 0);
+  if (fwd) {
+    for (int i = 0; i < n; i++)
+      for (int q = P[i]; q < P[i + 1]; q++) lev[i] = std::max(lev[i], lev[C1[q] - 1] + 1);
+  } else {
+    for (int i = n - 1; i >= 0; i--)
+      for (int q = P[i]; q < P[i + 1]; q++) lev[i] = std::max(lev[i], lev[C1[q] - 1] + 1);
+  }
+  h.exp.assign(n, 0);
+  for (int i = 0; i < n; i++)
+    for (int q = P[i]; q < P[i + 1]; q++) {
+      const int j = C1[q] - 1;
+      if (blk[j] != blk[i]) h.exp[j] = 1;
+    }
+  std::vector<int> lstep(n, 0);
+  h.bstep.assign(G + 1, 0);
+  h.spos.clear();
+  h.row.clear();
+  h.row.reserve(n);
+  std::vector<int> step_blk;
+  for (int b = 0; b < G; b++) {
+    h.bstep[b] = (int)h.spos.size();
+    std::vector<int> rows;
+    for (int i = rstart[b]; i < rstart[b + 1]; i++) rows.push_back(i);
+    std::stable_sort(rows.begin(), rows.end(), [&](int x, int y) { return lev[x] < lev[y]; });
+    int prev = -1, ls = -1;
+    for (int i : rows) {
+      if (lev[i] != prev) {
+        h.spos.push_back((int)h.row.size());
+        step_blk.push_back(b);
+        prev = lev[i];
+        ++ls;
+      }
+      lstep[i] = ls;
+      h.row.push_back(i);
+    }
+    h.maxsteps = std::max(h.maxsteps, ls + 1);
+  }
+  const int nstep = (int)h.spos.size();
+  h.bstep[G] = nstep;
+  h.spos.push_back(n);
+  h.ptr.assign(n + 1, 0);
+  h.col.clear();
+  h.map.clear();
+  for (int pos = 0; pos < n; pos++) {
+    const int i = h.row[pos];
+    h.ptr[pos] = (int)h.col.size();
+    for (int q = P[i]; q < P[i + 1]; q++) {
+      h.col.push_back(C1[q] - 1);
+      h.map.push_back((int)(canon_off + q));
+    }
+  }
+  h.ptr[n] = (int)h.col.size();
+  h.rptr.assign(nstep + 1, 0);
+  h.rblk.clear();
+  h.rneed.clear();
+  h.pub.assign(nstep, 0);
+  std::vector<int> runmax(G, 0), need(G, 0), touched;
+  for (int b = 0; b < G; b++) {
+    std::vector<int> seen;
+    for (int st = h.bstep[b]; st < h.bstep[b + 1]; st++) {
+      h.rptr[st] = (int)h.rblk.size();
+      touched.clear();
+      for (int pos = h.spos[st]; pos < h.spos[st + 1]; pos++) {
+        const int i = h.row[pos];
+        if (h.exp[i]) h.pub[st] = 1;
+        for (int q = P[i]; q < P[i + 1]; q++) {
+          const int j = C1[q] - 1;
+          const int ob = blk[j];
+          if (ob == b) continue;
+          if (need[ob] == 0) touched.push_back(ob);
+          need[ob] = std::max(need[ob], lstep[j] + 1);
+        }
+      }
+      for (int ob : touched) {
+        if (need[ob] > runmax[ob]) {
+          if (runmax[ob] == 0) seen.push_back(ob);
+          runmax[ob] = need[ob];
+          h.rblk.push_back(ob);
+          h.rneed.push_back(need[ob]);
+        }
+        need[ob] = 0;
+      }
+    }
+    for (int ob : seen) runmax[ob] = 0;
+  }
+  h.rptr[nstep] = (int)h.rblk.size();
+}
+
+int upload_side(const SideHost &h, DevSide &d) {
+  int rc;
+  if ((rc = upload(&d.bstep, h.bstep.data(), h.bstep.size())) || (rc = upload(&d.spos, h.spos.data(), h.spos.size())) ||
+      (rc = upload(&d.row, h.row.data(), h.row.size())) || (rc = upload(&d.ptr, h.ptr.data(), h.ptr.size())) ||
+      (rc = upload(&d.col, h.col.data(), h.col.size())) || (rc = upload(&d.map, h.map.data(), h.map.size())) ||
+      (rc = upload(&d.rptr, h.rptr.data(), h.rptr.size())) || (rc = upload(&d.rblk, h.rblk.data(), h.rblk.size())) ||
+      (rc = upload(&d.rneed, h.rneed.data(), h.rneed.size())) || (rc = upload(&d.pub, h.pub.data(), h.pub.size())) ||
+      (rc = upload(&d.exp, h.exp.data(), h.exp.size())) || (rc = dalloc(&d.val64, h.col.size())) ||
+      (rc = dalloc(&d.val32, h.col.size())))
+    return rc;
+  d.nnz = (long long)h.col.size();
+  return AFX_OK;
+}
+
+void free_side(DevSide &d) {
+  dfree(d.bstep); dfree(d.spos); dfree(d.row); dfree(d.ptr); dfree(d.col); dfree(d.map); dfree(d.rptr);
+  dfree(d.rblk); dfree(d.rneed); dfree(d.pub); dfree(d.exp); dfree(d.val64); dfree(d.val32);
+}
+
+afx::SweepSide side_view(const DevSide &d, bool fp64) {
+  afx::SweepSide v{};
+  v.bstep = d.bstep; v.spos = d.spos; v.row = d.row; v.ptr = d.ptr; v.col = d.col;
+  v.val = fp64 ? (const void *)d.val64 : (const void *)d.val32;
+  v.rptr = d.rptr; v.rblk = d.rblk; v.rneed = d.rneed; v.pub = d.pub; v.exp = d.exp;
+  return v;
+}
+
+int build_sweeps(afx_system s) {
+  const int n = s->n;
+  int G = std::max(1, std::min(s->ncu, (n + 63) / 64));
+  if (const char *e = getenv("AFX_SWEEP_BLOCKS")) G = std::max(1, std::min(atoi(e), n));
+  s->Gs = G;
+  std::vector<int> rstart(G + 1), blk(n);
+  for (int b = 0; b <= G; b++) rstart[b] = (int)((long long)b * n / G);
+  for (int b = 0; b < G; b++)
+    for (int i = rstart[b]; i < rstart[b + 1]; i++) blk[i] = b;
+  SideHost fw, bw;
+  build_side(n, G, rstart, blk, s->INL, s->IAL, (long long)n + s->nU, true, fw);
+  build_side(n, G, rstart, blk, s->INU, s->IAU, n, false, bw);
+  s->sweep_stride = std::max(fw.maxsteps, bw.maxsteps) + 1;
+  s->rows_per_block = 0;
+  for (int b = 0; b < G; b++) s->rows_per_block = std::max(s->rows_per_block, rstart[b + 1] - rstart[b]);
+  int rc;
+  if ((rc = upload(&s->d_rstart, rstart.data(), rstart.size()))) return rc;
+  if ((rc = upload_side(fw, s->fw)) || (rc = upload_side(bw, s->bw))) return rc;
+  if ((rc = dalloc(&s->d_progress, G)) || (rc = dalloc(&s->d_bar, 64)) || (rc = dalloc(&s->d_err, 1))) return rc;
+  HIPCHK(hipMemset(s->d_bar, 0, 64 * sizeof(unsigned)));
+  HIPCHK(hipMemset(s->d_err, 0, sizeof(int)));
+  return AFX_OK;
+}
+
+// Workspace of one Krylov solve; TP is the precision of the preconditioner vector z.
+template <typename TP>
 struct Work {
-  cx<T> *R = nullptr, *Z = nullptr, *P = nullptr, *Q = nullptr;
+  cx<double> *R = nullptr, *P = nullptr, *Q = nullptr;
+  cx<TP> *Z = nullptr;
   cx<double> *part_rho = nullptr, *part_c1 = nullptr, *fin_rho = nullptr, *fin_c1 = nullptr, *rho1 = nullptr;
   double *part_rr = nullptr, *fin_rr = nullptr, *bnrm2 = nullptr, *resid = nullptr, *norms = nullptr;
-  double *part_a = nullptr, *part_b = nullptr;
+  double *part_a = nullptr, *part_b = nullptr, *wcol = nullptr;
   int *conv = nullptr, *iters = nullptr, *status = nullptr, *col2f = nullptr;
-  T *wcol = nullptr;
-  double *wcol64 = nullptr;
   int *h_status = nullptr;
   ~Work() {
     dfree(R); dfree(Z); dfree(P); dfree(Q);
     dfree(part_rho); dfree(part_c1); dfree(fin_rho); dfree(fin_c1); dfree(rho1);
     dfree(part_rr); dfree(fin_rr); dfree(bnrm2); dfree(resid); dfree(norms); dfree(part_a); dfree(part_b);
-    dfree(conv); dfree(iters); dfree(status); dfree(col2f); dfree(wcol); dfree(wcol64);
+    dfree(conv); dfree(iters); dfree(status); dfree(col2f); dfree(wcol);
     if (h_status) (void)hipHostFree(h_status);
   }
   int alloc(long long n, int k, int G) {
@@ -358,8 +514,7 @@ struct Work {
     if ((rc = dalloc(&fin_rho, k)) || (rc = dalloc(&fin_c1, k)) || (rc = dalloc(&fin_rr, k))) return rc;
     if ((rc = dalloc(&rho1, k)) || (rc = dalloc(&bnrm2, k)) || (rc = dalloc(&resid, k))) return rc;
     if ((rc = dalloc(&norms, 2 * (size_t)k)) || (rc = dalloc(&conv, k)) || (rc = dalloc(&iters, k))) return rc;
-    if ((rc = dalloc(&status, 2)) || (rc = dalloc(&col2f, k)) || (rc = dalloc(&wcol, k)) || (rc = dalloc(&wcol64, k)))
-      return rc;
+    if ((rc = dalloc(&status, 2)) || (rc = dalloc(&col2f, k)) || (rc = dalloc(&wcol, k))) return rc;
     if (hipHostMalloc((void **)&h_status, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
       return fail(AFX_ERR_HIP, "hipHostMalloc");
     return AFX_OK;
@@ -382,127 +537,63 @@ __global__ void k_init_state(int k, const double *norms, double *bnrm2, int *con
   rho1[c] = cx<double>{0.0, 0.0};
 }
 
-template <typename T>
-__global__ void k_zero_cols(long long n, int k, const int *conv, cx<T> *X) {
+__global__ void k_zero_cols(long long n, int k, const int *conv, cx<double> *X) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   if (i >= n * k) return;
-  if (conv[i % k] == 2) X[i] = cx<T>{T(0), T(0)};
+  if (conv[i % k] == 2) X[i] = cx<double>{0.0, 0.0};
 }
 
-template <typename T>
-__global__ void k_cast_w(int k, const double *w, T *out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < k) out[c] = (T)w[c];
+template <typename TP>
+int lds_rows_for(afx_system s, bool multi) {
+  if (multi) return 0;
+  int dev = 0, maxlds = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
+  const size_t need = ((size_t)s->Gs * 4 + 15) / 16 * 16 + (size_t)s->rows_per_block * sizeof(cx<TP>) + 6144;
+  if (getenv("AFX_NO_LDS")) return 0;
+  return (need <= (size_t)maxlds) ? s->rows_per_block : 0;
 }
 
-// R = B (FP32 inner solve with x = 0) and per-block partial ||b||^2 in FP64
-template <typename T>
-__global__ void __launch_bounds__(256) k_copy_norm(long long n, int k, int kp, const cx<T> *B, cx<T> *R,
-                                                   double *part_a, double *part_b) {
-  __shared__ double lds[256];
-  const int t = threadIdx.x;
-  const int c = t % kp;
-  const long long slot = ((long long)blockIdx.x * 256 + t) / kp;
-  const long long nslot = ((long long)gridDim.x * 256) / kp;
-  double bb = 0.0;
-  if (c < k)
-    for (long long row = slot; row < n; row += nslot) {
-      const size_t ix = (size_t)row * k + c;
-      const cx<T> b = B[ix];
-      R[ix] = b;
-      bb += (double)b.r * (double)b.r + (double)b.i * (double)b.i;
-    }
-  lds[t] = bb;
-  __syncthreads();
-  for (int s = (256 / kp) >> 1; s >= 1; s >>= 1) {
-    if (t / kp < s) lds[t] += lds[t + s * kp];
-    __syncthreads();
-  }
-  if (t < kp && t < k) {
-    part_a[(size_t)blockIdx.x * k + t] = lds[t];
-    part_b[(size_t)blockIdx.x * k + t] = lds[t];
-  }
-}
-
-__global__ void k_norm_finish2(int G, int k, const double *pa, const double *pb, double *norms) {
-  __shared__ double la[256], lb[256];
-  const int c = blockIdx.x;
-  double a = 0.0, b = 0.0;
-  for (int g = threadIdx.x; g < G; g += 256) {
-    a += pa[(size_t)g * k + c];
-    b += pb[(size_t)g * k + c];
-  }
-  la[threadIdx.x] = a;
-  lb[threadIdx.x] = b;
-  __syncthreads();
-  for (int s = 128; s >= 1; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      la[threadIdx.x] += la[threadIdx.x + s];
-      lb[threadIdx.x] += lb[threadIdx.x + s];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    norms[2 * c] = la[0];
-    norms[2 * c + 1] = lb[0];
-  }
-}
-
-}  // namespace
-
-namespace {
-
-// One Krylov solve in precision T on device vectors B (const) and X (in/out), [n][k] layout.
-// For T = double the initial residual is b - A x in FP64; for T = float the caller passes
-// x = 0 so the initial residual is b.
-template <typename T>
-int krylov(afx_system s, int k, const cx<T> *B, cx<T> *X, const std::vector<double> &wcol,
+// One COCG solve on device vectors B (const) and X (in/out, the initial guess), [n][k] layout,
+// with the preconditioner in precision TP.
+template <typename TP>
+int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::vector<double> &wcol,
            const std::vector<int> &col2f, int nf, int pc, double tol, int maxit, int chunk, afx_stats *stats,
            int *host_conv) {
-  const bool fp64 = std::is_same<T, double>::value;
+  const bool fp64 = std::is_same<TP, double>::value;
   const bool multi = k > 1;
   const int kp = multi ? next_pow2(k) : 1;
   if (kp > 256) return fail(AFX_ERR_ARG, "at most 256 right-hand sides per launch");
-  const int G = grid_for(s, multi, fp64);
-  if (G <= 0) return fail(AFX_ERR_HIP, "no cooperative grid available");
-  Work<T> w;
+  const int G = s->Gs;
+  Work<TP> w;
   int rc = w.alloc(s->n, k, G);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(w.col2f, col2f.data(), k * sizeof(int), hipMemcpyHostToDevice, s->stream));
-  HIPCHK(hipMemcpyAsync(w.wcol64, wcol.data(), k * sizeof(double), hipMemcpyHostToDevice, s->stream));
-  hipLaunchKernelGGL(k_cast_w<T>, dim3((k + 255) / 256), dim3(256), 0, s->stream, k, w.wcol64, w.wcol);
-  HIPCHK(hipGetLastError());
-  // initial residual and ||b||^2 (COCG13_single.f90:139-169)
-  if (fp64) {
-    afx::SpmvParams sp{};
-    sp.n = s->n;
-    sp.k = k;
-    sp.kp = kp;
-    sp.rowptr = s->d_rowptr;
-    sp.col = s->d_col;
-    sp.val = s->d_val64;
-    sp.wcol = w.wcol64;
-    sp.X = (const cx<double> *)X;
-    sp.B = (const cx<double> *)B;
-    sp.Y = (cx<double> *)w.R;
-    sp.part_rr = w.part_a;
-    sp.part_bb = w.part_b;
-    HIPCHK(afx::launch_residual_norms(sp, G, w.norms, s->stream));
-  } else {
-    // FP32 inner solve: x = 0 on entry, so r0 = b
-    hipLaunchKernelGGL(k_copy_norm<T>, dim3(G), dim3(256), 0, s->stream, (long long)s->n, k, kp, B, w.R, w.part_a,
-                       w.part_b);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_norm_finish2, dim3(k), dim3(256), 0, s->stream, G, k, w.part_a, w.part_b, w.norms);
-    HIPCHK(hipGetLastError());
-  }
+  HIPCHK(hipMemcpyAsync(w.wcol, wcol.data(), k * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemsetAsync(s->d_progress, 0, (size_t)G * sizeof(unsigned), s->stream));
+  HIPCHK(hipMemsetAsync(s->d_err, 0, sizeof(int), s->stream));
+  // initial residual r0 = b - A x and ||b||^2 (COCG13_single.f90:139-169)
+  afx::SpmvParams sp{};
+  sp.n = s->n;
+  sp.k = k;
+  sp.kp = kp;
+  sp.rowptr = s->d_rowptr;
+  sp.col = s->d_col;
+  sp.val = s->d_val64;
+  sp.wcol = w.wcol;
+  sp.X = X;
+  sp.B = B;
+  sp.Y = w.R;
+  sp.part_rr = w.part_a;
+  sp.part_bb = w.part_b;
+  HIPCHK(afx::launch_residual_norms(sp, G, w.norms, s->stream));
   hipLaunchKernelGGL(k_init_state, dim3((k + 255) / 256), dim3(256), 0, s->stream, k, w.norms, w.bnrm2, w.conv,
                      w.iters, w.resid, w.rho1, w.status);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_zero_cols<T>, dim3((unsigned)(((long long)s->n * k + 255) / 256)), dim3(256), 0, s->stream,
+  hipLaunchKernelGGL(k_zero_cols, dim3((unsigned)(((long long)s->n * k + 255) / 256)), dim3(256), 0, s->stream,
                      (long long)s->n, k, w.conv, X);
   HIPCHK(hipGetLastError());
-  afx::CocgParams<T> p{};
+  afx::CocgParams<TP> p{};
   p.n = s->n;
   p.k = k;
   p.kp = kp;
@@ -511,20 +602,15 @@ int krylov(afx_system s, int k, const cx<T> *B, cx<T> *X, const std::vector<doub
   p.dist_reduce = ((long long)G * k > 4096) ? 1 : 0;
   p.rowptr = s->d_rowptr;
   p.col = s->d_col;
-  p.val = fp64 ? (const cx<T> *)s->d_val64 : (const cx<T> *)s->d_val32;
-  p.nlevL = s->nlevL;
-  p.nlevU = s->nlevU;
-  p.lvlL = s->d_lvlL;
-  p.rowL = s->d_rowL;
-  p.Lptr = s->d_Lptr;
-  p.Lcol = s->d_Lcol;
-  p.Lval = fp64 ? (const cx<T> *)s->d_Lval64 : (const cx<T> *)s->d_Lval32;
-  p.lvlU = s->d_lvlU;
-  p.rowU = s->d_rowU;
-  p.Uptr = s->d_Uptr;
-  p.Ucol = s->d_Ucol;
-  p.Uval = fp64 ? (const cx<T> *)s->d_Uval64 : (const cx<T> *)s->d_Uval32;
-  p.DD = fp64 ? (const cx<T> *)s->d_DD : (const cx<T> *)s->d_DD32;
+  p.val = s->d_val64;
+  p.Gs = G;
+  p.lds_rows = lds_rows_for<TP>(s, multi);
+  p.stride = s->sweep_stride;
+  p.rstart = s->d_rstart;
+  p.fw = side_view(s->fw, fp64);
+  p.bw = side_view(s->bw, fp64);
+  p.progress = s->d_progress;
+  p.DD = fp64 ? (const cx<TP> *)s->d_DD : (const cx<TP> *)s->d_DD32;
   p.col2f = w.col2f;
   p.wcol = w.wcol;
   p.B = B;
@@ -545,10 +631,16 @@ int krylov(afx_system s, int k, const cx<T> *B, cx<T> *X, const std::vector<doub
   p.iters = w.iters;
   p.resid = w.resid;
   p.status = w.status;
+  p.bar.cnt = s->d_bar;
+  p.bar.gen = s->d_bar + 32;
+  p.bar.err = s->d_err;
+  p.err = s->d_err;
   p.tol = tol;
   p.maxit = maxit;
   p.chunk = chunk > 0 ? chunk : 200;
   const int lpr = fp64 ? s->lpr64 : s->lpr32;
+  const bool verbose = getenv("AFX_VERBOSE") != nullptr;
+  const auto t_start = std::chrono::steady_clock::now();
   for (;;) {
     hipError_t e = fp64 ? afx::launch_cocg64(*(afx::CocgParams<double> *)&p, lpr, G, s->stream)
                         : afx::launch_cocg32(*(afx::CocgParams<float> *)&p, lpr, G, s->stream);
@@ -558,8 +650,17 @@ int krylov(afx_system s, int k, const cx<T> *B, cx<T> *X, const std::vector<doub
     }
     HIPCHK(hipMemcpyAsync(w.h_status, w.status, 2 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
+    if (verbose) {
+      const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+      fprintf(stderr, "afx: %s k=%d G=%d lpr=%d lds=%d it=%d done=%d %.3fs (%.1f us/it)\n",
+              fp64 ? "fp64" : "fp32-pc", k, G, lpr, p.lds_rows, w.h_status[0], w.h_status[1], dt,
+              w.h_status[0] ? 1e6 * dt / w.h_status[0] : 0.0);
+    }
     if (w.h_status[1] || w.h_status[0] >= maxit) break;
   }
+  int herr = 0;
+  HIPCHK(hipMemcpy(&herr, s->d_err, sizeof(int), hipMemcpyDeviceToHost));
+  if (herr) return fail(AFX_ERR_HIP, "a bounded device wait gave up (synchronisation error)");
   std::vector<int> conv(k), iters(k);
   std::vector<double> resid(k);
   HIPCHK(hipMemcpy(conv.data(), w.conv, k * sizeof(int), hipMemcpyDeviceToHost));
@@ -599,7 +700,9 @@ int create_common(afx_system s) {
   s->nnz = (long long)s->n + s->nU + s->nL;
   if (s->nnz > 0x7fffffffLL) return fail(AFX_ERR_ARG, "more than 2^31 stored entries");
   HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-  return build_device_pattern(s);
+  int rc = build_device_pattern(s);
+  if (rc) return rc;
+  return build_sweeps(s);
 }
 
 void release(afx_system s) {
@@ -613,6 +716,9 @@ void release(afx_system s) {
   dfree(s->d_xyz); dfree(s->d_fxyz); dfree(s->d_n4); dfree(s->d_n6); dfree(s->d_group); dfree(s->d_vxyz);
   dfree(s->d_cptr); dfree(s->d_cidx); dfree(s->d_rptr); dfree(s->d_ridx); dfree(s->d_mirror);
   dfree(s->d_rhs_unit);
+  free_side(s->fw);
+  free_side(s->bw);
+  dfree(s->d_rstart); dfree(s->d_progress); dfree(s->d_bar); dfree(s->d_err);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   s->stream = nullptr;
 }
@@ -706,96 +812,38 @@ int solve_device_impl(afx_system s, int k, const cx<double> *dB, cx<double> *dX,
   std::vector<afx_stats> st(k);
   for (auto &x : st) x = afx_stats{0, 0, 0.0, -1.0, 0};
   int ret = AFX_OK;
-  if (o->precision == AFX_FP64) {
-    ret = krylov<double>(s, k, dB, dX, wcol, col2f, nf, kind, o->tol, o->maxit, o->chunk, st.data(), nullptr);
+  // COCG with an FP64 (AFX_FP64) or complex-FP32 (AFX_MIXED) preconditioner; the recurrence,
+  // SpMV and scalars are FP64. With check_true, the FP64 true residual is recomputed and the
+  // solve restarts from the current iterate (an FP64 refinement step) while it exceeds tol.
+  const int outer_max = o->check_true ? std::max(1, o->max_outer) : 1;
+  std::vector<int> total(k, 0);
+  for (int outer = 0; outer < outer_max; ++outer) {
+    std::vector<afx_stats> ist(k);
+    int budget = o->maxit;
+    for (int c = 0; c < k; c++) budget = std::min(budget, std::max(1, o->maxit - total[c]));
+    ret = (o->precision == AFX_FP64)
+              ? krylov<double>(s, k, dB, dX, wcol, col2f, nf, kind, o->tol, budget, o->chunk, ist.data(), nullptr)
+              : krylov<float>(s, k, dB, dX, wcol, col2f, nf, kind, o->tol, budget, o->chunk, ist.data(), nullptr);
     if (ret != AFX_OK && ret != AFX_ERR_MAXIT && ret != AFX_ERR_BREAKDOWN) return ret;
-  } else {
-    // FP64 iterative refinement around complex-FP32 COCG
-    const int G = 2 * s->ncu;
-    const size_t nk = (size_t)s->n * k;
-    cx<double> *dR = nullptr;
-    cx<float> *dR32 = nullptr, *dD32 = nullptr;
-    double *dw = nullptr, *pa = nullptr, *pb = nullptr, *norms = nullptr, *scale = nullptr, *inv = nullptr;
-    auto cleanup = [&]() {
-      dfree(dR); dfree(dR32); dfree(dD32); dfree(dw); dfree(pa); dfree(pb); dfree(norms); dfree(scale); dfree(inv);
-    };
-    if ((rc = dalloc(&dR, nk)) || (rc = dalloc(&dR32, nk)) || (rc = dalloc(&dD32, nk)) ||
-        (rc = upload(&dw, wcol.data(), k)) || (rc = dalloc(&pa, (size_t)G * k)) || (rc = dalloc(&pb, (size_t)G * k)) ||
-        (rc = dalloc(&norms, 2 * (size_t)k)) || (rc = dalloc(&scale, k)) || (rc = dalloc(&inv, k))) {
-      cleanup();
-      return rc;
-    }
-    std::vector<int> total(k, 0), conv(k, 0);
-    std::vector<double> hn(2 * (size_t)k);
-    int outer = 0;
-    for (;;) {
-      afx::SpmvParams sp{};
-      sp.n = s->n;
-      sp.k = k;
-      sp.kp = k > 1 ? next_pow2(k) : 1;
-      sp.rowptr = s->d_rowptr;
-      sp.col = s->d_col;
-      sp.val = s->d_val64;
-      sp.wcol = dw;
-      sp.X = dX;
-      sp.B = dB;
-      sp.Y = dR;
-      sp.part_rr = pa;
-      sp.part_bb = pb;
-      hipError_t e = afx::launch_residual_norms(sp, G, norms, s->stream);
-      if (e == hipSuccess) e = hipMemcpyAsync(hn.data(), norms, hn.size() * 8, hipMemcpyDeviceToHost, s->stream);
-      if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-      if (e != hipSuccess) {
-        cleanup();
-        return fail(AFX_ERR_HIP, hipGetErrorString(e));
-      }
-      bool all = true;
-      int maxdone = 0;
-      for (int c = 0; c < k; c++) {
-        const double res = hn[2 * c + 1] > 0 ? std::sqrt(hn[2 * c] / hn[2 * c + 1]) : 0.0;
-        st[c].resid = res;
-        st[c].true_resid = res;
-        if (!(res <= o->tol)) all = false;
-        maxdone = std::max(maxdone, total[c]);
-      }
-      if (all || outer >= o->max_outer || maxdone >= o->maxit) break;
-      hipLaunchKernelGGL(k_inv_or_zero, dim3((k + 255) / 256), dim3(256), 0, s->stream, k, norms, o->tol, scale, inv);
-      e = hipGetLastError();
-      if (e == hipSuccess) e = afx::launch_scale_cols(s->n, k, dR, scale, dR32, s->stream);
-      if (e == hipSuccess) e = hipMemsetAsync(dD32, 0, nk * sizeof(cx<float>), s->stream);
-      if (e != hipSuccess) {
-        cleanup();
-        return fail(AFX_ERR_HIP, hipGetErrorString(e));
-      }
-      std::vector<afx_stats> ist(k);
-      const int budget = std::max(1, o->maxit - maxdone);
-      rc = krylov<float>(s, k, dR32, dD32, wcol, col2f, nf, kind, o->inner_tol, budget, o->chunk, ist.data(),
-                         conv.data());
-      if (rc != AFX_OK && rc != AFX_ERR_MAXIT && rc != AFX_ERR_BREAKDOWN) {
-        cleanup();
-        return rc;
-      }
-      for (int c = 0; c < k; c++)
-        if (conv[c] != 2) total[c] += std::min(ist[c].iter, budget);
-      e = afx::launch_axpy_scaled(s->n, k, dD32, inv, dX, s->stream);
-      if (e != hipSuccess) {
-        cleanup();
-        return fail(AFX_ERR_HIP, hipGetErrorString(e));
-      }
-      ++outer;
-    }
     for (int c = 0; c < k; c++) {
-      st[c].iter = total[c];
+      total[c] += (outer == 0 || ist[c].iter > 0) ? std::min(ist[c].iter, budget) : 0;
+      st[c] = ist[c];
+      st[c].iter = (outer == 0) ? ist[c].iter : total[c];
       st[c].outer = outer;
-      st[c].error = (st[c].resid <= o->tol) ? 0 : AFX_ERR_MAXIT;
-      if (st[c].error && ret == AFX_OK) ret = st[c].error;
     }
-    cleanup();
-  }
-  if (o->check_true && o->precision == AFX_FP64) {
+    if (!o->check_true) break;
     std::vector<double> tr(k);
     if ((rc = afx_true_residual_device(s, k, dB, dX, omega_scale, tr.data()))) return rc;
-    for (int c = 0; c < k; c++) st[c].true_resid = tr[c];
+    bool ok = true;
+    for (int c = 0; c < k; c++) {
+      st[c].true_resid = tr[c];
+      if (!(tr[c] <= o->tol)) ok = false;
+    }
+    if (ok || ret == AFX_ERR_BREAKDOWN) break;
+    bool budget_left = false;
+    for (int c = 0; c < k; c++)
+      if (total[c] < o->maxit) budget_left = true;
+    if (!budget_left) break;
   }
   if (stats)
     for (int c = 0; c < k; c++) stats[c] = st[c];
@@ -1288,10 +1336,14 @@ void lambda_h(const double x[4][3], const double *p, double a[4]) {
 extern "C" {
 
 int afx_autotune(afx_system s, int nrhs, int verbose) {
+  // Rows per wavefront (lanes per row `lpr`) of the single-column kernel, per preconditioner
+  // precision: time 20 iterations of each candidate on this system. The grid is the sweep
+  // partition (one block per row range), so only the lane mapping is tuned.
   if (!s || nrhs <= 0 || nrhs > 256) return fail(AFX_ERR_ARG, "bad arguments");
   if (!s->have_values) return fail(AFX_ERR_ARG, "no matrix values");
+  if (nrhs > 1) return AFX_OK;  // the multi-column kernel maps one lane per (row, column)
   const int kind = s->pc_valid ? s->pc_kind : AFX_PC_SSOR;
-  const size_t nk = (size_t)s->n * nrhs;
+  const size_t nk = (size_t)s->n;
   std::vector<cx<double>> hb(nk, cx<double>{1.0, 0.5});
   cx<double> *dB = nullptr, *dX = nullptr;
   int rc;
@@ -1302,43 +1354,33 @@ int afx_autotune(afx_system s, int nrhs, int verbose) {
   afx_opts o;
   afx_default_opts(&o);
   o.precond = kind;
+  o.sigma_diag = s->pc_sd;
+  o.sigma = s->pc_sig;
   o.tol = 0.0;
   o.maxit = 20;
   o.check_true = 0;
-  o.nset = s->pc_valid ? 0 : 1;
-  const bool multi = nrhs > 1;
-  std::vector<int> lprs = multi ? std::vector<int>{1} : std::vector<int>{2, 4, 8, 16};
   for (int prec = 0; prec < 2; prec++) {
     const bool fp64 = prec == 0;
     double best = 1e300;
-    int best_lpr = 8, best_g = 0;
-    for (int lpr : lprs) {
+    int best_lpr = 8;
+    for (int lpr : {2, 4, 8, 16}) {
       (fp64 ? s->lpr64 : s->lpr32) = lpr;
-      const int gmax = afx::max_coop_grid(lpr, multi, fp64);
-      for (int g : {s->ncu, 2 * s->ncu, 4 * s->ncu, gmax}) {
-        if (g <= 0 || g > gmax) continue;
-        int &gs = fp64 ? (multi ? s->grid_multi64 : s->grid_single64) : (multi ? s->grid_multi32 : s->grid_single32);
-        gs = g;
-        if (hipMemset(dX, 0, nk * sizeof(cx<double>)) != hipSuccess) break;
-        o.precision = fp64 ? AFX_FP64 : AFX_MIXED;
-        o.max_outer = 1;
-        o.inner_tol = 0.0;
-        auto t0 = std::chrono::steady_clock::now();
-        std::vector<afx_stats> st(nrhs);
-        int r2 = solve_device_impl(s, nrhs, dB, dX, nullptr, &o, st.data());
-        (void)hipDeviceSynchronize();
-        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        if (r2 != AFX_OK && r2 != AFX_ERR_MAXIT) continue;
-        if (verbose) fprintf(stderr, "afx_autotune %s lpr=%d grid=%d: %.3f ms / 20 it\n", fp64 ? "fp64" : "fp32", lpr, g, dt * 1e3);
-        if (dt < best) {
-          best = dt;
-          best_lpr = lpr;
-          best_g = g;
-        }
+      if (hipMemset(dX, 0, nk * sizeof(cx<double>)) != hipSuccess) break;
+      o.precision = fp64 ? AFX_FP64 : AFX_MIXED;
+      (void)hipDeviceSynchronize();
+      auto t0 = std::chrono::steady_clock::now();
+      std::vector<afx_stats> st(1);
+      int r2 = solve_device_impl(s, 1, dB, dX, nullptr, &o, st.data());
+      (void)hipDeviceSynchronize();
+      const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (r2 != AFX_OK && r2 != AFX_ERR_MAXIT) continue;
+      if (verbose) fprintf(stderr, "afx_autotune %s lpr=%d: %.3f ms / 20 it\n", fp64 ? "fp64" : "fp32-pc", lpr, dt * 1e3);
+      if (dt < best) {
+        best = dt;
+        best_lpr = lpr;
       }
     }
     (fp64 ? s->lpr64 : s->lpr32) = best_lpr;
-    (fp64 ? (multi ? s->grid_multi64 : s->grid_single64) : (multi ? s->grid_multi32 : s->grid_single32)) = best_g;
   }
   dfree(dB);
   dfree(dX);

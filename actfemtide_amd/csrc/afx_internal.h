@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/afx.h"
+#include "afx_sync.h"
 
 namespace afx {
 
@@ -17,26 +18,44 @@ struct alignas(2 * sizeof(T)) cx {
   T r, i;
 };
 
-template <typename T>
+// One direction of the pipelined sweep (forward over the strict lower triangle, backward over
+// the strict upper), laid out in block-step order.
+struct SweepSide {
+  const int *bstep;  // [G+1] first step of each block
+  const int *spos;   // [nstep+1] first position of each step
+  const int *row;    // [n] row at each position
+  const int *ptr;    // [n+1] entry range at each position
+  const int *col;    // [nnz_side] column (0-based) of each entry
+  const void *val;   // [nnz_side] cx<TP> (K, M) values
+  const int *rptr;   // [nstep+1] requirement range of each step
+  const int *rblk;   // requirement: block to wait for
+  const int *rneed;  // requirement: steps that block must have published
+  const unsigned char *pub;  // [nstep] 1: another block waits on this step
+  const unsigned char *exp;  // [n] by row id: 1: another block reads this row's z
+};
+
+template <typename TP>
 struct CocgParams {
   int n, k, kp, nf;  // rows, columns, padded columns (lanes per row slot), distinct frequencies
   int pc;            // AFX_PC_*
   int dist_reduce;   // 1: block c reduces column c then grid sync; 0: every block reduces
-  // full CSR (row order: diagonal, strict upper, strict lower)
+  // full CSR, FP64 (row order: diagonal, strict upper, strict lower)
   const int *rowptr, *col;
-  const cx<T> *val;
-  // sweeps, rows in level order
-  int nlevL, nlevU;
-  const int *lvlL, *rowL, *Lptr, *Lcol;
-  const cx<T> *Lval;
-  const int *lvlU, *rowU, *Uptr, *Ucol;
-  const cx<T> *Uval;
-  const cx<T> *DD;   // [n][nf]
-  const int *col2f;  // [k]
-  const T *wcol;     // [k] omega of each column
+  const cx<double> *val;
+  // pipelined sweeps
+  int Gs;            // blocks of the launch == sweep blocks
+  int lds_rows;      // > 0: own-block z staged in LDS (single column)
+  int stride;        // epoch stride of the progress counters (> steps of any block)
+  const int *rstart; // [G+1] row range of each block
+  SweepSide fw, bw;
+  unsigned *progress;  // [G]
+  const cx<TP> *DD;    // [n][nf]
+  const int *col2f;    // [k]
+  const double *wcol;  // [k] omega of each column
   // vectors [n][k]
-  const cx<T> *B;
-  cx<T> *X, *R, *Z, *P, *Q;
+  const cx<double> *B;
+  cx<double> *X, *R, *P, *Q;
+  cx<TP> *Z;
   // reductions [G][k] and finals [k]
   cx<double> *part_rho, *part_c1;
   double *part_rr;
@@ -49,8 +68,10 @@ struct CocgParams {
   int *iters;
   double *resid;
   int *status;  // [0] iterations done so far, [1] all columns finished
+  GridBar bar;
+  int *err;     // bounded waits that gave up
   double tol;
-  int maxit, chunk, init;
+  int maxit, chunk;
 };
 
 struct SpmvParams {
@@ -67,7 +88,7 @@ struct SpmvParams {
 // launchers (afx_solver.hip)
 hipError_t launch_cocg64(const CocgParams<double> &p, int lpr, int grid, hipStream_t s);
 hipError_t launch_cocg32(const CocgParams<float> &p, int lpr, int grid, hipStream_t s);
-int max_coop_grid(int lpr, bool multi, bool fp64);
+int max_coop_grid(int lpr, bool multi, bool fp64, size_t dyn);
 int max_coop_grid_ic();
 hipError_t launch_spmv(const SpmvParams &p, hipStream_t s);
 hipError_t launch_residual_norms(const SpmvParams &p, int grid, double *res_out_dev, hipStream_t s);

@@ -2,13 +2,16 @@
 //
 // The whole preconditioned COCG iteration of COCG13_single (src/solver/COCG13_single.f90:181-323)
 // runs inside ONE persistent cooperative launch per chunk of iterations:
-//   z = M^-1 r  -- level-scheduled forward (strict lower) and backward (strict upper) sweeps,
-//                  one grid barrier per dependency level, rows of a level stored contiguously;
+//   z = M^-1 r  -- pipelined forward (strict lower) and backward (strict upper) sweeps: block b
+//                  owns the contiguous rows [rstart[b], rstart[b+1]) and walks them in steps of
+//                  increasing dependency level, keeping its own z in LDS; a step waits only on
+//                  the progress counters of the blocks it reads from (point-to-point, no grid
+//                  barrier per level) and publishes its own counter when another block needs it;
 //   rho = r^T z -- fused into the backward sweep, reduced deterministically across blocks;
 //   p = z + beta p;  q = A p (CSR, `lpr` lanes per row) fused with p^T q;
 //   x += alpha p, r -= alpha q fused with ||r||^2 and the per-column convergence test.
-// Bilinear forms are unconjugated (COCG); every reduction and every scalar (alpha, beta, rho)
-// is FP64, also when vectors and matrix are complex FP32 (the mixed-precision inner solve).
+// Bilinear forms are unconjugated (COCG); SpMV, reductions, alpha, beta, rho and all vectors but
+// z are FP64. The preconditioner (sweep values, DD, z) is complex FP64 or complex FP32 (TP).
 //
 // Several right-hand sides (columns) share each matrix read: vectors are stored row-interleaved
 // [n][k], a lane owns one (row, column) pair, and each column runs its own recurrence with its
@@ -19,6 +22,7 @@
 #include <vector>
 
 #include "afx_internal.h"
+#include "afx_sync.h"
 
 namespace cg = cooperative_groups;
 
@@ -58,18 +62,22 @@ template <typename T>
 __device__ __forceinline__ cx<T> down(cx<double> a) {
   return {(T)a.r, (T)a.i};
 }
+// a / b as (a * conj(b)) / |b|^2 with b scaled by an exact power of two (C99 Annex G form,
+// the complex division the reference's compiled Fortran performs): the scaling avoids
+// overflow without changing the rounding of the textbook formula.
 __device__ __forceinline__ cx<double> cdivd(cx<double> a, cx<double> b) {
-  // Smith's algorithm
-  cx<double> r;
-  if (fabs(b.r) >= fabs(b.i)) {
-    double t = b.i / b.r, d = b.r + b.i * t;
-    r.r = (a.r + a.i * t) / d;
-    r.i = (a.i - a.r * t) / d;
-  } else {
-    double t = b.r / b.i, d = b.r * t + b.i;
-    r.r = (a.r * t + a.i) / d;
-    r.i = (a.i * t - a.r) / d;
+  double c = b.r, d = b.i;
+  const double m = fmax(fabs(c), fabs(d));
+  int e = 0;
+  if (m > 0.0 && isfinite(m)) {
+    e = ilogb(m);
+    c = scalbn(c, -e);
+    d = scalbn(d, -e);
   }
+  const double den = __dmul_rn(c, c) + __dmul_rn(d, d);
+  cx<double> r;
+  r.r = scalbn((__dmul_rn(a.r, c) + __dmul_rn(a.i, d)) / den, -e);
+  r.i = scalbn((__dmul_rn(a.i, c) - __dmul_rn(a.r, d)) / den, -e);
   return r;
 }
 __device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 64); }
@@ -83,6 +91,33 @@ __device__ __forceinline__ cx<T> group_sum(cx<T> v) {
     v.i += shfl_xor_d(v.i, m);
   }
   return v;
+}
+
+// z loads / stores that bypass the non-coherent caches (agent-scope relaxed atomics):
+// the sweeps exchange z between blocks through them.
+__device__ __forceinline__ cx<double> zld(const cx<double> *p) {
+  unsigned long long *q = (unsigned long long *)p;
+  const unsigned long long a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned long long b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return {__longlong_as_double((long long)a), __longlong_as_double((long long)b)};
+}
+__device__ __forceinline__ cx<float> zld(const cx<float> *p) {
+  const unsigned long long a =
+      __hip_atomic_load((unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  cx<float> r;
+  __builtin_memcpy(&r, &a, 8);
+  return r;
+}
+__device__ __forceinline__ void zst(cx<double> *p, cx<double> v) {
+  unsigned long long *q = (unsigned long long *)p;
+  __hip_atomic_store(q, (unsigned long long)__double_as_longlong(v.r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(q + 1, (unsigned long long)__double_as_longlong(v.i), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void zst(cx<float> *p, cx<float> v) {
+  unsigned long long a;
+  __builtin_memcpy(&a, &v, 8);
+  __hip_atomic_store((unsigned long long *)p, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ double zero_of(double) { return 0.0; }
@@ -108,13 +143,13 @@ __device__ V block_cols(V v, V *lds, int kp) {
 
 // Grid-wide per-column reduction with identical results in every block.
 template <typename V>
-__device__ V grid_cols(V v, V *part, V *fin, V *lds, int k, int kp, int c, bool dist,
-                       cg::grid_group &grid) {
+__device__ V grid_cols(V v, V *part, V *fin, V *lds, int k, int kp, int c, bool dist, const GridBar &bar,
+                       unsigned &phase) {
   const int t = threadIdx.x;
   const int G = gridDim.x;
   V bs = block_cols(v, lds, kp);
   if (t < kp && t < k) part[(size_t)blockIdx.x * k + t] = bs;
-  grid.sync();
+  grid_barrier(bar, phase);
   if (!dist) {
     V s = zero_of(v);
     if (c < k)
@@ -127,18 +162,111 @@ __device__ V grid_cols(V v, V *part, V *fin, V *lds, int k, int kp, int c, bool 
     s = block_cols(s, lds, 1);
     if (t == 0) fin[cc] = s;
   }
-  grid.sync();
+  grid_barrier(bar, phase);
   return (c < k) ? fin[c] : zero_of(v);
 }
 
 // all columns finished? identical in every block because the states are
 __device__ bool block_all(bool mine) { return __syncthreads_and(mine ? 1 : 0) != 0; }
 
-template <typename T, int LPR, bool MULTI>
-__global__ void __launch_bounds__(256) k_cocg(CocgParams<T> p) {
+// Wait until block `ob` has published at least `need` (progress counters are monotonic across
+// sweeps; comparisons are wrap-safe). `known` caches the last value seen per block.
+__device__ __forceinline__ void wait_progress(const unsigned *progress, unsigned *known, int ob, unsigned need,
+                                              int *err) {
+  if ((int)(known[ob] - need) >= 0) return;
+  unsigned spins = 0;
+  unsigned v = __hip_atomic_load(&progress[ob], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while ((int)(v - need) < 0) {
+    __builtin_amdgcn_s_sleep(1);
+    if (++spins > AFX_SPIN_LIMIT) {
+      atomicOr(err, 2);
+      break;
+    }
+    v = __hip_atomic_load(&progress[ob], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  known[ob] = v;
+}
+
+// One pipelined triangular sweep (FWD: z = D~^-1 (r - L z); else z -= D~^-1 U z), see header.
+template <typename TP, int LPR, bool MULTI, bool FWD>
+__device__ void sweep(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> *zs, unsigned *known, unsigned epoch,
+                      int c, bool cvalid, TP w, int f, cx<double> &rho_loc) {
+  const int b = blockIdx.x;
+  const int t = threadIdx.x;
+  const int k = p.k;
+  const int nf = p.nf;
+  const int rs = p.rstart[b], re = p.rstart[b + 1];
+  const bool lds = !MULTI && p.lds_rows > 0;
+  const cx<TP> *val = (const cx<TP> *)sd.val;
+  const int kp = MULTI ? p.kp : 1;
+  const int slot = MULTI ? t / kp : t / LPR;
+  const int nslot = MULTI ? 256 / kp : 256 / LPR;
+  const int lane = MULTI ? 0 : t % LPR;
+  const int s0 = sd.bstep[b], s1 = sd.bstep[b + 1];
+  for (int s = s0; s < s1; ++s) {
+    const int q0 = sd.rptr[s], q1 = sd.rptr[s + 1];
+    for (int q = q0 + t; q < q1; q += 256) wait_progress(p.progress, known, sd.rblk[q], epoch + sd.rneed[q], p.err);
+    __syncthreads();
+    const int lo = sd.spos[s], hi = sd.spos[s + 1];
+    for (int pos = lo + slot; pos < hi; pos += nslot) {
+      const int row = sd.row[pos];
+      const int eb = sd.ptr[pos], ee = sd.ptr[pos + 1];
+      if (MULTI) {
+        if (cvalid) {
+          cx<TP> acc = {TP(0), TP(0)};
+          for (int q = eb; q < ee; ++q) acc = cfma(mval(val[q], w), zld(&p.Z[(size_t)sd.col[q] * k + c]), acc);
+          const size_t ix = (size_t)row * k + c;
+          cx<TP> z;
+          if (FWD) {
+            z = cmul(down<TP>(p.R[ix]) - acc, p.DD[(size_t)row * nf + f]);
+          } else {
+            z = zld(&p.Z[ix]) - cmul(p.DD[(size_t)row * nf + f], acc);
+            rho_loc = addv(rho_loc, cmul(p.R[ix], up(z)));
+          }
+          zst(&p.Z[ix], z);
+        }
+      } else {
+        cx<TP> acc = {TP(0), TP(0)};
+        for (int q = eb + lane; q < ee; q += LPR) {
+          const int cl = sd.col[q];
+          const cx<TP> zc = (lds && cl >= rs && cl < re) ? zs[cl - rs] : zld(&p.Z[cl]);
+          acc = cfma(mval(val[q], w), zc, acc);
+        }
+        acc = group_sum<LPR>(acc);
+        if (lane == 0) {
+          cx<TP> z;
+          if (FWD) {
+            z = cmul(down<TP>(p.R[row]) - acc, p.DD[row]);
+          } else {
+            const cx<TP> zo = lds ? zs[row - rs] : zld(&p.Z[row]);
+            z = zo - cmul(p.DD[row], acc);
+            rho_loc = addv(rho_loc, cmul(p.R[row], up(z)));
+          }
+          if (lds) zs[row - rs] = z;
+          if (!lds || sd.exp[row]) zst(&p.Z[row], z);
+        }
+      }
+    }
+    if (sd.pub[s]) {
+      // the exported z went out through write-through atomic stores: once every wave has
+      // drained them, the counter may be raised
+      wave_drain();
+      __syncthreads();
+      if (t == 0)
+        __hip_atomic_store(&p.progress[b], epoch + (unsigned)(s - s0 + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __syncthreads();
+    }
+  }
+}
+
+template <typename TP, int LPR, bool MULTI>
+__global__ void __launch_bounds__(256) k_cocg(CocgParams<TP> p) {
   __shared__ cx<double> lds_c[256];
   __shared__ double lds_d[256];
-  cg::grid_group grid = cg::this_grid();
+  extern __shared__ unsigned char dyn[];
+  unsigned *known = (unsigned *)dyn;                           // [G]
+  cx<TP> *zs = (cx<TP> *)(dyn + ((gridDim.x * 4 + 15) / 16) * 16);  // [lds_rows]
   const int t = threadIdx.x;
   const long long gtid = (long long)blockIdx.x * 256 + t;
   const long long nthr = (long long)gridDim.x * 256;
@@ -149,11 +277,17 @@ __global__ void __launch_bounds__(256) k_cocg(CocgParams<T> p) {
   const long long slot = MULTI ? gtid / kp : gtid / LPR;
   const long long nslot = MULTI ? nthr / kp : nthr / LPR;
   const int lane = MULTI ? 0 : (int)(gtid % LPR);
-  const T w = cvalid ? p.wcol[c] : T(0);
+  const double w = cvalid ? p.wcol[c] : 0.0;
+  const TP wT = (TP)w;
   const int f = cvalid ? p.col2f[c] : 0;
-  const int nf = p.nf;
-  const bool sweep = (p.pc == AFX_PC_IC || p.pc == AFX_PC_SSOR);
+  const bool sweep_pc = (p.pc == AFX_PC_IC || p.pc == AFX_PC_SSOR);
   const bool dist = p.dist_reduce != 0;
+  const int rs = p.rstart[blockIdx.x], re = p.rstart[blockIdx.x + 1];
+  unsigned phase = 0;
+  if (t == 0) phase = __hip_atomic_load(p.bar.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int ob = t; ob < (int)gridDim.x; ob += 256)
+    known[ob] = __hip_atomic_load(&p.progress[ob], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
 
   int st = cvalid ? p.conv[c] : 1;
   const double bn = cvalid ? p.bnrm2[c] : 1.0;
@@ -168,90 +302,51 @@ __global__ void __launch_bounds__(256) k_cocg(CocgParams<T> p) {
     ++it;
     // ---- z = M^-1 r (COCG13_single.f90:186-241) and rho = r^T z (:246-250) ----
     cx<double> rho_loc = {0.0, 0.0};
-    if (sweep) {
-      for (int lv = 0; lv < p.nlevL; ++lv) {
-        const int lo = p.lvlL[lv], hi = p.lvlL[lv + 1];
-        for (long long pos = lo + slot; pos < hi; pos += nslot) {
-          const int row = p.rowL[pos];
-          const int qb = p.Lptr[pos], qe = p.Lptr[pos + 1];
-          if (MULTI) {
-            if (cvalid) {
-              cx<T> acc = {T(0), T(0)};
-              for (int q = qb; q < qe; ++q) acc = cfma(mval(p.Lval[q], w), p.Z[(size_t)p.Lcol[q] * k + c], acc);
-              const size_t ix = (size_t)row * k + c;
-              p.Z[ix] = cmul(p.R[ix] - acc, p.DD[(size_t)row * nf + f]);
-            }
-          } else {
-            cx<T> acc = {T(0), T(0)};
-            for (int q = qb + lane; q < qe; q += LPR) acc = cfma(mval(p.Lval[q], w), p.Z[p.Lcol[q]], acc);
-            acc = group_sum<LPR>(acc);
-            if (lane == 0) p.Z[row] = cmul(p.R[row] - acc, p.DD[row]);
-          }
-        }
-        grid.sync();
-      }
-      for (int lv = 0; lv < p.nlevU; ++lv) {
-        const int lo = p.lvlU[lv], hi = p.lvlU[lv + 1];
-        for (long long pos = lo + slot; pos < hi; pos += nslot) {
-          const int row = p.rowU[pos];
-          const int qb = p.Uptr[pos], qe = p.Uptr[pos + 1];
-          if (MULTI) {
-            if (cvalid) {
-              cx<T> acc = {T(0), T(0)};
-              for (int q = qb; q < qe; ++q) acc = cfma(mval(p.Uval[q], w), p.Z[(size_t)p.Ucol[q] * k + c], acc);
-              const size_t ix = (size_t)row * k + c;
-              const cx<T> z = p.Z[ix] - cmul(p.DD[(size_t)row * nf + f], acc);
-              p.Z[ix] = z;
-              rho_loc = addv(rho_loc, cmul(up(p.R[ix]), up(z)));
-            }
-          } else {
-            cx<T> acc = {T(0), T(0)};
-            for (int q = qb + lane; q < qe; q += LPR) acc = cfma(mval(p.Uval[q], w), p.Z[p.Ucol[q]], acc);
-            acc = group_sum<LPR>(acc);
-            if (lane == 0) {
-              const cx<T> z = p.Z[row] - cmul(p.DD[row], acc);
-              p.Z[row] = z;
-              rho_loc = addv(rho_loc, cmul(up(p.R[row]), up(z)));
-            }
-          }
-        }
-        grid.sync();
-      }
+    if (sweep_pc) {
+      const unsigned ep = (unsigned)(2 * it - 1) * (unsigned)p.stride;
+      sweep<TP, LPR, MULTI, true>(p, p.fw, zs, known, ep, c, cvalid, wT, f, rho_loc);
+      grid_barrier(p.bar, phase);
+      sweep<TP, LPR, MULTI, false>(p, p.bw, zs, known, ep + (unsigned)p.stride, c, cvalid, wT, f, rho_loc);
+      if (!MULTI && p.lds_rows > 0)
+        for (int r = rs + t; r < re; r += 256) p.Z[r] = zs[r - rs];
     } else {
       if (MULTI) {
         if (cvalid)
           for (long long row = slot; row < p.n; row += nslot) {
             const size_t ix = (size_t)row * k + c;
-            const cx<T> r = p.R[ix];
-            const cx<T> z = (p.pc == AFX_PC_DIAG) ? cmul(r, p.DD[(size_t)row * nf + f]) : r;
+            const cx<double> r = p.R[ix];
+            const cx<TP> z = (p.pc == AFX_PC_DIAG) ? cmul(down<TP>(r), p.DD[(size_t)row * p.nf + f]) : down<TP>(r);
             p.Z[ix] = z;
-            rho_loc = addv(rho_loc, cmul(up(r), up(z)));
+            rho_loc = addv(rho_loc, cmul(r, up(z)));
           }
       } else {
         for (long long row = gtid; row < p.n; row += nthr) {
-          const cx<T> r = p.R[row];
-          const cx<T> z = (p.pc == AFX_PC_DIAG) ? cmul(r, p.DD[row]) : r;
+          const cx<double> r = p.R[row];
+          const cx<TP> z = (p.pc == AFX_PC_DIAG) ? cmul(down<TP>(r), p.DD[row]) : down<TP>(r);
           p.Z[row] = z;
-          rho_loc = addv(rho_loc, cmul(up(r), up(z)));
+          rho_loc = addv(rho_loc, cmul(r, up(z)));
         }
       }
     }
-    const cx<double> rho = grid_cols(rho_loc, p.part_rho, p.fin_rho, lds_c, k, kp, c, dist, grid);
+    const cx<double> rho = grid_cols(rho_loc, p.part_rho, p.fin_rho, lds_c, k, kp, c, dist, p.bar, phase);
 
     // ---- p = z (first iteration) or z + beta p (:258-268) ----
     const bool first = (it == 1);
     const cx<double> beta = first ? cx<double>{0.0, 0.0} : cdivd(rho, rho1);
-    const cx<T> betaT = down<T>(beta);
     if (MULTI) {
       if (cvalid)
         for (long long row = slot; row < p.n; row += nslot) {
           const size_t ix = (size_t)row * k + c;
-          p.P[ix] = first ? p.Z[ix] : cfma(betaT, p.P[ix], p.Z[ix]);
+          const cx<double> z = up(p.Z[ix]);
+          p.P[ix] = first ? z : cfma(beta, p.P[ix], z);
         }
     } else {
-      for (long long row = gtid; row < p.n; row += nthr) p.P[row] = first ? p.Z[row] : cfma(betaT, p.P[row], p.Z[row]);
+      for (long long row = gtid; row < p.n; row += nthr) {
+        const cx<double> z = up(p.Z[row]);
+        p.P[row] = first ? z : cfma(beta, p.P[row], z);
+      }
     }
-    grid.sync();
+    grid_barrier(p.bar, phase);
 
     // ---- q = A p (:273-288) and c1 = p^T q (:293-296) ----
     cx<double> c1_loc = {0.0, 0.0};
@@ -259,23 +354,23 @@ __global__ void __launch_bounds__(256) k_cocg(CocgParams<T> p) {
       const int qb = p.rowptr[row], qe = p.rowptr[row + 1];
       if (MULTI) {
         if (cvalid) {
-          cx<T> acc = {T(0), T(0)};
+          cx<double> acc = {0.0, 0.0};
           for (int q = qb; q < qe; ++q) acc = cfma(mval(p.val[q], w), p.P[(size_t)p.col[q] * k + c], acc);
           const size_t ix = (size_t)row * k + c;
           p.Q[ix] = acc;
-          c1_loc = addv(c1_loc, cmul(up(p.P[ix]), up(acc)));
+          c1_loc = addv(c1_loc, cmul(p.P[ix], acc));
         }
       } else {
-        cx<T> acc = {T(0), T(0)};
+        cx<double> acc = {0.0, 0.0};
         for (int q = qb + lane; q < qe; q += LPR) acc = cfma(mval(p.val[q], w), p.P[p.col[q]], acc);
         acc = group_sum<LPR>(acc);
         if (lane == 0) {
           p.Q[row] = acc;
-          c1_loc = addv(c1_loc, cmul(up(p.P[row]), up(acc)));
+          c1_loc = addv(c1_loc, cmul(p.P[row], acc));
         }
       }
     }
-    const cx<double> c1 = grid_cols(c1_loc, p.part_c1, p.fin_c1, lds_c, k, kp, c, dist, grid);
+    const cx<double> c1 = grid_cols(c1_loc, p.part_c1, p.fin_c1, lds_c, k, kp, c, dist, p.bar, phase);
     const cx<double> alpha = cdivd(rho, c1);
     const bool run = (st == 0);
     if (run && (!(c1.r != 0.0 || c1.i != 0.0) || !isfinite(alpha.r) || !isfinite(alpha.i) ||
@@ -284,8 +379,7 @@ __global__ void __launch_bounds__(256) k_cocg(CocgParams<T> p) {
       its = it;
     }
     const bool upd = (st == 0);
-    const cx<T> alphaT = down<T>(alpha);
-    const cx<T> malphaT = {-alphaT.r, -alphaT.i};
+    const cx<double> malpha = {-alpha.r, -alpha.i};
 
     // ---- x += alpha p, r -= alpha q (:304-307), ||r||^2 (:310-313) ----
     double rr_loc = 0.0;
@@ -293,20 +387,20 @@ __global__ void __launch_bounds__(256) k_cocg(CocgParams<T> p) {
       if (cvalid && upd)
         for (long long row = slot; row < p.n; row += nslot) {
           const size_t ix = (size_t)row * k + c;
-          p.X[ix] = cfma(alphaT, p.P[ix], p.X[ix]);
-          const cx<T> r = cfma(malphaT, p.Q[ix], p.R[ix]);
+          p.X[ix] = cfma(alpha, p.P[ix], p.X[ix]);
+          const cx<double> r = cfma(malpha, p.Q[ix], p.R[ix]);
           p.R[ix] = r;
-          rr_loc += (double)r.r * (double)r.r + (double)r.i * (double)r.i;
+          rr_loc += r.r * r.r + r.i * r.i;
         }
     } else if (upd) {
       for (long long row = gtid; row < p.n; row += nthr) {
-        p.X[row] = cfma(alphaT, p.P[row], p.X[row]);
-        const cx<T> r = cfma(malphaT, p.Q[row], p.R[row]);
+        p.X[row] = cfma(alpha, p.P[row], p.X[row]);
+        const cx<double> r = cfma(malpha, p.Q[row], p.R[row]);
         p.R[row] = r;
-        rr_loc += (double)r.r * (double)r.r + (double)r.i * (double)r.i;
+        rr_loc += r.r * r.r + r.i * r.i;
       }
     }
-    const double rr = grid_cols(rr_loc, p.part_rr, p.fin_rr, lds_d, k, kp, c, dist, grid);
+    const double rr = grid_cols(rr_loc, p.part_rr, p.fin_rr, lds_d, k, kp, c, dist, p.bar, phase);
     if (upd) {
       res = sqrt(rr / bn);  // :315 RESID = sqrt(DNRM2/BNRM2)
       if (res <= p.tol) {
@@ -334,21 +428,31 @@ __global__ void __launch_bounds__(256) k_cocg(CocgParams<T> p) {
   }
 }
 
-template <typename T, int LPR, bool MULTI>
-static hipError_t launch_one(const CocgParams<T> &p, int grid, hipStream_t s) {
-  void *args[] = {(void *)&p};
-  return hipLaunchCooperativeKernel((const void *)k_cocg<T, LPR, MULTI>, dim3(grid), dim3(256), args, 0, s);
+template <typename TP>
+size_t cocg_dyn_lds(const CocgParams<TP> &p, int grid) {
+  return ((size_t)grid * 4 + 15) / 16 * 16 + (size_t)p.lds_rows * sizeof(cx<TP>);
 }
 
-template <typename T>
-static hipError_t launch_t(const CocgParams<T> &p, int lpr, int grid, hipStream_t s) {
-  if (p.k > 1) return launch_one<T, 1, true>(p, grid, s);
+template <typename TP, int LPR, bool MULTI>
+static hipError_t launch_one(const CocgParams<TP> &p, int grid, hipStream_t s) {
+  void *args[] = {(void *)&p};
+  const size_t dyn = cocg_dyn_lds(p, grid);
+  hipError_t e = hipFuncSetAttribute((const void *)k_cocg<TP, LPR, MULTI>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)dyn);
+  if (e != hipSuccess) return e;
+  return hipLaunchCooperativeKernel((const void *)k_cocg<TP, LPR, MULTI>, dim3(grid), dim3(256), args, dyn, s);
+}
+
+template <typename TP>
+static hipError_t launch_t(const CocgParams<TP> &p, int lpr, int grid, hipStream_t s) {
+  if (grid != p.Gs) return hipErrorInvalidValue;
+  if (p.k > 1) return launch_one<TP, 1, true>(p, grid, s);
   switch (lpr) {
-    case 1: return launch_one<T, 1, false>(p, grid, s);
-    case 2: return launch_one<T, 2, false>(p, grid, s);
-    case 4: return launch_one<T, 4, false>(p, grid, s);
-    case 8: return launch_one<T, 8, false>(p, grid, s);
-    case 16: return launch_one<T, 16, false>(p, grid, s);
+    case 1: return launch_one<TP, 1, false>(p, grid, s);
+    case 2: return launch_one<TP, 2, false>(p, grid, s);
+    case 4: return launch_one<TP, 4, false>(p, grid, s);
+    case 8: return launch_one<TP, 8, false>(p, grid, s);
+    case 16: return launch_one<TP, 16, false>(p, grid, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -356,30 +460,33 @@ static hipError_t launch_t(const CocgParams<T> &p, int lpr, int grid, hipStream_
 hipError_t launch_cocg64(const CocgParams<double> &p, int lpr, int grid, hipStream_t s) { return launch_t(p, lpr, grid, s); }
 hipError_t launch_cocg32(const CocgParams<float> &p, int lpr, int grid, hipStream_t s) { return launch_t(p, lpr, grid, s); }
 
-template <typename T, int LPR, bool MULTI>
-static int occ_one() {
+template <typename TP, int LPR, bool MULTI>
+static int occ_one(size_t dyn) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_cocg<T, LPR, MULTI>, 256, 0) != hipSuccess)
+  if (hipFuncSetAttribute((const void *)k_cocg<TP, LPR, MULTI>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) !=
+      hipSuccess)
+    return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_cocg<TP, LPR, MULTI>, 256, dyn) != hipSuccess)
     return 0;
   return nb;
 }
 
-// Largest co-resident grid (blocks) for the cooperative launch.
-int max_coop_grid(int lpr, bool multi, bool fp64) {
+// Largest co-resident grid (blocks) for the cooperative launch with `dyn` bytes of LDS.
+int max_coop_grid(int lpr, bool multi, bool fp64, size_t dyn) {
   int dev = 0, ncu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   int nb = 0;
   if (fp64) {
-    if (multi) nb = occ_one<double, 1, true>();
-    else nb = (lpr == 1) ? occ_one<double, 1, false>() : (lpr == 2) ? occ_one<double, 2, false>()
-            : (lpr == 4) ? occ_one<double, 4, false>() : (lpr == 8) ? occ_one<double, 8, false>()
-                                                       : occ_one<double, 16, false>();
+    if (multi) nb = occ_one<double, 1, true>(dyn);
+    else nb = (lpr == 1) ? occ_one<double, 1, false>(dyn) : (lpr == 2) ? occ_one<double, 2, false>(dyn)
+            : (lpr == 4) ? occ_one<double, 4, false>(dyn) : (lpr == 8) ? occ_one<double, 8, false>(dyn)
+                                                          : occ_one<double, 16, false>(dyn);
   } else {
-    if (multi) nb = occ_one<float, 1, true>();
-    else nb = (lpr == 1) ? occ_one<float, 1, false>() : (lpr == 2) ? occ_one<float, 2, false>()
-            : (lpr == 4) ? occ_one<float, 4, false>() : (lpr == 8) ? occ_one<float, 8, false>()
-                                                      : occ_one<float, 16, false>();
+    if (multi) nb = occ_one<float, 1, true>(dyn);
+    else nb = (lpr == 1) ? occ_one<float, 1, false>(dyn) : (lpr == 2) ? occ_one<float, 2, false>(dyn)
+            : (lpr == 4) ? occ_one<float, 4, false>(dyn) : (lpr == 8) ? occ_one<float, 8, false>(dyn)
+                                                         : occ_one<float, 16, false>(dyn);
   }
   return nb * ncu;
 }

@@ -40,3 +40,36 @@ def gpu_available():
         return torch.cuda.is_available()
     except Exception:
         return False
+
+
+def crust_params(case, ncell_max=None, seed=0):
+    """Parameters = crust cells (6 tets each): CSR (par_ptr, par_tet 0-based), cell ids."""
+    crust = np.nonzero(case.group >= 3)[0]
+    cells, inv = np.unique(case.cell_of_tet[crust], return_inverse=True)
+    order = np.argsort(inv, kind="stable")
+    tets = crust[order]
+    counts = np.bincount(inv)
+    ptr = np.concatenate([[0], np.cumsum(counts)])
+    if ncell_max is not None and ncell_max < len(cells):
+        rng = np.random.default_rng(seed)
+        pick = np.sort(rng.choice(len(cells), ncell_max, replace=False))
+        sel = [tets[ptr[m]:ptr[m + 1]] for m in pick]
+        ptr = np.concatenate([[0], np.cumsum([len(s) for s in sel])])
+        return ptr.astype(np.int32), np.concatenate(sel).astype(np.int32), cells[pick]
+    return ptr.astype(np.int32), tets.astype(np.int32), cells
+
+
+def seafloor_sites(case, n, seed):
+    """Seeded sites 1 m below the sea-bottom plane, strictly inside a tet."""
+    rng = np.random.default_rng(seed)
+    h = case.h
+    xout, yout = case.nx * h / 2, case.ny * h / 2
+    pts = []
+    for _ in range(n):
+        i = rng.integers(1, case.nx - 1)
+        j = rng.integers(1, case.ny - 1)
+        fx, fy = rng.uniform(0.13, 0.87, size=2)
+        if abs(fx - fy) < 0.05:
+            fy = min(0.87, fy + 0.1)
+        pts.append([-xout + (i + fx) * h, -yout + (j + fy) * h, -h - 0.001])
+    return np.array(pts)

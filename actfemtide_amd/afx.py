@@ -32,7 +32,7 @@ AFX_ERR_GEOM = -500
 AFX_ERR_PATTERN = -600
 AFX_ERR_OUTSIDE = -700
 
-PC = {"NONE": 0, "IC": 1, "ILU": 1, "SSOR": 2, "DIAG": 3}
+PC = {"NONE": 0, "IC": 1, "ILU": 1, "SSOR": 2, "DIAG": 3, "SSOR_BJ": 4}
 FP64, MIXED = 0, 1
 
 
@@ -75,6 +75,7 @@ SIGNATURES = {
     "afx_solve_device": ([_VP, _I, _VP, _VP, _VP, C.POINTER(afx_opts), _VP], _I),
     "afx_true_residual_device": ([_VP, _I, _VP, _VP, _VP, _VP], _I),
     "afx_autotune": ([_VP, _I, _I], _I),
+    "afx_kernel_timing": ([_VP, _I, _VP, _VP, _VP], _I),
     "afx_spmv": ([_VP, _VP, _VP], _I),
     "afx_precond_apply": ([_VP, _VP, _VP], _I),
     "afx_get_precond_diag": ([_VP, _VP], _I),
@@ -257,6 +258,12 @@ class System:
 
     def autotune(self, nrhs=1, verbose=False):
         _err(lib().afx_autotune(self.h, nrhs, 1 if verbose else 0))
+
+    def kernel_timing(self, reset=True):
+        """(launches, iterations, kernel_ms) of the persistent COCG kernel since the last reset."""
+        la, it, ms = C.c_longlong(), C.c_longlong(), C.c_double()
+        _err(lib().afx_kernel_timing(self.h, 1 if reset else 0, C.byref(la), C.byref(it), C.byref(ms)))
+        return la.value, it.value, ms.value
 
     def solve(self, B, X0=None, **opts):
         """Solve A X = B (host arrays, columns = right-hand sides); X0 is the initial guess."""

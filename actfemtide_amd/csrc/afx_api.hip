@@ -72,6 +72,11 @@ struct DevSide {
   cx<double> *val64 = nullptr;
   cx<float> *val32 = nullptr;
   long long nnz = 0;
+  int ew = 0, rmax = 0;
+  int *ecol = nullptr, *emap = nullptr;
+  cx<double> *eval64 = nullptr, *dpos64 = nullptr;
+  cx<float> *eval32 = nullptr, *dpos32 = nullptr;
+  int2 *req = nullptr;
 };
 
 struct afx_system_s {
@@ -115,11 +120,14 @@ struct afx_system_s {
   // pipelined sweeps
   int Gs = 0, sweep_stride = 0, rows_per_block = 0;
   int *d_rstart = nullptr;
-  DevSide fw, bw;
+  DevSide fw, bw, bjfw, bjbw;
   unsigned *d_progress = nullptr, *d_bar = nullptr;
   int *d_err = nullptr;
   // launch configuration (autotuned)
   int lpr64 = 8, lpr32 = 8;
+  // launch timing of the persistent COCG kernel since the last afx_kernel_timing reset
+  long long prof_launches = 0, prof_iters = 0;
+  double prof_ms = 0.0;
   int ncu = 0;
 };
 
@@ -272,6 +280,12 @@ int ensure_derived(afx_system s) {
   HIPCHK(afx::launch_gather_vals((int)s->nU, s->d_mapU, s->d_canon, s->d_Uval64, s->d_Uval32, s->stream));
   HIPCHK(afx::launch_gather_vals((int)s->fw.nnz, s->fw.map, s->d_canon, s->fw.val64, s->fw.val32, s->stream));
   HIPCHK(afx::launch_gather_vals((int)s->bw.nnz, s->bw.map, s->d_canon, s->bw.val64, s->bw.val32, s->stream));
+  HIPCHK(afx::launch_gather_vals(s->n * s->fw.ew, s->fw.emap, s->d_canon, s->fw.eval64, s->fw.eval32, s->stream));
+  HIPCHK(afx::launch_gather_vals(s->n * s->bw.ew, s->bw.emap, s->d_canon, s->bw.eval64, s->bw.eval32, s->stream));
+  for (DevSide *d : {&s->bjfw, &s->bjbw}) {
+    HIPCHK(afx::launch_gather_vals((int)d->nnz, d->map, s->d_canon, d->val64, d->val32, s->stream));
+    HIPCHK(afx::launch_gather_vals(s->n * d->ew, d->emap, s->d_canon, d->eval64, d->eval32, s->stream));
+  }
   s->derived_valid = true;
   return AFX_OK;
 }
@@ -298,6 +312,12 @@ int ensure_precond(afx_system s, int kind, double sd, double sig, const std::vec
                                            s->d_lvlL, s->d_rowL, s->d_Lptr, s->d_Lcol, s->d_Lval64, s->d_INU,
                                            s->d_canon + s->n, d_wf, s->d_DD, grid, s->stream);
   if (e == hipSuccess) e = afx::launch_cast_vec((long long)s->n * nf, s->d_DD, s->d_DD32, s->stream);
+  if (e == hipSuccess && nf == 1) e = afx::launch_dpos(s->n, s->fw.row, s->d_DD, s->fw.dpos64, s->fw.dpos32, s->stream);
+  if (e == hipSuccess && nf == 1) e = afx::launch_dpos(s->n, s->bw.row, s->d_DD, s->bw.dpos64, s->bw.dpos32, s->stream);
+  if (e == hipSuccess && nf == 1)
+    e = afx::launch_dpos(s->n, s->bjfw.row, s->d_DD, s->bjfw.dpos64, s->bjfw.dpos32, s->stream);
+  if (e == hipSuccess && nf == 1)
+    e = afx::launch_dpos(s->n, s->bjbw.row, s->d_DD, s->bjbw.dpos64, s->bjbw.dpos32, s->stream);
   hipError_t e2 = hipStreamSynchronize(s->stream);
   dfree(d_wf);
   if (e != hipSuccess || e2 != hipSuccess) {
@@ -343,10 +363,33 @@ struct SideHost {
   std::vector<int> bstep, spos, row, ptr, col, map, rptr, rblk, rneed;
   std::vector<unsigned char> pub, exp;
   int maxsteps = 0;
+  // fixed-width (padded) entries and per-step requirement slots
+  int ew = 0, rmax = 0;
+  std::vector<int> ecol, emap;
+  std::vector<int2> req;
 };
 
 void build_side(int n, int G, const std::vector<int> &rstart, const std::vector<int> &blk,
-                const std::vector<int> &P, const std::vector<int> &C1, long long canon_off, bool fwd, SideHost &h) {
+                const std::vector<int> &P0, const std::vector<int> &C0, long long canon_off, bool fwd, SideHost &h,
+                bool local_only = false) {
+  // local_only: drop the couplings between blocks (block-Jacobi), keeping canonical indices
+  std::vector<int> P, C1, qmap;
+  if (local_only) {
+    P.assign(n + 1, 0);
+    for (int i = 0; i < n; i++) {
+      for (int q = P0[i]; q < P0[i + 1]; q++)
+        if (blk[C0[q] - 1] == blk[i]) {
+          C1.push_back(C0[q]);
+          qmap.push_back(q);
+        }
+      P[i + 1] = (int)C1.size();
+    }
+  } else {
+    P = P0;
+    C1 = C0;
+    qmap.resize(C0.size());
+    for (size_t q = 0; q < C0.size(); q++) qmap[q] = (int)q;
+  }
   std::vector<int> lev(n, 0);
   if (fwd) {
     for (int i = 0; i < n; i++)
@@ -396,7 +439,7 @@ void build_side(int n, int G, const std::vector<int> &rstart, const std::vector<
     h.ptr[pos] = (int)h.col.size();
     for (int q = P[i]; q < P[i + 1]; q++) {
       h.col.push_back(C1[q] - 1);
-      h.map.push_back((int)(canon_off + q));
+      h.map.push_back((int)(canon_off + qmap[q]));
     }
   }
   h.ptr[n] = (int)h.col.size();
@@ -434,6 +477,22 @@ void build_side(int n, int G, const std::vector<int> &rstart, const std::vector<
     for (int ob : seen) runmax[ob] = 0;
   }
   h.rptr[nstep] = (int)h.rblk.size();
+  h.ew = 0;
+  for (int pos = 0; pos < n; pos++) h.ew = std::max(h.ew, h.ptr[pos + 1] - h.ptr[pos]);
+  h.ew = std::max(h.ew, 1);
+  h.ecol.assign((size_t)n * h.ew, -1);
+  h.emap.assign((size_t)n * h.ew, -1);
+  for (int pos = 0; pos < n; pos++)
+    for (int q = h.ptr[pos]; q < h.ptr[pos + 1]; q++) {
+      h.ecol[(size_t)pos * h.ew + (q - h.ptr[pos])] = h.col[q];
+      h.emap[(size_t)pos * h.ew + (q - h.ptr[pos])] = h.map[q];
+    }
+  h.rmax = 1;
+  for (int st = 0; st < nstep; st++) h.rmax = std::max(h.rmax, h.rptr[st + 1] - h.rptr[st]);
+  h.req.assign((size_t)nstep * h.rmax, make_int2(-1, 0));
+  for (int st = 0; st < nstep; st++)
+    for (int q = h.rptr[st]; q < h.rptr[st + 1]; q++)
+      h.req[(size_t)st * h.rmax + (q - h.rptr[st])] = make_int2(h.rblk[q], h.rneed[q]);
 }
 
 int upload_side(const SideHost &h, DevSide &d) {
@@ -447,12 +506,20 @@ int upload_side(const SideHost &h, DevSide &d) {
       (rc = dalloc(&d.val32, h.col.size())))
     return rc;
   d.nnz = (long long)h.col.size();
+  d.ew = h.ew;
+  d.rmax = h.rmax;
+  const size_t ne = h.ecol.size();
+  if ((rc = upload(&d.ecol, h.ecol.data(), ne)) || (rc = upload(&d.emap, h.emap.data(), ne)) ||
+      (rc = dalloc(&d.eval64, ne)) || (rc = dalloc(&d.eval32, ne)) || (rc = dalloc(&d.dpos64, h.row.size())) ||
+      (rc = dalloc(&d.dpos32, h.row.size())) || (rc = upload(&d.req, h.req.data(), h.req.size())))
+    return rc;
   return AFX_OK;
 }
 
 void free_side(DevSide &d) {
   dfree(d.bstep); dfree(d.spos); dfree(d.row); dfree(d.ptr); dfree(d.col); dfree(d.map); dfree(d.rptr);
   dfree(d.rblk); dfree(d.rneed); dfree(d.pub); dfree(d.exp); dfree(d.val64); dfree(d.val32);
+  dfree(d.ecol); dfree(d.emap); dfree(d.eval64); dfree(d.eval32); dfree(d.dpos64); dfree(d.dpos32); dfree(d.req);
 }
 
 afx::SweepSide side_view(const DevSide &d, bool fp64) {
@@ -460,6 +527,14 @@ afx::SweepSide side_view(const DevSide &d, bool fp64) {
   v.bstep = d.bstep; v.spos = d.spos; v.row = d.row; v.ptr = d.ptr; v.col = d.col;
   v.val = fp64 ? (const void *)d.val64 : (const void *)d.val32;
   v.rptr = d.rptr; v.rblk = d.rblk; v.rneed = d.rneed; v.pub = d.pub; v.exp = d.exp;
+  // the fixed-width path needs at most 16 entries per position (the E * lanes of the kernel)
+  const bool ell = d.ew <= 16 && !getenv("AFX_NO_ELL");
+  v.ew = d.ew;
+  v.ecol = ell ? d.ecol : nullptr;
+  v.eval = fp64 ? (const void *)d.eval64 : (const void *)d.eval32;
+  v.dpos = fp64 ? (const void *)d.dpos64 : (const void *)d.dpos32;
+  v.rmax = d.rmax;
+  v.req = d.req;
   return v;
 }
 
@@ -475,12 +550,17 @@ int build_sweeps(afx_system s) {
   SideHost fw, bw;
   build_side(n, G, rstart, blk, s->INL, s->IAL, (long long)n + s->nU, true, fw);
   build_side(n, G, rstart, blk, s->INU, s->IAU, n, false, bw);
-  s->sweep_stride = std::max(fw.maxsteps, bw.maxsteps) + 1;
+  SideHost bjfw, bjbw;
+  build_side(n, G, rstart, blk, s->INL, s->IAL, (long long)n + s->nU, true, bjfw, true);
+  build_side(n, G, rstart, blk, s->INU, s->IAU, n, false, bjbw, true);
+  s->sweep_stride = std::max(std::max(fw.maxsteps, bw.maxsteps), std::max(bjfw.maxsteps, bjbw.maxsteps)) + 1;
   s->rows_per_block = 0;
   for (int b = 0; b < G; b++) s->rows_per_block = std::max(s->rows_per_block, rstart[b + 1] - rstart[b]);
   int rc;
   if ((rc = upload(&s->d_rstart, rstart.data(), rstart.size()))) return rc;
-  if ((rc = upload_side(fw, s->fw)) || (rc = upload_side(bw, s->bw))) return rc;
+  if ((rc = upload_side(fw, s->fw)) || (rc = upload_side(bw, s->bw)) || (rc = upload_side(bjfw, s->bjfw)) ||
+      (rc = upload_side(bjbw, s->bjbw)))
+    return rc;
   if ((rc = dalloc(&s->d_progress, G)) || (rc = dalloc(&s->d_bar, 64)) || (rc = dalloc(&s->d_err, 1))) return rc;
   HIPCHK(hipMemset(s->d_bar, 0, 64 * sizeof(unsigned)));
   HIPCHK(hipMemset(s->d_err, 0, sizeof(int)));
@@ -607,8 +687,9 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   p.lds_rows = lds_rows_for<TP>(s, multi);
   p.stride = s->sweep_stride;
   p.rstart = s->d_rstart;
-  p.fw = side_view(s->fw, fp64);
-  p.bw = side_view(s->bw, fp64);
+  const bool bj = (pc == AFX_PC_SSOR_BJ);
+  p.fw = side_view(bj ? s->bjfw : s->fw, fp64);
+  p.bw = side_view(bj ? s->bjbw : s->bw, fp64);
   p.progress = s->d_progress;
   p.DD = fp64 ? (const cx<TP> *)s->d_DD : (const cx<TP> *)s->d_DD32;
   p.col2f = w.col2f;
@@ -641,15 +722,29 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   const int lpr = fp64 ? s->lpr64 : s->lpr32;
   const bool verbose = getenv("AFX_VERBOSE") != nullptr;
   const auto t_start = std::chrono::steady_clock::now();
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  HIPCHK(hipEventCreate(&ev0));
+  HIPCHK(hipEventCreate(&ev1));
+  int it_before = 0;
   for (;;) {
+    HIPCHK(hipEventRecord(ev0, s->stream));
     hipError_t e = fp64 ? afx::launch_cocg64(*(afx::CocgParams<double> *)&p, lpr, G, s->stream)
                         : afx::launch_cocg32(*(afx::CocgParams<float> *)&p, lpr, G, s->stream);
     if (e != hipSuccess) {
+      (void)hipEventDestroy(ev0);
+      (void)hipEventDestroy(ev1);
       g_err = std::string("cooperative COCG launch: ") + hipGetErrorString(e);
       return AFX_ERR_HIP;
     }
+    HIPCHK(hipEventRecord(ev1, s->stream));
     HIPCHK(hipMemcpyAsync(w.h_status, w.status, 2 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+    s->prof_launches += 1;
+    s->prof_ms += ms;
+    s->prof_iters += w.h_status[0] - it_before;
+    it_before = w.h_status[0];
     if (verbose) {
       const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
       fprintf(stderr, "afx: %s k=%d G=%d lpr=%d lds=%d it=%d done=%d %.3fs (%.1f us/it)\n",
@@ -658,6 +753,8 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
     }
     if (w.h_status[1] || w.h_status[0] >= maxit) break;
   }
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
   int herr = 0;
   HIPCHK(hipMemcpy(&herr, s->d_err, sizeof(int), hipMemcpyDeviceToHost));
   if (herr) return fail(AFX_ERR_HIP, "a bounded device wait gave up (synchronisation error)");
@@ -718,6 +815,8 @@ void release(afx_system s) {
   dfree(s->d_rhs_unit);
   free_side(s->fw);
   free_side(s->bw);
+  free_side(s->bjfw);
+  free_side(s->bjbw);
   dfree(s->d_rstart); dfree(s->d_progress); dfree(s->d_bar); dfree(s->d_err);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   s->stream = nullptr;
@@ -1334,6 +1433,19 @@ void lambda_h(const double x[4][3], const double *p, double a[4]) {
 }  // namespace
 
 extern "C" {
+
+int afx_kernel_timing(afx_system s, int reset, long long *launches, long long *iterations, double *kernel_ms) {
+  if (!s) return fail(AFX_ERR_ARG, "null system");
+  if (launches) *launches = s->prof_launches;
+  if (iterations) *iterations = s->prof_iters;
+  if (kernel_ms) *kernel_ms = s->prof_ms;
+  if (reset) {
+    s->prof_launches = 0;
+    s->prof_iters = 0;
+    s->prof_ms = 0.0;
+  }
+  return AFX_OK;
+}
 
 int afx_autotune(afx_system s, int nrhs, int verbose) {
   // Rows per wavefront (lanes per row `lpr`) of the single-column kernel, per preconditioner

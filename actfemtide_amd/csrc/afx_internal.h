@@ -32,6 +32,13 @@ struct SweepSide {
   const int *rneed;  // requirement: steps that block must have published
   const unsigned char *pub;  // [nstep] 1: another block waits on this step
   const unsigned char *exp;  // [n] by row id: 1: another block reads this row's z
+  // fixed-width copy for the single-column LDS path: entry e of position pos at pos*ew + e
+  int ew;                    // entries per position (padding: column -1, value 0)
+  const int *ecol;
+  const void *eval;          // cx<TP>
+  const void *dpos;          // [n] cx<TP>: DD in position order
+  int rmax;                  // requirement slots per step
+  const int2 *req;           // [nstep][rmax] (block, steps); block -1 = unused
 };
 
 template <typename TP>
@@ -105,6 +112,7 @@ hipError_t launch_precond_apply_levels(int n, int kind, int nlevL, const int *lv
                                        const cx<double> *r, cx<double> *z, hipStream_t s);
 hipError_t launch_gather_vals(int m, const int *map, const cx<double> *canon, cx<double> *out,
                               cx<float> *out32, hipStream_t s);
+hipError_t launch_dpos(int n, const int *row, const cx<double> *DD, cx<double> *d64, cx<float> *d32, hipStream_t s);
 hipError_t launch_cast_vec(long long m, const cx<double> *in, cx<float> *out, hipStream_t s);
 hipError_t launch_axpy_f32_to_f64(long long m, const cx<float> *d, cx<double> *x, hipStream_t s);
 hipError_t launch_scale_cols(long long n, int k, const cx<double> *in, const double *scale,

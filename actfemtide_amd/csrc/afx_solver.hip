@@ -260,6 +260,96 @@ __device__ void sweep(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> *zs, 
   }
 }
 
+// Single-column sweep with the block's z (and, on the forward pass, r) staged in LDS and the
+// next step's metadata prefetched into registers: a step then costs one LDS round trip plus the
+// exchange with other blocks, instead of a chain of dependent global loads.
+// zs[row - rs] holds r_i until row i is processed (forward) and z_i afterwards.
+template <typename TP, int LPR, int E>
+struct StepRegs {
+  int row;
+  int col[E];
+  cx<TP> val[E];
+  cx<TP> dd;
+};
+
+template <typename TP, int LPR, int E>
+__device__ __forceinline__ void prefetch_step(const SweepSide &sd, int pos, int hi, int lane, StepRegs<TP, LPR, E> &r) {
+  if (pos < hi) {
+    r.row = sd.row[pos];
+    r.dd = ((const cx<TP> *)sd.dpos)[pos];
+    const int *ec = sd.ecol + (size_t)pos * sd.ew;
+    const cx<TP> *ev = (const cx<TP> *)sd.eval + (size_t)pos * sd.ew;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int q = lane + e * LPR;
+      r.col[e] = (q < sd.ew) ? ec[q] : -1;
+      r.val[e] = (q < sd.ew) ? ev[q] : cx<TP>{TP(0), TP(0)};
+    }
+  } else {
+    r.row = -1;
+  }
+}
+
+template <typename TP, int LPR, int E, bool FWD>
+__device__ void sweep_lds(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> *zs, unsigned *known, unsigned epoch,
+                          TP w) {
+  const int b = blockIdx.x;
+  const int t = threadIdx.x;
+  const int rs = p.rstart[b], re = p.rstart[b + 1];
+  const int slot = t / LPR, lane = t % LPR;
+  constexpr int NSLOT = 256 / LPR;
+  const int s0 = sd.bstep[b], s1 = sd.bstep[b + 1];
+  StepRegs<TP, LPR, E> cur, nxt;
+  int2 rq_cur = make_int2(-1, 0), rq_nxt = make_int2(-1, 0);
+  if (s0 < s1) {
+    prefetch_step(sd, sd.spos[s0] + slot, sd.spos[s0 + 1], lane, cur);
+    if (t < sd.rmax) rq_cur = sd.req[(size_t)s0 * sd.rmax + t];
+  }
+  for (int s = s0; s < s1; ++s) {
+    if (rq_cur.x >= 0) wait_progress(p.progress, known, rq_cur.x, epoch + (unsigned)rq_cur.y, p.err);
+    const int lo = sd.spos[s], hi = sd.spos[s + 1];
+    __syncthreads();
+    if (s + 1 < s1) {
+      prefetch_step(sd, sd.spos[s + 1] + slot, sd.spos[s + 2], lane, nxt);
+      if (t < sd.rmax) rq_nxt = sd.req[(size_t)(s + 1) * sd.rmax + t];
+    }
+    // first row of this slot from the prefetched registers, any further rows directly
+    for (int pos = lo + slot; pos < hi; pos += NSLOT) {
+      StepRegs<TP, LPR, E> extra;
+      const bool first = (pos == lo + slot);
+      if (!first) prefetch_step(sd, pos, hi, lane, extra);
+      const StepRegs<TP, LPR, E> &r = first ? cur : extra;
+      cx<TP> acc = {TP(0), TP(0)};
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        const int cl = r.col[e];
+        if (cl >= 0) {
+          const cx<TP> zc = (cl >= rs && cl < re) ? zs[cl - rs] : zld(&p.Z[cl]);
+          acc = cfma(mval(r.val[e], w), zc, acc);
+        }
+      }
+      acc = group_sum<LPR>(acc);
+      if (lane == 0) {
+        const int row = r.row;
+        const cx<TP> zo = zs[row - rs];
+        const cx<TP> z = FWD ? cmul(zo - acc, r.dd) : zo - cmul(r.dd, acc);
+        zs[row - rs] = z;
+        if (sd.exp[row]) zst(&p.Z[row], z);
+      }
+    }
+    if (sd.pub[s]) {
+      wave_drain();
+      __syncthreads();
+      if (t == 0)
+        __hip_atomic_store(&p.progress[b], epoch + (unsigned)(s - s0 + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    cur = nxt;
+    rq_cur = rq_nxt;
+    rq_nxt = make_int2(-1, 0);
+  }
+  __syncthreads();
+}
+
 template <typename TP, int LPR, bool MULTI>
 __global__ void __launch_bounds__(256) k_cocg(CocgParams<TP> p) {
   __shared__ cx<double> lds_c[256];
@@ -280,7 +370,7 @@ __global__ void __launch_bounds__(256) k_cocg(CocgParams<TP> p) {
   const double w = cvalid ? p.wcol[c] : 0.0;
   const TP wT = (TP)w;
   const int f = cvalid ? p.col2f[c] : 0;
-  const bool sweep_pc = (p.pc == AFX_PC_IC || p.pc == AFX_PC_SSOR);
+  const bool sweep_pc = (p.pc == AFX_PC_IC || p.pc == AFX_PC_SSOR || p.pc == AFX_PC_SSOR_BJ);
   const bool dist = p.dist_reduce != 0;
   const int rs = p.rstart[blockIdx.x], re = p.rstart[blockIdx.x + 1];
   unsigned phase = 0;
@@ -304,11 +394,23 @@ __global__ void __launch_bounds__(256) k_cocg(CocgParams<TP> p) {
     cx<double> rho_loc = {0.0, 0.0};
     if (sweep_pc) {
       const unsigned ep = (unsigned)(2 * it - 1) * (unsigned)p.stride;
-      sweep<TP, LPR, MULTI, true>(p, p.fw, zs, known, ep, c, cvalid, wT, f, rho_loc);
-      grid_barrier(p.bar, phase);
-      sweep<TP, LPR, MULTI, false>(p, p.bw, zs, known, ep + (unsigned)p.stride, c, cvalid, wT, f, rho_loc);
-      if (!MULTI && p.lds_rows > 0)
-        for (int r = rs + t; r < re; r += 256) p.Z[r] = zs[r - rs];
+      if (!MULTI && p.lds_rows > 0 && p.fw.ecol != nullptr) {
+        for (int r = rs + t; r < re; r += 256) zs[r - rs] = down<TP>(p.R[r]);
+        sweep_lds<TP, LPR, 16 / LPR + (LPR == 16 ? 1 : 0), true>(p, p.fw, zs, known, ep, wT);
+        grid_barrier(p.bar, phase);
+        sweep_lds<TP, LPR, 16 / LPR + (LPR == 16 ? 1 : 0), false>(p, p.bw, zs, known, ep + (unsigned)p.stride, wT);
+        for (int r = rs + t; r < re; r += 256) {
+          const cx<TP> z = zs[r - rs];
+          p.Z[r] = z;
+          rho_loc = addv(rho_loc, cmul(p.R[r], up(z)));
+        }
+      } else {
+        sweep<TP, LPR, MULTI, true>(p, p.fw, zs, known, ep, c, cvalid, wT, f, rho_loc);
+        grid_barrier(p.bar, phase);
+        sweep<TP, LPR, MULTI, false>(p, p.bw, zs, known, ep + (unsigned)p.stride, c, cvalid, wT, f, rho_loc);
+        if (!MULTI && p.lds_rows > 0)
+          for (int r = rs + t; r < re; r += 256) p.Z[r] = zs[r - rs];
+      }
     } else {
       if (MULTI) {
         if (cvalid)
@@ -642,7 +744,7 @@ hipError_t launch_precond_setup(int n, int nf, int kind, double sd, double sig, 
                     (void *)&Lcol, (void *)&Lval, (void *)&INU, (void *)&AUkm, (void *)&wf, (void *)&DD};
     return hipLaunchCooperativeKernel((const void *)k_precond_ic, dim3(grid), dim3(256), args, 0, s);
   }
-  if (kind == AFX_PC_SSOR || kind == AFX_PC_DIAG) {
+  if (kind == AFX_PC_SSOR || kind == AFX_PC_DIAG || kind == AFX_PC_SSOR_BJ) {
     long long m = (long long)n * nf;
     hipLaunchKernelGGL(k_precond_diag, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, n, nf, sd, Dkm, wf, DD);
     return hipGetLastError();
@@ -711,7 +813,8 @@ hipError_t launch_precond_apply_levels(int n, int kind, int nlevL, const int *lv
 __global__ void k_gather(int m, const int *map, const cx<double> *canon, cx<double> *out, cx<float> *out32) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= m) return;
-  const cx<double> v = canon[map[i]];
+  const int src = map[i];
+  const cx<double> v = (src >= 0) ? canon[src] : cx<double>{0.0, 0.0};
   if (out) out[i] = v;
   if (out32) out32[i] = cx<float>{(float)v.r, (float)v.i};
 }
@@ -720,6 +823,20 @@ hipError_t launch_gather_vals(int m, const int *map, const cx<double> *canon, cx
                               hipStream_t s) {
   if (m <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_gather, dim3((m + 255) / 256), dim3(256), 0, s, m, map, canon, out, out32);
+  return hipGetLastError();
+}
+
+// position-ordered copy of the preconditioner diagonal (single frequency)
+__global__ void k_dpos(int n, const int *row, const cx<double> *DD, cx<double> *d64, cx<float> *d32) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const cx<double> v = DD[row[i]];
+  d64[i] = v;
+  d32[i] = cx<float>{(float)v.r, (float)v.i};
+}
+hipError_t launch_dpos(int n, const int *row, const cx<double> *DD, cx<double> *d64, cx<float> *d32, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dpos, dim3((n + 255) / 256), dim3(256), 0, s, n, row, DD, d64, d32);
   return hipGetLastError();
 }
 

@@ -35,6 +35,8 @@ extern "C" {
 #define AFX_PC_IC 1   /* 'IC' / 'ILU': diagonal-modified ILU (D-ILU) */
 #define AFX_PC_SSOR 2 /* 'SSOR' */
 #define AFX_PC_DIAG 3 /* 'DIAG' */
+#define AFX_PC_SSOR_BJ 4 /* SSOR on the diagonal blocks of the device row partition (no reference
+                            counterpart: changes the preconditioner, keeps the COCG iteration) */
 
 /* arithmetic of the Krylov loop */
 #define AFX_FP64 0  /* all-FP64 COCG */
@@ -137,6 +139,10 @@ int afx_solve_device(afx_system s, int nrhs, const void *dB, void *dX, const dou
 /* FP64 true residual ||b - A x|| / ||b|| per column of device, row-interleaved B and X. */
 int afx_true_residual_device(afx_system s, int nrhs, const void *dB, const void *dX,
                              const double *omega_scale, double *res);
+
+/* Launch timing (HIP events on the system's stream) of the persistent COCG kernel accumulated
+   since the last reset: launches, COCG iterations they ran, and total kernel milliseconds. */
+int afx_kernel_timing(afx_system s, int reset, long long *launches, long long *iterations, double *kernel_ms);
 
 /* Autotune rows-per-wavefront and persistent grid size for nrhs columns on this system. */
 int afx_autotune(afx_system s, int nrhs, int verbose);

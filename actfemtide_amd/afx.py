@@ -7,6 +7,7 @@ The reference entry points this module mirrors (same argument meaning and error 
   forward_bxyz         src/solver_mpi/forward_bxyz_mpi.f90:3-67  -> forward_bxyz()
   PREPOBSCOEFF         src/solver_mpi/n_ebfem_bxyz_mpi.f90:653-726 -> System.prepobscoeff()
   CALOBSEBCOMP/RESP    src/solver_mpi/n_ebfem_bxyz_mpi.f90:397-423,570-609 -> calobsebcomp()
+  solvePARDISOinv      src/src_inv/solvePARDISOinv.f90:5-244     -> solve_pardiso_inv()
   genjacobian1         src/src_inv_ap/m_jacobian_ap.f90:174-386  -> System.jacobian_device()
 There is no CPU fallback: every call goes through the HIP library, and loading fails loudly
 when libafx.so is missing.
@@ -74,6 +75,7 @@ SIGNATURES = {
     "afx_solve": ([_VP, _I, _VP, _I, _VP, _I, C.POINTER(afx_opts), _VP], _I),
     "afx_solve_device": ([_VP, _I, _VP, _VP, _VP, C.POINTER(afx_opts), _VP], _I),
     "afx_true_residual_device": ([_VP, _I, _VP, _VP, _VP, _VP], _I),
+    "afx_scale_initial_guess_device": ([_VP, _I, _VP, _VP, _VP, _VP, _VP, _VP], _I),
     "afx_autotune": ([_VP, _I, _I], _I),
     "afx_kernel_timing": ([_VP, _I, _VP, _VP, _VP], _I),
     "afx_spmv": ([_VP, _VP, _VP], _I),
@@ -84,6 +86,7 @@ SIGNATURES = {
     "afx_solve_cg": ([_I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP], _I),
     "afx_prepobscoeff": ([_VP, _I, _VP, _VP, _VP, _VP, _VP], _I),
     "afx_obs_apply_device": ([_VP, _I, _VP, _VP, _I, _VP, _VP], _I),
+    "afx_adjoint_rhs_device": ([_VP, _I, _VP, _VP, _VP, _VP], _I),
     "afx_jacobian_device": ([_VP, _I, _VP, _VP, _I, _VP, _VP, _VP, _D, _D, _VP], _I),
 }
 
@@ -300,6 +303,18 @@ class System:
                                             _p(sc), _p(res)))
         return res
 
+    def scale_initial_guess_device(self, dB, dX, omega_scale=None):
+        """Minimal-residual scaling of the initial guesses dX in place (zero where scaling does
+        not lower the residual below the zero guess's); returns (gamma, res_given, res_scaled)."""
+        k = 1 if dB.dim() == 1 else dB.shape[1]
+        g = np.zeros(2 * k)
+        r0 = np.zeros(k)
+        r1 = np.zeros(k)
+        sc = None if omega_scale is None else _f64(omega_scale)
+        _err(lib().afx_scale_initial_guess_device(self.h, k, C.c_void_p(dB.data_ptr()), C.c_void_p(dX.data_ptr()),
+                                                  _p(sc), _p(g), _p(r0), _p(r1)))
+        return g[0::2] + 1j * g[1::2], r0, r1
+
     def prepobscoeff(self, xyzobs):
         """PREPOBSCOEFF: (item (nobs,6) 1-based, val_e (3,nobs,6), val_b (3,nobs,6), iele (nobs,))."""
         xo = _f64(xyzobs).reshape(-1, 3)
@@ -319,6 +334,11 @@ class System:
         out = np.zeros((nobs, ncol), np.complex128)
         _err(lib().afx_obs_apply_device(self.h, nobs, _p(item), _p(val), ncol, C.c_void_p(dX.data_ptr()), _p(out)))
         return out
+
+    def adjoint_rhs_device(self, stack, item, val, dB):
+        """P^T columns of solvePARDISOinv into dB (n, nobs) complex128 device tensor."""
+        st, it, v = _i32(stack), _i32(item), _f64(val)
+        _err(lib().afx_adjoint_rhs_device(self.h, len(st) - 1, _p(st), _p(it), _p(v), C.c_void_p(dB.data_ptr())))
 
     def jacobian_device(self, dU, dx, par_ptr, par_tet, sigma_tet, omega, dlnsig_dm, dJ):
         """genjacobian1 accumulation: dJ (npar, nobs) complex128 device tensor is overwritten."""
@@ -364,6 +384,54 @@ def solve_cg(D, AL, INL, IAL, AU, INU, IAU, rhs):
     if rc not in (AFX_OK, AFX_ERR_MAXIT):
         _err(rc)
     return X
+
+
+def solve_pardiso_inv(s: System, rhs, PTR, iflag_comp=None, threshold=1e-10, max_cols=256, **opts):
+    """solvePARDISOinv (src/src_inv/solvePARDISOinv.f90:5-244) on the device system `s`.
+
+    rhs (n, nsr) complex; PTR: list of up to 5 observation matrices, each a tuple
+    (stack (nobs+1,) 0-based, item 1-based, val) = real CRS [nobs x n]; iflag_comp selects the
+    components (default: all given). Returns (xout (n, nsr), ut) with ut[c] the adjoint solutions
+    A^-1 P_o^T as a scipy CSR matrix [nobs x n] keeping |u| > threshold (convcomp_full2crs,
+    m_matrix.f90:405-451; 0 for unselected components), and the solver stats of every column.
+    The forward and adjoint solves use COCG to `tol` (default 1e-8) instead of a factorisation;
+    adjoint columns run in batches of at most `max_cols` per persistent launch."""
+    import scipy.sparse as sp
+    import torch
+    dev = torch.device("cuda")
+    rhs = np.asarray(rhs, np.complex128)
+    if rhs.ndim == 1:
+        rhs = rhs[:, None]
+    n, nsr = rhs.shape
+    if n != s.n:
+        raise AfxError(AFX_ERR_ARG, "rhs rows != system size")
+    opts = dict(dict(precond="SSOR", tol=1e-8, maxit=400000, chunk=2000), **opts)
+    dB = torch.from_numpy(np.ascontiguousarray(rhs)).to(dev)
+    dX = torch.zeros_like(dB)
+    st_f, rc = s.solve_device(dB, dX, **opts)
+    xout = dX.cpu().numpy()
+    flags = list(iflag_comp) if iflag_comp is not None else [1] * len(PTR)
+    ut, stats = [], [st_f]
+    for c, P in enumerate(PTR):
+        if not flags[c] or P is None:
+            ut.append(0)
+            continue
+        stack, item, val = (np.asarray(a) for a in P)
+        nobs = len(stack) - 1
+        U = np.zeros((n, nobs), np.complex128)
+        for o0 in range(0, nobs, max_cols):
+            o1 = min(nobs, o0 + max_cols)
+            sub = (stack[o0:o1 + 1] - stack[o0], item[stack[o0]:stack[o1]], val[stack[o0]:stack[o1]])
+            dRHS = torch.empty((n, o1 - o0), dtype=torch.complex128, device=dev)
+            s.adjoint_rhs_device(*sub, dRHS)
+            dU = torch.zeros_like(dRHS)
+            st_a, _ = s.solve_device(dRHS, dU, **opts)
+            stats.append(st_a)
+            U[:, o0:o1] = dU.cpu().numpy()
+        keep = np.abs(U) > threshold
+        rows, cols = np.nonzero(keep.T)
+        ut.append(sp.csr_matrix((U.T[rows, cols], (rows, cols)), shape=(nobs, n)))
+    return xout, ut, stats
 
 
 def forward_system(case, omega: float, sigma_tet=None) -> tuple:

@@ -67,16 +67,14 @@ int next_pow2(int k) {
 
 struct DevSide {
   int *bstep = nullptr, *spos = nullptr, *row = nullptr, *ptr = nullptr, *col = nullptr, *map = nullptr;
-  int *rptr = nullptr, *rblk = nullptr, *rneed = nullptr;
-  unsigned char *pub = nullptr, *exp = nullptr;
+  unsigned char *exp = nullptr, *expp = nullptr;
   cx<double> *val64 = nullptr;
   cx<float> *val32 = nullptr;
   long long nnz = 0;
-  int ew = 0, rmax = 0;
+  int ew = 0;
   int *ecol = nullptr, *emap = nullptr;
   cx<double> *eval64 = nullptr, *dpos64 = nullptr;
   cx<float> *eval32 = nullptr, *dpos32 = nullptr;
-  int2 *req = nullptr;
 };
 
 struct afx_system_s {
@@ -118,10 +116,10 @@ struct afx_system_s {
   bool have_slots = false;
   cx<double> *d_rhs_unit = nullptr;  // tidal RHS of the last assembly
   // pipelined sweeps
-  int Gs = 0, sweep_stride = 0, rows_per_block = 0;
+  int Gs = 0, rows_per_block = 0;
   int *d_rstart = nullptr;
   DevSide fw, bw, bjfw, bjbw;
-  unsigned *d_progress = nullptr, *d_bar = nullptr;
+  unsigned *d_bar = nullptr;
   int *d_err = nullptr;
   // launch configuration (autotuned)
   int lpr64 = 8, lpr32 = 8;
@@ -358,15 +356,14 @@ void column_freqs(afx_system s, int k, const double *scale, std::vector<double> 
 }
 
 // Pipelined-sweep layout of one triangle (see afx_solver.hip): rows of each block sorted by
-// dependency level into steps, the step requirements on other blocks, publish / export flags.
+// dependency level into steps, and the export flag of every row another block reads.
 struct SideHost {
-  std::vector<int> bstep, spos, row, ptr, col, map, rptr, rblk, rneed;
-  std::vector<unsigned char> pub, exp;
+  std::vector<int> bstep, spos, row, ptr, col, map;
+  std::vector<unsigned char> exp, expp;
   int maxsteps = 0;
-  // fixed-width (padded) entries and per-step requirement slots
-  int ew = 0, rmax = 0;
+  // fixed-width (padded) entries
+  int ew = 0;
   std::vector<int> ecol, emap;
-  std::vector<int2> req;
 };
 
 void build_side(int n, int G, const std::vector<int> &rstart, const std::vector<int> &blk,
@@ -443,40 +440,8 @@ void build_side(int n, int G, const std::vector<int> &rstart, const std::vector<
     }
   }
   h.ptr[n] = (int)h.col.size();
-  h.rptr.assign(nstep + 1, 0);
-  h.rblk.clear();
-  h.rneed.clear();
-  h.pub.assign(nstep, 0);
-  std::vector<int> runmax(G, 0), need(G, 0), touched;
-  for (int b = 0; b < G; b++) {
-    std::vector<int> seen;
-    for (int st = h.bstep[b]; st < h.bstep[b + 1]; st++) {
-      h.rptr[st] = (int)h.rblk.size();
-      touched.clear();
-      for (int pos = h.spos[st]; pos < h.spos[st + 1]; pos++) {
-        const int i = h.row[pos];
-        if (h.exp[i]) h.pub[st] = 1;
-        for (int q = P[i]; q < P[i + 1]; q++) {
-          const int j = C1[q] - 1;
-          const int ob = blk[j];
-          if (ob == b) continue;
-          if (need[ob] == 0) touched.push_back(ob);
-          need[ob] = std::max(need[ob], lstep[j] + 1);
-        }
-      }
-      for (int ob : touched) {
-        if (need[ob] > runmax[ob]) {
-          if (runmax[ob] == 0) seen.push_back(ob);
-          runmax[ob] = need[ob];
-          h.rblk.push_back(ob);
-          h.rneed.push_back(need[ob]);
-        }
-        need[ob] = 0;
-      }
-    }
-    for (int ob : seen) runmax[ob] = 0;
-  }
-  h.rptr[nstep] = (int)h.rblk.size();
+  h.expp.resize(n);
+  for (int pos = 0; pos < n; pos++) h.expp[pos] = h.exp[h.row[pos]];
   h.ew = 0;
   for (int pos = 0; pos < n; pos++) h.ew = std::max(h.ew, h.ptr[pos + 1] - h.ptr[pos]);
   h.ew = std::max(h.ew, 1);
@@ -487,12 +452,6 @@ void build_side(int n, int G, const std::vector<int> &rstart, const std::vector<
       h.ecol[(size_t)pos * h.ew + (q - h.ptr[pos])] = h.col[q];
       h.emap[(size_t)pos * h.ew + (q - h.ptr[pos])] = h.map[q];
     }
-  h.rmax = 1;
-  for (int st = 0; st < nstep; st++) h.rmax = std::max(h.rmax, h.rptr[st + 1] - h.rptr[st]);
-  h.req.assign((size_t)nstep * h.rmax, make_int2(-1, 0));
-  for (int st = 0; st < nstep; st++)
-    for (int q = h.rptr[st]; q < h.rptr[st + 1]; q++)
-      h.req[(size_t)st * h.rmax + (q - h.rptr[st])] = make_int2(h.rblk[q], h.rneed[q]);
 }
 
 int upload_side(const SideHost &h, DevSide &d) {
@@ -500,41 +459,37 @@ int upload_side(const SideHost &h, DevSide &d) {
   if ((rc = upload(&d.bstep, h.bstep.data(), h.bstep.size())) || (rc = upload(&d.spos, h.spos.data(), h.spos.size())) ||
       (rc = upload(&d.row, h.row.data(), h.row.size())) || (rc = upload(&d.ptr, h.ptr.data(), h.ptr.size())) ||
       (rc = upload(&d.col, h.col.data(), h.col.size())) || (rc = upload(&d.map, h.map.data(), h.map.size())) ||
-      (rc = upload(&d.rptr, h.rptr.data(), h.rptr.size())) || (rc = upload(&d.rblk, h.rblk.data(), h.rblk.size())) ||
-      (rc = upload(&d.rneed, h.rneed.data(), h.rneed.size())) || (rc = upload(&d.pub, h.pub.data(), h.pub.size())) ||
-      (rc = upload(&d.exp, h.exp.data(), h.exp.size())) || (rc = dalloc(&d.val64, h.col.size())) ||
+      (rc = upload(&d.exp, h.exp.data(), h.exp.size())) || (rc = upload(&d.expp, h.expp.data(), h.expp.size())) || (rc = dalloc(&d.val64, h.col.size())) ||
       (rc = dalloc(&d.val32, h.col.size())))
     return rc;
   d.nnz = (long long)h.col.size();
   d.ew = h.ew;
-  d.rmax = h.rmax;
   const size_t ne = h.ecol.size();
   if ((rc = upload(&d.ecol, h.ecol.data(), ne)) || (rc = upload(&d.emap, h.emap.data(), ne)) ||
       (rc = dalloc(&d.eval64, ne)) || (rc = dalloc(&d.eval32, ne)) || (rc = dalloc(&d.dpos64, h.row.size())) ||
-      (rc = dalloc(&d.dpos32, h.row.size())) || (rc = upload(&d.req, h.req.data(), h.req.size())))
+      (rc = dalloc(&d.dpos32, h.row.size())))
     return rc;
   return AFX_OK;
 }
 
 void free_side(DevSide &d) {
-  dfree(d.bstep); dfree(d.spos); dfree(d.row); dfree(d.ptr); dfree(d.col); dfree(d.map); dfree(d.rptr);
-  dfree(d.rblk); dfree(d.rneed); dfree(d.pub); dfree(d.exp); dfree(d.val64); dfree(d.val32);
-  dfree(d.ecol); dfree(d.emap); dfree(d.eval64); dfree(d.eval32); dfree(d.dpos64); dfree(d.dpos32); dfree(d.req);
+  dfree(d.bstep); dfree(d.spos); dfree(d.row); dfree(d.ptr); dfree(d.col); dfree(d.map);
+  dfree(d.exp); dfree(d.expp); dfree(d.val64); dfree(d.val32);
+  dfree(d.ecol); dfree(d.emap); dfree(d.eval64); dfree(d.eval32); dfree(d.dpos64); dfree(d.dpos32);
 }
 
 afx::SweepSide side_view(const DevSide &d, bool fp64) {
   afx::SweepSide v{};
   v.bstep = d.bstep; v.spos = d.spos; v.row = d.row; v.ptr = d.ptr; v.col = d.col;
   v.val = fp64 ? (const void *)d.val64 : (const void *)d.val32;
-  v.rptr = d.rptr; v.rblk = d.rblk; v.rneed = d.rneed; v.pub = d.pub; v.exp = d.exp;
+  v.exp = d.exp;
+  v.expp = d.expp;
   // the fixed-width path needs at most 16 entries per position (the E * lanes of the kernel)
   const bool ell = d.ew <= 16 && !getenv("AFX_NO_ELL");
   v.ew = d.ew;
   v.ecol = ell ? d.ecol : nullptr;
   v.eval = fp64 ? (const void *)d.eval64 : (const void *)d.eval32;
   v.dpos = fp64 ? (const void *)d.dpos64 : (const void *)d.dpos32;
-  v.rmax = d.rmax;
-  v.req = d.req;
   return v;
 }
 
@@ -553,7 +508,6 @@ int build_sweeps(afx_system s) {
   SideHost bjfw, bjbw;
   build_side(n, G, rstart, blk, s->INL, s->IAL, (long long)n + s->nU, true, bjfw, true);
   build_side(n, G, rstart, blk, s->INU, s->IAU, n, false, bjbw, true);
-  s->sweep_stride = std::max(std::max(fw.maxsteps, bw.maxsteps), std::max(bjfw.maxsteps, bjbw.maxsteps)) + 1;
   s->rows_per_block = 0;
   for (int b = 0; b < G; b++) s->rows_per_block = std::max(s->rows_per_block, rstart[b + 1] - rstart[b]);
   int rc;
@@ -561,7 +515,7 @@ int build_sweeps(afx_system s) {
   if ((rc = upload_side(fw, s->fw)) || (rc = upload_side(bw, s->bw)) || (rc = upload_side(bjfw, s->bjfw)) ||
       (rc = upload_side(bjbw, s->bjbw)))
     return rc;
-  if ((rc = dalloc(&s->d_progress, G)) || (rc = dalloc(&s->d_bar, 64)) || (rc = dalloc(&s->d_err, 1))) return rc;
+  if ((rc = dalloc(&s->d_bar, 64)) || (rc = dalloc(&s->d_err, 1))) return rc;
   HIPCHK(hipMemset(s->d_bar, 0, 64 * sizeof(unsigned)));
   HIPCHK(hipMemset(s->d_err, 0, sizeof(int)));
   return AFX_OK;
@@ -571,14 +525,14 @@ int build_sweeps(afx_system s) {
 template <typename TP>
 struct Work {
   cx<double> *R = nullptr, *P = nullptr, *Q = nullptr;
-  cx<TP> *Z = nullptr;
+  cx<TP> *Z = nullptr, *mbf = nullptr, *mbb = nullptr;
   cx<double> *part_rho = nullptr, *part_c1 = nullptr, *fin_rho = nullptr, *fin_c1 = nullptr, *rho1 = nullptr;
   double *part_rr = nullptr, *fin_rr = nullptr, *bnrm2 = nullptr, *resid = nullptr, *norms = nullptr;
   double *part_a = nullptr, *part_b = nullptr, *wcol = nullptr;
   int *conv = nullptr, *iters = nullptr, *status = nullptr, *col2f = nullptr;
   int *h_status = nullptr;
   ~Work() {
-    dfree(R); dfree(Z); dfree(P); dfree(Q);
+    dfree(R); dfree(Z); dfree(P); dfree(Q); dfree(mbf); dfree(mbb);
     dfree(part_rho); dfree(part_c1); dfree(fin_rho); dfree(fin_c1); dfree(rho1);
     dfree(part_rr); dfree(fin_rr); dfree(bnrm2); dfree(resid); dfree(norms); dfree(part_a); dfree(part_b);
     dfree(conv); dfree(iters); dfree(status); dfree(col2f); dfree(wcol);
@@ -629,7 +583,7 @@ int lds_rows_for(afx_system s, bool multi) {
   int dev = 0, maxlds = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
-  const size_t need = ((size_t)s->Gs * 4 + 15) / 16 * 16 + (size_t)s->rows_per_block * sizeof(cx<TP>) + 6144;
+  const size_t need = (size_t)s->rows_per_block * sizeof(cx<TP>) + 6144;
   if (getenv("AFX_NO_LDS")) return 0;
   return (need <= (size_t)maxlds) ? s->rows_per_block : 0;
 }
@@ -650,7 +604,6 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(w.col2f, col2f.data(), k * sizeof(int), hipMemcpyHostToDevice, s->stream));
   HIPCHK(hipMemcpyAsync(w.wcol, wcol.data(), k * sizeof(double), hipMemcpyHostToDevice, s->stream));
-  HIPCHK(hipMemsetAsync(s->d_progress, 0, (size_t)G * sizeof(unsigned), s->stream));
   HIPCHK(hipMemsetAsync(s->d_err, 0, sizeof(int), s->stream));
   // initial residual r0 = b - A x and ||b||^2 (COCG13_single.f90:139-169)
   afx::SpmvParams sp{};
@@ -685,12 +638,19 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   p.val = s->d_val64;
   p.Gs = G;
   p.lds_rows = lds_rows_for<TP>(s, multi);
-  p.stride = s->sweep_stride;
   p.rstart = s->d_rstart;
   const bool bj = (pc == AFX_PC_SSOR_BJ);
   p.fw = side_view(bj ? s->bjfw : s->fw, fp64);
   p.bw = side_view(bj ? s->bjbw : s->bw, fp64);
-  p.progress = s->d_progress;
+  // sweep mailboxes, every slot empty at the start of the solve (kept empty between iterations)
+  if (pc == AFX_PC_IC || pc == AFX_PC_SSOR || pc == AFX_PC_SSOR_BJ) {
+    if ((rc = dalloc(&w.mbf, (size_t)s->n * k)) || (rc = dalloc(&w.mbb, (size_t)s->n * k))) return rc;
+    const long long words = (long long)s->n * k * (long long)sizeof(cx<TP>) / 8;
+    HIPCHK(afx::launch_mb_fill(words, (unsigned long long *)w.mbf, s->stream));
+    HIPCHK(afx::launch_mb_fill(words, (unsigned long long *)w.mbb, s->stream));
+  }
+  p.mbf = w.mbf;
+  p.mbb = w.mbb;
   p.DD = fp64 ? (const cx<TP> *)s->d_DD : (const cx<TP> *)s->d_DD32;
   p.col2f = w.col2f;
   p.wcol = w.wcol;
@@ -817,7 +777,7 @@ void release(afx_system s) {
   free_side(s->bw);
   free_side(s->bjfw);
   free_side(s->bjbw);
-  dfree(s->d_rstart); dfree(s->d_progress); dfree(s->d_bar); dfree(s->d_err);
+  dfree(s->d_rstart); dfree(s->d_bar); dfree(s->d_err);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   s->stream = nullptr;
 }
@@ -1371,6 +1331,80 @@ int afx_true_residual_device(afx_system s, int nrhs, const void *dB, const void 
   return AFX_OK;
 }
 
+int afx_scale_initial_guess_device(afx_system s, int nrhs, const void *dB, void *dX, const double *omega_scale,
+                                   double *gamma, double *res_given, double *res_scaled) {
+  if (!s || !dB || !dX || nrhs <= 0) return fail(AFX_ERR_ARG, "bad arguments");
+  int rc = ensure_derived(s);
+  if (rc) return rc;
+  const int kp = nrhs > 1 ? next_pow2(nrhs) : 1;
+  if (kp > 256) return fail(AFX_ERR_ARG, "at most 256 right-hand sides");
+  std::vector<double> wcol, wf;
+  std::vector<int> col2f;
+  column_freqs(s, nrhs, omega_scale, wcol, wf, col2f);
+  const int G = 2 * s->ncu;
+  cx<double> *dY = nullptr, *dg = nullptr;
+  double *dw = nullptr, *part = nullptr;
+  if ((rc = dalloc(&dY, (size_t)s->n * nrhs)) || (rc = upload(&dw, wcol.data(), nrhs)) ||
+      (rc = dalloc(&part, (size_t)G * nrhs * 4)) || (rc = dalloc(&dg, (size_t)nrhs))) {
+    dfree(dY); dfree(dw); dfree(part); dfree(dg);
+    return rc;
+  }
+  afx::SpmvParams sp{};
+  sp.n = s->n;
+  sp.k = nrhs;
+  sp.kp = kp;
+  sp.rowptr = s->d_rowptr;
+  sp.col = s->d_col;
+  sp.val = s->d_val64;
+  sp.wcol = dw;
+  sp.X = (const cx<double> *)dX;
+  sp.Y = dY;
+  std::vector<double> h((size_t)G * nrhs * 4);
+  hipError_t e = afx::launch_spmv(sp, s->stream);
+  if (e == hipSuccess) e = afx::launch_warm_dots(s->n, nrhs, kp, G, dY, (const cx<double> *)dB, part, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, h.size() * 8, hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  std::vector<cx<double>> g(nrhs);
+  if (e == hipSuccess) {
+    for (int c = 0; c < nrhs; c++) {
+      double a = 0, bi = 0, yy = 0, bb = 0;  // fixed summation order over blocks
+      for (int q = 0; q < G; q++) {
+        const double *v = &h[((size_t)q * nrhs + c) * 4];
+        a += v[0];
+        bi += v[1];
+        yy += v[2];
+        bb += v[3];
+      }
+      // ||b - y||^2 = bb - 2 Re(y^H b) + yy; with x -> gamma x, gamma = y^H b / y^H y:
+      // ||b - gamma y||^2 = bb - |y^H b|^2 / yy
+      const double r0 = bb > 0 ? std::sqrt(std::max(0.0, bb - 2.0 * a + yy) / bb) : 0.0;
+      double r1 = 1.0;
+      cx<double> gc{0.0, 0.0};
+      if (bb > 0 && yy > 0) {
+        const double rs = std::sqrt(std::max(0.0, bb - (a * a + bi * bi) / yy) / bb);
+        if (rs < 1.0) {  // lowers the zero guess's relative residual: keep the scaled guess
+          gc = cx<double>{a / yy, bi / yy};
+          r1 = rs;
+        }
+      }
+      if (bb == 0) r1 = 0.0;
+      g[c] = gc;
+      if (gamma) {
+        gamma[2 * c] = gc.r;
+        gamma[2 * c + 1] = gc.i;
+      }
+      if (res_given) res_given[c] = r0;
+      if (res_scaled) res_scaled[c] = r1;
+    }
+    e = hipMemcpyAsync(dg, g.data(), nrhs * sizeof(cx<double>), hipMemcpyHostToDevice, s->stream);
+  }
+  if (e == hipSuccess) e = afx::launch_scale_cols_cx(s->n, nrhs, dg, (cx<double> *)dX, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  dfree(dY); dfree(dw); dfree(part); dfree(dg);
+  if (e != hipSuccess) return fail(AFX_ERR_HIP, hipGetErrorString(e));
+  return AFX_OK;
+}
+
 }  // extern "C"
 
 // -----------------------------------------------------------------------------------------
@@ -1657,6 +1691,29 @@ int afx_obs_apply_device(afx_system s, int nobs, const int *item, const double *
   if (e == hipSuccess) e = hipMemcpyAsync(out, dout, (size_t)nobs * ncol * 16, hipMemcpyDeviceToHost, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   dfree(di); dfree(dv); dfree(dout);
+  if (e != hipSuccess) return fail(AFX_ERR_HIP, hipGetErrorString(e));
+  return AFX_OK;
+}
+
+int afx_adjoint_rhs_device(afx_system s, int nobs, const int *stack, const int *item, const double *val, void *dB) {
+  if (!s || !stack || !item || !val || !dB || nobs <= 0) return fail(AFX_ERR_ARG, "bad arguments");
+  if (stack[0] != 0) return fail(AFX_ERR_ARG, "stack must start at 0");
+  for (int o = 0; o < nobs; o++)
+    if (stack[o + 1] < stack[o]) return fail(AFX_ERR_ARG, "stack must be non-decreasing");
+  const int nnz = stack[nobs];
+  for (int q = 0; q < nnz; q++)
+    if (item[q] < 1 || item[q] > s->n) return fail(AFX_ERR_ARG, "item out of range");
+  int *dst = nullptr, *dit = nullptr;
+  double *dv = nullptr;
+  int rc;
+  if ((rc = upload(&dst, stack, (size_t)nobs + 1)) || (rc = upload(&dit, item, (size_t)std::max(nnz, 1))) ||
+      (rc = upload(&dv, val, (size_t)std::max(nnz, 1)))) {
+    dfree(dst); dfree(dit); dfree(dv);
+    return rc;
+  }
+  hipError_t e = afx::launch_adjoint_rhs(s->n, nobs, dst, dit, dv, (cx<double> *)dB, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  dfree(dst); dfree(dit); dfree(dv);
   if (e != hipSuccess) return fail(AFX_ERR_HIP, hipGetErrorString(e));
   return AFX_OK;
 }

@@ -298,6 +298,29 @@ hipError_t launch_obs_apply(int nobs, const int *item, const double *val, int nc
   return hipGetLastError();
 }
 
+// Adjoint right-hand sides of solvePARDISOinv (src/src_inv/solvePARDISOinv.f90:176-187): column o
+// of B is row o of the real CRS observation matrix (stack 0-based offsets, item 1-based edges)
+// scattered into a dense n-vector (assignment, as b1(P%item(j), i) = P%val(j)); B row-interleaved.
+__global__ void k_adjoint_rhs(int nobs, const int *stack, const int *item, const double *val, cx<double> *B) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= nobs) return;
+  for (int q = stack[o]; q < stack[o + 1]; ++q) B[(size_t)(item[q] - 1) * nobs + o] = cx<double>{val[q], 0.0};
+}
+__global__ void k_zero_cx(long long m, cx<double> *B) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < m) B[i] = cx<double>{0.0, 0.0};
+}
+hipError_t launch_adjoint_rhs(int n, int nobs, const int *stack, const int *item, const double *val, cx<double> *B,
+                              hipStream_t s) {
+  const long long m = (long long)n * nobs;
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_cx, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m, B);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_adjoint_rhs, dim3((nobs + 255) / 256), dim3(256), 0, s, nobs, stack, item, val, B);
+  return hipGetLastError();
+}
+
 // Jacobian, step 1 (genjacobian1 :292-322): per selected tet q,
 //   w_q[i] = -(dBB/dm) * sum_j Y(i,j) x[e_j],  dBB/dm = i*omega*mu0*sigma*L0^2*dlnsig
 __global__ void k_jac_w(int nsel, const int *tets, const double *xyz, const int *n4, const int *n6line,

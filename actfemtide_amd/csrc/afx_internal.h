@@ -27,18 +27,13 @@ struct SweepSide {
   const int *ptr;    // [n+1] entry range at each position
   const int *col;    // [nnz_side] column (0-based) of each entry
   const void *val;   // [nnz_side] cx<TP> (K, M) values
-  const int *rptr;   // [nstep+1] requirement range of each step
-  const int *rblk;   // requirement: block to wait for
-  const int *rneed;  // requirement: steps that block must have published
-  const unsigned char *pub;  // [nstep] 1: another block waits on this step
   const unsigned char *exp;  // [n] by row id: 1: another block reads this row's z
+  const unsigned char *expp; // [n] the same flag by position
   // fixed-width copy for the single-column LDS path: entry e of position pos at pos*ew + e
   int ew;                    // entries per position (padding: column -1, value 0)
   const int *ecol;
   const void *eval;          // cx<TP>
   const void *dpos;          // [n] cx<TP>: DD in position order
-  int rmax;                  // requirement slots per step
-  const int2 *req;           // [nstep][rmax] (block, steps); block -1 = unused
 };
 
 template <typename TP>
@@ -52,10 +47,9 @@ struct CocgParams {
   // pipelined sweeps
   int Gs;            // blocks of the launch == sweep blocks
   int lds_rows;      // > 0: own-block z staged in LDS (single column)
-  int stride;        // epoch stride of the progress counters (> steps of any block)
   const int *rstart; // [G+1] row range of each block
   SweepSide fw, bw;
-  unsigned *progress;  // [G]
+  cx<TP> *mbf, *mbb;   // [n][k] mailboxes of the forward / backward sweep (MB_EMPTY when idle)
   const cx<TP> *DD;    // [n][nf]
   const int *col2f;    // [k]
   const double *wcol;  // [k] omega of each column
@@ -114,6 +108,10 @@ hipError_t launch_gather_vals(int m, const int *map, const cx<double> *canon, cx
                               cx<float> *out32, hipStream_t s);
 hipError_t launch_dpos(int n, const int *row, const cx<double> *DD, cx<double> *d64, cx<float> *d32, hipStream_t s);
 hipError_t launch_cast_vec(long long m, const cx<double> *in, cx<float> *out, hipStream_t s);
+hipError_t launch_mb_fill(long long words, unsigned long long *mb, hipStream_t s);
+hipError_t launch_warm_dots(long long n, int k, int kp, int grid, const cx<double> *Y, const cx<double> *B,
+                            double *part, hipStream_t s);
+hipError_t launch_scale_cols_cx(long long n, int k, const cx<double> *g, cx<double> *X, hipStream_t s);
 hipError_t launch_axpy_f32_to_f64(long long m, const cx<float> *d, cx<double> *x, hipStream_t s);
 hipError_t launch_scale_cols(long long n, int k, const cx<double> *in, const double *scale,
                              cx<float> *out, hipStream_t s);
@@ -147,6 +145,8 @@ hipError_t launch_findelement(int nobs, const double *pts, int ntet, const doubl
                               int *first, hipStream_t s);
 hipError_t launch_obs_apply(int nobs, const int *item, const double *val, int ncol, const cx<double> *X,
                             cx<double> *out, hipStream_t s);
+hipError_t launch_adjoint_rhs(int n, int nobs, const int *stack, const int *item, const double *val, cx<double> *B,
+                              hipStream_t s);
 hipError_t launch_jac_w(int ntet_sel, const int *tets, const double *xyz, const int *n4, const int *n6line,
                         const double *sigma, double omega, double dlnsig, const cx<double> *x, cx<double> *w,
                         int *edges, hipStream_t s);

@@ -4,9 +4,9 @@
 // runs inside ONE persistent cooperative launch per chunk of iterations:
 //   z = M^-1 r  -- pipelined forward (strict lower) and backward (strict upper) sweeps: block b
 //                  owns the contiguous rows [rstart[b], rstart[b+1]) and walks them in steps of
-//                  increasing dependency level, keeping its own z in LDS; a step waits only on
-//                  the progress counters of the blocks it reads from (point-to-point, no grid
-//                  barrier per level) and publishes its own counter when another block needs it;
+//                  increasing dependency level, keeping its own z in LDS; z values another block
+//                  needs are posted to mailbox slots that the reader polls (point-to-point, no
+//                  grid barrier per level, no progress counters);
 //   rho = r^T z -- fused into the backward sweep, reduced deterministically across blocks;
 //   p = z + beta p;  q = A p (CSR, `lpr` lanes per row) fused with p^T q;
 //   x += alpha p, r -= alpha q fused with ||r||^2 and the per-column convergence test.
@@ -93,31 +93,75 @@ __device__ __forceinline__ cx<T> group_sum(cx<T> v) {
   return v;
 }
 
-// z loads / stores that bypass the non-coherent caches (agent-scope relaxed atomics):
-// the sweeps exchange z between blocks through them.
-__device__ __forceinline__ cx<double> zld(const cx<double> *p) {
-  unsigned long long *q = (unsigned long long *)p;
-  const unsigned long long a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const unsigned long long b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Mailbox exchange of z between blocks. An exported z value travels through a mailbox slot that
+// holds MB_EMPTY (the all-ones bit pattern: a NaN that no arithmetic produces) until the owner
+// writes the value; a reader polls the slot itself, so the value is its own ready flag and one
+// hand-off costs a write-through store plus the reader's load -- no progress counter and no wait
+// for store completion before raising one. Slots are accessed with agent-scope relaxed 64-bit
+// atomics (they bypass the per-XCD non-coherent caches). The owner writes MB_EMPTY back once every
+// reader is past the sweep (after the next grid barrier). A poll gives up after AFX_SPIN_LIMIT
+// tries, or soon after another wait gave up, and flags err: results are then not trusted.
+constexpr unsigned long long MB_EMPTY = ~0ull;
+
+__device__ __forceinline__ unsigned long long mb_ld(const unsigned long long *q) {
+  return __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void mb_st(unsigned long long *q, unsigned long long v) {
+  __hip_atomic_store(q, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool mb_give_up(unsigned spins, int *err) {
+  if (spins > AFX_SPIN_LIMIT) {
+    atomicOr(err, 2);
+    return true;
+  }
+  return (spins & 1023u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+__device__ __forceinline__ cx<double> mb_get(const cx<double> *p, int *err) {
+  const unsigned long long *q = (const unsigned long long *)p;
+  unsigned long long a = mb_ld(q), b = mb_ld(q + 1);
+  unsigned spins = 0;
+  while (a == MB_EMPTY || b == MB_EMPTY) {
+    if (mb_give_up(++spins, err)) break;
+    __builtin_amdgcn_s_sleep(1);
+    if (a == MB_EMPTY) a = mb_ld(q);
+    if (b == MB_EMPTY) b = mb_ld(q + 1);
+  }
   return {__longlong_as_double((long long)a), __longlong_as_double((long long)b)};
 }
-__device__ __forceinline__ cx<float> zld(const cx<float> *p) {
-  const unsigned long long a =
-      __hip_atomic_load((unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ cx<float> mb_get(const cx<float> *p, int *err) {
+  const unsigned long long *q = (const unsigned long long *)p;
+  unsigned long long a = mb_ld(q);
+  unsigned spins = 0;
+  while (a == MB_EMPTY) {
+    if (mb_give_up(++spins, err)) break;
+    __builtin_amdgcn_s_sleep(1);
+    a = mb_ld(q);
+  }
   cx<float> r;
   __builtin_memcpy(&r, &a, 8);
   return r;
 }
-__device__ __forceinline__ void zst(cx<double> *p, cx<double> v) {
+__device__ __forceinline__ void mb_put(cx<double> *p, cx<double> v) {
   unsigned long long *q = (unsigned long long *)p;
-  __hip_atomic_store(q, (unsigned long long)__double_as_longlong(v.r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(q + 1, (unsigned long long)__double_as_longlong(v.i), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
+  mb_st(q, (unsigned long long)__double_as_longlong(v.r));
+  mb_st(q + 1, (unsigned long long)__double_as_longlong(v.i));
 }
-__device__ __forceinline__ void zst(cx<float> *p, cx<float> v) {
+__device__ __forceinline__ void mb_put(cx<float> *p, cx<float> v) {
   unsigned long long a;
   __builtin_memcpy(&a, &v, 8);
-  __hip_atomic_store((unsigned long long *)p, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  mb_st((unsigned long long *)p, a);
+}
+__device__ __forceinline__ void mb_clear(cx<double> *p) {
+  mb_st((unsigned long long *)p, MB_EMPTY);
+  mb_st((unsigned long long *)p + 1, MB_EMPTY);
+}
+__device__ __forceinline__ void mb_clear(cx<float> *p) { mb_st((unsigned long long *)p, MB_EMPTY); }
+
+// empty the block's own exported slots of one mailbox (rows [rs, re), k columns)
+template <typename TP>
+__device__ void mb_clear_own(cx<TP> *mb, const unsigned char *exp, int rs, int re, int k) {
+  for (long long i = (long long)rs * k + threadIdx.x; i < (long long)re * k; i += 256)
+    if (exp[i / k]) mb_clear(&mb[i]);
 }
 
 __device__ __forceinline__ double zero_of(double) { return 0.0; }
@@ -169,28 +213,12 @@ __device__ V grid_cols(V v, V *part, V *fin, V *lds, int k, int kp, int c, bool 
 // all columns finished? identical in every block because the states are
 __device__ bool block_all(bool mine) { return __syncthreads_and(mine ? 1 : 0) != 0; }
 
-// Wait until block `ob` has published at least `need` (progress counters are monotonic across
-// sweeps; comparisons are wrap-safe). `known` caches the last value seen per block.
-__device__ __forceinline__ void wait_progress(const unsigned *progress, unsigned *known, int ob, unsigned need,
-                                              int *err) {
-  if ((int)(known[ob] - need) >= 0) return;
-  unsigned spins = 0;
-  unsigned v = __hip_atomic_load(&progress[ob], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  while ((int)(v - need) < 0) {
-    __builtin_amdgcn_s_sleep(1);
-    if (++spins > AFX_SPIN_LIMIT) {
-      atomicOr(err, 2);
-      break;
-    }
-    v = __hip_atomic_load(&progress[ob], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  known[ob] = v;
-}
-
-// One pipelined triangular sweep (FWD: z = D~^-1 (r - L z); else z -= D~^-1 U z), see header.
+// One pipelined triangular sweep (FWD: z = D~^-1 (r - L z); else z -= D~^-1 U z) over the CSR
+// side layout, any column count. Own rows are read from LDS (zs, single column) or Z; rows of
+// other blocks are polled from the mailbox mb; exported rows are posted to mb.
 template <typename TP, int LPR, bool MULTI, bool FWD>
-__device__ void sweep(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> *zs, unsigned *known, unsigned epoch,
-                      int c, bool cvalid, TP w, int f, cx<double> &rho_loc) {
+__device__ void sweep(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> *zs, cx<TP> *mb, int c, bool cvalid,
+                      TP w, int f, cx<double> &rho_loc) {
   const int b = blockIdx.x;
   const int t = threadIdx.x;
   const int k = p.k;
@@ -203,10 +231,8 @@ __device__ void sweep(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> *zs, 
   const int nslot = MULTI ? 256 / kp : 256 / LPR;
   const int lane = MULTI ? 0 : t % LPR;
   const int s0 = sd.bstep[b], s1 = sd.bstep[b + 1];
+  __syncthreads();
   for (int s = s0; s < s1; ++s) {
-    const int q0 = sd.rptr[s], q1 = sd.rptr[s + 1];
-    for (int q = q0 + t; q < q1; q += 256) wait_progress(p.progress, known, sd.rblk[q], epoch + sd.rneed[q], p.err);
-    __syncthreads();
     const int lo = sd.spos[s], hi = sd.spos[s + 1];
     for (int pos = lo + slot; pos < hi; pos += nslot) {
       const int row = sd.row[pos];
@@ -214,22 +240,29 @@ __device__ void sweep(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> *zs, 
       if (MULTI) {
         if (cvalid) {
           cx<TP> acc = {TP(0), TP(0)};
-          for (int q = eb; q < ee; ++q) acc = cfma(mval(val[q], w), zld(&p.Z[(size_t)sd.col[q] * k + c]), acc);
+          for (int q = eb; q < ee; ++q) {
+            const int cl = sd.col[q];
+            const size_t cix = (size_t)cl * k + c;
+            const cx<TP> zc = (cl >= rs && cl < re) ? p.Z[cix] : mb_get(&mb[cix], p.err);
+            acc = cfma(mval(val[q], w), zc, acc);
+          }
           const size_t ix = (size_t)row * k + c;
           cx<TP> z;
           if (FWD) {
             z = cmul(down<TP>(p.R[ix]) - acc, p.DD[(size_t)row * nf + f]);
           } else {
-            z = zld(&p.Z[ix]) - cmul(p.DD[(size_t)row * nf + f], acc);
+            z = p.Z[ix] - cmul(p.DD[(size_t)row * nf + f], acc);
             rho_loc = addv(rho_loc, cmul(p.R[ix], up(z)));
           }
-          zst(&p.Z[ix], z);
+          p.Z[ix] = z;
+          if (sd.exp[row]) mb_put(&mb[ix], z);
         }
       } else {
         cx<TP> acc = {TP(0), TP(0)};
         for (int q = eb + lane; q < ee; q += LPR) {
           const int cl = sd.col[q];
-          const cx<TP> zc = (lds && cl >= rs && cl < re) ? zs[cl - rs] : zld(&p.Z[cl]);
+          const bool own = cl >= rs && cl < re;
+          const cx<TP> zc = own ? (lds ? zs[cl - rs] : p.Z[cl]) : mb_get(&mb[cl], p.err);
           acc = cfma(mval(val[q], w), zc, acc);
         }
         acc = group_sum<LPR>(acc);
@@ -238,35 +271,36 @@ __device__ void sweep(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> *zs, 
           if (FWD) {
             z = cmul(down<TP>(p.R[row]) - acc, p.DD[row]);
           } else {
-            const cx<TP> zo = lds ? zs[row - rs] : zld(&p.Z[row]);
+            const cx<TP> zo = lds ? zs[row - rs] : p.Z[row];
             z = zo - cmul(p.DD[row], acc);
             rho_loc = addv(rho_loc, cmul(p.R[row], up(z)));
           }
-          if (lds) zs[row - rs] = z;
-          if (!lds || sd.exp[row]) zst(&p.Z[row], z);
+          if (lds)
+            zs[row - rs] = z;
+          else
+            p.Z[row] = z;
+          if (sd.exp[row]) mb_put(&mb[row], z);
         }
       }
     }
-    if (sd.pub[s]) {
-      // the exported z went out through write-through atomic stores: once every wave has
-      // drained them, the counter may be raised
-      wave_drain();
-      __syncthreads();
-      if (t == 0)
-        __hip_atomic_store(&p.progress[b], epoch + (unsigned)(s - s0 + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      __syncthreads();
-    }
+    __syncthreads();
   }
 }
 
-// Single-column sweep with the block's z (and, on the forward pass, r) staged in LDS and the
-// next step's metadata prefetched into registers: a step then costs one LDS round trip plus the
-// exchange with other blocks, instead of a chain of dependent global loads.
+// Single-column sweep with the block's z (and, on the forward pass, r) staged in LDS.
 // zs[row - rs] holds r_i until row i is processed (forward) and z_i afterwards.
+// Software pipeline, per step s of the block:
+//   * the step bounds of s+4 are loaded (scalar loads; the step barrier waits for them),
+//   * the fixed-width entries of step s+2 are loaded into registers,
+//   * the mailbox slots that step s+1 reads from other blocks are loaded raw (no wait): these
+//     loads are issued before step s posts its own z, so waiting for them never waits for a
+//     write-through store to complete; a slot still empty when step s+1 uses it is polled.
+// In steady state a step then costs one LDS round trip, the row arithmetic and the barrier:
+// the mailbox latency is paid once per chain of blocks, not once per dependency level.
 template <typename TP, int LPR, int E>
 struct StepRegs {
-  int row;
+  int row;  // -1: this slot has no row in the step
+  int exp;  // another block reads this row's z
   int col[E];
   cx<TP> val[E];
   cx<TP> dd;
@@ -276,6 +310,7 @@ template <typename TP, int LPR, int E>
 __device__ __forceinline__ void prefetch_step(const SweepSide &sd, int pos, int hi, int lane, StepRegs<TP, LPR, E> &r) {
   if (pos < hi) {
     r.row = sd.row[pos];
+    r.exp = sd.expp[pos];
     r.dd = ((const cx<TP> *)sd.dpos)[pos];
     const int *ec = sd.ecol + (size_t)pos * sd.ew;
     const cx<TP> *ev = (const cx<TP> *)sd.eval + (size_t)pos * sd.ew;
@@ -290,41 +325,75 @@ __device__ __forceinline__ void prefetch_step(const SweepSide &sd, int pos, int 
   }
 }
 
+// raw (non-polling) mailbox loads of the remote entries of a prefetched row
+__device__ __forceinline__ void mb_raw(const cx<double> *p, cx<double> &v) {
+  const unsigned long long *q = (const unsigned long long *)p;
+  v.r = __longlong_as_double((long long)mb_ld(q));
+  v.i = __longlong_as_double((long long)mb_ld(q + 1));
+}
+__device__ __forceinline__ void mb_raw(const cx<float> *p, cx<float> &v) {
+  const unsigned long long a = mb_ld((const unsigned long long *)p);
+  __builtin_memcpy(&v, &a, 8);
+}
+__device__ __forceinline__ bool mb_empty(cx<double> v) {
+  return (unsigned long long)__double_as_longlong(v.r) == MB_EMPTY ||
+         (unsigned long long)__double_as_longlong(v.i) == MB_EMPTY;
+}
+__device__ __forceinline__ bool mb_empty(cx<float> v) {
+  unsigned long long a;
+  __builtin_memcpy(&a, &v, 8);
+  return a == MB_EMPTY;
+}
+
+template <typename TP, int LPR, int E>
+__device__ __forceinline__ void mb_fetch(const StepRegs<TP, LPR, E> &r, int rs, int re, const cx<TP> *mb,
+                                         cx<TP> (&z)[E]) {
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int cl = r.col[e];
+    if (r.row >= 0 && cl >= 0 && (cl < rs || cl >= re)) mb_raw(&mb[cl], z[e]);
+  }
+}
+
 template <typename TP, int LPR, int E, bool FWD>
-__device__ void sweep_lds(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> *zs, unsigned *known, unsigned epoch,
-                          TP w) {
+__device__ void sweep_lds(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> *zs, cx<TP> *mb, TP w) {
   const int b = blockIdx.x;
   const int t = threadIdx.x;
   const int rs = p.rstart[b], re = p.rstart[b + 1];
   const int slot = t / LPR, lane = t % LPR;
   constexpr int NSLOT = 256 / LPR;
   const int s0 = sd.bstep[b], s1 = sd.bstep[b + 1];
-  StepRegs<TP, LPR, E> cur, nxt;
-  int2 rq_cur = make_int2(-1, 0), rq_nxt = make_int2(-1, 0);
-  if (s0 < s1) {
-    prefetch_step(sd, sd.spos[s0] + slot, sd.spos[s0 + 1], lane, cur);
-    if (t < sd.rmax) rq_cur = sd.req[(size_t)s0 * sd.rmax + t];
-  }
+  auto sp = [&](int s) { return sd.spos[s < s1 ? s : s1]; };
+  int l0 = sp(s0), l1 = sp(s0 + 1), l2 = sp(s0 + 2), l3 = sp(s0 + 3);
+  StepRegs<TP, LPR, E> cur, nx1;
+  cx<TP> zr[E], zr1[E];
+  prefetch_step(sd, l0 + slot, l1, lane, cur);
+  prefetch_step(sd, l1 + slot, l2, lane, nx1);
+  mb_fetch(cur, rs, re, mb, zr);
+  __syncthreads();
   for (int s = s0; s < s1; ++s) {
-    if (rq_cur.x >= 0) wait_progress(p.progress, known, rq_cur.x, epoch + (unsigned)rq_cur.y, p.err);
-    const int lo = sd.spos[s], hi = sd.spos[s + 1];
-    __syncthreads();
-    if (s + 1 < s1) {
-      prefetch_step(sd, sd.spos[s + 1] + slot, sd.spos[s + 2], lane, nxt);
-      if (t < sd.rmax) rq_nxt = sd.req[(size_t)(s + 1) * sd.rmax + t];
-    }
-    // first row of this slot from the prefetched registers, any further rows directly
-    for (int pos = lo + slot; pos < hi; pos += NSLOT) {
+    const int l4 = sp(s + 4);
+    mb_fetch(nx1, rs, re, mb, zr1);
+    StepRegs<TP, LPR, E> nx2;
+    prefetch_step(sd, l2 + slot, l3, lane, nx2);
+    // first row of this slot from the pipelined registers, any further rows directly
+    for (int pos = l0 + slot; pos < l1; pos += NSLOT) {
+      const bool first = (pos == l0 + slot);
       StepRegs<TP, LPR, E> extra;
-      const bool first = (pos == lo + slot);
-      if (!first) prefetch_step(sd, pos, hi, lane, extra);
+      if (!first) prefetch_step(sd, pos, l1, lane, extra);
       const StepRegs<TP, LPR, E> &r = first ? cur : extra;
       cx<TP> acc = {TP(0), TP(0)};
 #pragma unroll
       for (int e = 0; e < E; e++) {
         const int cl = r.col[e];
         if (cl >= 0) {
-          const cx<TP> zc = (cl >= rs && cl < re) ? zs[cl - rs] : zld(&p.Z[cl]);
+          cx<TP> zc;
+          if (cl >= rs && cl < re) {
+            zc = zs[cl - rs];
+          } else {
+            zc = first ? zr[e] : cx<TP>{TP(0), TP(0)};
+            if (!first || mb_empty(zc)) zc = mb_get(&mb[cl], p.err);
+          }
           acc = cfma(mval(r.val[e], w), zc, acc);
         }
       }
@@ -334,20 +403,19 @@ __device__ void sweep_lds(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> *
         const cx<TP> zo = zs[row - rs];
         const cx<TP> z = FWD ? cmul(zo - acc, r.dd) : zo - cmul(r.dd, acc);
         zs[row - rs] = z;
-        if (sd.exp[row]) zst(&p.Z[row], z);
+        if (r.exp) mb_put(&mb[row], z);
       }
     }
-    if (sd.pub[s]) {
-      wave_drain();
-      __syncthreads();
-      if (t == 0)
-        __hip_atomic_store(&p.progress[b], epoch + (unsigned)(s - s0 + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    cur = nxt;
-    rq_cur = rq_nxt;
-    rq_nxt = make_int2(-1, 0);
+    __syncthreads();
+    cur = nx1;
+    nx1 = nx2;
+#pragma unroll
+    for (int e = 0; e < E; e++) zr[e] = zr1[e];
+    l0 = l1;
+    l1 = l2;
+    l2 = l3;
+    l3 = l4;
   }
-  __syncthreads();
 }
 
 template <typename TP, int LPR, bool MULTI>
@@ -355,8 +423,7 @@ __global__ void __launch_bounds__(256) k_cocg(CocgParams<TP> p) {
   __shared__ cx<double> lds_c[256];
   __shared__ double lds_d[256];
   extern __shared__ unsigned char dyn[];
-  unsigned *known = (unsigned *)dyn;                           // [G]
-  cx<TP> *zs = (cx<TP> *)(dyn + ((gridDim.x * 4 + 15) / 16) * 16);  // [lds_rows]
+  cx<TP> *zs = (cx<TP> *)dyn;  // [lds_rows]
   const int t = threadIdx.x;
   const long long gtid = (long long)blockIdx.x * 256 + t;
   const long long nthr = (long long)gridDim.x * 256;
@@ -375,8 +442,6 @@ __global__ void __launch_bounds__(256) k_cocg(CocgParams<TP> p) {
   const int rs = p.rstart[blockIdx.x], re = p.rstart[blockIdx.x + 1];
   unsigned phase = 0;
   if (t == 0) phase = __hip_atomic_load(p.bar.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (int ob = t; ob < (int)gridDim.x; ob += 256)
-    known[ob] = __hip_atomic_load(&p.progress[ob], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
 
   int st = cvalid ? p.conv[c] : 1;
@@ -393,21 +458,24 @@ __global__ void __launch_bounds__(256) k_cocg(CocgParams<TP> p) {
     // ---- z = M^-1 r (COCG13_single.f90:186-241) and rho = r^T z (:246-250) ----
     cx<double> rho_loc = {0.0, 0.0};
     if (sweep_pc) {
-      const unsigned ep = (unsigned)(2 * it - 1) * (unsigned)p.stride;
+      // forward sweep, grid barrier, then the forward mailbox is emptied (every reader is past
+      // it) while the backward sweep runs on its own mailbox
       if (!MULTI && p.lds_rows > 0 && p.fw.ecol != nullptr) {
         for (int r = rs + t; r < re; r += 256) zs[r - rs] = down<TP>(p.R[r]);
-        sweep_lds<TP, LPR, 16 / LPR + (LPR == 16 ? 1 : 0), true>(p, p.fw, zs, known, ep, wT);
+        sweep_lds<TP, LPR, 16 / LPR + (LPR == 16 ? 1 : 0), true>(p, p.fw, zs, p.mbf, wT);
         grid_barrier(p.bar, phase);
-        sweep_lds<TP, LPR, 16 / LPR + (LPR == 16 ? 1 : 0), false>(p, p.bw, zs, known, ep + (unsigned)p.stride, wT);
+        mb_clear_own(p.mbf, p.fw.exp, rs, re, 1);
+        sweep_lds<TP, LPR, 16 / LPR + (LPR == 16 ? 1 : 0), false>(p, p.bw, zs, p.mbb, wT);
         for (int r = rs + t; r < re; r += 256) {
           const cx<TP> z = zs[r - rs];
           p.Z[r] = z;
           rho_loc = addv(rho_loc, cmul(p.R[r], up(z)));
         }
       } else {
-        sweep<TP, LPR, MULTI, true>(p, p.fw, zs, known, ep, c, cvalid, wT, f, rho_loc);
+        sweep<TP, LPR, MULTI, true>(p, p.fw, zs, p.mbf, c, cvalid, wT, f, rho_loc);
         grid_barrier(p.bar, phase);
-        sweep<TP, LPR, MULTI, false>(p, p.bw, zs, known, ep + (unsigned)p.stride, c, cvalid, wT, f, rho_loc);
+        mb_clear_own(p.mbf, p.fw.exp, rs, re, MULTI ? k : 1);
+        sweep<TP, LPR, MULTI, false>(p, p.bw, zs, p.mbb, c, cvalid, wT, f, rho_loc);
         if (!MULTI && p.lds_rows > 0)
           for (int r = rs + t; r < re; r += 256) p.Z[r] = zs[r - rs];
       }
@@ -431,6 +499,8 @@ __global__ void __launch_bounds__(256) k_cocg(CocgParams<TP> p) {
       }
     }
     const cx<double> rho = grid_cols(rho_loc, p.part_rho, p.fin_rho, lds_c, k, kp, c, dist, p.bar, phase);
+    // every block is past the backward sweep: empty its mailbox for the next iteration
+    if (sweep_pc) mb_clear_own(p.mbb, p.bw.exp, rs, re, MULTI ? k : 1);
 
     // ---- p = z (first iteration) or z + beta p (:258-268) ----
     const bool first = (it == 1);
@@ -532,7 +602,8 @@ __global__ void __launch_bounds__(256) k_cocg(CocgParams<TP> p) {
 
 template <typename TP>
 size_t cocg_dyn_lds(const CocgParams<TP> &p, int grid) {
-  return ((size_t)grid * 4 + 15) / 16 * 16 + (size_t)p.lds_rows * sizeof(cx<TP>);
+  (void)grid;
+  return (size_t)p.lds_rows * sizeof(cx<TP>);
 }
 
 template <typename TP, int LPR, bool MULTI>
@@ -671,6 +742,46 @@ hipError_t launch_spmv(const SpmvParams &p, hipStream_t s) {
   q.part_rr = nullptr;
   q.part_bb = nullptr;
   hipLaunchKernelGGL(k_spmv, dim3((unsigned)g), dim3(256), 0, s, q);
+  return hipGetLastError();
+}
+
+// Per-block partial sums for the minimal-residual scaling of an initial guess, per column c:
+// part[(g*k + c)*4 + {0,1,2,3}] = sum over rows of Re conj(y) b, Im conj(y) b, |y|^2, |b|^2.
+__global__ void __launch_bounds__(256) k_warm_dots(long long n, int k, int kp, const cx<double> *Y,
+                                                   const cx<double> *B, double *part) {
+  __shared__ double lds[256];
+  const int t = threadIdx.x;
+  const int c = t % kp;
+  const bool cvalid = c < k;
+  const long long slot = ((long long)blockIdx.x * 256 + t) / kp;
+  const long long nslot = (long long)gridDim.x * 256 / kp;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (cvalid)
+    for (long long row = slot; row < n; row += nslot) {
+      const cx<double> y = Y[row * k + c], b = B[row * k + c];
+      v[0] += y.r * b.r + y.i * b.i;
+      v[1] += y.r * b.i - y.i * b.r;
+      v[2] += y.r * y.r + y.i * y.i;
+      v[3] += b.r * b.r + b.i * b.i;
+    }
+  for (int q = 0; q < 4; q++) {
+    const double sum = block_cols(v[q], lds, kp);
+    if (t < kp && cvalid) part[((size_t)blockIdx.x * k + t) * 4 + q] = sum;
+  }
+}
+hipError_t launch_warm_dots(long long n, int k, int kp, int grid, const cx<double> *Y, const cx<double> *B,
+                            double *part, hipStream_t s) {
+  hipLaunchKernelGGL(k_warm_dots, dim3(grid), dim3(256), 0, s, n, k, kp, Y, B, part);
+  return hipGetLastError();
+}
+
+// X[row][c] *= g[c]
+__global__ void k_scale_cols_cx(long long n, int k, const cx<double> *g, cx<double> *X) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n * k) X[i] = cmul(g[i % k], X[i]);
+}
+hipError_t launch_scale_cols_cx(long long n, int k, const cx<double> *g, cx<double> *X, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_cols_cx, dim3((unsigned)((n * k + 255) / 256)), dim3(256), 0, s, n, k, g, X);
   return hipGetLastError();
 }
 
@@ -847,6 +958,17 @@ __global__ void k_cast(long long m, const cx<double> *in, cx<float> *out) {
 hipError_t launch_cast_vec(long long m, const cx<double> *in, cx<float> *out, hipStream_t s) {
   if (m <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_cast, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m, in, out);
+  return hipGetLastError();
+}
+
+// every mailbox slot to MB_EMPTY (a fill kernel, so the solve can sit in a captured graph)
+__global__ void k_mb_fill(long long words, unsigned long long *mb) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < words) mb[i] = MB_EMPTY;
+}
+hipError_t launch_mb_fill(long long words, unsigned long long *mb, hipStream_t s) {
+  if (words <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mb_fill, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, words, mb);
   return hipGetLastError();
 }
 

@@ -1,0 +1,168 @@
+"""Learned warm start for the tidal forward solve: data generation with the method itself,
+training on the surrogate kernels, and warm-started COCG with fallback.
+
+There is no reference counterpart (the reference has no learned code). The pipeline only
+changes the initial guess X of the COCG solve (COCG13_single.f90:9-36, X in/out):
+  1. conductivity models: crust log10 sigma as a Gaussian random field (mesh.gaussian_random_field)
+     with the same air / sea layers, each with its own seeded tidal source;
+  2. each model is assembled and solved on the GPU (this library's COCG) and the edge solution
+     is projected to per-cell constant fields (the training targets);
+  3. the encoder-decoder (nn.Surrogate) is trained with Adam on those pairs;
+  4. for a new model the prediction is averaged onto edges (Dirichlet edges zero), scaled by the
+     minimal-residual factor (afx_scale_initial_guess_device, which falls back to X = 0 when the
+     prediction does not lower ||b - A X|| / ||b|| below 1), and COCG finishes the solve to the
+     same tolerance as a cold start.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import afx, mesh, nn
+
+
+def grf_case(grid, seed, std=0.7, corr_cells=4.0, sigma_sea=3.3):
+    """A box case whose crust log10 sigma is a Gaussian random field (mean -2)."""
+    nx, ny, nz = grid
+    f = mesh.gaussian_random_field(nz, ny, nx, seed, mean=-2.0, std=std, corr_cells=corr_cells)
+    return mesh.tide_case(nx, ny, nz, seed=seed, log10_sigma_field=f, sigma_sea=sigma_sea)
+
+
+@dataclass
+class Sample:
+    seed: int
+    feat: object          # bf16 [ncell][cin] device tensor
+    target: object        # f32 [ncell][6] device tensor (scaled cell fields)
+    b: object             # complex128 [n] device right-hand side (after boundary conditions)
+    x: object             # complex128 [n] device solution
+    iters: int
+    seconds: float
+
+
+class Dataset:
+    """Solved models on one grid. All models share the mesh, the pattern and one device System;
+    only conductivities and sources change."""
+
+    def __init__(self, grid, omega, device="cuda", cin=8):
+        import torch
+        self.grid = tuple(grid)
+        self.omega = omega
+        self.cin = cin
+        self.dev = torch.device(device)
+        base = grf_case(grid, 0)
+        self.s = afx.System.from_n6line(base.nline, base.n6line)
+        self.s.set_mesh(base)
+        xout, yout, zmin, zmax = base.bounds
+        self.s.assemble(base.sigma_tet(), omega)
+        self.bc = self.s.genbc(base.line, zmin, zmax, xout, yout)
+        self.bc_dev = torch.from_numpy(np.ascontiguousarray(self.bc, np.uint8)).to(self.dev)
+        self.maps = nn.GridMaps(base, device=device)
+        self.scale = None
+        self.samples: list[Sample] = []
+
+    def system_for(self, case):
+        """Assemble and constrain the device system for `case`; returns the host rhs."""
+        self.s.set_mesh(case)
+        b = self.s.assemble(case.sigma_tet(), self.omega)
+        return self.s.set_bc(self.bc, b)
+
+    def features(self, case):
+        import torch
+        f = nn.cell_features(case, self.cin)
+        return torch.from_numpy(f).to(self.dev).to(torch.bfloat16)
+
+    def add(self, seed, tol=1e-6, maxit=200000, precond="SSOR"):
+        import torch
+        case = grf_case(self.grid, seed)
+        b = self.system_for(case)
+        self.s.precond_setup(precond)
+        db = torch.from_numpy(np.ascontiguousarray(b, np.complex128)).to(self.dev)
+        dx = torch.zeros_like(db)
+        t0 = time.perf_counter()
+        st, _ = self.s.solve_device(db, dx, precond=precond, tol=tol, maxit=maxit, chunk=2000)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        if self.scale is None:  # RMS of the first model's cell fields
+            F1 = self.maps.edges_to_cells(dx, 1.0).cpu().numpy().astype(np.float64)
+            self.scale = float(np.sqrt(np.mean(F1 ** 2))) or 1.0
+        tgt = self.maps.edges_to_cells(dx, self.scale)
+        smp = Sample(seed, self.features(case), tgt, db, dx, st[0].iter, dt)
+        self.samples.append(smp)
+        return smp
+
+
+@dataclass
+class TrainLog:
+    losses: list = field(default_factory=list)
+    seconds: float = 0.0
+    steps: int = 0
+
+
+def train(net: nn.Surrogate, data: Dataset, idx, steps=300, batch=4, lr=2e-3, lr_min=1e-4, seed=0, log_every=50,
+          printer=None):
+    """Adam on the masked-free MSE between the network output and the scaled cell fields."""
+    import torch
+    rng = np.random.default_rng(seed)
+    nz, ny, nx = data.grid[2], data.grid[1], data.grid[0]
+    V1 = nx * ny * nz
+    log = TrainLog()
+    t0 = time.perf_counter()
+    idx = list(idx)
+    for step in range(steps):
+        pick = rng.choice(len(idx), size=min(batch, len(idx)), replace=False)
+        xs = torch.cat([data.samples[idx[i]].feat for i in pick], 0)
+        ts = torch.cat([data.samples[idx[i]].target for i in pick], 0)
+        nb = len(pick)
+        out = net.forward(xs, (nb, nz, ny, nx))
+        loss, grad = net.loss(out, ts)
+        net.backward(grad)
+        frac = step / max(steps - 1, 1)
+        net.step(lr=lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * frac)))
+        if step % log_every == 0 or step == steps - 1:
+            l = float(loss.item())
+            log.losses.append((step, l))
+            if printer:
+                printer(f"step {step} loss {l:.4e}")
+    torch.cuda.synchronize()
+    log.seconds = time.perf_counter() - t0
+    log.steps = steps
+    _ = V1
+    return log
+
+
+def predict_x0(net: nn.Surrogate, data: Dataset, feat):
+    """Network prediction averaged onto edges (Dirichlet edges zero), complex128 [n]."""
+    nz, ny, nx = data.grid[2], data.grid[1], data.grid[0]
+    out = net.forward(feat, (1, nz, ny, nx))
+    return data.maps.cells_to_edges(out, data.scale, ldf=32, zero=data.bc_dev)
+
+
+def warm_solve(net, data: Dataset, case, tol=1e-8, precond="SSOR", maxit=400000):
+    """Solve `case` cold (X = 0) and warm (surrogate X, scaled, with fallback); returns a dict with
+    iterations, times, initial residuals and the FP64 true residuals of both."""
+    import torch
+    b = data.system_for(case)
+    data.s.precond_setup(precond)
+    db = torch.from_numpy(np.ascontiguousarray(b, np.complex128)).to(data.dev)
+    x_cold = torch.zeros_like(db)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st_c, _ = data.s.solve_device(db, x_cold, precond=precond, tol=tol, maxit=maxit, chunk=2000)
+    torch.cuda.synchronize()
+    t_cold = time.perf_counter() - t0
+    feat = data.features(case)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x_warm = predict_x0(net, data, feat)
+    g, r_given, r_scaled = data.s.scale_initial_guess_device(db, x_warm)
+    st_w, _ = data.s.solve_device(db, x_warm, precond=precond, tol=tol, maxit=maxit, chunk=2000)
+    torch.cuda.synchronize()
+    t_warm = time.perf_counter() - t0
+    res = data.s.true_residual_device(db, x_warm)
+    res_c = data.s.true_residual_device(db, x_cold)
+    return dict(iters_cold=st_c[0].iter, iters_warm=st_w[0].iter, t_cold=t_cold, t_warm=t_warm,
+                r0_given=float(r_given[0]), r0_scaled=float(r_scaled[0]), gamma=complex(g[0]),
+                true_cold=float(res_c[0]), true_warm=float(res[0]), fallback=bool(g[0] == 0))

@@ -140,6 +140,16 @@ int afx_solve_device(afx_system s, int nrhs, const void *dB, void *dX, const dou
 int afx_true_residual_device(afx_system s, int nrhs, const void *dB, const void *dX,
                              const double *omega_scale, double *res);
 
+/* Minimal-residual scaling of device initial guesses X (row-interleaved, nrhs columns), for the
+   learned warm start (no reference counterpart; the reference always starts COCG13_single from
+   the X it is given, COCG13_single.f90:139-145). Per column: y = A x, gamma = y^H b / y^H y;
+   X <- gamma X when that lowers ||b - A X|| / ||b|| below 1 (the zero guess's value), else
+   X <- 0 (fallback). gamma (2*nrhs, re/im), res_given and res_scaled (nrhs, relative residual
+   of X as given and as returned) may be NULL. */
+int afx_scale_initial_guess_device(afx_system s, int nrhs, const void *dB, void *dX,
+                                   const double *omega_scale, double *gamma, double *res_given,
+                                   double *res_scaled);
+
 /* Launch timing (HIP events on the system's stream) of the persistent COCG kernel accumulated
    since the last reset: launches, COCG iterations they ran, and total kernel milliseconds. */
 int afx_kernel_timing(afx_system s, int reset, long long *launches, long long *iterations, double *kernel_ms);
@@ -185,6 +195,13 @@ int afx_obs_apply_device(afx_system s, int nobs, const int *item, const double *
                          double *out);
 
 /* ---- adjoint Jacobian -------------------------------------------------------------- */
+
+/* Adjoint right-hand sides of solvePARDISOinv (src/src_inv/solvePARDISOinv.f90:176-187): the
+   observation matrix PTR (real CRS, nobs rows over the n edges: stack(nobs+1) 0-based offsets,
+   item 1-based edge ids, val) transposed into nobs dense device columns, row-interleaved
+   (element (i, o) at i*nobs + o, the afx_solve_device layout). Solving with afx_solve_device
+   gives the adjoint fields u_o = A^-1 P_o^T that solvePARDISOinv returns as ut. */
+int afx_adjoint_rhs_device(afx_system s, int nobs, const int *stack, const int *item, const double *val, void *dB);
 
 /* genjacobian1 (src/src_inv_ap/m_jacobian_ap.f90:174-386) without thresholding:
      J[m, o] = sum_{t in m} sum_j U[e_j(t), o] * w_t[j],

@@ -1,0 +1,128 @@
+! afx_mod.f90 -- Fortran iso_c_binding interface to libafx.so (include/afx.h).
+!
+! A Fortran caller of the reference (type(global_matrix) arrays, complex(8) vectors,
+! 1-based index arrays) passes its arrays straight through: complex(8) has the interleaved
+! (re, im) layout the C ABI expects, INU/INL are (0:n) offsets, IAU/IAL hold 1-based ids.
+!
+!   call afx_cocg13_single_f(...)   replaces   call COCG13_single(...)   (COCG13_single.f90:9-36)
+!   call afx_solve_cg_f(...)        replaces   call solveCG(...)         (forward_bxyz_mpi.f90:70-94)
+!   afx_solve(...) on a handle      replaces   call solvePARDISO(...)    (solvePARDISO.f90:6-16)
+module afx
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: AFX_OK = 0, AFX_ERR_MAXIT = -100, AFX_ERR_BADOPT = 101
+  integer(c_int), parameter :: AFX_ERR_BREAKDOWN = -200, AFX_ERR_HIP = -300, AFX_ERR_ARG = -400
+  integer(c_int), parameter :: AFX_PC_NONE = 0, AFX_PC_IC = 1, AFX_PC_SSOR = 2, AFX_PC_DIAG = 3
+  integer(c_int), parameter :: AFX_FP64 = 0, AFX_MIXED = 1
+
+  type, bind(C) :: afx_opts
+    integer(c_int) :: precond
+    real(c_double) :: sigma_diag, sigma, tol
+    integer(c_int) :: maxit, nset, precision
+    real(c_double) :: inner_tol
+    integer(c_int) :: max_outer, chunk, check_true
+  end type afx_opts
+
+  type, bind(C) :: afx_stats
+    integer(c_int) :: iter, error
+    real(c_double) :: resid, true_resid
+    integer(c_int) :: outer
+  end type afx_stats
+
+  interface
+    subroutine afx_default_opts(o) bind(C, name="afx_default_opts")
+      import :: afx_opts
+      type(afx_opts), intent(out) :: o
+    end subroutine
+
+    integer(c_int) function afx_create(n, INU, IAU, INL, IAL, h) bind(C, name="afx_create")
+      import :: c_int, c_ptr
+      integer(c_int), value :: n
+      integer(c_int), intent(in) :: INU(*), IAU(*), INL(*), IAL(*)
+      type(c_ptr), intent(out) :: h
+    end function
+
+    integer(c_int) function afx_create_from_n6line(nline, ntet, n6line, h) bind(C, name="afx_create_from_n6line")
+      import :: c_int, c_ptr
+      integer(c_int), value :: nline, ntet
+      integer(c_int), intent(in) :: n6line(*)
+      type(c_ptr), intent(out) :: h
+    end function
+
+    integer(c_int) function afx_destroy(h) bind(C, name="afx_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+
+    integer(c_int) function afx_set_values(h, D, AU, AL) bind(C, name="afx_set_values")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(in) :: D(*), AU(*), AL(*)
+    end function
+
+    integer(c_int) function afx_precond_setup(h, kind, sigma_diag, sigma) bind(C, name="afx_precond_setup")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: kind
+      real(c_double), value :: sigma_diag, sigma
+    end function
+
+    ! solvePARDISO(doftot, nsr, A, rhs, xout, ip): rhs(doftot,nsr), xout(doftot,nsr) column-major
+    integer(c_int) function afx_solve(h, nrhs, B, ldb, X, ldx, opts, stats) bind(C, name="afx_solve")
+      import :: c_int, c_ptr, c_double_complex, afx_opts, afx_stats
+      type(c_ptr), value :: h
+      integer(c_int), value :: nrhs, ldb, ldx
+      complex(c_double_complex), intent(in) :: B(*)
+      complex(c_double_complex), intent(inout) :: X(*)
+      type(afx_opts), intent(in) :: opts
+      type(afx_stats), intent(out) :: stats(*)
+    end function
+
+    integer(c_int) function afx_cocg13_single(NP, NPL, NPU, D, AL, INL, IAL, AU, INU, IAU, B, X, PRECOND, &
+                                              SIGMA_DIAG, SIGMA, RESID, ITER, ERROR, NSET) &
+                                              bind(C, name="afx_cocg13_single")
+      import :: c_int, c_double, c_double_complex, c_char
+      integer(c_int), value :: NP, NPL, NPU, NSET
+      complex(c_double_complex), intent(inout) :: D(*), AL(*), AU(*), B(*), X(*)
+      integer(c_int), intent(in) :: INL(*), IAL(*), INU(*), IAU(*)
+      character(kind=c_char), intent(in) :: PRECOND(*)
+      real(c_double), value :: SIGMA_DIAG, SIGMA
+      real(c_double), intent(inout) :: RESID
+      integer(c_int), intent(inout) :: ITER
+      integer(c_int), intent(out) :: ERROR
+    end function
+
+    integer(c_int) function afx_solve_cg(doftot, NPL, NPU, D, AL, INL, IAL, AU, INU, IAU, rhs, X) &
+                                         bind(C, name="afx_solve_cg")
+      import :: c_int, c_double_complex
+      integer(c_int), value :: doftot, NPL, NPU
+      complex(c_double_complex), intent(in) :: D(*), AL(*), AU(*), rhs(*)
+      integer(c_int), intent(in) :: INL(*), IAL(*), INU(*), IAU(*)
+      complex(c_double_complex), intent(out) :: X(*)
+    end function
+  end interface
+
+contains
+
+  ! The reference's COCG13_single call, argument for argument (character(*) PRECOND padded
+  ! to the 20 characters the C side reads).
+  subroutine afx_cocg13_single_f(NP, NPL, NPU, D, AL, INL, IAL, AU, INU, IAU, B, X, PRECOND, &
+                                 SIGMA_DIAG, SIGMA, RESID, ITER, ERROR, NSET)
+    integer(4), intent(in) :: NP, NPL, NPU, NSET
+    complex(8), intent(inout) :: D(NP), AL(NPU), AU(NPU), B(NP), X(NP)
+    integer(4), intent(in) :: INL(0:NP), IAL(NPL), INU(0:NP), IAU(NPU)
+    character(len=*), intent(in) :: PRECOND
+    real(8), intent(in) :: SIGMA_DIAG, SIGMA
+    real(8), intent(inout) :: RESID
+    integer(4), intent(inout) :: ITER
+    integer(4), intent(out) :: ERROR
+    character(kind=c_char, len=20) :: pc
+    integer(c_int) :: rc
+    pc = PRECOND
+    rc = afx_cocg13_single(NP, NPL, NPU, D, AL, INL, IAL, AU, INU, IAU, B, X, pc, SIGMA_DIAG, SIGMA, &
+                           RESID, ITER, ERROR, NSET)
+    if (rc /= AFX_OK .and. rc /= AFX_ERR_MAXIT .and. rc /= AFX_ERR_BADOPT) ERROR = rc
+  end subroutine afx_cocg13_single_f
+
+end module afx

@@ -32,3 +32,17 @@ def test_default_opts_without_gpu():
     from actfemtide_amd import afx
     o = afx.default_opts()
     assert o.precond == afx.PC["SSOR"] and o.tol == 1e-8 and o.nset == 1 and o.precision == afx.FP64
+
+
+def declared_nn():
+    src = open(os.path.join(ROOT, "include", "afx_nn.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?\s*(afx_nn_\w+)\s*\(", src, re.M)))
+
+
+def test_nn_header_exports_and_binding():
+    from actfemtide_amd import afx, nn
+    names = declared_nn()
+    assert "afx_nn_gemm_nt" in names and len(names) >= 15
+    lib = ctypes.CDLL(afx.LIB_PATH)
+    assert not [n for n in names if not hasattr(lib, n)]
+    assert sorted(nn.SIGNATURES) == names

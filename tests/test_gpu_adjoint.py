@@ -1,0 +1,107 @@
+"""solvePARDISOinv mirror (forward + adjoint multi-RHS solves) and the end-to-end adjoint
+Jacobian on the GPU.
+
+Pinned by the adjoint identity u_o^T b = P_o x (A complex symmetric, so A^-T = A^-1), the
+reciprocity of P A^-1 P^T, a direct sparse solve of the oracle's matrix, and central finite
+differences of the observed fields in log10 sigma."""
+import numpy as np
+import pytest
+
+from tests.conftest import OMEGA_M2
+
+pytestmark = pytest.mark.gpu
+
+
+def _ptr_list(item, ve, vb):
+    """PTR(1:5) of the forward-inversion driver: Bx, By, Bz (face rows) and Ex, Ey (edge rows),
+    each a real CRS [nobs x nline] with 6 entries per site."""
+    nobs = item.shape[0]
+    stack = np.arange(0, 6 * nobs + 1, 6, dtype=np.int32)
+    it = item.reshape(-1).astype(np.int32)
+    return [(stack, it, vb[c].reshape(-1)) for c in range(3)] + [(stack, it, ve[c].reshape(-1)) for c in range(2)]
+
+
+def test_solve_pardiso_inv_identities(case_small):
+    import scipy.sparse.linalg as spla
+    from actfemtide_amd import afx
+    from oracle import oracle as O
+    from tests.conftest import seafloor_sites
+    s, b, bc = afx.forward_system(case_small, OMEGA_M2)
+    sites = seafloor_sites(case_small, 12, 21)
+    item, ve, vb, _ = s.prepobscoeff(sites)
+    PTR = _ptr_list(item, ve, vb)
+    xout, ut, stats = afx.solve_pardiso_inv(s, b, PTR, iflag_comp=[1, 1, 1, 1, 0])
+    assert ut[4] == 0
+    x = xout[:, 0]
+    o = O.forward_system(case_small, OMEGA_M2)
+    A = O.to_scipy(o["INU"], o["IAU"], o["INL"], o["IAL"], o["D"], o["AU"], o["AL"]).tocsc()
+    lu = spla.splu(A)
+    x_ref = lu.solve(o["b"])
+    assert np.linalg.norm(x - x_ref) <= 1e-6 * np.linalg.norm(x_ref)
+    Pd = []
+    for c in range(4):
+        st, it, val = PTR[c]
+        P = np.zeros((len(st) - 1, s.n))
+        for r in range(len(st) - 1):
+            P[r, it[st[r]:st[r + 1]] - 1] = val[st[r]:st[r + 1]]
+        Pd.append(P)
+        U = ut[c].toarray()                      # rows u_o^T
+        # adjoint identity: u_o^T b = P_o x
+        lhs = U @ b
+        rhs = P @ x
+        assert np.abs(lhs - rhs).max() <= 1e-6 * np.abs(rhs).max(), c
+        # against the direct solve of the oracle's matrix
+        U_ref = np.array([lu.solve(p.astype(complex)) for p in P])
+        err = np.abs(U - U_ref).max() / np.abs(U_ref).max()
+        print(f"component {c}: nnz(ut) {ut[c].nnz}, max |u - u_ref| / max|u_ref| = {err:.2e}")
+        assert err < 1e-6
+        # threshold: every kept entry above 1e-10, every dropped one at most 1e-10
+        assert np.abs(ut[c].data).min() > 1e-10
+        dropped = np.abs(U_ref)[U == 0]
+        assert dropped.size == 0 or dropped.max() <= 1e-10 * (1 + 1e-6)
+    # reciprocity between the E-field rows and the B-field rows
+    G = Pd[0] @ ut[3].toarray().T   # P_Bx A^-1 P_Ex^T
+    H = (Pd[3] @ ut[0].toarray().T).T
+    assert np.abs(G - H).max() <= 1e-6 * np.abs(G).max()
+
+
+def test_adjoint_jacobian_matches_finite_differences(case_tiny):
+    """Device adjoint Jacobian of the Bz field at 3 sites w.r.t. log10 sigma of crust cells,
+    against central differences (step 1e-4) of device forward solves."""
+    import torch
+    from actfemtide_amd import afx
+    from tests.conftest import crust_params, seafloor_sites
+    c = case_tiny
+    s, b, bc = afx.forward_system(c, OMEGA_M2)
+    item, ve, vb, _ = s.prepobscoeff(seafloor_sites(c, 3, 7))
+    PTR = _ptr_list(item, ve, vb)
+    xout, ut, _ = afx.solve_pardiso_inv(s, b, [PTR[2]], tol=1e-11)
+    x = xout[:, 0]
+    par_ptr, par_tet, cells = crust_params(c, ncell_max=4, seed=3)
+    dev = torch.device("cuda:0")
+    U = ut[0].toarray()
+    dU = torch.from_numpy(np.ascontiguousarray(U.T)).to(dev)
+    dx = torch.from_numpy(x).to(dev)
+    dJ = torch.zeros((len(par_ptr) - 1, U.shape[0]), dtype=torch.complex128, device=dev)
+    sig = c.sigma_tet()
+    s.jacobian_device(dU, dx, par_ptr, par_tet, sig, OMEGA_M2, np.log(10.0), dJ)
+    J = dJ.cpu().numpy()                     # d(P x) / d log10 sigma, (npar, nobs)
+    st, it, val = PTR[2]
+    P = np.zeros((len(st) - 1, s.n))
+    for r in range(len(st) - 1):
+        P[r, it[st[r]:st[r + 1]] - 1] = val[st[r]:st[r + 1]]
+    h = 1e-4
+    for m in range(len(par_ptr) - 1):
+        tets = par_tet[par_ptr[m]:par_ptr[m + 1]]
+        f = []
+        for sgn in (1, -1):
+            sg = sig.copy()
+            sg[tets] *= 10.0 ** (sgn * h)
+            s2, b2, _ = afx.forward_system(c, OMEGA_M2, sigma_tet=sg)
+            x2, st2 = s2.solve(b2, precond="SSOR", tol=1e-11, maxit=100000)
+            f.append(P @ x2[:, 0] if x2.ndim == 2 else P @ x2)
+            s2.close()
+        fd = (f[0] - f[1]) / (2 * h)
+        err = np.abs(J[m] - fd).max() / np.abs(fd).max()
+        print(f"parameter {m}: adjoint vs central difference {err:.2e}")
+        assert err < 1e-4

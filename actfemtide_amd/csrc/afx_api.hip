@@ -72,7 +72,8 @@ struct DevSide {
   cx<float> *val32 = nullptr;
   long long nnz = 0;
   int ew = 0;
-  int *ecol = nullptr, *emap = nullptr;
+  int *ecol = nullptr, *emap = nullptr, *epos = nullptr;
+  int smax = 0;
   cx<double> *eval64 = nullptr, *dpos64 = nullptr;
   cx<float> *eval32 = nullptr, *dpos32 = nullptr;
 };
@@ -361,9 +362,9 @@ struct SideHost {
   std::vector<int> bstep, spos, row, ptr, col, map;
   std::vector<unsigned char> exp, expp;
   int maxsteps = 0;
-  // fixed-width (padded) entries
-  int ew = 0;
-  std::vector<int> ecol, emap;
+  // fixed-width (padded) entries; epos: position of a same-block column row, else -1
+  int ew = 0, smax = 0;
+  std::vector<int> ecol, emap, epos;
 };
 
 void build_side(int n, int G, const std::vector<int> &rstart, const std::vector<int> &blk,
@@ -447,10 +448,16 @@ void build_side(int n, int G, const std::vector<int> &rstart, const std::vector<
   h.ew = std::max(h.ew, 1);
   h.ecol.assign((size_t)n * h.ew, -1);
   h.emap.assign((size_t)n * h.ew, -1);
+  h.epos.assign((size_t)n * h.ew, -1);
+  std::vector<int> rowpos(n);
+  for (int pos = 0; pos < n; pos++) rowpos[h.row[pos]] = pos;
+  h.smax = 0;
+  for (int st = 0; st < nstep; st++) h.smax = std::max(h.smax, h.spos[st + 1] - h.spos[st]);
   for (int pos = 0; pos < n; pos++)
     for (int q = h.ptr[pos]; q < h.ptr[pos + 1]; q++) {
       h.ecol[(size_t)pos * h.ew + (q - h.ptr[pos])] = h.col[q];
       h.emap[(size_t)pos * h.ew + (q - h.ptr[pos])] = h.map[q];
+      if (blk[h.col[q]] == blk[h.row[pos]]) h.epos[(size_t)pos * h.ew + (q - h.ptr[pos])] = rowpos[h.col[q]];
     }
 }
 
@@ -465,7 +472,9 @@ int upload_side(const SideHost &h, DevSide &d) {
   d.nnz = (long long)h.col.size();
   d.ew = h.ew;
   const size_t ne = h.ecol.size();
+  d.smax = h.smax;
   if ((rc = upload(&d.ecol, h.ecol.data(), ne)) || (rc = upload(&d.emap, h.emap.data(), ne)) ||
+      (rc = upload(&d.epos, h.epos.data(), ne)) ||
       (rc = dalloc(&d.eval64, ne)) || (rc = dalloc(&d.eval32, ne)) || (rc = dalloc(&d.dpos64, h.row.size())) ||
       (rc = dalloc(&d.dpos32, h.row.size())))
     return rc;
@@ -475,7 +484,7 @@ int upload_side(const SideHost &h, DevSide &d) {
 void free_side(DevSide &d) {
   dfree(d.bstep); dfree(d.spos); dfree(d.row); dfree(d.ptr); dfree(d.col); dfree(d.map);
   dfree(d.exp); dfree(d.expp); dfree(d.val64); dfree(d.val32);
-  dfree(d.ecol); dfree(d.emap); dfree(d.eval64); dfree(d.eval32); dfree(d.dpos64); dfree(d.dpos32);
+  dfree(d.ecol); dfree(d.emap); dfree(d.epos); dfree(d.eval64); dfree(d.eval32); dfree(d.dpos64); dfree(d.dpos32);
 }
 
 afx::SweepSide side_view(const DevSide &d, bool fp64) {
@@ -487,6 +496,8 @@ afx::SweepSide side_view(const DevSide &d, bool fp64) {
   // the fixed-width path needs at most 16 entries per position (the E * lanes of the kernel)
   const bool ell = d.ew <= 16 && !getenv("AFX_NO_ELL");
   v.ew = d.ew;
+  v.epos = d.epos;
+  v.smax = d.smax;
   v.ecol = ell ? d.ecol : nullptr;
   v.eval = fp64 ? (const void *)d.eval64 : (const void *)d.eval32;
   v.dpos = fp64 ? (const void *)d.dpos64 : (const void *)d.dpos32;
@@ -588,6 +599,23 @@ int lds_rows_for(afx_system s, bool multi) {
   return (need <= (size_t)maxlds) ? s->rows_per_block : 0;
 }
 
+// LDS ring (positions) of the several-column sweep: the largest power of two whose k columns
+// fit in 128 KB, at least 5*smax + 64 positions (the kernel reads the ring only for sources less
+// than W - 3*smax positions back, and loads older ones from global memory one step ahead, which
+// needs them two steps back); 0 = not used.
+template <typename TP>
+int ring_w_for(afx_system s, int k, int pc) {
+  if (k <= 1 || getenv("AFX_NO_RING")) return 0;
+  if (pc != AFX_PC_IC && pc != AFX_PC_SSOR && pc != AFX_PC_SSOR_BJ) return 0;
+  const bool bj = (pc == AFX_PC_SSOR_BJ);
+  const DevSide &fw = bj ? s->bjfw : s->fw, &bw = bj ? s->bjbw : s->bw;
+  if (fw.ew > 16 || bw.ew > 16 || 2 * k > 256) return 0;
+  int W = 1;
+  while ((size_t)(2 * W) * k * sizeof(cx<TP>) <= 131072) W *= 2;
+  const int smax = std::max(fw.smax, bw.smax);
+  return (W >= 5 * smax + 64) ? W : 0;
+}
+
 // One COCG solve on device vectors B (const) and X (in/out, the initial guess), [n][k] layout,
 // with the preconditioner in precision TP.
 template <typename TP>
@@ -638,6 +666,7 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   p.val = s->d_val64;
   p.Gs = G;
   p.lds_rows = lds_rows_for<TP>(s, multi);
+  p.ring_w = ring_w_for<TP>(s, k, pc);
   p.rstart = s->d_rstart;
   const bool bj = (pc == AFX_PC_SSOR_BJ);
   p.fw = side_view(bj ? s->bjfw : s->fw, fp64);
@@ -681,6 +710,17 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   p.chunk = chunk > 0 ? chunk : 200;
   const int lpr = fp64 ? s->lpr64 : s->lpr32;
   const bool verbose = getenv("AFX_VERBOSE") != nullptr;
+  // AFX_PROFILE: per-block phase timing of the persistent kernel, summarised on stderr
+  unsigned long long *dbg = nullptr;
+  struct DbgFree {
+    unsigned long long *&q;
+    ~DbgFree() { dfree(q); }
+  } dbg_free{dbg};
+  if (getenv("AFX_PROFILE")) {
+    if ((rc = dalloc(&dbg, (size_t)G * 8))) return rc;
+    HIPCHK(hipMemsetAsync(dbg, 0, (size_t)G * 8 * sizeof(unsigned long long), s->stream));
+  }
+  p.dbg = dbg;
   const auto t_start = std::chrono::steady_clock::now();
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   HIPCHK(hipEventCreate(&ev0));
@@ -707,14 +747,30 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
     it_before = w.h_status[0];
     if (verbose) {
       const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
-      fprintf(stderr, "afx: %s k=%d G=%d lpr=%d lds=%d it=%d done=%d %.3fs (%.1f us/it)\n",
-              fp64 ? "fp64" : "fp32-pc", k, G, lpr, p.lds_rows, w.h_status[0], w.h_status[1], dt,
+      fprintf(stderr, "afx: %s k=%d G=%d lpr=%d lds=%d ring=%d it=%d done=%d %.3fs (%.1f us/it)\n",
+              fp64 ? "fp64" : "fp32-pc", k, G, lpr, p.lds_rows, p.ring_w, w.h_status[0], w.h_status[1], dt,
               w.h_status[0] ? 1e6 * dt / w.h_status[0] : 0.0);
     }
     if (w.h_status[1] || w.h_status[0] >= maxit) break;
   }
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
+  if (dbg) {
+    std::vector<unsigned long long> h((size_t)G * 8);
+    HIPCHK(hipMemcpy(h.data(), dbg, h.size() * 8, hipMemcpyDeviceToHost));
+    const double its = std::max(1, w.h_status[0]);
+    double mean[6] = {0, 0, 0, 0, 0, 0}, mx[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < G; b++)
+      for (int q = 0; q < 6; q++) {
+        const double v = (double)h[(size_t)b * 8 + q] / its * (q < 5 ? 0.01 : 1.0);  // 100 MHz -> us
+        mean[q] += v / G;
+        mx[q] = std::max(mx[q], v);
+      }
+    fprintf(stderr,
+            "afx profile (us/iteration, mean/max over %d blocks): fw sweep %.1f/%.1f  barrier %.1f/%.1f  "
+            "bw sweep %.1f/%.1f  rest %.1f/%.1f  total %.1f/%.1f  mailbox polls/iteration/block %.1f/%.0f\n",
+            G, mean[0], mx[0], mean[1], mx[1], mean[2], mx[2], mean[3], mx[3], mean[4], mx[4], mean[5], mx[5]);
+  }
   int herr = 0;
   HIPCHK(hipMemcpy(&herr, s->d_err, sizeof(int), hipMemcpyDeviceToHost));
   if (herr) return fail(AFX_ERR_HIP, "a bounded device wait gave up (synchronisation error)");

@@ -34,6 +34,9 @@ struct SweepSide {
   const int *ecol;
   const void *eval;          // cx<TP>
   const void *dpos;          // [n] cx<TP>: DD in position order
+  const int *epos;           // [n * ew] position of the entry's column row when it is a row of the
+                             // same block, -1 otherwise (ring-buffer sweep of several columns)
+  int smax;                  // largest number of positions in one step
 };
 
 template <typename TP>
@@ -47,6 +50,7 @@ struct CocgParams {
   // pipelined sweeps
   int Gs;            // blocks of the launch == sweep blocks
   int lds_rows;      // > 0: own-block z staged in LDS (single column)
+  int ring_w;        // > 0: several columns, z of the last ring_w positions kept in an LDS ring
   const int *rstart; // [G+1] row range of each block
   SweepSide fw, bw;
   cx<TP> *mbf, *mbb;   // [n][k] mailboxes of the forward / backward sweep (MB_EMPTY when idle)
@@ -73,6 +77,7 @@ struct CocgParams {
   int *err;     // bounded waits that gave up
   double tol;
   int maxit, chunk;
+  unsigned long long *dbg;  // optional [G][8] phase timing (wall clock ticks) and poll counts
 };
 
 struct SpmvParams {

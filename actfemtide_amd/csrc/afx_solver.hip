@@ -19,6 +19,7 @@
 // (K, M) pairs and the entry used is K + i*w*M, so several tidal constituents share one launch.
 #include <hip/hip_cooperative_groups.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "afx_internal.h"
@@ -299,31 +300,12 @@ __device__ void sweep(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> *zs, 
 // the mailbox latency is paid once per chain of blocks, not once per dependency level.
 template <typename TP, int LPR, int E>
 struct StepRegs {
-  int row;  // -1: this slot has no row in the step
+  int row;
   int exp;  // another block reads this row's z
   int col[E];
   cx<TP> val[E];
   cx<TP> dd;
 };
-
-template <typename TP, int LPR, int E>
-__device__ __forceinline__ void prefetch_step(const SweepSide &sd, int pos, int hi, int lane, StepRegs<TP, LPR, E> &r) {
-  if (pos < hi) {
-    r.row = sd.row[pos];
-    r.exp = sd.expp[pos];
-    r.dd = ((const cx<TP> *)sd.dpos)[pos];
-    const int *ec = sd.ecol + (size_t)pos * sd.ew;
-    const cx<TP> *ev = (const cx<TP> *)sd.eval + (size_t)pos * sd.ew;
-#pragma unroll
-    for (int e = 0; e < E; e++) {
-      const int q = lane + e * LPR;
-      r.col[e] = (q < sd.ew) ? ec[q] : -1;
-      r.val[e] = (q < sd.ew) ? ev[q] : cx<TP>{TP(0), TP(0)};
-    }
-  } else {
-    r.row = -1;
-  }
-}
 
 // raw (non-polling) mailbox loads of the remote entries of a prefetched row
 __device__ __forceinline__ void mb_raw(const cx<double> *p, cx<double> &v) {
@@ -345,81 +327,264 @@ __device__ __forceinline__ bool mb_empty(cx<float> v) {
   return a == MB_EMPTY;
 }
 
+// Unconditional loads of the pipelined registers of one position (posc must be a valid position
+// index: the caller clamps it and keeps validity separately). Keeping every load outside control
+// flow matters: a load whose destination merges with another value at a branch join is copied,
+// and the copy waits for the load -- which would serialise the software pipeline.
 template <typename TP, int LPR, int E>
-__device__ __forceinline__ void mb_fetch(const StepRegs<TP, LPR, E> &r, int rs, int re, const cx<TP> *mb,
-                                         cx<TP> (&z)[E]) {
+__device__ __forceinline__ void load_step(const SweepSide &sd, int posc, int lane, StepRegs<TP, LPR, E> &r) {
+  r.row = sd.row[posc];
+  r.exp = sd.expp[posc];
+  r.dd = ((const cx<TP> *)sd.dpos)[posc];
+  const int *ec = sd.ecol + (size_t)posc * sd.ew;
+  const cx<TP> *ev = (const cx<TP> *)sd.eval + (size_t)posc * sd.ew;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int q = min(lane + e * LPR, sd.ew - 1);
+    r.col[e] = ec[q];
+    r.val[e] = ev[q];
+  }
+}
+
+// raw mailbox loads for the remote entries of a loaded row (other entries load slot 0, unused)
+template <typename TP, int LPR, int E>
+__device__ __forceinline__ void mb_fetch(const StepRegs<TP, LPR, E> &r, bool valid, const bool (&qok)[E], int rs,
+                                         int re, const cx<TP> *mb, cx<TP> (&z)[E]) {
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int cl = r.col[e];
-    if (r.row >= 0 && cl >= 0 && (cl < rs || cl >= re)) mb_raw(&mb[cl], z[e]);
+    const bool rem = valid && qok[e] && cl >= 0 && (cl < rs || cl >= re);
+    mb_raw(&mb[rem ? cl : 0], z[e]);
   }
 }
 
 template <typename TP, int LPR, int E, bool FWD>
-__device__ void sweep_lds(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> *zs, cx<TP> *mb, TP w) {
+__device__ void sweep_lds(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> *zs, cx<TP> *mb, TP w,
+                          unsigned &nfallback) {
   const int b = blockIdx.x;
   const int t = threadIdx.x;
   const int rs = p.rstart[b], re = p.rstart[b + 1];
   const int slot = t / LPR, lane = t % LPR;
   constexpr int NSLOT = 256 / LPR;
   const int s0 = sd.bstep[b], s1 = sd.bstep[b + 1];
-  auto sp = [&](int s) { return sd.spos[s < s1 ? s : s1]; };
-  int l0 = sp(s0), l1 = sp(s0 + 1), l2 = sp(s0 + 2), l3 = sp(s0 + 3);
-  StepRegs<TP, LPR, E> cur, nx1;
-  cx<TP> zr[E], zr1[E];
-  prefetch_step(sd, l0 + slot, l1, lane, cur);
-  prefetch_step(sd, l1 + slot, l2, lane, nx1);
-  mb_fetch(cur, rs, re, mb, zr);
-  __syncthreads();
-  for (int s = s0; s < s1; ++s) {
-    const int l4 = sp(s + 4);
-    mb_fetch(nx1, rs, re, mb, zr1);
-    StepRegs<TP, LPR, E> nx2;
-    prefetch_step(sd, l2 + slot, l3, lane, nx2);
-    // first row of this slot from the pipelined registers, any further rows directly
-    for (int pos = l0 + slot; pos < l1; pos += NSLOT) {
-      const bool first = (pos == l0 + slot);
-      StepRegs<TP, LPR, E> extra;
-      if (!first) prefetch_step(sd, pos, l1, lane, extra);
-      const StepRegs<TP, LPR, E> &r = first ? cur : extra;
-      cx<TP> acc = {TP(0), TP(0)};
+  bool qok[E];
 #pragma unroll
-      for (int e = 0; e < E; e++) {
-        const int cl = r.col[e];
-        if (cl >= 0) {
-          cx<TP> zc;
-          if (cl >= rs && cl < re) {
-            zc = zs[cl - rs];
-          } else {
-            zc = first ? zr[e] : cx<TP>{TP(0), TP(0)};
-            if (!first || mb_empty(zc)) zc = mb_get(&mb[cl], p.err);
+  for (int e = 0; e < E; e++) qok[e] = lane + e * LPR < sd.ew;
+  // step bounds are uniform: scalar loads
+  auto sp = [&](int s) { return sd.spos[__builtin_amdgcn_readfirstlane(s < s1 ? s : s1)]; };
+  // a clamped, always valid position of the step [lo, hi) (or of the block's last position)
+  auto clampp = [&](int lo, int hi) { return max(0, min(lo + slot, hi - 1)); };
+  int l0 = sp(s0), l1 = sp(s0 + 1), l2 = sp(s0 + 2), l3 = sp(s0 + 3);
+  // three register sets rotate by position in the unrolled loop below, so a set is never copied
+  // (a copy would wait for its loads to arrive, collapsing the pipeline to one step)
+  StepRegs<TP, LPR, E> R0, R1, R2;
+  cx<TP> Z0[E], Z1[E], Z2[E];
+  load_step(sd, clampp(l0, l1), lane, R0);
+  load_step(sd, clampp(l1, l2), lane, R1);
+  mb_fetch(R0, l0 + slot < l1, qok, rs, re, mb, Z0);
+  __syncthreads();
+  auto do_row = [&](const StepRegs<TP, LPR, E> &r, const cx<TP>(&zr)[E], auto pre) {
+    cx<TP> acc = {TP(0), TP(0)};
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int cl = r.col[e];
+      if (qok[e] && cl >= 0) {
+        cx<TP> zc;
+        if (cl >= rs && cl < re) {
+          zc = zs[cl - rs];
+        } else {
+          zc = decltype(pre)::value ? zr[e] : cx<TP>{TP(0), TP(0)};
+          if (!decltype(pre)::value || mb_empty(zc)) {
+            zc = mb_get(&mb[cl], p.err);
+            ++nfallback;
           }
-          acc = cfma(mval(r.val[e], w), zc, acc);
         }
-      }
-      acc = group_sum<LPR>(acc);
-      if (lane == 0) {
-        const int row = r.row;
-        const cx<TP> zo = zs[row - rs];
-        const cx<TP> z = FWD ? cmul(zo - acc, r.dd) : zo - cmul(r.dd, acc);
-        zs[row - rs] = z;
-        if (r.exp) mb_put(&mb[row], z);
+        acc = cfma(mval(r.val[e], w), zc, acc);
       }
     }
+    acc = group_sum<LPR>(acc);
+    if (lane == 0) {
+      const int row = r.row;
+      const cx<TP> zo = zs[row - rs];
+      const cx<TP> z = FWD ? cmul(zo - acc, r.dd) : zo - cmul(r.dd, acc);
+      zs[row - rs] = z;
+      if (r.exp) mb_put(&mb[row], z);
+    }
+  };
+  auto step = [&](const StepRegs<TP, LPR, E> &cur, const StepRegs<TP, LPR, E> &nx1, StepRegs<TP, LPR, E> &nx2,
+                  const cx<TP>(&zr)[E], cx<TP>(&zr1)[E], int l4) {
+    mb_fetch(nx1, l1 + slot < l2, qok, rs, re, mb, zr1);
+    load_step(sd, clampp(l2, l3), lane, nx2);
+    // first row of this slot from the pipelined registers, any further rows directly (each call
+    // site names its own register set: selecting one by reference would put both in scratch)
+    if (l0 + slot < l1) do_row(cur, zr, std::true_type{});
+    for (int pos = l0 + slot + NSLOT; pos < l1; pos += NSLOT) {
+      StepRegs<TP, LPR, E> extra;
+      load_step(sd, pos, lane, extra);
+      do_row(extra, zr, std::false_type{});
+    }
     __syncthreads();
-    cur = nx1;
-    nx1 = nx2;
-#pragma unroll
-    for (int e = 0; e < E; e++) zr[e] = zr1[e];
     l0 = l1;
     l1 = l2;
     l2 = l3;
     l3 = l4;
+  };
+  for (int s = s0; s < s1; s += 3) {
+    step(R0, R1, R2, Z0, Z1, sp(s + 4));
+    if (s + 1 >= s1) break;
+    step(R1, R2, R0, Z1, Z2, sp(s + 5));
+    if (s + 2 >= s1) break;
+    step(R2, R0, R1, Z2, Z0, sp(s + 6));
+  }
+}
+
+// Several columns (k > 1) share each sweep step: RL = LPR * k lanes per row, lane (c, l) takes
+// column c and entries l, l + LPR, ... of the row. The z of the block's last ring_w positions
+// (all columns) live in an LDS ring indexed by position; an entry whose source is a row of the
+// same block less than ring_w - 3*smax positions earlier reads the ring (its slot is rewritten at
+// least two steps later), an older one reads Z in global memory (ring_w >= 5*smax + 64, so it
+// was written at least two steps earlier), a row of another block reads the mailbox.
+// The same three-stage register pipeline as the single-column sweep hides the loads.
+template <typename TP, int LPR, int E>
+struct MRegs {
+  int row;
+  int exp;
+  int col[E];
+  int epos[E];
+  cx<TP> val[E];
+};
+
+template <typename TP, int LPR, int E>
+__device__ __forceinline__ void load_mstep(const SweepSide &sd, int posc, int lane, MRegs<TP, LPR, E> &r) {
+  r.row = sd.row[posc];
+  r.exp = sd.expp[posc];
+  const size_t base = (size_t)posc * sd.ew;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int q = min(lane + e * LPR, sd.ew - 1);
+    r.col[e] = sd.ecol[base + q];
+    r.epos[e] = sd.epos[base + q];
+    r.val[e] = ((const cx<TP> *)sd.eval)[base + q];
+  }
+}
+
+// stage-1 loads of a loaded row: remote z (mailbox, raw), far own z (Z), the row's input value
+// and its DD; every load unconditional (unused ones read index 0)
+template <typename TP, int LPR, int E, bool FWD>
+__device__ __forceinline__ void mfetch(const CocgParams<TP> &p, const MRegs<TP, LPR, E> &r, bool valid,
+                                       const bool (&qok)[E], int q, int near_lim, int c, int f, const cx<TP> *mb,
+                                       cx<TP> (&zx)[E], cx<TP> &zin, cx<TP> &dd) {
+  const int k = p.k;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int cl = r.col[e];
+    const bool use = valid && qok[e] && cl >= 0;
+    const bool rem = use && r.epos[e] < 0;
+    const bool far = use && r.epos[e] >= 0 && q - r.epos[e] >= near_lim;
+    const size_t ix = (size_t)(use ? cl : 0) * k + c;
+    if (rem)
+      mb_raw(&mb[ix], zx[e]);
+    else
+      zx[e] = p.Z[far ? ix : 0];
+  }
+  const size_t rx = (size_t)(valid ? r.row : 0) * k + c;
+  zin = FWD ? down<TP>(p.R[rx]) : p.Z[rx];
+  dd = p.DD[(size_t)(valid ? r.row : 0) * p.nf + f];
+}
+
+template <typename TP, int LPR, int E, bool FWD>
+__device__ void sweep_ring(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> *ring, cx<TP> *mb,
+                           unsigned &nfallback) {
+  const int b = blockIdx.x;
+  const int t = threadIdx.x;
+  const int k = p.k;
+  const int RL = LPR * k;
+  const int nslot = 256 / RL;
+  const int slot = t / RL;
+  const bool active = slot < nslot;
+  const int c = active ? (t % RL) / LPR : 0;  // this lane's column, its frequency and DD set
+  const int lane = t % LPR;
+  const TP w = (TP)p.wcol[c];
+  const int f = p.col2f[c];
+  const int s0 = sd.bstep[b], s1 = sd.bstep[b + 1];
+  const int W = p.ring_w;
+  const int near_lim = W - 3 * sd.smax;
+  bool qok[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) qok[e] = lane + e * LPR < sd.ew;
+  auto sp = [&](int s) { return sd.spos[__builtin_amdgcn_readfirstlane(s < s1 ? s : s1)]; };
+  auto clampp = [&](int lo, int hi) { return max(0, min(lo + slot, hi - 1)); };
+  int l0 = sp(s0), l1 = sp(s0 + 1), l2 = sp(s0 + 2), l3 = sp(s0 + 3);
+  MRegs<TP, LPR, E> R0, R1, R2;
+  cx<TP> X0[E], X1[E], X2[E], I0, I1, I2, D0, D1, D2;
+  load_mstep(sd, clampp(l0, l1), lane, R0);
+  load_mstep(sd, clampp(l1, l2), lane, R1);
+  mfetch<TP, LPR, E, FWD>(p, R0, active && l0 + slot < l1, qok, l0 + slot, near_lim, c, f, mb, X0, I0, D0);
+  __syncthreads();
+  auto do_row = [&](int q, const MRegs<TP, LPR, E> &r, const cx<TP>(&zx)[E], cx<TP> zin, cx<TP> dd) {
+    cx<TP> acc = {TP(0), TP(0)};
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int cl = r.col[e];
+      if (qok[e] && cl >= 0) {
+        const int ps = r.epos[e];
+        cx<TP> zc;
+        if (ps >= 0 && q - ps < near_lim) {
+          zc = ring[(size_t)(ps & (W - 1)) * k + c];
+        } else if (ps >= 0) {
+          zc = zx[e];
+        } else {
+          zc = zx[e];
+          if (mb_empty(zc)) {
+            zc = mb_get(&mb[(size_t)cl * k + c], p.err);
+            ++nfallback;
+          }
+        }
+        acc = cfma(mval(r.val[e], w), zc, acc);
+      }
+    }
+    acc = group_sum<LPR>(acc);
+    if (lane == 0) {
+      const cx<TP> z = FWD ? cmul(zin - acc, dd) : zin - cmul(dd, acc);
+      const size_t ix = (size_t)r.row * k + c;
+      ring[(size_t)(q & (W - 1)) * k + c] = z;
+      p.Z[ix] = z;
+      if (r.exp) mb_put(&mb[ix], z);
+    }
+  };
+  auto step = [&](const MRegs<TP, LPR, E> &cur, const MRegs<TP, LPR, E> &nx1, MRegs<TP, LPR, E> &nx2,
+                  const cx<TP>(&zx)[E], cx<TP> zin, cx<TP> dd, cx<TP>(&zx1)[E], cx<TP> &zin1, cx<TP> &dd1, int l4) {
+    mfetch<TP, LPR, E, FWD>(p, nx1, active && l1 + slot < l2, qok, l1 + slot, near_lim, c, f, mb, zx1, zin1, dd1);
+    load_mstep(sd, clampp(l2, l3), lane, nx2);
+    if (active && l0 + slot < l1) do_row(l0 + slot, cur, zx, zin, dd);
+    if (active)
+      for (int pos = l0 + slot + nslot; pos < l1; pos += nslot) {
+        MRegs<TP, LPR, E> ex;
+        cx<TP> xx[E], xi, xd;
+        load_mstep(sd, pos, lane, ex);
+        mfetch<TP, LPR, E, FWD>(p, ex, true, qok, pos, near_lim, c, f, mb, xx, xi, xd);
+        do_row(pos, ex, xx, xi, xd);
+      }
+    __syncthreads();
+    l0 = l1;
+    l1 = l2;
+    l2 = l3;
+    l3 = l4;
+  };
+  for (int s = s0; s < s1; s += 3) {
+    step(R0, R1, R2, X0, I0, D0, X1, I1, D1, sp(s + 4));
+    if (s + 1 >= s1) break;
+    step(R1, R2, R0, X1, I1, D1, X2, I2, D2, sp(s + 5));
+    if (s + 2 >= s1) break;
+    step(R2, R0, R1, X2, I2, D2, X0, I0, D0, sp(s + 6));
   }
 }
 
 template <typename TP, int LPR, bool MULTI>
-__global__ void __launch_bounds__(256) k_cocg(CocgParams<TP> p) {
+// One block of 256 threads per CU (the cooperative grid never exceeds the CU count), so each
+// SIMD runs a single wave of this kernel: let it use the whole register file instead of
+// spilling the software-pipelined sweep registers to scratch.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) k_cocg(CocgParams<TP> p) {
   __shared__ cx<double> lds_c[256];
   __shared__ double lds_d[256];
   extern __shared__ unsigned char dyn[];
@@ -452,9 +617,14 @@ __global__ void __launch_bounds__(256) k_cocg(CocgParams<TP> p) {
   int it = p.status[0];
   const int it_end = min(it + p.chunk, p.maxit);
   bool alldone = block_all(st != 0);
+  // optional per-block phase timing (wall clock, 100 MHz) and mailbox poll count
+  const bool prof = p.dbg != nullptr;
+  unsigned long long tfw = 0, tbw = 0, tbar = 0, trest = 0, titer = 0;
+  unsigned nfb = 0;
 
   while (!alldone && it < it_end) {
     ++it;
+    const unsigned long long ci0 = prof ? wall_clock64() : 0;
     // ---- z = M^-1 r (COCG13_single.f90:186-241) and rho = r^T z (:246-250) ----
     cx<double> rho_loc = {0.0, 0.0};
     if (sweep_pc) {
@@ -462,15 +632,35 @@ __global__ void __launch_bounds__(256) k_cocg(CocgParams<TP> p) {
       // it) while the backward sweep runs on its own mailbox
       if (!MULTI && p.lds_rows > 0 && p.fw.ecol != nullptr) {
         for (int r = rs + t; r < re; r += 256) zs[r - rs] = down<TP>(p.R[r]);
-        sweep_lds<TP, LPR, 16 / LPR + (LPR == 16 ? 1 : 0), true>(p, p.fw, zs, p.mbf, wT);
+        const unsigned long long c0 = prof ? wall_clock64() : 0;
+        sweep_lds<TP, LPR, 16 / LPR + (LPR == 16 ? 1 : 0), true>(p, p.fw, zs, p.mbf, wT, nfb);
+        const unsigned long long c1 = prof ? wall_clock64() : 0;
         grid_barrier(p.bar, phase);
         mb_clear_own(p.mbf, p.fw.exp, rs, re, 1);
-        sweep_lds<TP, LPR, 16 / LPR + (LPR == 16 ? 1 : 0), false>(p, p.bw, zs, p.mbb, wT);
+        const unsigned long long c2 = prof ? wall_clock64() : 0;
+        sweep_lds<TP, LPR, 16 / LPR + (LPR == 16 ? 1 : 0), false>(p, p.bw, zs, p.mbb, wT, nfb);
+        if (prof) {
+          const unsigned long long c3 = wall_clock64();
+          tfw += c1 - c0;
+          tbar += c2 - c1;
+          tbw += c3 - c2;
+        }
         for (int r = rs + t; r < re; r += 256) {
           const cx<TP> z = zs[r - rs];
           p.Z[r] = z;
           rho_loc = addv(rho_loc, cmul(p.R[r], up(z)));
         }
+      } else if (MULTI && p.ring_w > 0) {
+        sweep_ring<TP, 2, 8, true>(p, p.fw, zs, p.mbf, nfb);
+        grid_barrier(p.bar, phase);
+        mb_clear_own(p.mbf, p.fw.exp, rs, re, k);
+        sweep_ring<TP, 2, 8, false>(p, p.bw, zs, p.mbb, nfb);
+        __syncthreads();
+        if (cvalid)
+          for (long long row = rs + t / kp; row < re; row += 256 / kp) {
+            const size_t ix = (size_t)row * k + c;
+            rho_loc = addv(rho_loc, cmul(p.R[ix], up(p.Z[ix])));
+          }
       } else {
         sweep<TP, LPR, MULTI, true>(p, p.fw, zs, p.mbf, c, cvalid, wT, f, rho_loc);
         grid_barrier(p.bar, phase);
@@ -585,6 +775,19 @@ __global__ void __launch_bounds__(256) k_cocg(CocgParams<TP> p) {
     }
     rho1 = rho;
     alldone = block_all(st != 0);
+    if (prof) titer += wall_clock64() - ci0;
+  }
+  if (prof) {
+    trest = titer - tfw - tbw - tbar;
+    if (t == 0) {
+      unsigned long long *d = p.dbg + (size_t)blockIdx.x * 8;
+      d[0] += tfw;
+      d[1] += tbar;
+      d[2] += tbw;
+      d[3] += trest;
+      d[4] += titer;
+    }
+    atomicAdd(p.dbg + (size_t)blockIdx.x * 8 + 5, (unsigned long long)nfb);
   }
   if (blockIdx.x == 0) {
     if (cvalid && t < kp) {
@@ -603,6 +806,7 @@ __global__ void __launch_bounds__(256) k_cocg(CocgParams<TP> p) {
 template <typename TP>
 size_t cocg_dyn_lds(const CocgParams<TP> &p, int grid) {
   (void)grid;
+  if (p.ring_w > 0) return (size_t)p.ring_w * p.k * sizeof(cx<TP>);
   return (size_t)p.lds_rows * sizeof(cx<TP>);
 }
 

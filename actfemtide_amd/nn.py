@@ -484,6 +484,13 @@ class GridMaps:
         eptr = np.cumsum(eptr).astype(np.int32)
         ln = case.line.astype(np.int64) - 1
         evec = (case.xyz[ln[:, 1]] - case.xyz[ln[:, 0]]).astype(np.float64)
+        # edges every incident cell of which is air (group 1): sigma_air leaves their gradient
+        # components almost undetermined, so a prediction there only adds error
+        cell_air = np.zeros(ncell, bool)
+        cell_air[case.cell_of_tet[case.group == 1]] = True
+        not_air = np.zeros(nline, np.int64)
+        np.add.at(not_air, pe[:, 1], (~cell_air[pe[:, 0]]).astype(np.int64))
+        self.air_only = torch.from_numpy((not_air == 0).astype(np.uint8)).to(dev)
         self.ncell, self.nline = ncell, nline
         self.dims = (case.nz, case.ny, case.nx)
         self.cptr = torch.from_numpy(cptr).to(dev)

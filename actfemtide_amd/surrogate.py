@@ -133,16 +133,20 @@ def train(net: nn.Surrogate, data: Dataset, idx, steps=300, batch=4, lr=2e-3, lr
     return log
 
 
-def predict_x0(net: nn.Surrogate, data: Dataset, feat):
-    """Network prediction averaged onto edges (Dirichlet edges zero), complex128 [n]."""
+def predict_x0(net: nn.Surrogate, data: Dataset, feat, air=False):
+    """Network prediction averaged onto edges, complex128 [n]; zero on Dirichlet edges and,
+    unless `air`, on edges that touch only air cells."""
     nz, ny, nx = data.grid[2], data.grid[1], data.grid[0]
     out = net.forward(feat, (1, nz, ny, nx))
-    return data.maps.cells_to_edges(out, data.scale, ldf=32, zero=data.bc_dev)
+    zero = data.bc_dev if air else (data.bc_dev | data.maps.air_only)
+    return data.maps.cells_to_edges(out, data.scale, ldf=32, zero=zero)
 
 
-def warm_solve(net, data: Dataset, case, tol=1e-8, precond="SSOR", maxit=400000):
-    """Solve `case` cold (X = 0) and warm (surrogate X, scaled, with fallback); returns a dict with
-    iterations, times, initial residuals and the FP64 true residuals of both."""
+def warm_solve(net, data: Dataset, case, tol=1e-8, precond="SSOR", maxit=400000, accept=0.5, air=False):
+    """Solve `case` cold (X = 0) and warm (surrogate X, minimal-residual scaled) and return a dict
+    with iterations, times, initial residuals and the FP64 true residuals of both. The warm start
+    is used only when its scaled relative residual is at most `accept`; otherwise the solve
+    starts from X = 0 (the fallback), so a poor prediction costs one SpMV, not iterations."""
     import torch
     b = data.system_for(case)
     data.s.precond_setup(precond)
@@ -156,8 +160,11 @@ def warm_solve(net, data: Dataset, case, tol=1e-8, precond="SSOR", maxit=400000)
     feat = data.features(case)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    x_warm = predict_x0(net, data, feat)
+    x_warm = predict_x0(net, data, feat, air=air)
     g, r_given, r_scaled = data.s.scale_initial_guess_device(db, x_warm)
+    fallback = bool(g[0] == 0) or float(r_scaled[0]) > accept
+    if fallback:
+        x_warm.zero_()
     st_w, _ = data.s.solve_device(db, x_warm, precond=precond, tol=tol, maxit=maxit, chunk=2000)
     torch.cuda.synchronize()
     t_warm = time.perf_counter() - t0
@@ -165,4 +172,4 @@ def warm_solve(net, data: Dataset, case, tol=1e-8, precond="SSOR", maxit=400000)
     res_c = data.s.true_residual_device(db, x_cold)
     return dict(iters_cold=st_c[0].iter, iters_warm=st_w[0].iter, t_cold=t_cold, t_warm=t_warm,
                 r0_given=float(r_given[0]), r0_scaled=float(r_scaled[0]), gamma=complex(g[0]),
-                true_cold=float(res_c[0]), true_warm=float(res[0]), fallback=bool(g[0] == 0))
+                true_cold=float(res_c[0]), true_warm=float(res[0]), fallback=fallback)

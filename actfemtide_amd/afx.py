@@ -8,7 +8,9 @@ The reference entry points this module mirrors (same argument meaning and error 
   PREPOBSCOEFF         src/solver_mpi/n_ebfem_bxyz_mpi.f90:653-726 -> System.prepobscoeff()
   CALOBSEBCOMP/RESP    src/solver_mpi/n_ebfem_bxyz_mpi.f90:397-423,570-609 -> calobsebcomp()
   solvePARDISOinv      src/src_inv/solvePARDISOinv.f90:5-244     -> solve_pardiso_inv()
-  genjacobian1         src/src_inv_ap/m_jacobian_ap.f90:174-386  -> System.jacobian_device()
+  genjacobian1         src/src_inv_ap/m_jacobian_ap.f90:174-386  -> System.jacobian_device(),
+                                                                    jacobian_ampphase_device()
+  getnewmodel          src/src_inv_ap/m_jacobian_ap.f90:623-787  -> gn_update()
 There is no CPU fallback: every call goes through the HIP library, and loading fails loudly
 when libafx.so is missing.
 """
@@ -88,6 +90,10 @@ SIGNATURES = {
     "afx_obs_apply_device": ([_VP, _I, _VP, _VP, _I, _VP, _VP], _I),
     "afx_adjoint_rhs_device": ([_VP, _I, _VP, _VP, _VP, _VP], _I),
     "afx_jacobian_device": ([_VP, _I, _VP, _VP, _I, _VP, _VP, _VP, _D, _D, _VP], _I),
+    "afx_jacobian_ampphase_device": ([_I, _I, _VP, _VP, _VP], _I),
+    "afx_gn_update_device": ([_I, _I, _VP, _VP, _VP, _VP, _VP, _D, _VP, _VP, _VP, _VP, _VP, _VP], _I),
+    "afx_dgemm_nt_device": ([_I, _I, _I, _VP, _I, _VP, _I, _VP, _I, _I], _I),
+    "afx_inv_last_error": ([], C.c_char_p),
 }
 
 _lib = None
@@ -432,6 +438,44 @@ def solve_pardiso_inv(s: System, rhs, PTR, iflag_comp=None, threshold=1e-10, max
         rows, cols = np.nonzero(keep.T)
         ut.append(sp.csr_matrix((U.T[rows, cols], (rows, cols)), shape=(nobs, n)))
     return xout, ut, stats
+
+
+def _inv_err(code):
+    if code != AFX_OK:
+        raise AfxError(code, lib().afx_inv_last_error().decode())
+
+
+def jacobian_ampphase_device(dJ, dbe, dout):
+    """genjacobian1's real rows (m_jacobian_ap.f90:338-345): dJ (npar, nobs) complex128, dbe (nobs,)
+    complex128, dout (2, nobs, npar) float64 -- all torch device tensors."""
+    npar, nobs = dJ.shape
+    _inv_err(lib().afx_jacobian_ampphase_device(npar, nobs, C.c_void_p(dJ.data_ptr()), C.c_void_p(dbe.data_ptr()),
+                                                C.c_void_p(dout.data_ptr())))
+
+
+def dgemm_nt_device(dA, dB, dC, accumulate=False):
+    M, K = dA.shape
+    N = dB.shape[0]
+    _inv_err(lib().afx_dgemm_nt_device(M, N, K, C.c_void_p(dA.data_ptr()), K, C.c_void_p(dB.data_ptr()), K,
+                                       C.c_void_p(dC.data_ptr()), N, 1 if accumulate else 0))
+
+
+def gn_update(dJ, Cm, cd, alpha, dobs, dcal, model, model_ref):
+    """getnewmodel + solvebeta (src/src_inv_ap/m_jacobian_ap.f90:623-787) on the device: dJ a
+    (ndat, nmodel) float64 torch device tensor, Cm a scipy sparse symmetric (nmodel, nmodel)
+    matrix, the rest host arrays. Returns (model_new, beta)."""
+    import scipy.sparse as sp
+    ndat, nmodel = dJ.shape
+    Cm = sp.csr_matrix(Cm)
+    ptr, col, val = _i32(Cm.indptr), _i32(Cm.indices), _f64(Cm.data)
+    cd, dobs, dcal = _f64(cd), _f64(dobs), _f64(dcal)
+    model, model_ref = _f64(model), _f64(model_ref)
+    out = np.zeros(nmodel)
+    beta = np.zeros(ndat)
+    _inv_err(lib().afx_gn_update_device(ndat, nmodel, C.c_void_p(dJ.data_ptr()), _p(ptr), _p(col), _p(val), _p(cd),
+                                        float(alpha), _p(dobs), _p(dcal), _p(model), _p(model_ref), _p(out),
+                                        _p(beta)))
+    return out, beta
 
 
 def forward_system(case, omega: float, sigma_tet=None) -> tuple:

@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libafx.so")
-SOURCES = ["afx_solver.hip", "afx_assembly.hip", "afx_api.hip", "afx_nn.hip"]
+SOURCES = ["afx_solver.hip", "afx_assembly.hip", "afx_api.hip", "afx_nn.hip", "afx_inv.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 FLAGS = [

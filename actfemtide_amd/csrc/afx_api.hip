@@ -609,7 +609,7 @@ int ring_w_for(afx_system s, int k, int pc) {
   if (pc != AFX_PC_IC && pc != AFX_PC_SSOR && pc != AFX_PC_SSOR_BJ) return 0;
   const bool bj = (pc == AFX_PC_SSOR_BJ);
   const DevSide &fw = bj ? s->bjfw : s->fw, &bw = bj ? s->bjbw : s->bw;
-  if (fw.ew > 16 || bw.ew > 16 || 2 * k > 256) return 0;
+  if (fw.ew > 16 || bw.ew > 16 || k > AFX_RING_KC) return 0;
   int W = 1;
   while ((size_t)(2 * W) * k * sizeof(cx<TP>) <= 131072) W *= 2;
   const int smax = std::max(fw.smax, bw.smax);

@@ -11,6 +11,11 @@
 
 namespace afx {
 
+// most columns the LDS-ring sweep takes in one launch (2 lanes per row and column)
+#ifndef AFX_RING_KC
+#define AFX_RING_KC 128
+#endif
+
 // complex number as a (re, im) pair; for matrix values the same pair holds (K, M) with the
 // effective entry K + i*w*M for a column whose angular-frequency factor is w.
 template <typename T>

@@ -1,0 +1,295 @@
+// Gauss-Newton data-space model update on the device: the real Jacobian rows of genjacobian1's
+// amplitude / phase derivatives (src/src_inv_ap/m_jacobian_ap.f90:338-345), the normal matrix
+// C = J Cm J^T + alpha Cd and the update of getnewmodel (:623-787) with solvebeta's C beta = X.
+//
+// The J Cm J^T product is a real dense contraction (ndat x nmodel by nmodel x ndat) and runs on
+// FP64 MFMA (v_mfma_f64_16x16x4f64); C is factorised by a column-by-column Cholesky on the
+// device and the two triangular solves run in one block each.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "afx_internal.h"
+
+using afx::cx;
+
+namespace {
+
+typedef double double4v __attribute__((ext_vector_type(4)));
+
+// C[m][n] = sum_k A[m][k] B[n][k] (+ C when accumulate), FP64. One wave computes a 32x32 tile as
+// 2x2 MFMA 16x16 tiles; 4 waves per block cover 64x64. Operand layout of the 16x16x4 MFMA: lane l
+// supplies A[row l%16][k l/16] and B[k l/16][col l%16]; it holds D[4*(l/16)+i][l%16], i = 0..3.
+__global__ void __launch_bounds__(256) k_dgemm_nt(const double *__restrict__ A, int lda, const double *__restrict__ B,
+                                                  int ldb, int M, int N, int K, double *__restrict__ C, int ldc,
+                                                  int accumulate) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m0 = blockIdx.x * 64 + (wave >> 1) * 32;
+  const int n0 = blockIdx.y * 64 + (wave & 1) * 32;
+  const int r = lane & 15, kk = lane >> 4;
+  double4v acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = double4v{0.0, 0.0, 0.0, 0.0};
+  const int ma0 = m0 + r, ma1 = m0 + 16 + r, nb0 = n0 + r, nb1 = n0 + 16 + r;
+  const double *pa0 = A + (size_t)min(ma0, M - 1) * lda, *pa1 = A + (size_t)min(ma1, M - 1) * lda;
+  const double *pb0 = B + (size_t)min(nb0, N - 1) * ldb, *pb1 = B + (size_t)min(nb1, N - 1) * ldb;
+  for (int k = 0; k < K; k += 4) {
+    const int kc = k + kk;
+    const bool kin = kc < K;
+    const double a0 = (kin && ma0 < M) ? pa0[kc] : 0.0, a1 = (kin && ma1 < M) ? pa1[kc] : 0.0;
+    const double b0 = (kin && nb0 < N) ? pb0[kc] : 0.0, b1 = (kin && nb1 < N) ? pb1[kc] : 0.0;
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int m = m0 + 16 * i + 4 * kk + q, n = n0 + 16 * j + r;
+        if (m < M && n < N) {
+          double *c = C + (size_t)m * ldc + n;
+          *c = accumulate ? *c + acc[i][j][q] : acc[i][j][q];
+        }
+      }
+}
+
+// G[d][i] = sum_j J[d][j] Cm[i][j] (Cm CSR; symmetric Cm gives G = J Cm)
+__global__ void k_spmm_rows(int ndat, int nmodel, const double *J, const int *ptr, const int *col, const double *val,
+                            double *G) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)ndat * nmodel) return;
+  const int d = (int)(t / nmodel), i = (int)(t % nmodel);
+  const double *jr = J + (size_t)d * nmodel;
+  double s = 0.0;
+  for (int q = ptr[i]; q < ptr[i + 1]; ++q) s += val[q] * jr[col[q]];
+  G[t] = s;
+}
+
+__global__ void k_add_diag(int n, double *C, int ldc, const double *d, double alpha) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) C[(size_t)i * ldc + i] += alpha * d[i];
+}
+
+// y[d] = sum_i A[d][i] x[i] (one block per row)
+__global__ void k_gemv_rows(int nrow, int ncol, const double *A, const double *x, double *y) {
+  __shared__ double red[256];
+  const int d = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < ncol; i += 256) s += A[(size_t)d * ncol + i] * x[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) y[d] = red[0];
+}
+
+// y[i] = sum_d A[d][i] x[d] (threads over i, coalesced rows)
+__global__ void k_gemv_cols(int nrow, int ncol, const double *A, const double *x, double *y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncol) return;
+  double s = 0.0;
+  for (int d = 0; d < nrow; ++d) s += A[(size_t)d * ncol + i] * x[d];
+  y[i] = s;
+}
+
+// Cholesky, column j, step 1: A[i][j] -= sum_{k<j} A[i][k] A[j][k] for i >= j (one block per i)
+__global__ void k_chol_dots(int n, int j, double *A) {
+  __shared__ double red[256];
+  const int i = j + blockIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int k = threadIdx.x; k < j; k += 256) s += A[(size_t)i * n + k] * A[(size_t)j * n + k];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) A[(size_t)i * n + j] -= red[0];
+}
+
+// step 2: A[i][j] /= sqrt(A[j][j]) for i > j (the pivot stays in A[j][j] until k_chol_diag: no
+// later column reads a diagonal entry); a pivot <= 0 sets *bad
+__global__ void k_chol_scale(int n, int j, double *A, int *bad) {
+  const int i = j + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const double p = A[(size_t)j * n + j];
+  if (!(p > 0.0)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *bad = j + 1;
+    return;
+  }
+  if (i < n) A[(size_t)i * n + j] /= sqrt(p);
+}
+
+__global__ void k_chol_diag(int n, double *A) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) A[(size_t)j * n + j] = sqrt(A[(size_t)j * n + j]);
+}
+
+// L y = b then L^T x = y, L lower (row-major, n x n); one block of 1024 threads
+__global__ void __launch_bounds__(1024) k_chol_solve(int n, const double *L, const double *b, double *x, double *y) {
+  __shared__ double red[1024];
+  const int t = threadIdx.x;
+  for (int i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (int k = t; k < i; k += 1024) s += L[(size_t)i * n + k] * y[k];
+    red[t] = s;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+      if (t < w) red[t] += red[t + w];
+      __syncthreads();
+    }
+    if (t == 0) y[i] = (b[i] - red[0]) / L[(size_t)i * n + i];
+    __syncthreads();
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = 0.0;
+    for (int k = i + 1 + t; k < n; k += 1024) s += L[(size_t)k * n + i] * x[k];
+    red[t] = s;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+      if (t < w) red[t] += red[t + w];
+      __syncthreads();
+    }
+    if (t == 0) x[i] = (y[i] - red[0]) / L[(size_t)i * n + i];
+    __syncthreads();
+  }
+}
+
+// genjacobian1 :338-345: rows o of dJ (npar, nobs complex, o fastest) divided by be[o]:
+// amp row o: Re(dbe/be)/ln10, phase row o: Im(dbe/be)*180/pi -> out [2][nobs][npar]
+__global__ void k_ampphase(int npar, int nobs, const cx<double> *J, const cx<double> *be, double *out) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)npar * nobs) return;
+  const int m = (int)(t / nobs), o = (int)(t % nobs);
+  const cx<double> a = J[t], b = be[o];
+  const double den = b.r * b.r + b.i * b.i;
+  const double qr = (a.r * b.r + a.i * b.i) / den, qi = (a.i * b.r - a.r * b.i) / den;
+  out[(size_t)o * npar + m] = qr / log(10.0);
+  out[((size_t)nobs + o) * npar + m] = qi * 180.0 / M_PI;
+}
+
+thread_local std::string g_inv_err;
+
+int fail(int code, const char *m) {
+  g_inv_err = m;
+  return code;
+}
+
+template <typename T>
+int dalloc(T **p, size_t n) {
+  if (hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return fail(AFX_ERR_HIP, "hipMalloc");
+  return AFX_OK;
+}
+template <typename T>
+int upload(T **p, const T *h, size_t n) {
+  int rc = dalloc(p, n);
+  if (rc) return rc;
+  if (n && hipMemcpy(*p, h, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return fail(AFX_ERR_HIP, "copy");
+  return AFX_OK;
+}
+template <typename T>
+void dfree(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *afx_inv_last_error(void) { return g_inv_err.c_str(); }
+
+int afx_dgemm_nt_device(int M, int N, int K, const double *dA, int lda, const double *dB, int ldb, double *dC, int ldc,
+                        int accumulate) {
+  if (M <= 0 || N <= 0 || K <= 0 || !dA || !dB || !dC || lda < K || ldb < K || ldc < N)
+    return fail(AFX_ERR_ARG, "dgemm: bad arguments");
+  hipLaunchKernelGGL(k_dgemm_nt, dim3((M + 63) / 64, (N + 63) / 64), dim3(256), 0, 0, dA, lda, dB, ldb, M, N, K, dC,
+                     ldc, accumulate);
+  if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return fail(AFX_ERR_HIP, "dgemm");
+  return AFX_OK;
+}
+
+int afx_jacobian_ampphase_device(int npar, int nobs, const void *dJ, const void *dbe, double *dout) {
+  if (npar <= 0 || nobs <= 0 || !dJ || !dbe || !dout) return fail(AFX_ERR_ARG, "ampphase: bad arguments");
+  const long long m = (long long)npar * nobs;
+  hipLaunchKernelGGL(k_ampphase, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, 0, npar, nobs,
+                     (const cx<double> *)dJ, (const cx<double> *)dbe, dout);
+  if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return fail(AFX_ERR_HIP, "ampphase");
+  return AFX_OK;
+}
+
+int afx_gn_update_device(int ndat, int nmodel, const double *dJ, const int *cm_ptr, const int *cm_col,
+                         const double *cm_val, const double *cd, double alpha, const double *dobs, const double *dcal,
+                         const double *model, const double *model_ref, double *model_new, double *beta_out) {
+  if (ndat <= 0 || nmodel <= 0 || !dJ || !cm_ptr || !cm_col || !cm_val || !cd || !dobs || !dcal || !model ||
+      !model_ref || !model_new)
+    return fail(AFX_ERR_ARG, "gn_update: bad arguments");
+  const int nnz = cm_ptr[nmodel];
+  for (int q = 0; q < nnz; q++)
+    if (cm_col[q] < 0 || cm_col[q] >= nmodel) return fail(AFX_ERR_ARG, "gn_update: Cm column out of range");
+  int *dptr = nullptr, *dcol = nullptr, *dbad = nullptr;
+  double *dval = nullptr, *dG = nullptr, *dC = nullptr, *dcd = nullptr, *ddm = nullptr, *dJM = nullptr;
+  double *dX = nullptr, *dy = nullptr, *dbeta = nullptr, *dmod = nullptr;
+  int rc;
+  auto cleanup = [&]() {
+    dfree(dptr); dfree(dcol); dfree(dbad); dfree(dval); dfree(dG); dfree(dC); dfree(dcd); dfree(ddm);
+    dfree(dJM); dfree(dX); dfree(dy); dfree(dbeta); dfree(dmod);
+  };
+  std::vector<double> dm(nmodel);
+  for (int i = 0; i < nmodel; i++) dm[i] = model[i] - model_ref[i];  // getnewmodel :712
+  if ((rc = upload(&dptr, cm_ptr, (size_t)nmodel + 1)) || (rc = upload(&dcol, cm_col, (size_t)nnz)) ||
+      (rc = upload(&dval, cm_val, (size_t)nnz)) || (rc = dalloc(&dG, (size_t)ndat * nmodel)) ||
+      (rc = dalloc(&dC, (size_t)ndat * ndat)) || (rc = upload(&dcd, cd, (size_t)ndat)) ||
+      (rc = upload(&ddm, dm.data(), (size_t)nmodel)) || (rc = dalloc(&dJM, (size_t)ndat)) ||
+      (rc = dalloc(&dX, (size_t)ndat)) || (rc = dalloc(&dy, (size_t)ndat)) || (rc = dalloc(&dbeta, (size_t)ndat)) ||
+      (rc = dalloc(&dmod, (size_t)nmodel)) || (rc = dalloc(&dbad, 1))) {
+    cleanup();
+    return rc;
+  }
+  bool ok = hipMemset(dbad, 0, sizeof(int)) == hipSuccess;
+  // G = J Cm (= (Cm J^T)^T, the CMJT of getnewmodel :699-701); C = G J^T + alpha Cd (:704-711)
+  const long long ng = (long long)ndat * nmodel;
+  hipLaunchKernelGGL(k_spmm_rows, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, 0, ndat, nmodel, dJ, dptr, dcol,
+                     dval, dG);
+  hipLaunchKernelGGL(k_dgemm_nt, dim3((ndat + 63) / 64, (ndat + 63) / 64), dim3(256), 0, 0, dG, nmodel, dJ, nmodel,
+                     ndat, ndat, nmodel, dC, ndat, 0);
+  hipLaunchKernelGGL(k_add_diag, dim3((ndat + 255) / 256), dim3(256), 0, 0, ndat, dC, ndat, dcd, alpha);
+  // X = dobs - dcal + J (m - m_ref) (:713-715)
+  hipLaunchKernelGGL(k_gemv_rows, dim3(ndat), dim3(256), 0, 0, ndat, nmodel, dJ, ddm, dJM);
+  std::vector<double> JM(ndat), X(ndat);
+  ok = ok && hipGetLastError() == hipSuccess && hipMemcpy(JM.data(), dJM, ndat * 8, hipMemcpyDeviceToHost) == hipSuccess;
+  for (int d = 0; d < ndat; d++) X[d] = dobs[d] - dcal[d] + JM[d];
+  ok = ok && hipMemcpy(dX, X.data(), ndat * 8, hipMemcpyHostToDevice) == hipSuccess;
+  // C beta = X (solvebeta): Cholesky C = L L^T in place, then two triangular solves
+  for (int j = 0; ok && j < ndat; j++) {
+    hipLaunchKernelGGL(k_chol_dots, dim3(ndat - j), dim3(256), 0, 0, ndat, j, dC);
+    hipLaunchKernelGGL(k_chol_scale, dim3((ndat - j + 255) / 256), dim3(256), 0, 0, ndat, j, dC, dbad);
+  }
+  hipLaunchKernelGGL(k_chol_diag, dim3((ndat + 255) / 256), dim3(256), 0, 0, ndat, dC);
+  hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(1024), 0, 0, ndat, dC, dX, dbeta, dy);
+  // m_new = m_ref + Cm J^T beta = m_ref + G^T beta (:724-733)
+  hipLaunchKernelGGL(k_gemv_cols, dim3((nmodel + 255) / 256), dim3(256), 0, 0, ndat, nmodel, dG, dbeta, dmod);
+  int bad = 0;
+  ok = ok && hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
+       hipMemcpy(&bad, dbad, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess &&
+       hipMemcpy(model_new, dmod, (size_t)nmodel * 8, hipMemcpyDeviceToHost) == hipSuccess &&
+       (!beta_out || hipMemcpy(beta_out, dbeta, (size_t)ndat * 8, hipMemcpyDeviceToHost) == hipSuccess);
+  cleanup();
+  if (!ok) return fail(AFX_ERR_HIP, "gn_update: HIP error");
+  if (bad) return fail(AFX_ERR_BREAKDOWN, "gn_update: C is not positive definite");
+  for (int i = 0; i < nmodel; i++) model_new[i] += model_ref[i];
+  return AFX_OK;
+}
+
+}  // extern "C"

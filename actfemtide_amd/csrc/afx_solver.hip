@@ -327,6 +327,29 @@ __device__ __forceinline__ bool mb_empty(cx<float> v) {
   return a == MB_EMPTY;
 }
 
+// Wait until none of the N slots marked `rem` holds MB_EMPTY: every pending slot is reloaded in
+// the same round (one memory round trip per retry for all of them, not one per slot).
+template <typename TP, int N>
+__device__ __forceinline__ void mb_poll(const cx<TP> *mb, const size_t (&idx)[N], const bool (&rem)[N],
+                                        cx<TP> (&z)[N], int *err, unsigned &nfallback) {
+  bool any = false;
+#pragma unroll
+  for (int i = 0; i < N; i++) any |= rem[i] && mb_empty(z[i]);
+  if (!any) return;
+  ++nfallback;
+  unsigned spins = 0;
+  for (;;) {
+#pragma unroll
+    for (int i = 0; i < N; i++)
+      if (rem[i] && mb_empty(z[i])) mb_raw(&mb[idx[i]], z[i]);
+    any = false;
+#pragma unroll
+    for (int i = 0; i < N; i++) any |= rem[i] && mb_empty(z[i]);
+    if (!any || mb_give_up(++spins, err)) break;
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
 // Unconditional loads of the pipelined registers of one position (posc must be a valid position
 // index: the caller clamps it and keeps validity separately). Keeping every load outside control
 // flow matters: a load whose destination merges with another value at a branch join is copied,
@@ -384,24 +407,29 @@ __device__ void sweep_lds(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> *
   mb_fetch(R0, l0 + slot < l1, qok, rs, re, mb, Z0);
   __syncthreads();
   auto do_row = [&](const StepRegs<TP, LPR, E> &r, const cx<TP>(&zr)[E], auto pre) {
-    cx<TP> acc = {TP(0), TP(0)};
+    cx<TP> zv[E];
+    bool rem[E];
+    size_t idx[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
       const int cl = r.col[e];
-      if (qok[e] && cl >= 0) {
-        cx<TP> zc;
-        if (cl >= rs && cl < re) {
-          zc = zs[cl - rs];
-        } else {
-          zc = decltype(pre)::value ? zr[e] : cx<TP>{TP(0), TP(0)};
-          if (!decltype(pre)::value || mb_empty(zc)) {
-            zc = mb_get(&mb[cl], p.err);
-            ++nfallback;
-          }
-        }
-        acc = cfma(mval(r.val[e], w), zc, acc);
+      const bool use = qok[e] && cl >= 0;
+      const bool own = cl >= rs && cl < re;
+      rem[e] = use && !own;
+      idx[e] = rem[e] ? (size_t)cl : 0;
+      if (use && own) {
+        zv[e] = zs[cl - rs];
+      } else if (decltype(pre)::value) {
+        zv[e] = zr[e];
+      } else {
+        mb_raw(&mb[idx[e]], zv[e]);
       }
     }
+    mb_poll(mb, idx, rem, zv, p.err, nfallback);
+    cx<TP> acc = {TP(0), TP(0)};
+#pragma unroll
+    for (int e = 0; e < E; e++)
+      if (qok[e] && r.col[e] >= 0) acc = cfma(mval(r.val[e], w), zv[e], acc);
     acc = group_sum<LPR>(acc);
     if (lane == 0) {
       const int row = r.row;
@@ -438,12 +466,12 @@ __device__ void sweep_lds(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> *
   }
 }
 
-// Several columns (k > 1) share each sweep step: RL = LPR * k lanes per row, lane (c, l) takes
-// column c and entries l, l + LPR, ... of the row. The z of the block's last ring_w positions
-// (all columns) live in an LDS ring indexed by position; an entry whose source is a row of the
-// same block less than ring_w - 3*smax positions earlier reads the ring (its slot is rewritten at
-// least two steps later), an older one reads Z in global memory (ring_w >= 5*smax + 64, so it
-// was written at least two steps earlier), a row of another block reads the mailbox.
+// Several columns (2 <= k <= 128) share each sweep step: RL = LPR * k lanes per row, lane (c, l)
+// takes column c and entries l, l + LPR, ... of the row. The z of the block's last ring_w
+// positions (all columns) live in an LDS ring indexed by position; an entry whose source is a
+// row of the same block less than ring_w - 3*smax positions earlier reads the ring (its slot is
+// rewritten at least two steps later), an older one reads Z in global memory (ring_w >= 5*smax +
+// 64, so it was written at least two steps earlier), a row of another block reads the mailbox.
 // The same three-stage register pipeline as the single-column sweep hides the loads.
 template <typename TP, int LPR, int E>
 struct MRegs {
@@ -523,9 +551,15 @@ __device__ void sweep_ring(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> 
   __syncthreads();
   auto do_row = [&](int q, const MRegs<TP, LPR, E> &r, const cx<TP>(&zx)[E], cx<TP> zin, cx<TP> dd) {
     cx<TP> acc = {TP(0), TP(0)};
+    cx<TP> zv[E];
+    bool rem[E];
+    size_t idx[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
       const int cl = r.col[e];
+      rem[e] = qok[e] && cl >= 0 && r.epos[e] < 0;
+      idx[e] = rem[e] ? (size_t)cl * k + c : 0;
+      zv[e] = cx<TP>{TP(0), TP(0)};
       if (qok[e] && cl >= 0) {
         const int ps = r.epos[e];
         cx<TP> zc;
@@ -535,14 +569,14 @@ __device__ void sweep_ring(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> 
           zc = zx[e];
         } else {
           zc = zx[e];
-          if (mb_empty(zc)) {
-            zc = mb_get(&mb[(size_t)cl * k + c], p.err);
-            ++nfallback;
-          }
         }
-        acc = cfma(mval(r.val[e], w), zc, acc);
+        zv[e] = zc;
       }
     }
+    mb_poll(mb, idx, rem, zv, p.err, nfallback);
+#pragma unroll
+    for (int e = 0; e < E; e++)
+      if (qok[e] && r.col[e] >= 0) acc = cfma(mval(r.val[e], w), zv[e], acc);
     acc = group_sum<LPR>(acc);
     if (lane == 0) {
       const cx<TP> z = FWD ? cmul(zin - acc, dd) : zin - cmul(dd, acc);
@@ -651,10 +685,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
           rho_loc = addv(rho_loc, cmul(p.R[r], up(z)));
         }
       } else if (MULTI && p.ring_w > 0) {
+        const unsigned long long c0 = prof ? wall_clock64() : 0;
         sweep_ring<TP, 2, 8, true>(p, p.fw, zs, p.mbf, nfb);
+        const unsigned long long c1 = prof ? wall_clock64() : 0;
         grid_barrier(p.bar, phase);
         mb_clear_own(p.mbf, p.fw.exp, rs, re, k);
+        const unsigned long long c2 = prof ? wall_clock64() : 0;
         sweep_ring<TP, 2, 8, false>(p, p.bw, zs, p.mbb, nfb);
+        if (prof) {
+          const unsigned long long c3 = wall_clock64();
+          tfw += c1 - c0;
+          tbar += c2 - c1;
+          tbw += c3 - c2;
+        }
         __syncthreads();
         if (cvalid)
           for (long long row = rs + t / kp; row < re; row += 256 / kp) {
@@ -712,9 +755,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
 
     // ---- q = A p (:273-288) and c1 = p^T q (:293-296) ----
     cx<double> c1_loc = {0.0, 0.0};
-    for (long long row = slot; row < p.n; row += nslot) {
-      const int qb = p.rowptr[row], qe = p.rowptr[row + 1];
-      if (MULTI) {
+    if (MULTI) {
+      for (long long row = slot; row < p.n; row += nslot) {
+        const int qb = p.rowptr[row], qe = p.rowptr[row + 1];
         if (cvalid) {
           cx<double> acc = {0.0, 0.0};
           for (int q = qb; q < qe; ++q) acc = cfma(mval(p.val[q], w), p.P[(size_t)p.col[q] * k + c], acc);
@@ -722,13 +765,49 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
           p.Q[ix] = acc;
           c1_loc = addv(c1_loc, cmul(p.P[ix], acc));
         }
-      } else {
-        cx<double> acc = {0.0, 0.0};
-        for (int q = qb + lane; q < qe; q += LPR) acc = cfma(mval(p.val[q], w), p.P[p.col[q]], acc);
-        acc = group_sum<LPR>(acc);
-        if (lane == 0) {
-          p.Q[row] = acc;
-          c1_loc = addv(c1_loc, cmul(p.P[row], acc));
+      }
+    } else {
+      // SU rows of this slot in flight together: every load of the group is issued before the
+      // first one is waited for (one wave per SIMD has little else to hide memory latency);
+      // entries of a row are summed in the same order as a row-at-a-time loop
+      constexpr int SU = 4, SE = (24 + LPR - 1) / LPR;
+      for (long long row0 = slot; row0 < p.n; row0 += SU * nslot) {
+        int qb[SU], qe[SU];
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+          const long long r = row0 + u * nslot;
+          const long long rc = r < p.n ? r : p.n - 1;
+          qb[u] = p.rowptr[rc];
+          qe[u] = r < p.n ? p.rowptr[rc + 1] : qb[u];
+        }
+        cx<double> acc[SU];
+        cx<double> vv[SU][SE], pv[SU][SE];
+#pragma unroll
+        for (int u = 0; u < SU; u++)
+#pragma unroll
+          for (int e = 0; e < SE; e++) {
+            const int q = qb[u] + lane + e * LPR;
+            const int qc = q < qe[u] ? q : qb[u];
+            vv[u][e] = p.val[qc];
+            pv[u][e] = p.P[p.col[qc]];
+          }
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+          acc[u] = cx<double>{0.0, 0.0};
+#pragma unroll
+          for (int e = 0; e < SE; e++)
+            if (qb[u] + lane + e * LPR < qe[u]) acc[u] = cfma(mval(vv[u][e], w), pv[u][e], acc[u]);
+          for (int q = qb[u] + lane + SE * LPR; q < qe[u]; q += LPR)
+            acc[u] = cfma(mval(p.val[q], w), p.P[p.col[q]], acc[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+          const long long r = row0 + u * nslot;
+          const cx<double> a = group_sum<LPR>(acc[u]);
+          if (lane == 0 && r < p.n) {
+            p.Q[r] = a;
+            c1_loc = addv(c1_loc, cmul(p.P[r], a));
+          }
         }
       }
     }

@@ -1,22 +1,21 @@
 """Benchmark: edge-FEM solves/s to a 1e-8 relative residual on one MI355X.
 
-Workload: 64x64x48 cells of 2.5 km (1,179,648 tets, 1,407,152 edges), crust log10 sigma ~
-N(-2, 0.5) per cell plus 3 spheres of 1 S/m (seed 1002), sea 3.3 S/m, air 1e-8 S/m, tidal
-v x F source, tangential E = 0, SSOR-preconditioned COCG (the preconditioner the reference's
-solveCG uses; its 'IC' stagnates on this system). Mesh, pattern, assembly and boundary
-conditions are done before the timed region; inputs are resident in HBM.
+Workload (the 1.4M-edge scale-up case): 64x64x48 cells of 2.5 km (1,179,648 tets, 1,407,152
+edges), crust log10 sigma ~ N(-2, 0.5) per cell plus 3 spheres of 1 S/m (seed 1002), sea
+3.3 S/m, air 1e-8 S/m, tidal v x F source at M2, tangential E = 0, one right-hand side.
+Mesh, pattern, assembly, boundary conditions and the preconditioner setup happen before the
+timed region; inputs are resident in HBM.
 
-A step is one batched solve of the 5 tidal constituents (M2, S2, N2, K1, O1): 5 columns of
-one persistent launch sharing every matrix read, each column with its own omega (the matrix
-is held as (K, M) and K + i*omega*M is formed on the fly), all-FP64 COCG, each column brought
-to an FP64 true residual ||b - A x||/||b|| <= 1e-8 (recomputed at the end; the solve restarts
-from its iterate if the recurrence drifted). The complex-FP32 preconditioner variant is not
-used: FP32 rounding of the curl-curl entries breaks the discrete gradient null space and
-multiplies the iteration count on this system.
-The plain method -- all-FP64 COCG on the single M2 right-hand side from x0 = 0 -- is timed
-beside it in the same run and reported as `plain_fp64`.
+A step is one solve from x0 = 0 to an FP64 true residual ||b - A x||/||b|| <= 1e-8 (recomputed
+on the device at the end) with SSOR-preconditioned COCG in FP64 (SSOR is the preconditioner the
+reference's solveCG passes; its 'IC' stagnates on this system). The whole COCG iteration runs in
+one persistent launch per 2000 iterations.
 
-`python bench.py [--gpus 1] [--steps K] [--warmup W] [--size cfg2|cfg3|small]`
+Variants, off by default because each adds minutes (their results are documented):
+  --mixed    complex-FP32 preconditioner inside FP64 refinement (same tolerance);
+  --batched  the 5 tidal constituents (M2, S2, N2, K1, O1) in one launch, per-column omega.
+
+`python bench.py [--gpus 1] [--steps K] [--warmup W] [--size cfg2|cfg3|small] [--mixed] [--batched]`
 """
 from __future__ import annotations
 
@@ -54,14 +53,30 @@ def bytes_per_iteration(n, nU, nL, k, z_bytes):
     return spmv + sweeps + vec
 
 
+def measured_traffic(workload, iterations_per_launch):
+    """HBM bytes per launch of k_cocg from the committed rocprofv3 --pmc summary of this workload
+    (FETCH_SIZE and WRITE_SIZE per COCG iteration, scaled by the counters' reading for a copy of
+    known size), or None when no summary exists."""
+    path = os.path.join(ROOT, "profiles", f"k_cocg_{workload}_pmc.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        m = json.load(f)
+    return int(m["hbm_bytes_per_iteration"] * iterations_per_launch), os.path.relpath(path, ROOT)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--size", default="cfg2", choices=sorted(SIZES))
-    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mixed", action="store_true", help="also time the complex-FP32-preconditioner solve")
+    ap.add_argument("--batched", action="store_true", help="also time the 5 constituents in one launch")
+    ap.add_argument("--maxit", type=int, default=400000, help="iteration cap of each solve")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="after the solves, copy this many bytes device-to-device (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: each system is solved on a single MI355X; several GPUs would "
@@ -85,63 +100,83 @@ def main():
     s, b, bc = afx.forward_system(case, w0)
     log(f"system: n={s.n} nU={s.npu} levels L/U={s.nlev_l}/{s.nlev_u} ({time.time() - t0:.1f}s)")
     n = s.n
-    k = len(CONSTITUENTS)
-    dB1 = torch.from_numpy(b.astype(np.complex128)).to(dev)
-    dBk = dB1.reshape(n, 1).repeat(1, k).contiguous()
-    scale = omegas / w0
+    dB = torch.from_numpy(b.astype(np.complex128)).to(dev)
     s.precond_setup("SSOR")
     s.autotune(1)
     torch.cuda.synchronize()
-    opts = dict(precond="SSOR", tol=1e-8, maxit=400000, chunk=2000, precision=afx.FP64, max_outer=4)
+    opts = dict(precond="SSOR", tol=1e-8, maxit=args.maxit, chunk=2000, precision=afx.FP64)
 
-    dX = torch.zeros((n, k), dtype=torch.complex128, device=dev)
+    dX = torch.zeros(n, dtype=torch.complex128, device=dev)
     for i in range(args.warmup):
         dX.zero_()
-        s.solve_device(dBk, dX, omega_scale=scale, **opts)
+        s.solve_device(dB, dX, **opts)
+    s.kernel_timing(reset=True)
     torch.cuda.synchronize()
-    steps = []
+    stats = []
     t_all = time.perf_counter()
     for i in range(args.steps):
         dX.zero_()
-        st, rc = s.solve_device(dBk, dX, omega_scale=scale, **opts)
-        steps.append(st)
+        st, rc = s.solve_device(dB, dX, **opts)
+        stats.append(st[0])
     torch.cuda.synchronize()
-    t_fast = time.perf_counter() - t_all
-    res_fast = s.true_residual_device(dBk, dX, omega_scale=scale)
-    solves = args.steps * k
-    value = solves / t_fast
-    s.kernel_timing(reset=True)
-    log(f"batched FP64: {solves} solves in {t_fast:.2f}s, iters={[x.iter for x in steps[-1]]}, "
-        f"true residual max={res_fast.max():.2e}")
+    t_run = time.perf_counter() - t_all
+    launches, iters, kms = s.kernel_timing(reset=True)
+    true_res = float(s.true_residual_device(dB, dX)[0])
+    value = args.steps / t_run
+    its = stats[-1].iter
+    log(f"FP64 SSOR-COCG: {args.steps} solve(s) in {t_run:.2f}s, {its} iterations, true residual "
+        f"{true_res:.2e}; k_cocg {launches} launches {kms:.0f} ms")
 
-    plain = None
-    roof = None
-    if not args.no_plain:
-        dx1 = torch.zeros(n, dtype=torch.complex128, device=dev)
+    # roofline of the persistent COCG kernel: algorithmic bytes of the iterations each launch ran
+    # over the launch's measured duration (HIP events on the library's stream)
+    by = bytes_per_iteration(n, s.npu, s.npl, 1, 16)
+    achieved = by * iters / (kms / 1e3) / 1e9
+    ipl = iters / max(launches, 1)
+    traffic, tsrc = measured_traffic(f"{nx}x{ny}x{nz}", ipl)
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                kernel="k_cocg (persistent COCG, FP64, 1 column)",
+                bytes_per_iteration=by, avg_launch_ms=round(kms / max(launches, 1), 3),
+                iterations_per_launch=round(ipl, 1), us_per_iteration=round(1e3 * kms / max(iters, 1), 1),
+                traffic_source=tsrc)
+
+    variants = {}
+    if args.mixed:
+        dXm = torch.zeros(n, dtype=torch.complex128, device=dev)
         torch.cuda.synchronize()
-        tp = time.perf_counter()
-        stp, _ = s.solve_device(dB1, dx1, precond="SSOR", tol=1e-8, maxit=400000, chunk=2000, precision=afx.FP64)
+        tm = time.perf_counter()
+        stm, rcm = s.solve_device(dB, dXm, **dict(opts, precision=afx.MIXED, max_outer=8))
         torch.cuda.synchronize()
-        tp = time.perf_counter() - tp
-        launches, iters, kms = s.kernel_timing(reset=True)
-        plain = dict(solves_per_s=round(1.0 / tp, 6), iters=stp[0].iter, true_resid=stp[0].true_resid,
-                     seconds=round(tp, 2), kernel_launches=launches, kernel_ms=round(kms, 1))
-        log(f"plain FP64: {tp:.2f}s, {stp[0].iter} iters, true residual {stp[0].true_resid:.2e}, "
-            f"k_cocg {launches} launches {kms:.0f} ms")
-        # x solutions of the plain and batched paths agree at the M2 column
-        # roofline of the persistent COCG kernel: algorithmic bytes of the iterations each launch
-        # ran / the launch's measured duration (HIP events on the library's stream)
-        by = bytes_per_iteration(n, s.npu, s.npl, 1, 16)
-        achieved = by * iters / (kms / 1e3) / 1e9
-        roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_cocg (persistent COCG, FP64, 1 column)",
-                    bytes_per_iteration=by, avg_launch_ms=round(kms / max(launches, 1), 3),
-                    iterations_per_launch=round(iters / max(launches, 1), 1))
+        tm = time.perf_counter() - tm
+        variants["mixed_fp32_preconditioner"] = dict(
+            solves_per_s=round(1.0 / tm, 6), seconds=round(tm, 2), iters=stm[0].iter, outer=stm[0].outer,
+            true_resid=float(s.true_residual_device(dB, dXm)[0]), rc=rcm)
+        log(f"mixed: {tm:.2f}s, {stm[0].iter} iterations")
+    if args.batched:
+        k = len(CONSTITUENTS)
+        dBk = dB.reshape(n, 1).repeat(1, k).contiguous()
+        dXk = torch.zeros((n, k), dtype=torch.complex128, device=dev)
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        stb, rcb = s.solve_device(dBk, dXk, omega_scale=omegas / w0, **opts)
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - tb
+        variants["batched_5_constituents"] = dict(
+            solves_per_s=round(k / tb, 6), seconds=round(tb, 2), iters=[x.iter for x in stb],
+            true_resid_max=float(s.true_residual_device(dBk, dXk, omega_scale=omegas / w0).max()))
+        log(f"batched: {k} solves in {tb:.2f}s, iterations {[x.iter for x in stb]}")
+
+    if args.copy_bytes:
+        src = torch.empty(args.copy_bytes // 8, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy of {args.copy_bytes} bytes done")
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(case, w0, plain["iters"] if plain else max(x.iter for x in steps[-1]))
+        cpu = cpu_baseline(case, w0, its)
 
     out = {
         "metric": "edge-FEM solves/s to 1e-8 rel. residual, warm-start mixed vs plain COCG, 1 GPU",
@@ -150,28 +185,27 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * t_fast / args.steps, 1),
+        "ms_per_step": round(1e3 * t_run / args.steps, 1),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c128",
         "data": "synthetic",
-        "config": {"workload": f"{nx}x{ny}x{nz} cells ({case.ntet} tets, {case.nline} edges), 5 tidal "
-                               f"constituents batched in one launch, SSOR-COCG to 1e-8 true residual, x0 = 0",
-                   "edges": case.nline, "tets": case.ntet, "rhs_per_step": k, "parallelism": "1 GPU"},
-        "plain_fp64": plain,
-        "speedup_vs_plain": round(value / plain["solves_per_s"], 3) if plain else None,
-        "iters_batched": [x.iter for x in steps[-1]],
-        "true_residual_max": float(res_fast.max()),
+        "config": {"workload": f"{nx}x{ny}x{nz} cells ({case.ntet} tets, {case.nline} edges), one M2 right-hand "
+                               f"side, FP64 SSOR-COCG from x0 = 0 to a 1e-8 true residual",
+                   "edges": case.nline, "tets": case.ntet, "rhs_per_step": 1, "parallelism": "1 GPU"},
+        "iters": its,
+        "true_residual": true_res,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "variants": variants or None,
     }
     print(json.dumps(out), flush=True)
 
 
 def cpu_baseline(case, omega, gpu_iters):
-    """The oracle's serial COCG13 restatement (C, 1 core) on the same system: a bounded sample of
-    iterations, scaled to the iteration count the GPU needed for 1e-8."""
+    """The oracle's serial COCG13_single restatement (C, 1 core) on the same system: a bounded
+    sample of iterations, scaled to the iteration count the GPU needed for 1e-8."""
     from oracle import oracle as O
     t0 = time.time()
     o = O.forward_system(case, omega)

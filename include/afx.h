@@ -213,6 +213,34 @@ int afx_adjoint_rhs_device(afx_system s, int nobs, const int *stack, const int *
 int afx_jacobian_device(afx_system s, int nobs, const void *dU, const void *dx, int npar, const int *par_ptr,
                         const int *par_tet, const double *sigma_tet, double omega, double dlnsig_dm, void *dJ);
 
+/* ---- Gauss-Newton model update (inversion step after the Jacobian) --------------------- */
+
+/* genjacobian1's conversion of the adjoint accumulation into real Jacobian rows
+   (src/src_inv_ap/m_jacobian_ap.f90:338-345): for dJ (npar, nobs) complex device (o fastest,
+   the afx_jacobian_device output) and the observed fields dbe (nobs complex, device),
+   dout[0][o][m] = Re(dJ[m][o] / be[o]) / ln 10 and dout[1][o][m] = Im(dJ[m][o] / be[o]) * 180/pi
+   (device, [2][nobs][npar] doubles): the d log10|be| / dm and d phase[deg] / dm rows that
+   genjacobian2 stacks into JJ (:393-482). */
+int afx_jacobian_ampphase_device(int npar, int nobs, const void *dJ, const void *dbe, double *dout);
+
+/* getnewmodel (src/src_inv_ap/m_jacobian_ap.f90:623-787) with solvebeta: dJ the dense real
+   Jacobian (ndat, nmodel) row-major on the device; Cm (nmodel x nmodel, symmetric) as CSR
+   (cm_ptr 0-based offsets, cm_col 0-based), cd (ndat) the diagonal of Cd, all host;
+     C = J Cm J^T + alpha Cd (FP64 MFMA), X = dobs - dcal + J (model - model_ref),
+     C beta = X (device Cholesky), model_new = model_ref + Cm J^T beta.
+   beta_out (ndat, host) may be NULL. Returns AFX_ERR_BREAKDOWN when C is not positive definite. */
+int afx_gn_update_device(int ndat, int nmodel, const double *dJ, const int *cm_ptr, const int *cm_col,
+                         const double *cm_val, const double *cd, double alpha, const double *dobs, const double *dcal,
+                         const double *model, const double *model_ref, double *model_new, double *beta_out);
+
+/* C (M, N) = A (M, K) B (N, K)^T (+ C if accumulate), FP64 on MFMA, device row-major arrays --
+   the dense contraction of the update, exposed for its parity test. */
+int afx_dgemm_nt_device(int M, int N, int K, const double *dA, int lda, const double *dB, int ldb, double *dC, int ldc,
+                        int accumulate);
+
+/* message of the last failure of the three calls above on the calling thread */
+const char *afx_inv_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,12 @@ int next_pow2(int k) {
 struct DevSide {
   int *bstep = nullptr, *spos = nullptr, *row = nullptr, *ptr = nullptr, *col = nullptr, *map = nullptr;
   unsigned char *exp = nullptr, *expp = nullptr;
+  int *rowx = nullptr;
+  int maxsteps = 0;
+  int npos = 0;  // positions (the dummy position of the single-column layout is npos)
+  int *psrc = nullptr, *pmap = nullptr, *prow = nullptr;
+  cx<double> *pval64 = nullptr;
+  cx<float> *pval32 = nullptr;
   cx<double> *val64 = nullptr;
   cx<float> *val32 = nullptr;
   long long nnz = 0;
@@ -285,6 +291,9 @@ int ensure_derived(afx_system s) {
     HIPCHK(afx::launch_gather_vals((int)d->nnz, d->map, s->d_canon, d->val64, d->val32, s->stream));
     HIPCHK(afx::launch_gather_vals(s->n * d->ew, d->emap, s->d_canon, d->eval64, d->eval32, s->stream));
   }
+  for (DevSide *d : {&s->fw, &s->bw, &s->bjfw, &s->bjbw})
+    if (d->pmap)
+      HIPCHK(afx::launch_gather_vals((s->n + 1) * AFX_PIPE_W, d->pmap, s->d_canon, d->pval64, d->pval32, s->stream));
   s->derived_valid = true;
   return AFX_OK;
 }
@@ -361,7 +370,11 @@ void column_freqs(afx_system s, int k, const double *scale, std::vector<double> 
 struct SideHost {
   std::vector<int> bstep, spos, row, ptr, col, map;
   std::vector<unsigned char> exp, expp;
+  std::vector<int> rowx;  // row | (exported << 31) by position
   int maxsteps = 0;
+  // single-column sweep layout, positions 0..n-1 plus a dummy position n (see SweepSide)
+  bool pipe = false;
+  std::vector<int> psrc, pmap, prow;
   // fixed-width (padded) entries; epos: position of a same-block column row, else -1
   int ew = 0, smax = 0;
   std::vector<int> ecol, emap, epos;
@@ -388,13 +401,22 @@ void build_side(int n, int G, const std::vector<int> &rstart, const std::vector<
     qmap.resize(C0.size());
     for (size_t q = 0; q < C0.size(); q++) qmap[q] = (int)q;
   }
+  // Schedule key of each row: the longest dependency chain ending at it, where a dependency on
+  // a row of the same block counts 1 and one on another block's row counts `rw` (a mailbox hop
+  // costs several block steps). Rows of a block run in increasing key, so a row whose input
+  // comes from another block is placed after the block's independent work instead of stalling
+  // the step it would share with that work. Keys increase along every dependency, so rows of
+  // one key are independent and the smallest pending key of any block can always proceed.
+  int rw = 1;
+  if (const char *e = getenv("AFX_HOP_WEIGHT")) rw = std::max(1, std::min(atoi(e), 64));
   std::vector<int> lev(n, 0);
+  auto wdep = [&](int i, int j) { return blk[i] == blk[j] ? 1 : rw; };
   if (fwd) {
     for (int i = 0; i < n; i++)
-      for (int q = P[i]; q < P[i + 1]; q++) lev[i] = std::max(lev[i], lev[C1[q] - 1] + 1);
+      for (int q = P[i]; q < P[i + 1]; q++) lev[i] = std::max(lev[i], lev[C1[q] - 1] + wdep(i, C1[q] - 1));
   } else {
     for (int i = n - 1; i >= 0; i--)
-      for (int q = P[i]; q < P[i + 1]; q++) lev[i] = std::max(lev[i], lev[C1[q] - 1] + 1);
+      for (int q = P[i]; q < P[i + 1]; q++) lev[i] = std::max(lev[i], lev[C1[q] - 1] + wdep(i, C1[q] - 1));
   }
   h.exp.assign(n, 0);
   for (int i = 0; i < n; i++)
@@ -413,15 +435,19 @@ void build_side(int n, int G, const std::vector<int> &rstart, const std::vector<
     std::vector<int> rows;
     for (int i = rstart[b]; i < rstart[b + 1]; i++) rows.push_back(i);
     std::stable_sort(rows.begin(), rows.end(), [&](int x, int y) { return lev[x] < lev[y]; });
-    int prev = -1, ls = -1;
+    // a step holds rows of one level, at most AFX_STEP_ROWS of them (the single-column sweep
+    // gives each row of a step its own 8-lane group of the block)
+    int prev = -1, ls = -1, cnt = 0;
     for (int i : rows) {
-      if (lev[i] != prev) {
+      if (lev[i] != prev || cnt == AFX_STEP_ROWS) {
+        cnt = 0;
         h.spos.push_back((int)h.row.size());
         step_blk.push_back(b);
         prev = lev[i];
         ++ls;
       }
       lstep[i] = ls;
+      ++cnt;
       h.row.push_back(i);
     }
     h.maxsteps = std::max(h.maxsteps, ls + 1);
@@ -443,9 +469,12 @@ void build_side(int n, int G, const std::vector<int> &rstart, const std::vector<
   h.ptr[n] = (int)h.col.size();
   h.expp.resize(n);
   for (int pos = 0; pos < n; pos++) h.expp[pos] = h.exp[h.row[pos]];
+  h.rowx.resize(n);
+  for (int pos = 0; pos < n; pos++) h.rowx[pos] = (int)((unsigned)h.row[pos] | ((unsigned)h.expp[pos] << 31));
+  // even width (>= 2): a lane of the single-column sweep loads an aligned pair of entries
   h.ew = 0;
   for (int pos = 0; pos < n; pos++) h.ew = std::max(h.ew, h.ptr[pos + 1] - h.ptr[pos]);
-  h.ew = std::max(h.ew, 1);
+  h.ew = std::max(2, (h.ew + 1) & ~1);
   h.ecol.assign((size_t)n * h.ew, -1);
   h.emap.assign((size_t)n * h.ew, -1);
   h.epos.assign((size_t)n * h.ew, -1);
@@ -459,6 +488,33 @@ void build_side(int n, int G, const std::vector<int> &rstart, const std::vector<
       h.emap[(size_t)pos * h.ew + (q - h.ptr[pos])] = h.map[q];
       if (blk[h.col[q]] == blk[h.row[pos]]) h.epos[(size_t)pos * h.ew + (q - h.ptr[pos])] = rowpos[h.col[q]];
     }
+  // Single-column sweep layout: AFX_PIPE_W entries per position, entry source src >= 0 the LDS
+  // slot of a row of the same block, src < 0 the mailbox slot ~src of another block's row;
+  // padding reads the zero dummy slot (index = largest block) with value 0. prow = {row |
+  // exported << 31, LDS slot}. Position n is a dummy row (all padding, DD 0, not exported) that
+  // fills the unused row slots of a step, so the kernel needs no validity tests.
+  h.pipe = h.ew <= AFX_PIPE_W;
+  if (h.pipe) {
+    int dslot = 0;
+    for (int b = 0; b < G; b++) dslot = std::max(dslot, rstart[b + 1] - rstart[b]);
+    const size_t W = AFX_PIPE_W;
+    h.psrc.assign((size_t)(n + 1) * W, dslot);
+    h.pmap.assign((size_t)(n + 1) * W, -1);
+    h.prow.assign((size_t)(n + 1) * 2, 0);
+    for (int pos = 0; pos < n; pos++) {
+      const int i = h.row[pos], bi = blk[i];
+      h.prow[2 * (size_t)pos] = h.rowx[pos];
+      h.prow[2 * (size_t)pos + 1] = i - rstart[bi];
+      for (int q = h.ptr[pos]; q < h.ptr[pos + 1]; q++) {
+        const int j = h.col[q];
+        const size_t e = (size_t)pos * W + (q - h.ptr[pos]);
+        h.psrc[e] = (blk[j] == bi) ? j - rstart[bi] : ~j;
+        h.pmap[e] = h.map[q];
+      }
+    }
+    h.prow[2 * (size_t)n] = 0;
+    h.prow[2 * (size_t)n + 1] = dslot;
+  }
 }
 
 int upload_side(const SideHost &h, DevSide &d) {
@@ -466,25 +522,39 @@ int upload_side(const SideHost &h, DevSide &d) {
   if ((rc = upload(&d.bstep, h.bstep.data(), h.bstep.size())) || (rc = upload(&d.spos, h.spos.data(), h.spos.size())) ||
       (rc = upload(&d.row, h.row.data(), h.row.size())) || (rc = upload(&d.ptr, h.ptr.data(), h.ptr.size())) ||
       (rc = upload(&d.col, h.col.data(), h.col.size())) || (rc = upload(&d.map, h.map.data(), h.map.size())) ||
-      (rc = upload(&d.exp, h.exp.data(), h.exp.size())) || (rc = upload(&d.expp, h.expp.data(), h.expp.size())) || (rc = dalloc(&d.val64, h.col.size())) ||
+      (rc = upload(&d.exp, h.exp.data(), h.exp.size())) || (rc = upload(&d.expp, h.expp.data(), h.expp.size())) ||
+      (rc = upload(&d.rowx, h.rowx.data(), h.rowx.size())) || (rc = dalloc(&d.val64, h.col.size())) ||
       (rc = dalloc(&d.val32, h.col.size())))
     return rc;
   d.nnz = (long long)h.col.size();
   d.ew = h.ew;
+  d.maxsteps = h.maxsteps;
+  d.npos = (int)h.row.size();
   const size_t ne = h.ecol.size();
   d.smax = h.smax;
   if ((rc = upload(&d.ecol, h.ecol.data(), ne)) || (rc = upload(&d.emap, h.emap.data(), ne)) ||
       (rc = upload(&d.epos, h.epos.data(), ne)) ||
-      (rc = dalloc(&d.eval64, ne)) || (rc = dalloc(&d.eval32, ne)) || (rc = dalloc(&d.dpos64, h.row.size())) ||
-      (rc = dalloc(&d.dpos32, h.row.size())))
+      (rc = dalloc(&d.eval64, ne)) || (rc = dalloc(&d.eval32, ne)) || (rc = dalloc(&d.dpos64, h.row.size() + 1)) ||
+      (rc = dalloc(&d.dpos32, h.row.size() + 1)))
     return rc;
+  // the dummy position's DD stays zero
+  HIPCHK(hipMemset(d.dpos64, 0, (h.row.size() + 1) * sizeof(cx<double>)));
+  HIPCHK(hipMemset(d.dpos32, 0, (h.row.size() + 1) * sizeof(cx<float>)));
+  if (h.pipe) {
+    const size_t np = h.psrc.size();
+    if ((rc = upload(&d.psrc, h.psrc.data(), np)) || (rc = upload(&d.pmap, h.pmap.data(), np)) ||
+        (rc = upload(&d.prow, h.prow.data(), h.prow.size())) || (rc = dalloc(&d.pval64, np)) ||
+        (rc = dalloc(&d.pval32, np)))
+      return rc;
+  }
   return AFX_OK;
 }
 
 void free_side(DevSide &d) {
   dfree(d.bstep); dfree(d.spos); dfree(d.row); dfree(d.ptr); dfree(d.col); dfree(d.map);
-  dfree(d.exp); dfree(d.expp); dfree(d.val64); dfree(d.val32);
+  dfree(d.exp); dfree(d.expp); dfree(d.rowx); dfree(d.val64); dfree(d.val32);
   dfree(d.ecol); dfree(d.emap); dfree(d.epos); dfree(d.eval64); dfree(d.eval32); dfree(d.dpos64); dfree(d.dpos32);
+  dfree(d.psrc); dfree(d.pmap); dfree(d.prow); dfree(d.pval64); dfree(d.pval32);
 }
 
 afx::SweepSide side_view(const DevSide &d, bool fp64) {
@@ -493,6 +563,12 @@ afx::SweepSide side_view(const DevSide &d, bool fp64) {
   v.val = fp64 ? (const void *)d.val64 : (const void *)d.val32;
   v.exp = d.exp;
   v.expp = d.expp;
+  v.rowx = d.rowx;
+  v.maxsteps = d.maxsteps;
+  v.npos = d.npos;
+  v.psrc = getenv("AFX_NO_PIPE") ? nullptr : d.psrc;
+  v.prow = d.prow;
+  v.pval = fp64 ? (const void *)d.pval64 : (const void *)d.pval32;
   // the fixed-width path needs at most 16 entries per position (the E * lanes of the kernel)
   const bool ell = d.ew <= 16 && !getenv("AFX_NO_ELL");
   v.ew = d.ew;
@@ -509,8 +585,12 @@ int build_sweeps(afx_system s) {
   int G = std::max(1, std::min(s->ncu, (n + 63) / 64));
   if (const char *e = getenv("AFX_SWEEP_BLOCKS")) G = std::max(1, std::min(atoi(e), n));
   s->Gs = G;
+  // the sweeps may use fewer row slabs than the launch has blocks (a slab boundary costs a
+  // mailbox hand-off on the sweep's critical path); blocks past the last slab own no rows
+  int Gp = G;
+  if (const char *e = getenv("AFX_SWEEP_PARTS")) Gp = std::max(1, std::min(atoi(e), G));
   std::vector<int> rstart(G + 1), blk(n);
-  for (int b = 0; b <= G; b++) rstart[b] = (int)((long long)b * n / G);
+  for (int b = 0; b <= G; b++) rstart[b] = (int)((long long)std::min(b, Gp) * n / Gp);
   for (int b = 0; b < G; b++)
     for (int i = rstart[b]; i < rstart[b + 1]; i++) blk[i] = b;
   SideHost fw, bw;
@@ -594,9 +674,13 @@ int lds_rows_for(afx_system s, bool multi) {
   int dev = 0, maxlds = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
-  const size_t need = (size_t)s->rows_per_block * sizeof(cx<TP>) + 6144;
+  // z of the block's rows, the staged step bounds of both sweep directions, the static LDS
+  const int ms = std::max(s->fw.maxsteps + s->bw.maxsteps, s->bjfw.maxsteps + s->bjbw.maxsteps) + 2;
+  const size_t need =
+      (((size_t)(s->rows_per_block + 1) * sizeof(cx<TP>) + 15) & ~(size_t)15) + (size_t)ms * sizeof(int) + 6144;
   if (getenv("AFX_NO_LDS")) return 0;
-  return (need <= (size_t)maxlds) ? s->rows_per_block : 0;
+  // one slot past the largest block: the zero slot that padding entries read
+  return (need <= (size_t)maxlds) ? s->rows_per_block + 1 : 0;
 }
 
 // LDS ring (positions) of the several-column sweep: the largest power of two whose k columns
@@ -671,10 +755,12 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   const bool bj = (pc == AFX_PC_SSOR_BJ);
   p.fw = side_view(bj ? s->bjfw : s->fw, fp64);
   p.bw = side_view(bj ? s->bjbw : s->bw, fp64);
-  // sweep mailboxes, every slot empty at the start of the solve (kept empty between iterations)
+  // sweep mailboxes, every slot empty at the start of the solve (kept empty between iterations);
+  // one row more than n: the single-column sweep's unused loads may read slot n
   if (pc == AFX_PC_IC || pc == AFX_PC_SSOR || pc == AFX_PC_SSOR_BJ) {
-    if ((rc = dalloc(&w.mbf, (size_t)s->n * k)) || (rc = dalloc(&w.mbb, (size_t)s->n * k))) return rc;
-    const long long words = (long long)s->n * k * (long long)sizeof(cx<TP>) / 8;
+    const size_t slots = ((size_t)s->n + 1) * k;
+    if ((rc = dalloc(&w.mbf, slots)) || (rc = dalloc(&w.mbb, slots))) return rc;
+    const long long words = (long long)slots * (long long)sizeof(cx<TP>) / 8;
     HIPCHK(afx::launch_mb_fill(words, (unsigned long long *)w.mbf, s->stream));
     HIPCHK(afx::launch_mb_fill(words, (unsigned long long *)w.mbb, s->stream));
   }
@@ -759,17 +845,19 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
     std::vector<unsigned long long> h((size_t)G * 8);
     HIPCHK(hipMemcpy(h.data(), dbg, h.size() * 8, hipMemcpyDeviceToHost));
     const double its = std::max(1, w.h_status[0]);
-    double mean[6] = {0, 0, 0, 0, 0, 0}, mx[6] = {0, 0, 0, 0, 0, 0};
+    double mean[8] = {0, 0, 0, 0, 0, 0, 0, 0}, mx[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int b = 0; b < G; b++)
-      for (int q = 0; q < 6; q++) {
-        const double v = (double)h[(size_t)b * 8 + q] / its * (q < 5 ? 0.01 : 1.0);  // 100 MHz -> us
+      for (int q = 0; q < 8; q++) {
+        const double v = (double)h[(size_t)b * 8 + q] / its * (q != 5 ? 0.01 : 1.0);  // 100 MHz -> us
         mean[q] += v / G;
         mx[q] = std::max(mx[q], v);
       }
     fprintf(stderr,
             "afx profile (us/iteration, mean/max over %d blocks): fw sweep %.1f/%.1f  barrier %.1f/%.1f  "
-            "bw sweep %.1f/%.1f  rest %.1f/%.1f  total %.1f/%.1f  mailbox polls/iteration/block %.1f/%.0f\n",
-            G, mean[0], mx[0], mean[1], mx[1], mean[2], mx[2], mean[3], mx[3], mean[4], mx[4], mean[5], mx[5]);
+            "bw sweep %.1f/%.1f  rest %.1f/%.1f (rho wait %.1f/%.1f, after rho %.1f/%.1f)  total %.1f/%.1f  "
+            "mailbox polls/iteration/block %.1f/%.0f\n",
+            G, mean[0], mx[0], mean[1], mx[1], mean[2], mx[2], mean[3], mx[3], mean[6], mx[6], mean[7], mx[7],
+            mean[4], mx[4], mean[5], mx[5]);
   }
   int herr = 0;
   HIPCHK(hipMemcpy(&herr, s->d_err, sizeof(int), hipMemcpyDeviceToHost));

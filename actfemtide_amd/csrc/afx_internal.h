@@ -11,6 +11,11 @@
 
 namespace afx {
 
+// most rows in one step of a sweep (the single-column sweep gives each its own 8-lane group)
+#define AFX_STEP_ROWS 32
+// entries per position of the single-column sweep layout (8 lanes x 2)
+#define AFX_PIPE_W 16
+
 // most columns the LDS-ring sweep takes in one launch (2 lanes per row and column)
 #ifndef AFX_RING_KC
 #define AFX_RING_KC 128
@@ -34,6 +39,8 @@ struct SweepSide {
   const void *val;   // [nnz_side] cx<TP> (K, M) values
   const unsigned char *exp;  // [n] by row id: 1: another block reads this row's z
   const unsigned char *expp; // [n] the same flag by position
+  const int *rowx;           // [n] row | (exported << 31) by position
+  int maxsteps;              // most steps of one block
   // fixed-width copy for the single-column LDS path: entry e of position pos at pos*ew + e
   int ew;                    // entries per position (padding: column -1, value 0)
   const int *ecol;
@@ -42,6 +49,14 @@ struct SweepSide {
   const int *epos;           // [n * ew] position of the entry's column row when it is a row of the
                              // same block, -1 otherwise (ring-buffer sweep of several columns)
   int smax;                  // largest number of positions in one step
+  // single-column sweep layout (null when a row has more than AFX_PIPE_W entries): per position
+  // AFX_PIPE_W entry sources (>= 0: LDS slot of a same-block row; < 0: mailbox slot ~src;
+  // padding: the zero LDS slot) and values, and prow = {row | exported << 31, LDS slot}; position
+  // npos is a dummy row that writes only the zero slot
+  int npos;
+  const int *psrc;
+  const int *prow;
+  const void *pval;          // cx<TP>
 };
 
 template <typename TP>

@@ -22,7 +22,8 @@ typedef double double4v __attribute__((ext_vector_type(4)));
 
 // C[m][n] = sum_k A[m][k] B[n][k] (+ C when accumulate), FP64. One wave computes a 32x32 tile as
 // 2x2 MFMA 16x16 tiles; 4 waves per block cover 64x64. Operand layout of the 16x16x4 MFMA: lane l
-// supplies A[row l%16][k l/16] and B[k l/16][col l%16]; it holds D[4*(l/16)+i][l%16], i = 0..3.
+// supplies A[row l%16][k l/16] and B[k l/16][col l%16]; it holds D[l/16 + 4*i][l%16], i = 0..3 (the
+// FP64 variant interleaves rows, unlike the 4*(l/16)+i of the FP32 16x16 forms).
 __global__ void __launch_bounds__(256) k_dgemm_nt(const double *__restrict__ A, int lda, const double *__restrict__ B,
                                                   int ldb, int M, int N, int K, double *__restrict__ C, int ldc,
                                                   int accumulate) {
@@ -54,7 +55,7 @@ __global__ void __launch_bounds__(256) k_dgemm_nt(const double *__restrict__ A, 
     for (int j = 0; j < 2; j++)
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        const int m = m0 + 16 * i + 4 * kk + q, n = n0 + 16 * j + r;
+        const int m = m0 + 16 * i + kk + 4 * q, n = n0 + 16 * j + r;
         if (m < M && n < N) {
           double *c = C + (size_t)m * ldc + n;
           *c = accumulate ? *c + acc[i][j][q] : acc[i][j][q];

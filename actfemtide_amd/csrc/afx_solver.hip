@@ -81,15 +81,42 @@ __device__ __forceinline__ cx<double> cdivd(cx<double> a, cx<double> b) {
   r.i = scalbn((__dmul_rn(a.i, c) - __dmul_rn(a.r, d)) / den, -e);
   return r;
 }
-__device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 64); }
-__device__ __forceinline__ float shfl_xor_d(float v, int m) { return __shfl_xor(v, m, 64); }
+// Lane exchange inside aligned groups of 2..16 lanes through DPP (no LDS round trip).
+// (bound_ctrl set: no old value to merge; every source lane of these patterns is in the group)
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xF, 0xF, true);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
 
+// Sum over an aligned group of LPR lanes, every lane of the group getting the same bits:
+// quad_perm [1,0,3,2] and [2,3,0,1] (xor 1, xor 2), then row_half_mirror (lane i <- 7-i: the
+// other quad of 8) and row_mirror (lane i <- 15-i: the other half of 16). Every group's lanes
+// must execute it together.
 template <int LPR, typename T>
 __device__ __forceinline__ cx<T> group_sum(cx<T> v) {
-#pragma unroll
-  for (int m = LPR / 2; m >= 1; m >>= 1) {
-    v.r += shfl_xor_d(v.r, m);
-    v.i += shfl_xor_d(v.i, m);
+  static_assert(LPR == 1 || LPR == 2 || LPR == 4 || LPR == 8 || LPR == 16, "group of 1..16 lanes");
+  if constexpr (LPR >= 2) {
+    v.r += dpp_mov<0xB1>(v.r);
+    v.i += dpp_mov<0xB1>(v.i);
+  }
+  if constexpr (LPR >= 4) {
+    v.r += dpp_mov<0x4E>(v.r);
+    v.i += dpp_mov<0x4E>(v.i);
+  }
+  if constexpr (LPR >= 8) {
+    v.r += dpp_mov<0x141>(v.r);
+    v.i += dpp_mov<0x141>(v.i);
+  }
+  if constexpr (LPR >= 16) {
+    v.r += dpp_mov<0x140>(v.r);
+    v.i += dpp_mov<0x140>(v.i);
   }
   return v;
 }
@@ -288,25 +315,6 @@ __device__ void sweep(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> *zs, 
   }
 }
 
-// Single-column sweep with the block's z (and, on the forward pass, r) staged in LDS.
-// zs[row - rs] holds r_i until row i is processed (forward) and z_i afterwards.
-// Software pipeline, per step s of the block:
-//   * the step bounds of s+4 are loaded (scalar loads; the step barrier waits for them),
-//   * the fixed-width entries of step s+2 are loaded into registers,
-//   * the mailbox slots that step s+1 reads from other blocks are loaded raw (no wait): these
-//     loads are issued before step s posts its own z, so waiting for them never waits for a
-//     write-through store to complete; a slot still empty when step s+1 uses it is polled.
-// In steady state a step then costs one LDS round trip, the row arithmetic and the barrier:
-// the mailbox latency is paid once per chain of blocks, not once per dependency level.
-template <typename TP, int LPR, int E>
-struct StepRegs {
-  int row;
-  int exp;  // another block reads this row's z
-  int col[E];
-  cx<TP> val[E];
-  cx<TP> dd;
-};
-
 // raw (non-polling) mailbox loads of the remote entries of a prefetched row
 __device__ __forceinline__ void mb_raw(const cx<double> *p, cx<double> &v) {
   const unsigned long long *q = (const unsigned long long *)p;
@@ -350,119 +358,148 @@ __device__ __forceinline__ void mb_poll(const cx<TP> *mb, const size_t (&idx)[N]
   }
 }
 
-// Unconditional loads of the pipelined registers of one position (posc must be a valid position
-// index: the caller clamps it and keeps validity separately). Keeping every load outside control
-// flow matters: a load whose destination merges with another value at a branch join is copied,
-// and the copy waits for the load -- which would serialise the software pipeline.
-template <typename TP, int LPR, int E>
-__device__ __forceinline__ void load_step(const SweepSide &sd, int posc, int lane, StepRegs<TP, LPR, E> &r) {
-  r.row = sd.row[posc];
-  r.exp = sd.expp[posc];
-  r.dd = ((const cx<TP> *)sd.dpos)[posc];
-  const int *ec = sd.ecol + (size_t)posc * sd.ew;
-  const cx<TP> *ev = (const cx<TP> *)sd.eval + (size_t)posc * sd.ew;
+// Single-column sweep with the block's z (and, on the forward pass, r) staged in LDS.
+// zs[slot] holds r_i until row i is processed (forward) and z_i afterwards.
+//
+// Every step holds at most AFX_STEP_ROWS rows; row slot t/8 of the block takes one of them (or
+// the dummy position) with 8 lanes, lane l holding the entry pair (2l, 2l+1) of the 16-wide row.
+// The sweep is a software pipeline in which no wait targets a memory operation issued fewer than
+// PMB steps earlier (vmcnt retires in issue order, so a wait for a young load would also wait
+// for every older load and store). Step s
+//   * reads the z values of its rows' own-block entries from LDS,
+//   * requests the matrix data of step s+PMAT (positions from the step bounds staged in LDS) and
+//     the mailbox slots of step s+PMB (from entry sources that arrived PMAT-PMB steps ago) into
+//     the register sets step s-1 released -- issuing these overlaps the LDS reads,
+//   * computes its rows from the data requested at steps s-PMAT and s-PMB (a mailbox value
+//     still empty is polled: it waits for the producer's post), posts z to LDS and, when another
+//     block reads it, to the mailbox, and ends with the block barrier.
+// Each register set is named by a compile-time index of a 6-way unrolled loop, so no set is
+// ever copied (a copy would wait for the set's loads).
+namespace pipe {
+constexpr int L = 8;             // lanes per row
+constexpr int NS = 256 / L;      // row slots per block (== AFX_STEP_ROWS)
+constexpr int PMAT = 5;          // steps between requesting a row's matrix data and using it
+constexpr int PMB = 2;           // steps between requesting its mailbox slots and using them
+static_assert(NS == AFX_STEP_ROWS, "one row slot per row of a step");
+}  // namespace pipe
+
+template <typename TP>
+struct PSet {
+  int2 rw;     // {row | exported << 31, LDS slot}
+  int src[2];  // entry sources (>= 0 LDS slot, < 0 mailbox slot ~src)
+  cx<TP> val[2];
+  cx<TP> dd;
+};
+
+// requests of one row slot: the position lo + slot of the step [lo, hi), or the dummy position
+template <typename TP>
+__device__ __forceinline__ void pset_load(const SweepSide &sd, int lo, int hi, int slot, int l8, PSet<TP> &r) {
+  const int p0 = lo + slot;
+  const int pos = p0 < hi ? p0 : sd.npos;
+  const size_t e0 = (size_t)pos * AFX_PIPE_W + 2 * l8;
+  r.rw = ((const int2 *)sd.prow)[pos];
+  const int2 c = *(const int2 *)(sd.psrc + e0);
+  r.src[0] = c.x;
+  r.src[1] = c.y;
+  const cx<TP> *v = (const cx<TP> *)sd.pval + e0;
+  r.val[0] = v[0];
+  r.val[1] = v[1];
+  r.dd = ((const cx<TP> *)sd.dpos)[pos];
+}
+
+// whether any lane of the wave holds an entry of another block in set r (wave-uniform)
+template <typename TP>
+__device__ __forceinline__ bool pset_remote(const PSet<TP> &r) {
+  return __builtin_amdgcn_ballot_w64((r.src[0] | r.src[1]) < 0) != 0;
+}
+
+// raw mailbox loads for the entries of a loaded set; an entry that needs none loads slot src
+// (src <= n: any valid slot, the value is not used). Issued by every wave: a conditional issue
+// would make the compiler count these loads as absent and wait for younger loads after them.
+template <typename TP>
+__device__ __forceinline__ void pset_mb(const PSet<TP> &r, const cx<TP> *mb, cx<TP> (&z)[2]) {
 #pragma unroll
-  for (int e = 0; e < E; e++) {
-    const int q = min(lane + e * LPR, sd.ew - 1);
-    r.col[e] = ec[q];
-    r.val[e] = ev[q];
+  for (int e = 0; e < 2; e++) {
+    const int sc = r.src[e];
+    mb_raw(&mb[sc < 0 ? ~sc : sc], z[e]);
   }
 }
 
-// raw mailbox loads for the remote entries of a loaded row (other entries load slot 0, unused)
-template <typename TP, int LPR, int E>
-__device__ __forceinline__ void mb_fetch(const StepRegs<TP, LPR, E> &r, bool valid, const bool (&qok)[E], int rs,
-                                         int re, const cx<TP> *mb, cx<TP> (&z)[E]) {
-#pragma unroll
-  for (int e = 0; e < E; e++) {
-    const int cl = r.col[e];
-    const bool rem = valid && qok[e] && cl >= 0 && (cl < rs || cl >= re);
-    mb_raw(&mb[rem ? cl : 0], z[e]);
-  }
-}
-
-template <typename TP, int LPR, int E, bool FWD>
-__device__ void sweep_lds(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> *zs, cx<TP> *mb, TP w,
-                          unsigned &nfallback) {
+template <typename TP, bool FWD>
+__device__ void sweep_pipe(const CocgParams<TP> &p, const SweepSide &sd, const int *sb, cx<TP> *zs, cx<TP> *mb,
+                           TP w, unsigned &nfallback) {
+  using namespace pipe;
   const int b = blockIdx.x;
   const int t = threadIdx.x;
-  const int rs = p.rstart[b], re = p.rstart[b + 1];
-  const int slot = t / LPR, lane = t % LPR;
-  constexpr int NSLOT = 256 / LPR;
+  const int slot = t / L, l8 = t % L;
   const int s0 = sd.bstep[b], s1 = sd.bstep[b + 1];
-  bool qok[E];
+  // first position of step s (the block's end past its last step): staged in LDS
+  auto bnd = [&](int s) { return sb[min(s, s1) - s0]; };
+
+  PSet<TP> R0, R1, R2, R3, R4, R5;
+  cx<TP> M0[2], M1[2], M2[2];
+  pset_load(sd, bnd(s0), bnd(s0 + 1), slot, l8, R0);
+  pset_load(sd, bnd(s0 + 1), bnd(s0 + 2), slot, l8, R1);
+  pset_load(sd, bnd(s0 + 2), bnd(s0 + 3), slot, l8, R2);
+  pset_load(sd, bnd(s0 + 3), bnd(s0 + 4), slot, l8, R3);
+  pset_load(sd, bnd(s0 + 4), bnd(s0 + 5), slot, l8, R4);
+  pset_mb(R0, mb, M0);
+  pset_mb(R1, mb, M1);
+  int lo = bnd(s0 + 5);  // first position of the next step to request
+  int hi = bnd(s0 + 6);
+
+  // Step s: first the requests -- matrix data of step s+5 into rn and mailbox slots of step s+2
+  // into mn (both sets were released by step s-1; rm holds step s+2) -- then the rows of step s
+  // from r and m, so that issuing the requests overlaps the LDS reads of the step.
+  auto step = [&](int s, const PSet<TP> &r, const cx<TP>(&m)[2], PSet<TP> &rn, cx<TP>(&mn)[2],
+                  const PSet<TP> &rm) {
+    cx<TP> zl[2];
 #pragma unroll
-  for (int e = 0; e < E; e++) qok[e] = lane + e * LPR < sd.ew;
-  // step bounds are uniform: scalar loads
-  auto sp = [&](int s) { return sd.spos[__builtin_amdgcn_readfirstlane(s < s1 ? s : s1)]; };
-  // a clamped, always valid position of the step [lo, hi) (or of the block's last position)
-  auto clampp = [&](int lo, int hi) { return max(0, min(lo + slot, hi - 1)); };
-  int l0 = sp(s0), l1 = sp(s0 + 1), l2 = sp(s0 + 2), l3 = sp(s0 + 3);
-  // three register sets rotate by position in the unrolled loop below, so a set is never copied
-  // (a copy would wait for its loads to arrive, collapsing the pipeline to one step)
-  StepRegs<TP, LPR, E> R0, R1, R2;
-  cx<TP> Z0[E], Z1[E], Z2[E];
-  load_step(sd, clampp(l0, l1), lane, R0);
-  load_step(sd, clampp(l1, l2), lane, R1);
-  mb_fetch(R0, l0 + slot < l1, qok, rs, re, mb, Z0);
-  __syncthreads();
-  auto do_row = [&](const StepRegs<TP, LPR, E> &r, const cx<TP>(&zr)[E], auto pre) {
-    cx<TP> zv[E];
-    bool rem[E];
-    size_t idx[E];
+    for (int e = 0; e < 2; e++) zl[e] = zs[max(r.src[e], 0)];
+    const cx<TP> zo = zs[r.rw.y];
+    cx<TP> zv[2] = {zl[0], zl[1]};
+    const int hn = bnd(s + PMAT + 2);  // upper bound of the step requested next
+    if (pset_remote(r)) {
+      // entries of other blocks: the mailbox values requested PMB steps ago; an empty one is
+      // polled before this step's requests are issued, so that a poll's wait (vmcnt retires in
+      // issue order) does not include loads issued in this step
+      bool rem[2];
+      size_t idx[2];
 #pragma unroll
-    for (int e = 0; e < E; e++) {
-      const int cl = r.col[e];
-      const bool use = qok[e] && cl >= 0;
-      const bool own = cl >= rs && cl < re;
-      rem[e] = use && !own;
-      idx[e] = rem[e] ? (size_t)cl : 0;
-      if (use && own) {
-        zv[e] = zs[cl - rs];
-      } else if (decltype(pre)::value) {
-        zv[e] = zr[e];
-      } else {
-        mb_raw(&mb[idx[e]], zv[e]);
+      for (int e = 0; e < 2; e++) {
+        const int sc = r.src[e];
+        rem[e] = sc < 0;
+        idx[e] = rem[e] ? (size_t)~sc : 0;
+        zv[e] = rem[e] ? m[e] : zl[e];
       }
+      mb_poll(mb, idx, rem, zv, p.err, nfallback);
     }
-    mb_poll(mb, idx, rem, zv, p.err, nfallback);
-    cx<TP> acc = {TP(0), TP(0)};
-#pragma unroll
-    for (int e = 0; e < E; e++)
-      if (qok[e] && r.col[e] >= 0) acc = cfma(mval(r.val[e], w), zv[e], acc);
-    acc = group_sum<LPR>(acc);
-    if (lane == 0) {
-      const int row = r.row;
-      const cx<TP> zo = zs[row - rs];
+    pset_load(sd, lo, hi, slot, l8, rn);
+    pset_mb(rm, mb, mn);
+    lo = hi;
+    hi = hn;
+    cx<TP> acc = cfma(mval(r.val[0], w), zv[0], cx<TP>{TP(0), TP(0)});
+    acc = cfma(mval(r.val[1], w), zv[1], acc);
+    acc = group_sum<L>(acc);
+    if (l8 == 0) {
       const cx<TP> z = FWD ? cmul(zo - acc, r.dd) : zo - cmul(r.dd, acc);
-      zs[row - rs] = z;
-      if (r.exp) mb_put(&mb[row], z);
-    }
-  };
-  auto step = [&](const StepRegs<TP, LPR, E> &cur, const StepRegs<TP, LPR, E> &nx1, StepRegs<TP, LPR, E> &nx2,
-                  const cx<TP>(&zr)[E], cx<TP>(&zr1)[E], int l4) {
-    mb_fetch(nx1, l1 + slot < l2, qok, rs, re, mb, zr1);
-    load_step(sd, clampp(l2, l3), lane, nx2);
-    // first row of this slot from the pipelined registers, any further rows directly (each call
-    // site names its own register set: selecting one by reference would put both in scratch)
-    if (l0 + slot < l1) do_row(cur, zr, std::true_type{});
-    for (int pos = l0 + slot + NSLOT; pos < l1; pos += NSLOT) {
-      StepRegs<TP, LPR, E> extra;
-      load_step(sd, pos, lane, extra);
-      do_row(extra, zr, std::false_type{});
+      zs[r.rw.y] = z;
+      if (r.rw.x < 0) mb_put(&mb[r.rw.x & 0x7fffffff], z);
     }
     __syncthreads();
-    l0 = l1;
-    l1 = l2;
-    l2 = l3;
-    l3 = l4;
   };
-  for (int s = s0; s < s1; s += 3) {
-    step(R0, R1, R2, Z0, Z1, sp(s + 4));
+  for (int s = s0; s < s1; s += 6) {
+    step(s, R0, M0, R5, M2, R2);
     if (s + 1 >= s1) break;
-    step(R1, R2, R0, Z1, Z2, sp(s + 5));
+    step(s + 1, R1, M1, R0, M0, R3);
     if (s + 2 >= s1) break;
-    step(R2, R0, R1, Z2, Z0, sp(s + 6));
+    step(s + 2, R2, M2, R1, M1, R4);
+    if (s + 3 >= s1) break;
+    step(s + 3, R3, M0, R2, M2, R5);
+    if (s + 4 >= s1) break;
+    step(s + 4, R4, M1, R3, M0, R0);
+    if (s + 5 >= s1) break;
+    step(s + 5, R5, M2, R4, M1, R1);
   }
 }
 
@@ -614,10 +651,19 @@ __device__ void sweep_ring(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> 
   }
 }
 
-template <typename TP, int LPR, bool MULTI>
+// dynamic LDS of the single-column path: z of the block's rows, then the step bounds of the
+// forward and backward sweeps (16-byte aligned)
+template <typename TP>
+__host__ __device__ constexpr size_t lds_bounds_offset(int lds_rows) {
+  return ((size_t)lds_rows * sizeof(cx<TP>) + 15) & ~(size_t)15;
+}
+
 // One block of 256 threads per CU (the cooperative grid never exceeds the CU count), so each
 // SIMD runs a single wave of this kernel: let it use the whole register file instead of
-// spilling the software-pipelined sweep registers to scratch.
+// spilling the software-pipelined sweep registers to scratch. PROF selects the instantiation
+// that records per-phase timing (its timers would otherwise hold scalar registers that the
+// sweep loops need).
+template <typename TP, int LPR, bool MULTI, bool PROF>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) k_cocg(CocgParams<TP> p) {
   __shared__ cx<double> lds_c[256];
   __shared__ double lds_d[256];
@@ -638,7 +684,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
   const int f = cvalid ? p.col2f[c] : 0;
   const bool sweep_pc = (p.pc == AFX_PC_IC || p.pc == AFX_PC_SSOR || p.pc == AFX_PC_SSOR_BJ);
   const bool dist = p.dist_reduce != 0;
-  const int rs = p.rstart[blockIdx.x], re = p.rstart[blockIdx.x + 1];
+  const int pb = blockIdx.x;
+  const int rs = p.rstart[pb], re = p.rstart[pb + 1];
+  // single-column sweeps: the block's step bounds of both directions are staged in LDS once
+  const bool pipe_sweep = !MULTI && p.lds_rows > 0 && p.fw.psrc != nullptr && p.bw.psrc != nullptr;
+  int *sbf = (int *)(dyn + lds_bounds_offset<TP>(p.lds_rows));
+  int *sbb = sbf + p.fw.maxsteps + 1;
+  if (pipe_sweep) {
+    const int f0 = p.fw.bstep[pb], f1 = p.fw.bstep[pb + 1];
+    const int b0 = p.bw.bstep[pb], b1 = p.bw.bstep[pb + 1];
+    for (int i = t; i <= f1 - f0; i += 256) sbf[i] = p.fw.spos[f0 + i];
+    for (int i = t; i <= b1 - b0; i += 256) sbb[i] = p.bw.spos[b0 + i];
+  }
   unsigned phase = 0;
   if (t == 0) phase = __hip_atomic_load(p.bar.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
@@ -652,8 +709,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
   const int it_end = min(it + p.chunk, p.maxit);
   bool alldone = block_all(st != 0);
   // optional per-block phase timing (wall clock, 100 MHz) and mailbox poll count
-  const bool prof = p.dbg != nullptr;
-  unsigned long long tfw = 0, tbw = 0, tbar = 0, trest = 0, titer = 0;
+  constexpr bool prof = PROF;
+  unsigned long long tfw = 0, tbw = 0, tbar = 0, trest = 0, titer = 0, trho = 0, tpost = 0;
   unsigned nfb = 0;
 
   while (!alldone && it < it_end) {
@@ -664,15 +721,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     if (sweep_pc) {
       // forward sweep, grid barrier, then the forward mailbox is emptied (every reader is past
       // it) while the backward sweep runs on its own mailbox
-      if (!MULTI && p.lds_rows > 0 && p.fw.ecol != nullptr) {
+      if (pipe_sweep) {
         for (int r = rs + t; r < re; r += 256) zs[r - rs] = down<TP>(p.R[r]);
+        if (t == 0) zs[p.lds_rows - 1] = cx<TP>{TP(0), TP(0)};  // the slot padding entries read
         const unsigned long long c0 = prof ? wall_clock64() : 0;
-        sweep_lds<TP, LPR, 16 / LPR + (LPR == 16 ? 1 : 0), true>(p, p.fw, zs, p.mbf, wT, nfb);
+        __syncthreads();
+        sweep_pipe<TP, true>(p, p.fw, sbf, zs, p.mbf, wT, nfb);
         const unsigned long long c1 = prof ? wall_clock64() : 0;
         grid_barrier(p.bar, phase);
         mb_clear_own(p.mbf, p.fw.exp, rs, re, 1);
         const unsigned long long c2 = prof ? wall_clock64() : 0;
-        sweep_lds<TP, LPR, 16 / LPR + (LPR == 16 ? 1 : 0), false>(p, p.bw, zs, p.mbb, wT, nfb);
+        sweep_pipe<TP, false>(p, p.bw, sbb, zs, p.mbb, wT, nfb);
         if (prof) {
           const unsigned long long c3 = wall_clock64();
           tfw += c1 - c0;
@@ -731,7 +790,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         }
       }
     }
+    const unsigned long long cr0 = prof ? wall_clock64() : 0;
     const cx<double> rho = grid_cols(rho_loc, p.part_rho, p.fin_rho, lds_c, k, kp, c, dist, p.bar, phase);
+    const unsigned long long cr1 = prof ? wall_clock64() : 0;
+    if (prof) trho += cr1 - cr0;
     // every block is past the backward sweep: empty its mailbox for the next iteration
     if (sweep_pc) mb_clear_own(p.mbb, p.bw.exp, rs, re, MULTI ? k : 1);
 
@@ -769,8 +831,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     } else {
       // SU rows of this slot in flight together: every load of the group is issued before the
       // first one is waited for (one wave per SIMD has little else to hide memory latency);
-      // entries of a row are summed in the same order as a row-at-a-time loop
-      constexpr int SU = 4, SE = (24 + LPR - 1) / LPR;
+      // entries of a row are summed in the same order as a row-at-a-time loop (with 1 or 2 lanes
+      // per row, fewer rows and entries are held so that the registers do not spill)
+      constexpr int SU = LPR >= 4 ? 4 : 2, SE = LPR >= 4 ? (24 + LPR - 1) / LPR : 6;
       for (long long row0 = slot; row0 < p.n; row0 += SU * nslot) {
         int qb[SU], qe[SU];
 #pragma unroll
@@ -854,7 +917,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     }
     rho1 = rho;
     alldone = block_all(st != 0);
-    if (prof) titer += wall_clock64() - ci0;
+    if (prof) {
+      const unsigned long long ce = wall_clock64();
+      titer += ce - ci0;
+      tpost += ce - cr1;
+    }
   }
   if (prof) {
     trest = titer - tfw - tbw - tbar;
@@ -865,6 +932,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
       d[2] += tbw;
       d[3] += trest;
       d[4] += titer;
+      d[6] += trho;
+      d[7] += tpost;
     }
     atomicAdd(p.dbg + (size_t)blockIdx.x * 8 + 5, (unsigned long long)nfb);
   }
@@ -886,31 +955,37 @@ template <typename TP>
 size_t cocg_dyn_lds(const CocgParams<TP> &p, int grid) {
   (void)grid;
   if (p.ring_w > 0) return (size_t)p.ring_w * p.k * sizeof(cx<TP>);
-  return (size_t)p.lds_rows * sizeof(cx<TP>);
+  if (p.lds_rows <= 0) return 0;
+  return lds_bounds_offset<TP>(p.lds_rows) + (size_t)(p.fw.maxsteps + p.bw.maxsteps + 2) * sizeof(int);
 }
 
-template <typename TP, int LPR, bool MULTI>
+template <typename TP, int LPR, bool MULTI, bool PROF>
 static hipError_t launch_one(const CocgParams<TP> &p, int grid, hipStream_t s) {
   void *args[] = {(void *)&p};
   const size_t dyn = cocg_dyn_lds(p, grid);
-  hipError_t e = hipFuncSetAttribute((const void *)k_cocg<TP, LPR, MULTI>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)dyn);
+  const void *f = (const void *)k_cocg<TP, LPR, MULTI, PROF>;
+  hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
   if (e != hipSuccess) return e;
-  return hipLaunchCooperativeKernel((const void *)k_cocg<TP, LPR, MULTI>, dim3(grid), dim3(256), args, dyn, s);
+  return hipLaunchCooperativeKernel(f, dim3(grid), dim3(256), args, dyn, s);
+}
+
+template <typename TP, bool PROF>
+static hipError_t launch_p(const CocgParams<TP> &p, int lpr, int grid, hipStream_t s) {
+  if (p.k > 1) return launch_one<TP, 1, true, PROF>(p, grid, s);
+  switch (lpr) {
+    case 1: return launch_one<TP, 1, false, PROF>(p, grid, s);
+    case 2: return launch_one<TP, 2, false, PROF>(p, grid, s);
+    case 4: return launch_one<TP, 4, false, PROF>(p, grid, s);
+    case 8: return launch_one<TP, 8, false, PROF>(p, grid, s);
+    case 16: return launch_one<TP, 16, false, PROF>(p, grid, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 template <typename TP>
 static hipError_t launch_t(const CocgParams<TP> &p, int lpr, int grid, hipStream_t s) {
   if (grid != p.Gs) return hipErrorInvalidValue;
-  if (p.k > 1) return launch_one<TP, 1, true>(p, grid, s);
-  switch (lpr) {
-    case 1: return launch_one<TP, 1, false>(p, grid, s);
-    case 2: return launch_one<TP, 2, false>(p, grid, s);
-    case 4: return launch_one<TP, 4, false>(p, grid, s);
-    case 8: return launch_one<TP, 8, false>(p, grid, s);
-    case 16: return launch_one<TP, 16, false>(p, grid, s);
-    default: return hipErrorInvalidValue;
-  }
+  return p.dbg ? launch_p<TP, true>(p, lpr, grid, s) : launch_p<TP, false>(p, lpr, grid, s);
 }
 
 hipError_t launch_cocg64(const CocgParams<double> &p, int lpr, int grid, hipStream_t s) { return launch_t(p, lpr, grid, s); }
@@ -918,13 +993,15 @@ hipError_t launch_cocg32(const CocgParams<float> &p, int lpr, int grid, hipStrea
 
 template <typename TP, int LPR, bool MULTI>
 static int occ_one(size_t dyn) {
-  int nb = 0;
-  if (hipFuncSetAttribute((const void *)k_cocg<TP, LPR, MULTI>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) !=
-      hipSuccess)
+  int nb = 0, nbp = 0;
+  const void *f = (const void *)k_cocg<TP, LPR, MULTI, false>, *fp = (const void *)k_cocg<TP, LPR, MULTI, true>;
+  if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess ||
+      hipFuncSetAttribute(fp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess)
     return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_cocg<TP, LPR, MULTI>, 256, dyn) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 256, dyn) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbp, fp, 256, dyn) != hipSuccess)
     return 0;
-  return nb;
+  return std::min(nb, nbp);
 }
 
 // Largest co-resident grid (blocks) for the cooperative launch with `dyn` bytes of LDS.

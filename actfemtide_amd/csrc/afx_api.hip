@@ -803,8 +803,8 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
     ~DbgFree() { dfree(q); }
   } dbg_free{dbg};
   if (getenv("AFX_PROFILE")) {
-    if ((rc = dalloc(&dbg, (size_t)G * 8))) return rc;
-    HIPCHK(hipMemsetAsync(dbg, 0, (size_t)G * 8 * sizeof(unsigned long long), s->stream));
+    if ((rc = dalloc(&dbg, (size_t)G * AFX_DBG_SLOTS))) return rc;
+    HIPCHK(hipMemsetAsync(dbg, 0, (size_t)G * AFX_DBG_SLOTS * sizeof(unsigned long long), s->stream));
   }
   p.dbg = dbg;
   const auto t_start = std::chrono::steady_clock::now();
@@ -842,22 +842,22 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
   if (dbg) {
-    std::vector<unsigned long long> h((size_t)G * 8);
+    std::vector<unsigned long long> h((size_t)G * AFX_DBG_SLOTS);
     HIPCHK(hipMemcpy(h.data(), dbg, h.size() * 8, hipMemcpyDeviceToHost));
     const double its = std::max(1, w.h_status[0]);
-    double mean[8] = {0, 0, 0, 0, 0, 0, 0, 0}, mx[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double mean[10] = {0}, mx[10] = {0};
     for (int b = 0; b < G; b++)
-      for (int q = 0; q < 8; q++) {
-        const double v = (double)h[(size_t)b * 8 + q] / its * (q != 5 ? 0.01 : 1.0);  // 100 MHz -> us
+      for (int q = 0; q < 10; q++) {
+        const double v = (double)h[(size_t)b * AFX_DBG_SLOTS + q] / its * (q != 5 ? 0.01 : 1.0);  // 100 MHz -> us
         mean[q] += v / G;
         mx[q] = std::max(mx[q], v);
       }
     fprintf(stderr,
             "afx profile (us/iteration, mean/max over %d blocks): fw sweep %.1f/%.1f  barrier %.1f/%.1f  "
-            "bw sweep %.1f/%.1f  rest %.1f/%.1f (rho wait %.1f/%.1f, after rho %.1f/%.1f)  total %.1f/%.1f  "
-            "mailbox polls/iteration/block %.1f/%.0f\n",
+            "bw sweep %.1f/%.1f  rest %.1f/%.1f (rho wait %.1f/%.1f, after rho %.1f/%.1f: p update %.1f, "
+            "spmv+c1 %.1f)  total %.1f/%.1f  mailbox polls/iteration/block %.1f/%.0f\n",
             G, mean[0], mx[0], mean[1], mx[1], mean[2], mx[2], mean[3], mx[3], mean[6], mx[6], mean[7], mx[7],
-            mean[4], mx[4], mean[5], mx[5]);
+            mean[8], mean[9], mean[4], mx[4], mean[5], mx[5]);
   }
   int herr = 0;
   HIPCHK(hipMemcpy(&herr, s->d_err, sizeof(int), hipMemcpyDeviceToHost));

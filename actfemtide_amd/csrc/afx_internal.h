@@ -15,6 +15,8 @@ namespace afx {
 #define AFX_STEP_ROWS 32
 // entries per position of the single-column sweep layout (8 lanes x 2)
 #define AFX_PIPE_W 16
+// per-block profile counters of the persistent solve (AFX_PROFILE)
+#define AFX_DBG_SLOTS 16
 
 // most columns the LDS-ring sweep takes in one launch (2 lanes per row and column)
 #ifndef AFX_RING_KC
@@ -97,7 +99,7 @@ struct CocgParams {
   int *err;     // bounded waits that gave up
   double tol;
   int maxit, chunk;
-  unsigned long long *dbg;  // optional [G][8] phase timing (wall clock ticks) and poll counts
+  unsigned long long *dbg;  // optional [G][AFX_DBG_SLOTS] phase timing (wall clock ticks) and poll counts
 };
 
 struct SpmvParams {

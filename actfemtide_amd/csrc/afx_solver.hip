@@ -710,7 +710,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
   bool alldone = block_all(st != 0);
   // optional per-block phase timing (wall clock, 100 MHz) and mailbox poll count
   constexpr bool prof = PROF;
-  unsigned long long tfw = 0, tbw = 0, tbar = 0, trest = 0, titer = 0, trho = 0, tpost = 0;
+  unsigned long long tfw = 0, tbw = 0, tbar = 0, trest = 0, titer = 0, trho = 0, tpost = 0, tpu = 0, tsp = 0;
   unsigned nfb = 0;
 
   while (!alldone && it < it_end) {
@@ -722,7 +722,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
       // forward sweep, grid barrier, then the forward mailbox is emptied (every reader is past
       // it) while the backward sweep runs on its own mailbox
       if (pipe_sweep) {
-        for (int r = rs + t; r < re; r += 256) zs[r - rs] = down<TP>(p.R[r]);
+        // the block's r staged in LDS, 8 loads in flight per thread
+        for (int r0 = rs + t; r0 < re; r0 += 8 * 256) {
+          cx<double> v[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) v[u] = p.R[min(r0 + u * 256, re - 1)];
+#pragma unroll
+          for (int u = 0; u < 8; u++)
+            if (r0 + u * 256 < re) zs[r0 + u * 256 - rs] = down<TP>(v[u]);
+        }
         if (t == 0) zs[p.lds_rows - 1] = cx<TP>{TP(0), TP(0)};  // the slot padding entries read
         const unsigned long long c0 = prof ? wall_clock64() : 0;
         __syncthreads();
@@ -738,10 +746,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
           tbar += c2 - c1;
           tbw += c3 - c2;
         }
-        for (int r = rs + t; r < re; r += 256) {
-          const cx<TP> z = zs[r - rs];
-          p.Z[r] = z;
-          rho_loc = addv(rho_loc, cmul(p.R[r], up(z)));
+        // z written back and rho = r^T z accumulated in row order, 8 loads in flight per thread
+        for (int r0 = rs + t; r0 < re; r0 += 8 * 256) {
+          cx<double> rv[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) rv[u] = p.R[min(r0 + u * 256, re - 1)];
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const int r = r0 + u * 256;
+            if (r < re) {
+              const cx<TP> z = zs[r - rs];
+              p.Z[r] = z;
+              rho_loc = addv(rho_loc, cmul(rv[u], up(z)));
+            }
+          }
         }
       } else if (MULTI && p.ring_w > 0) {
         const unsigned long long c0 = prof ? wall_clock64() : 0;
@@ -808,12 +826,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
           p.P[ix] = first ? z : cfma(beta, p.P[ix], z);
         }
     } else {
-      for (long long row = gtid; row < p.n; row += nthr) {
-        const cx<double> z = up(p.Z[row]);
-        p.P[row] = first ? z : cfma(beta, p.P[row], z);
+      // 8 rows per thread in flight (one wave per SIMD: the loads must overlap)
+      for (long long r0 = gtid; r0 < p.n; r0 += 8 * nthr) {
+        cx<TP> zv[8];
+        cx<double> pv[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const long long r = r0 + u * nthr < p.n ? r0 + u * nthr : r0;
+          zv[u] = p.Z[r];
+          pv[u] = first ? cx<double>{0.0, 0.0} : p.P[r];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const long long r = r0 + u * nthr;
+          if (r < p.n) p.P[r] = first ? up(zv[u]) : cfma(beta, pv[u], up(zv[u]));
+        }
       }
     }
     grid_barrier(p.bar, phase);
+    const unsigned long long cp1 = prof ? wall_clock64() : 0;
+    if (prof) tpu += cp1 - cr1;
 
     // ---- q = A p (:273-288) and c1 = p^T q (:293-296) ----
     cx<double> c1_loc = {0.0, 0.0};
@@ -829,52 +861,88 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         }
       }
     } else {
-      // SU rows of this slot in flight together: every load of the group is issued before the
-      // first one is waited for (one wave per SIMD has little else to hide memory latency);
-      // entries of a row are summed in the same order as a row-at-a-time loop (with 1 or 2 lanes
-      // per row, fewer rows and entries are held so that the registers do not spill)
-      constexpr int SU = LPR >= 4 ? 4 : 2, SE = LPR >= 4 ? (24 + LPR - 1) / LPR : 6;
-      for (long long row0 = slot; row0 < p.n; row0 += SU * nslot) {
+      // Rows are taken in rounds of SU rows per row slot (row0 + u * nslot). A software pipeline
+      // over rounds: while round g is computed, the P gathers of round g, the values and
+      // columns of round g+1 and the row bounds of round g+2 are in flight; each register set
+      // is named by a compile-time index of a 2-way unrolled loop (no copies). Entries of a
+      // row are summed in the same order as a row-at-a-time loop.
+      constexpr int SU = LPR >= 8 ? 4 : 2, SE = LPR >= 4 ? (24 + LPR - 1) / LPR : 6;
+      struct RB {  // row bounds of one round
         int qb[SU], qe[SU];
+      };
+      struct VC {  // values and columns of one round (column -1: no entry)
+        int qb[SU], qe[SU];
+        int cl[SU][SE];
+        cx<double> vv[SU][SE];
+      };
+      const long long rstep = (long long)SU * nslot;
+      auto load_rb = [&](long long row0, RB &b) {
 #pragma unroll
         for (int u = 0; u < SU; u++) {
           const long long r = row0 + u * nslot;
           const long long rc = r < p.n ? r : p.n - 1;
-          qb[u] = p.rowptr[rc];
-          qe[u] = r < p.n ? p.rowptr[rc + 1] : qb[u];
+          b.qb[u] = p.rowptr[rc];
+          b.qe[u] = p.rowptr[rc + 1];
+          if (r >= p.n) b.qe[u] = b.qb[u];
         }
-        cx<double> acc[SU];
-        cx<double> vv[SU][SE], pv[SU][SE];
+      };
+      auto load_vc = [&](const RB &b, VC &v) {
 #pragma unroll
-        for (int u = 0; u < SU; u++)
+        for (int u = 0; u < SU; u++) {
+          v.qb[u] = b.qb[u];
+          v.qe[u] = b.qe[u];
 #pragma unroll
           for (int e = 0; e < SE; e++) {
-            const int q = qb[u] + lane + e * LPR;
-            const int qc = q < qe[u] ? q : qb[u];
-            vv[u][e] = p.val[qc];
-            pv[u][e] = p.P[p.col[qc]];
+            const int q = b.qb[u] + lane + e * LPR;
+            const bool in = q < b.qe[u];
+            const int qc = in ? q : 0;
+            v.vv[u][e] = p.val[qc];
+            const int cq = p.col[qc];  // loaded unconditionally (a fixed count of loads per round)
+            v.cl[u][e] = in ? cq : -1;
           }
+        }
+      };
+      auto round = [&](long long row0, const VC &v, VC &vn, const RB &bn, RB &bnn) {
+        cx<double> pv[SU][SE], pr[SU];
 #pragma unroll
         for (int u = 0; u < SU; u++) {
-          acc[u] = cx<double>{0.0, 0.0};
+#pragma unroll
+          for (int e = 0; e < SE; e++) pv[u][e] = p.P[max(v.cl[u][e], 0)];
+          const long long r = row0 + u * nslot;
+          pr[u] = p.P[r < p.n ? r : 0];
+        }
+        load_vc(bn, vn);
+        load_rb(row0 + 2 * rstep, bnn);
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+          cx<double> acc = {0.0, 0.0};
 #pragma unroll
           for (int e = 0; e < SE; e++)
-            if (qb[u] + lane + e * LPR < qe[u]) acc[u] = cfma(mval(vv[u][e], w), pv[u][e], acc[u]);
-          for (int q = qb[u] + lane + SE * LPR; q < qe[u]; q += LPR)
-            acc[u] = cfma(mval(p.val[q], w), p.P[p.col[q]], acc[u]);
-        }
-#pragma unroll
-        for (int u = 0; u < SU; u++) {
+            if (v.cl[u][e] >= 0) acc = cfma(mval(v.vv[u][e], w), pv[u][e], acc);
+          for (int q = v.qb[u] + lane + SE * LPR; q < v.qe[u]; q += LPR)
+            acc = cfma(mval(p.val[q], w), p.P[p.col[q]], acc);
           const long long r = row0 + u * nslot;
-          const cx<double> a = group_sum<LPR>(acc[u]);
+          const cx<double> a = group_sum<LPR>(acc);
           if (lane == 0 && r < p.n) {
             p.Q[r] = a;
-            c1_loc = addv(c1_loc, cmul(p.P[r], a));
+            c1_loc = addv(c1_loc, cmul(pr[u], a));
           }
         }
+      };
+      RB b0, b1;
+      VC v0, v1;
+      const long long first0 = slot;
+      load_rb(first0, b0);
+      load_rb(first0 + rstep, b1);
+      load_vc(b0, v0);
+      for (long long row0 = first0; row0 < p.n; row0 += 2 * rstep) {
+        round(row0, v0, v1, b1, b0);
+        if (row0 + rstep >= p.n) break;
+        round(row0 + rstep, v1, v0, b0, b1);
       }
     }
     const cx<double> c1 = grid_cols(c1_loc, p.part_c1, p.fin_c1, lds_c, k, kp, c, dist, p.bar, phase);
+    if (prof) tsp += wall_clock64() - cp1;
     const cx<double> alpha = cdivd(rho, c1);
     const bool run = (st == 0);
     if (run && (!(c1.r != 0.0 || c1.i != 0.0) || !isfinite(alpha.r) || !isfinite(alpha.i) ||
@@ -897,11 +965,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
           rr_loc += r.r * r.r + r.i * r.i;
         }
     } else if (upd) {
-      for (long long row = gtid; row < p.n; row += nthr) {
-        p.X[row] = cfma(alpha, p.P[row], p.X[row]);
-        const cx<double> r = cfma(malpha, p.Q[row], p.R[row]);
-        p.R[row] = r;
-        rr_loc += r.r * r.r + r.i * r.i;
+      // 4 rows per thread in flight; |r|^2 accumulated in row order
+      for (long long r0 = gtid; r0 < p.n; r0 += 4 * nthr) {
+        cx<double> xv[4], pv[4], qv[4], rv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const long long r = r0 + u * nthr < p.n ? r0 + u * nthr : r0;
+          xv[u] = p.X[r];
+          pv[u] = p.P[r];
+          qv[u] = p.Q[r];
+          rv[u] = p.R[r];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const long long row = r0 + u * nthr;
+          if (row < p.n) {
+            p.X[row] = cfma(alpha, pv[u], xv[u]);
+            const cx<double> r = cfma(malpha, qv[u], rv[u]);
+            p.R[row] = r;
+            rr_loc += r.r * r.r + r.i * r.i;
+          }
+        }
       }
     }
     const double rr = grid_cols(rr_loc, p.part_rr, p.fin_rr, lds_d, k, kp, c, dist, p.bar, phase);
@@ -926,7 +1010,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
   if (prof) {
     trest = titer - tfw - tbw - tbar;
     if (t == 0) {
-      unsigned long long *d = p.dbg + (size_t)blockIdx.x * 8;
+      unsigned long long *d = p.dbg + (size_t)blockIdx.x * AFX_DBG_SLOTS;
       d[0] += tfw;
       d[1] += tbar;
       d[2] += tbw;
@@ -934,8 +1018,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
       d[4] += titer;
       d[6] += trho;
       d[7] += tpost;
+      d[8] += tpu;
+      d[9] += tsp;
     }
-    atomicAdd(p.dbg + (size_t)blockIdx.x * 8 + 5, (unsigned long long)nfb);
+    atomicAdd(p.dbg + (size_t)blockIdx.x * AFX_DBG_SLOTS + 5, (unsigned long long)nfb);
   }
   if (blockIdx.x == 0) {
     if (cvalid && t < kp) {

@@ -379,7 +379,7 @@ namespace pipe {
 constexpr int L = 8;             // lanes per row
 constexpr int NS = 256 / L;      // row slots per block (== AFX_STEP_ROWS)
 constexpr int PMAT = 5;          // steps between requesting a row's matrix data and using it
-constexpr int PMB = 2;           // steps between requesting its mailbox slots and using them
+constexpr int PMB = 1;           // steps between requesting its mailbox slots and using them
 static_assert(NS == AFX_STEP_ROWS, "one row slot per row of a step");
 }  // namespace pipe
 
@@ -437,20 +437,20 @@ __device__ void sweep_pipe(const CocgParams<TP> &p, const SweepSide &sd, const i
   auto bnd = [&](int s) { return sb[min(s, s1) - s0]; };
 
   PSet<TP> R0, R1, R2, R3, R4, R5;
-  cx<TP> M0[2], M1[2], M2[2];
+  cx<TP> M0[2], M1[2];
   pset_load(sd, bnd(s0), bnd(s0 + 1), slot, l8, R0);
   pset_load(sd, bnd(s0 + 1), bnd(s0 + 2), slot, l8, R1);
   pset_load(sd, bnd(s0 + 2), bnd(s0 + 3), slot, l8, R2);
   pset_load(sd, bnd(s0 + 3), bnd(s0 + 4), slot, l8, R3);
   pset_load(sd, bnd(s0 + 4), bnd(s0 + 5), slot, l8, R4);
   pset_mb(R0, mb, M0);
-  pset_mb(R1, mb, M1);
   int lo = bnd(s0 + 5);  // first position of the next step to request
   int hi = bnd(s0 + 6);
 
-  // Step s: first the requests -- matrix data of step s+5 into rn and mailbox slots of step s+2
-  // into mn (both sets were released by step s-1; rm holds step s+2) -- then the rows of step s
-  // from r and m, so that issuing the requests overlaps the LDS reads of the step.
+  // Step s uses r (requested at step s-5) and m (mailbox slots requested at step s-1); it requests
+  // the mailbox slots of step s+1 into mn from rm, then the matrix data of step s+5 into rn (both
+  // sets were released by step s-1). The mailbox requests go first: a wait for them then never
+  // covers the matrix requests issued in the same step.
   auto step = [&](int s, const PSet<TP> &r, const cx<TP>(&m)[2], PSet<TP> &rn, cx<TP>(&mn)[2],
                   const PSet<TP> &rm) {
     cx<TP> zl[2];
@@ -474,8 +474,8 @@ __device__ void sweep_pipe(const CocgParams<TP> &p, const SweepSide &sd, const i
       }
       mb_poll(mb, idx, rem, zv, p.err, nfallback);
     }
-    pset_load(sd, lo, hi, slot, l8, rn);
     pset_mb(rm, mb, mn);
+    pset_load(sd, lo, hi, slot, l8, rn);
     lo = hi;
     hi = hn;
     cx<TP> acc = cfma(mval(r.val[0], w), zv[0], cx<TP>{TP(0), TP(0)});
@@ -489,17 +489,17 @@ __device__ void sweep_pipe(const CocgParams<TP> &p, const SweepSide &sd, const i
     __syncthreads();
   };
   for (int s = s0; s < s1; s += 6) {
-    step(s, R0, M0, R5, M2, R2);
+    step(s, R0, M0, R5, M1, R1);
     if (s + 1 >= s1) break;
-    step(s + 1, R1, M1, R0, M0, R3);
+    step(s + 1, R1, M1, R0, M0, R2);
     if (s + 2 >= s1) break;
-    step(s + 2, R2, M2, R1, M1, R4);
+    step(s + 2, R2, M0, R1, M1, R3);
     if (s + 3 >= s1) break;
-    step(s + 3, R3, M0, R2, M2, R5);
+    step(s + 3, R3, M1, R2, M0, R4);
     if (s + 4 >= s1) break;
-    step(s + 4, R4, M1, R3, M0, R0);
+    step(s + 4, R4, M0, R3, M1, R5);
     if (s + 5 >= s1) break;
-    step(s + 5, R5, M2, R4, M1, R1);
+    step(s + 5, R5, M1, R4, M0, R0);
   }
 }
 

@@ -606,8 +606,8 @@ int build_sweeps(afx_system s) {
   if ((rc = upload_side(fw, s->fw)) || (rc = upload_side(bw, s->bw)) || (rc = upload_side(bjfw, s->bjfw)) ||
       (rc = upload_side(bjbw, s->bjbw)))
     return rc;
-  if ((rc = dalloc(&s->d_bar, 64)) || (rc = dalloc(&s->d_err, 1))) return rc;
-  HIPCHK(hipMemset(s->d_bar, 0, 64 * sizeof(unsigned)));
+  if ((rc = dalloc(&s->d_bar, AFX_BAR_WORDS)) || (rc = dalloc(&s->d_err, 1))) return rc;
+  HIPCHK(hipMemset(s->d_bar, 0, AFX_BAR_WORDS * sizeof(unsigned)));
   HIPCHK(hipMemset(s->d_err, 0, sizeof(int)));
   return AFX_OK;
 }
@@ -789,6 +789,10 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   p.status = w.status;
   p.bar.cnt = s->d_bar;
   p.bar.gen = s->d_bar + 32;
+  p.bar.grp = s->d_bar + 64;
+  // counters and generation start from zero for every solve (a launch that gave up on a wait
+  // may have left them inconsistent)
+  HIPCHK(hipMemsetAsync(s->d_bar, 0, AFX_BAR_WORDS * sizeof(unsigned), s->stream));
   p.bar.err = s->d_err;
   p.err = s->d_err;
   p.tol = tol;

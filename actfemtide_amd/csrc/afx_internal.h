@@ -15,6 +15,10 @@ namespace afx {
 #define AFX_STEP_ROWS 32
 // entries per position of the single-column sweep layout (8 lanes x 2)
 #define AFX_PIPE_W 16
+// words of the grid barrier: top counter, generation, AFX_BAR_GROUPS group counters, each on
+// its own 128-byte line
+#define AFX_BAR_WORDS (32 * (2 + AFX_BAR_GROUPS))
+
 // per-block profile counters of the persistent solve (AFX_PROFILE)
 #define AFX_DBG_SLOTS 16
 

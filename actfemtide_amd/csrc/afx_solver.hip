@@ -185,11 +185,24 @@ __device__ __forceinline__ void mb_clear(cx<double> *p) {
 }
 __device__ __forceinline__ void mb_clear(cx<float> *p) { mb_st((unsigned long long *)p, MB_EMPTY); }
 
-// empty the block's own exported slots of one mailbox (rows [rs, re), k columns)
+// empty the block's own exported slots of one mailbox (rows [rs, re), k columns); the flags of
+// 8 slots per thread are loaded before the first is tested
 template <typename TP>
 __device__ void mb_clear_own(cx<TP> *mb, const unsigned char *exp, int rs, int re, int k) {
-  for (long long i = (long long)rs * k + threadIdx.x; i < (long long)re * k; i += 256)
-    if (exp[i / k]) mb_clear(&mb[i]);
+  const long long lo = (long long)rs * k, hi = (long long)re * k;
+  for (long long i0 = lo + threadIdx.x; i0 < hi; i0 += 8 * 256) {
+    unsigned char f[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const long long i = i0 + u * 256;
+      f[u] = exp[(i < hi ? i : i0) / k];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const long long i = i0 + u * 256;
+      if (i < hi && f[u]) mb_clear(&mb[i]);
+    }
+  }
 }
 
 __device__ __forceinline__ double zero_of(double) { return 0.0; }
@@ -719,8 +732,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     // ---- z = M^-1 r (COCG13_single.f90:186-241) and rho = r^T z (:246-250) ----
     cx<double> rho_loc = {0.0, 0.0};
     if (sweep_pc) {
-      // forward sweep, grid barrier, then the forward mailbox is emptied (every reader is past
-      // it) while the backward sweep runs on its own mailbox
+      // forward then backward sweep with no grid barrier between them: a block's backward sweep
+      // reads its own rows' forward z (LDS or Z) and other blocks' backward z (its own mailbox),
+      // never another block's forward result. Both mailboxes are emptied after the rho
+      // reduction, which every block reaches only after both sweeps.
       if (pipe_sweep) {
         // the block's r staged in LDS, 8 loads in flight per thread
         for (int r0 = rs + t; r0 < re; r0 += 8 * 256) {
@@ -736,8 +751,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         __syncthreads();
         sweep_pipe<TP, true>(p, p.fw, sbf, zs, p.mbf, wT, nfb);
         const unsigned long long c1 = prof ? wall_clock64() : 0;
-        grid_barrier(p.bar, phase);
-        mb_clear_own(p.mbf, p.fw.exp, rs, re, 1);
         const unsigned long long c2 = prof ? wall_clock64() : 0;
         sweep_pipe<TP, false>(p, p.bw, sbb, zs, p.mbb, wT, nfb);
         if (prof) {
@@ -765,8 +778,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         const unsigned long long c0 = prof ? wall_clock64() : 0;
         sweep_ring<TP, 2, 8, true>(p, p.fw, zs, p.mbf, nfb);
         const unsigned long long c1 = prof ? wall_clock64() : 0;
-        grid_barrier(p.bar, phase);
-        mb_clear_own(p.mbf, p.fw.exp, rs, re, k);
         const unsigned long long c2 = prof ? wall_clock64() : 0;
         sweep_ring<TP, 2, 8, false>(p, p.bw, zs, p.mbb, nfb);
         if (prof) {
@@ -783,8 +794,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
           }
       } else {
         sweep<TP, LPR, MULTI, true>(p, p.fw, zs, p.mbf, c, cvalid, wT, f, rho_loc);
-        grid_barrier(p.bar, phase);
-        mb_clear_own(p.mbf, p.fw.exp, rs, re, MULTI ? k : 1);
         sweep<TP, LPR, MULTI, false>(p, p.bw, zs, p.mbb, c, cvalid, wT, f, rho_loc);
         if (!MULTI && p.lds_rows > 0)
           for (int r = rs + t; r < re; r += 256) p.Z[r] = zs[r - rs];
@@ -812,8 +821,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     const cx<double> rho = grid_cols(rho_loc, p.part_rho, p.fin_rho, lds_c, k, kp, c, dist, p.bar, phase);
     const unsigned long long cr1 = prof ? wall_clock64() : 0;
     if (prof) trho += cr1 - cr0;
-    // every block is past the backward sweep: empty its mailbox for the next iteration
-    if (sweep_pc) mb_clear_own(p.mbb, p.bw.exp, rs, re, MULTI ? k : 1);
+    // every block is past both sweeps: empty its mailbox slots for the next iteration
+    if (sweep_pc) {
+      mb_clear_own(p.mbf, p.fw.exp, rs, re, MULTI ? k : 1);
+      mb_clear_own(p.mbb, p.bw.exp, rs, re, MULTI ? k : 1);
+    }
 
     // ---- p = z (first iteration) or z + beta p (:258-268) ----
     const bool first = (it == 1);

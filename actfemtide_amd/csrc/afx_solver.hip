@@ -185,22 +185,25 @@ __device__ __forceinline__ void mb_clear(cx<double> *p) {
 }
 __device__ __forceinline__ void mb_clear(cx<float> *p) { mb_st((unsigned long long *)p, MB_EMPTY); }
 
-// empty the block's own exported slots of one mailbox (rows [rs, re), k columns); the flags of
-// 8 slots per thread are loaded before the first is tested
+// empty the block's own exported slots of both mailboxes (rows [rs, re), k columns); the flags
+// of 8 slots per thread are loaded before the first is tested
 template <typename TP>
-__device__ void mb_clear_own(cx<TP> *mb, const unsigned char *exp, int rs, int re, int k) {
+__device__ void mb_clear_own2(cx<TP> *mba, const unsigned char *expa, cx<TP> *mbb, const unsigned char *expb, int rs,
+                              int re, int k) {
   const long long lo = (long long)rs * k, hi = (long long)re * k;
   for (long long i0 = lo + threadIdx.x; i0 < hi; i0 += 8 * 256) {
-    unsigned char f[8];
+    unsigned char fa[8], fb[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-      const long long i = i0 + u * 256;
-      f[u] = exp[(i < hi ? i : i0) / k];
+      const long long i = (i0 + u * 256 < hi ? i0 + u * 256 : i0) / k;
+      fa[u] = expa[i];
+      fb[u] = expb[i];
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       const long long i = i0 + u * 256;
-      if (i < hi && f[u]) mb_clear(&mb[i]);
+      if (i < hi && fa[u]) mb_clear(&mba[i]);
+      if (i < hi && fb[u]) mb_clear(&mbb[i]);
     }
   }
 }
@@ -822,10 +825,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     const unsigned long long cr1 = prof ? wall_clock64() : 0;
     if (prof) trho += cr1 - cr0;
     // every block is past both sweeps: empty its mailbox slots for the next iteration
-    if (sweep_pc) {
-      mb_clear_own(p.mbf, p.fw.exp, rs, re, MULTI ? k : 1);
-      mb_clear_own(p.mbb, p.bw.exp, rs, re, MULTI ? k : 1);
-    }
+    if (sweep_pc) mb_clear_own2(p.mbf, p.fw.exp, p.mbb, p.bw.exp, rs, re, MULTI ? k : 1);
 
     // ---- p = z (first iteration) or z + beta p (:258-268) ----
     const bool first = (it == 1);

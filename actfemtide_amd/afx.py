@@ -88,6 +88,7 @@ SIGNATURES = {
     "afx_solve_cg": ([_I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP], _I),
     "afx_prepobscoeff": ([_VP, _I, _VP, _VP, _VP, _VP, _VP], _I),
     "afx_obs_apply_device": ([_VP, _I, _VP, _VP, _I, _VP, _VP], _I),
+    "afx_ixyh": ([_I, _VP, _VP, _VP, _VP, _VP, _I, _VP, _VP, _D, _VP, _VP], _I),
     "afx_adjoint_rhs_device": ([_VP, _I, _VP, _VP, _VP, _VP], _I),
     "afx_jacobian_device": ([_VP, _I, _VP, _VP, _I, _VP, _VP, _VP, _D, _D, _VP], _I),
     "afx_jacobian_ampphase_device": ([_I, _I, _VP, _VP, _VP], _I),
@@ -105,6 +106,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with `python -m actfemtide_amd.build`")
+        # PyTorch carries its own HIP runtime under the same soname (libamdhip64.so.7). Loading it
+        # first makes libafx.so bind to that one instance; loaded the other way round, the process
+        # holds two HIP runtimes and the second to initialise finds no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (args, res) in SIGNATURES.items():
             f = getattr(L, name)
@@ -443,6 +451,23 @@ def solve_pardiso_inv(s: System, rhs, PTR, iflag_comp=None, threshold=1e-10, max
 def _inv_err(code):
     if code != AFX_OK:
         raise AfxError(code, lib().afx_inv_last_error().decode())
+
+
+def ixyh(xyzobs, exy, stack, item, val, vxyz, fxyz, sigma):
+    """IXYHOUT (src/solver_mpi/n_ebfem_bxyz_mpi.f90:192-301) on the device: returns (ixyh_e,
+    ixyh_vF, ixyh, vh), each (nobs, 2) complex, and h (nobs,). exy (nobs, 2) complex E at the
+    points; coeff_vF as CSR (stack from 0, item 1-based); vxyz (node, 3) complex; fxyz (node, 3)."""
+    xo = _f64(xyzobs).reshape(-1, 3)
+    nobs = xo.shape[0]
+    e = _c128(exy).reshape(nobs, 2)
+    st, it, v = _i32(stack), _i32(item), _f64(val)
+    vx = _c128(vxyz).reshape(-1, 3)
+    fx = _f64(fxyz).reshape(-1, 3)
+    out = np.zeros((nobs, 8), np.complex128)
+    h = np.zeros(nobs)
+    _err(lib().afx_ixyh(nobs, _p(xo), _p(e), _p(st), _p(it), _p(v), vx.shape[0], _p(vx), _p(fx), float(sigma),
+                        _p(out), _p(h)))
+    return out[:, 0:2], out[:, 2:4], out[:, 4:6], out[:, 6:8], h
 
 
 def jacobian_ampphase_device(dJ, dbe, dout):

@@ -1821,6 +1821,41 @@ int afx_prepobscoeff(afx_system s, int nobs, const double *xyzobs, int *item, do
   return AFX_OK;
 }
 
+int afx_ixyh(int nobs, const double *xyzobs, const double *exy, const int *stack, const int *item, const double *val,
+             int node, const double *vxyz, const double *fxyz, double sigma, double *out, double *h) {
+  if (nobs <= 0 || node <= 0 || !xyzobs || !exy || !stack || !item || !val || !vxyz || !fxyz || !out || !h)
+    return fail(AFX_ERR_ARG, "ixyh: bad arguments");
+  if (stack[0] != 0) return fail(AFX_ERR_ARG, "ixyh: stack must start at 0");
+  for (int o = 0; o < nobs; o++)
+    if (stack[o + 1] < stack[o]) return fail(AFX_ERR_ARG, "ixyh: stack must be non-decreasing");
+  const int nnz = stack[nobs];
+  for (int q = 0; q < nnz; q++)
+    if (item[q] < 1 || item[q] > node) return fail(AFX_ERR_ARG, "ixyh: item out of range");
+  std::vector<double> z(nobs);
+  for (int o = 0; o < nobs; o++) z[o] = xyzobs[3 * (size_t)o + 2];
+  double *dz = nullptr, *dval = nullptr, *dF = nullptr, *dh = nullptr;
+  int *dptr = nullptr, *ditem = nullptr;
+  cx<double> *dexy = nullptr, *dv = nullptr, *dout = nullptr;
+  auto cleanup = [&]() {
+    dfree(dz); dfree(dval); dfree(dF); dfree(dh); dfree(dptr); dfree(ditem); dfree(dexy); dfree(dv); dfree(dout);
+  };
+  int rc;
+  if ((rc = upload(&dz, z.data(), (size_t)nobs)) || (rc = upload(&dptr, stack, (size_t)nobs + 1)) ||
+      (rc = upload(&ditem, item, (size_t)nnz)) || (rc = upload(&dval, val, (size_t)nnz)) ||
+      (rc = upload(&dexy, (const cx<double> *)exy, 2 * (size_t)nobs)) ||
+      (rc = upload(&dv, (const cx<double> *)vxyz, 3 * (size_t)node)) || (rc = upload(&dF, fxyz, 3 * (size_t)node)) ||
+      (rc = dalloc(&dout, 8 * (size_t)nobs)) || (rc = dalloc(&dh, (size_t)nobs))) {
+    cleanup();
+    return rc;
+  }
+  hipError_t e = afx::launch_ixyh(nobs, dz, dexy, dptr, ditem, dval, dv, dF, sigma, dout, dh, 0);
+  if (e == hipSuccess) e = hipMemcpy(out, dout, 8 * (size_t)nobs * sizeof(cx<double>), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(h, dh, (size_t)nobs * sizeof(double), hipMemcpyDeviceToHost);
+  cleanup();
+  if (e != hipSuccess) return fail(AFX_ERR_HIP, hipGetErrorString(e));
+  return AFX_OK;
+}
+
 int afx_obs_apply_device(afx_system s, int nobs, const int *item, const double *val, int ncol, const void *dX,
                          double *out) {
   if (!s || !item || !val || !dX || !out) return fail(AFX_ERR_ARG, "null argument");

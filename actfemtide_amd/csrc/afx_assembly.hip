@@ -290,6 +290,57 @@ __global__ void k_obs_apply(int nobs, const int *item, const double *val, int nc
   }
   out[gid] = s;
 }
+// IXYHOUT (src/solver_mpi/n_ebfem_bxyz_mpi.f90:192-301): the seafloor current at observation
+// point o. h = -z (0 on land, :227-231); vF = v x F at the nodes of row o of coeff_vF (cr_outer,
+// src/common/m_outerinnerproduct.f90:45-53) interpolated by that row (mul_matcrs_cv order,
+// :244-251) as is v (x, y); then, in the reference's order of operations (:261-268),
+//   ixyh_e = sigma h E,  ixyh_vF = sigma h vF 1e-3,  ixyh = sigma h (E + vF 1e-3),  vh = h v.
+// out[o] = {ixyh_e (x, y), ixyh_vF (x, y), ixyh (x, y), vh (x, y)}; contraction off, so every
+// product and sum is rounded as written.
+__global__ void k_ixyh(int nobs, const double *zobs, const cx<double> *exy, const int *ptr, const int *item,
+                       const double *val, const cx<double> *v, const double *F, double sigma, cx<double> *out,
+                       double *hout) {
+#pragma clang fp contract(off)
+  const int o = blockIdx.x * 256 + threadIdx.x;
+  if (o >= nobs) return;
+  double h = -zobs[o];
+  if (h <= 0.0) h = 0.0;
+  cx<double> vf[2] = {{0.0, 0.0}, {0.0, 0.0}}, vh[2] = {{0.0, 0.0}, {0.0, 0.0}};
+  for (int q = ptr[o]; q < ptr[o + 1]; ++q) {
+    const int nd = item[q] - 1;
+    const double a = val[q];
+    const cx<double> vx = v[3 * (size_t)nd], vy = v[3 * (size_t)nd + 1], vz = v[3 * (size_t)nd + 2];
+    const double Fx = F[3 * (size_t)nd], Fy = F[3 * (size_t)nd + 1], Fz = F[3 * (size_t)nd + 2];
+    const cx<double> c0 = {vy.r * Fz - vz.r * Fy, vy.i * Fz - vz.i * Fy};
+    const cx<double> c1 = {vz.r * Fx - vx.r * Fz, vz.i * Fx - vx.i * Fz};
+    vf[0].r = vf[0].r + a * c0.r;
+    vf[0].i = vf[0].i + a * c0.i;
+    vf[1].r = vf[1].r + a * c1.r;
+    vf[1].i = vf[1].i + a * c1.i;
+    vh[0].r = vh[0].r + a * vx.r;
+    vh[0].i = vh[0].i + a * vx.i;
+    vh[1].r = vh[1].r + a * vy.r;
+    vh[1].i = vh[1].i + a * vy.i;
+  }
+  const double sh = sigma * h;
+  cx<double> *r = out + 8 * (size_t)o;
+  for (int c = 0; c < 2; c++) {
+    const cx<double> e = exy[2 * (size_t)o + c];
+    r[c] = {sh * e.r, sh * e.i};
+    r[2 + c] = {sh * vf[c].r * 1.0e-3, sh * vf[c].i * 1.0e-3};
+    r[4 + c] = {sh * (e.r + vf[c].r * 1.0e-3), sh * (e.i + vf[c].i * 1.0e-3)};
+    r[6 + c] = {h * vh[c].r, h * vh[c].i};
+  }
+  hout[o] = h;
+}
+hipError_t launch_ixyh(int nobs, const double *zobs, const cx<double> *exy, const int *ptr, const int *item,
+                       const double *val, const cx<double> *v, const double *F, double sigma, cx<double> *out,
+                       double *hout, hipStream_t s) {
+  if (nobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ixyh, dim3((nobs + 255) / 256), dim3(256), 0, s, nobs, zobs, exy, ptr, item, val, v, F, sigma,
+                     out, hout);
+  return hipGetLastError();
+}
 hipError_t launch_obs_apply(int nobs, const int *item, const double *val, int ncol, const cx<double> *X,
                             cx<double> *out, hipStream_t s) {
   const long long m = (long long)nobs * ncol;

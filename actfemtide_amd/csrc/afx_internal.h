@@ -174,6 +174,9 @@ hipError_t launch_genbc(int nline, const int *line, const double *xyz, double zm
                         double yout, unsigned char *bc, hipStream_t s);
 hipError_t launch_findelement(int nobs, const double *pts, int ntet, const double *xyz, const int *n4,
                               int *first, hipStream_t s);
+hipError_t launch_ixyh(int nobs, const double *zobs, const cx<double> *exy, const int *ptr, const int *item,
+                       const double *val, const cx<double> *v, const double *F, double sigma, cx<double> *out,
+                       double *hout, hipStream_t s);
 hipError_t launch_obs_apply(int nobs, const int *item, const double *val, int ncol, const cx<double> *X,
                             cx<double> *out, hipStream_t s);
 hipError_t launch_adjoint_rhs(int n, int nobs, const int *stack, const int *item, const double *val, cx<double> *B,

@@ -194,6 +194,16 @@ int afx_prepobscoeff(afx_system s, int nobs, const double *xyzobs, int *item, do
 int afx_obs_apply_device(afx_system s, int nobs, const int *item, const double *val, int ncol, const void *dX,
                          double *out);
 
+/* IXYHOUT (src/solver_mpi/n_ebfem_bxyz_mpi.f90:192-301): seafloor current at nobs points.
+   xyzobs (nobs, 3) [km]; exy (nobs, 2) complex E at the points (the reference rebuilds it from
+   CALOBSRESP's amplitude and phase, :219-225); coeff_vF as CSR (stack[nobs + 1] from 0, item
+   1-based node ids, val); vxyz (node, 3) complex [m/s] and fxyz (node, 3) [nT] (readvxyz /
+   readfxyz, m_readfvxyz.f90); sigma the ocean conductivity (the reference's literal 3.3, a
+   single-precision value, :221). Out (host): out (nobs, 8) complex = ixyh_e, ixyh_vF, ixyh and
+   vh, each (x, y); h (nobs) [km]. Does not need a system handle. */
+int afx_ixyh(int nobs, const double *xyzobs, const double *exy, const int *stack, const int *item, const double *val,
+             int node, const double *vxyz, const double *fxyz, double sigma, double *out, double *h);
+
 /* ---- adjoint Jacobian -------------------------------------------------------------- */
 
 /* Adjoint right-hand sides of solvePARDISOinv (src/src_inv/solvePARDISOinv.f90:176-187): the

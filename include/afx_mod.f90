@@ -7,6 +7,7 @@
 !   call afx_cocg13_single_f(...)   replaces   call COCG13_single(...)   (COCG13_single.f90:9-36)
 !   call afx_solve_cg_f(...)        replaces   call solveCG(...)         (forward_bxyz_mpi.f90:70-94)
 !   afx_solve(...) on a handle      replaces   call solvePARDISO(...)    (solvePARDISO.f90:6-16)
+!   afx_ixyh(...)                   computes   IXYHOUT's seafloor currents (n_ebfem_bxyz_mpi.f90:192-268)
 module afx
   use iso_c_binding
   implicit none
@@ -100,6 +101,21 @@ module afx
       complex(c_double_complex), intent(in) :: D(*), AL(*), AU(*), rhs(*)
       integer(c_int), intent(in) :: INL(*), IAL(*), INU(*), IAU(*)
       complex(c_double_complex), intent(out) :: X(*)
+    end function
+
+    ! IXYHOUT's arrays (n_ebfem_bxyz_mpi.f90:192-268) in the reference's own shapes:
+    ! xyzobs(3,nobs), exy(2,nobs), vxyz(3,node), fxyz(3,node); coeff_vF as stack(0:nobs) from 0,
+    ! item (1-based nodes), val. out(2,4,nobs) holds ixyh_e, ixyh_vF, ixyh, vh_obs for each point.
+    integer(c_int) function afx_ixyh(nobs, xyzobs, exy, stack, item, val, node, vxyz, fxyz, sigma, out, h) &
+                                     bind(C, name="afx_ixyh")
+      import :: c_int, c_double, c_double_complex
+      integer(c_int), value :: nobs, node
+      real(c_double), intent(in) :: xyzobs(*), val(*), fxyz(*)
+      complex(c_double_complex), intent(in) :: exy(*), vxyz(*)
+      integer(c_int), intent(in) :: stack(*), item(*)
+      real(c_double), value :: sigma
+      complex(c_double_complex), intent(out) :: out(*)
+      real(c_double), intent(out) :: h(*)
     end function
   end interface
 

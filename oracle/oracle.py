@@ -203,6 +203,51 @@ def obs_apply(item, val, x):
     return out
 
 
+def ixyhout(xyzobs, exy, stack, item, val, vxyz, fxyz, sigma):
+    """IXYHOUT (src/solver_mpi/n_ebfem_bxyz_mpi.f90:213-268), restated in numpy in the reference's
+    order of operations: h = -z, 0 on land (:227-231); vF = cr_outer(v, F) at the nodes
+    (m_outerinnerproduct.f90:45-53, :240-242); vF_obs, v_obs = mul_matcrs_cv rows in item order
+    (m_matrix.f90:83-102, :244-251); ixyh_e = (sigma h) E; ixyh_vF = ((sigma h) vF) 1e-3;
+    ixyh = (sigma h) (E + vF 1e-3); vh = h v (:261-268). Real and imaginary parts are computed
+    separately (a real times a complex). Returns (ixyh_e, ixyh_vF, ixyh, vh) each (nobs, 2)
+    complex, and h (nobs,)."""
+    xo = np.asarray(xyzobs, np.float64).reshape(-1, 3)
+    nobs = xo.shape[0]
+    exy = np.asarray(exy, np.complex128).reshape(nobs, 2)
+    v = np.asarray(vxyz, np.complex128).reshape(-1, 3)
+    F = np.asarray(fxyz, np.float64).reshape(-1, 3)
+    out = [np.zeros((nobs, 2), np.complex128) for _ in range(4)]
+    hs = np.zeros(nobs)
+    for o in range(nobs):
+        h = -xo[o, 2]
+        if h <= 0.0:
+            h = 0.0
+        vf = [[0.0, 0.0], [0.0, 0.0]]
+        vh = [[0.0, 0.0], [0.0, 0.0]]
+        for q in range(stack[o], stack[o + 1]):
+            nd = item[q] - 1
+            a = float(val[q])
+            vx, vy, vz = v[nd]
+            Fx, Fy, Fz = (float(t) for t in F[nd])
+            c0 = (vy.real * Fz - vz.real * Fy, vy.imag * Fz - vz.imag * Fy)
+            c1 = (vz.real * Fx - vx.real * Fz, vz.imag * Fx - vx.imag * Fz)
+            for c, cc in ((0, c0), (1, c1)):
+                vf[c][0] = vf[c][0] + a * cc[0]
+                vf[c][1] = vf[c][1] + a * cc[1]
+            for c, w in ((0, vx), (1, vy)):
+                vh[c][0] = vh[c][0] + a * w.real
+                vh[c][1] = vh[c][1] + a * w.imag
+        sh = sigma * h
+        for c in range(2):
+            e = exy[o, c]
+            out[0][o, c] = complex(sh * e.real, sh * e.imag)
+            out[1][o, c] = complex(sh * vf[c][0] * 1.0e-3, sh * vf[c][1] * 1.0e-3)
+            out[2][o, c] = complex(sh * (e.real + vf[c][0] * 1.0e-3), sh * (e.imag + vf[c][1] * 1.0e-3))
+            out[3][o, c] = complex(h * vh[c][0], h * vh[c][1])
+        hs[o] = h
+    return out[0], out[1], out[2], out[3], hs
+
+
 def jacobian(u, x, par_ptr, par_tet, case, omega, dsig_factor, sigma=None):
     """genjacobian1 restated (m_jacobian_ap.f90:174-386): dbe (nobs, npar) complex."""
     u = _c128(u)

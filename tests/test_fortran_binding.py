@@ -46,6 +46,36 @@ program t
 end program
 """
 
+PROG_IXYH = """
+program t
+  use afx
+  implicit none
+  integer(4), parameter :: nobs = 3, node = 4
+  real(8) :: xyzobs(3,nobs), fxyz(3,node), val(6), h(nobs)
+  complex(8) :: exy(2,nobs), vxyz(3,node), out(2,4,nobs)
+  integer(4) :: stack(0:nobs), item(6), i, rc
+  real(4) :: sig4
+  sig4 = 3.3
+  do i = 1, nobs
+    xyzobs(:,i) = (/ 1.d0*i, -2.d0*i, -0.5d0*i /)
+    exy(:,i) = (/ (1.d0, 2.d0)*i, (-3.d0, 0.5d0) /)
+  end do
+  xyzobs(3,3) = 0.25d0
+  do i = 1, node
+    vxyz(:,i) = (/ (0.2d0, -0.1d0), (0.05d0, 0.3d0), (0.d0, 0.d0) /)
+    fxyz(:,i) = (/ 20000.d0, -5000.d0, 40000.d0 /)
+  end do
+  stack = (/ 0, 2, 4, 6 /)
+  item = (/ 1, 2, 2, 3, 3, 4 /)
+  val = 0.5d0
+  rc = afx_ixyh(nobs, xyzobs, exy, stack, item, val, node, vxyz, fxyz, dble(sig4), out, h)
+  open(newunit=i, file='ixyh.bin', access='stream', form='unformatted', status='replace')
+  write(i) out, h
+  close(i)
+  print '(i0)', rc
+end program
+"""
+
 
 def _build(tmp, src):
     if not os.path.exists(FLANG):
@@ -89,3 +119,27 @@ def test_fortran_dropin_cocg13_single(tmp_path):
     assert int(err) == 0 and float(resid) <= 1e-8
     assert abs(int(it) - it_ref) <= max(3, 0.08 * it_ref)
     assert np.linalg.norm(x - g["x"]) <= 1e-5 * np.linalg.norm(g["x"])
+
+
+@pytest.mark.gpu
+def test_fortran_ixyh_reference_shapes(tmp_path):
+    """afx_ixyh called from Fortran with IXYHOUT's own array shapes: uniform v and F make
+    ixyh_vF = sigma h (v x F) 1e-3 and vh = h v; the third point is on land (h = 0)."""
+    exe = _build(str(tmp_path), PROG_IXYH)
+    r = subprocess.run([exe], capture_output=True, text=True, cwd=str(tmp_path), timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert int(r.stdout.split()[0]) == 0
+    raw = np.fromfile(os.path.join(str(tmp_path), "ixyh.bin"), np.uint8)
+    out = raw[:3 * 8 * 16].view(np.complex128).reshape(3, 4, 2)
+    h = raw[3 * 8 * 16:].view(np.float64)
+    sig = float(np.float32(3.3))
+    v = np.array([0.2 - 0.1j, 0.05 + 0.3j, 0.0])
+    F = np.array([20000.0, -5000.0, 40000.0])
+    vf = np.cross(v, F)[:2] * 1e-3
+    assert np.array_equal(h, [0.5, 1.0, 0.0])
+    for o in range(3):
+        e = np.array([(1 + 2j) * (o + 1), -3 + 0.5j])
+        assert np.allclose(out[o, 0], sig * h[o] * e, rtol=1e-15, atol=0)
+        assert np.allclose(out[o, 1], sig * h[o] * vf, rtol=1e-14, atol=0)
+        assert np.allclose(out[o, 2], out[o, 0] + out[o, 1], rtol=1e-15, atol=0)
+        assert np.allclose(out[o, 3], h[o] * v[:2], rtol=1e-15, atol=0)

@@ -81,6 +81,7 @@ SIGNATURES = {
     "afx_autotune": ([_VP, _I, _I], _I),
     "afx_kernel_timing": ([_VP, _I, _VP, _VP, _VP], _I),
     "afx_spmv": ([_VP, _VP, _VP], _I),
+    "afx_spmv_device": ([_VP, _VP, _VP, _VP], _I),
     "afx_precond_apply": ([_VP, _VP, _VP], _I),
     "afx_get_precond_diag": ([_VP, _VP], _I),
     "afx_cocg13_single": ([_I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.c_char_p, _D, _D,
@@ -272,6 +273,18 @@ class System:
         y = np.zeros(self.n, np.complex128)
         _err(lib().afx_spmv(self.h, _p(x), _p(y)))
         return y
+
+    def spmv_device(self, dX, dY, timed=False):
+        """dY = A dX on the device (torch complex128 tensors of n elements on the GPU). Returns the
+        launch's duration in ms (HIP events on the library's stream) when timed, else None."""
+        import torch
+        for t in (dX, dY):
+            if t.dtype != torch.complex128 or not t.is_cuda or not t.is_contiguous() or t.numel() != self.n:
+                raise ValueError("dX, dY: contiguous complex128 GPU tensors of n elements")
+        ms = C.c_double()
+        _err(lib().afx_spmv_device(self.h, C.c_void_p(dX.data_ptr()), C.c_void_p(dY.data_ptr()),
+                                   C.byref(ms) if timed else None))
+        return ms.value if timed else None
 
     def autotune(self, nrhs=1, verbose=False):
         _err(lib().afx_autotune(self.h, nrhs, 1 if verbose else 0))

@@ -87,6 +87,7 @@ struct DevSide {
 struct afx_system_s {
   int n = 0;
   long long nU = 0, nL = 0, nnz = 0;
+  int maxrow = 0;  // longest full-CSR row
   std::vector<int> INU, IAU, INL, IAL;  // reference layout (IAU/IAL 1-based)
   int nlevL = 0, nlevU = 0;
   hipStream_t stream = nullptr;
@@ -201,6 +202,8 @@ int build_device_pattern(afx_system s) {
     }
   }
   rowptr[n] = (int)pos;
+  s->maxrow = 0;
+  for (int i = 0; i < n; i++) s->maxrow = std::max(s->maxrow, rowptr[i + 1] - rowptr[i]);
   // sweeps
   std::vector<int> Lc(nL), Uc(nU);
   for (long long q = 0; q < nL; q++) Lc[q] = s->IAL[q] - 1;
@@ -1351,26 +1354,45 @@ int afx_spmv(afx_system s, const double *x, double *y) {
   int rc = ensure_derived(s);
   if (rc) return rc;
   cx<double> *dx = nullptr, *dy = nullptr;
-  double *dw = nullptr;
-  const double w = s->omega;
-  if ((rc = upload(&dx, (const cx<double> *)x, s->n)) || (rc = dalloc(&dy, s->n)) || (rc = upload(&dw, &w, 1))) {
-    dfree(dx); dfree(dy); dfree(dw);
+  if ((rc = upload(&dx, (const cx<double> *)x, s->n)) || (rc = dalloc(&dy, s->n))) {
+    dfree(dx); dfree(dy);
     return rc;
   }
-  afx::SpmvParams sp{};
-  sp.n = s->n;
-  sp.k = 1;
-  sp.kp = 1;
-  sp.rowptr = s->d_rowptr;
-  sp.col = s->d_col;
-  sp.val = s->d_val64;
-  sp.wcol = dw;
-  sp.X = dx;
-  sp.Y = dy;
-  hipError_t e = afx::launch_spmv(sp, s->stream);
+  hipError_t e = afx::launch_spmv_stream(s->n, s->maxrow, s->d_rowptr, s->d_col, s->d_val64, s->omega, dx, dy, s->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(y, dy, (size_t)s->n * 16, hipMemcpyDeviceToHost, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-  dfree(dx); dfree(dy); dfree(dw);
+  dfree(dx); dfree(dy);
+  if (e != hipSuccess) return fail(AFX_ERR_HIP, hipGetErrorString(e));
+  return AFX_OK;
+}
+
+int afx_spmv_device(afx_system s, const void *dX, void *dY, double *kernel_ms) {
+  if (!s || !dX || !dY) return fail(AFX_ERR_ARG, "null argument");
+  if (dX == dY) return fail(AFX_ERR_ARG, "x and y must not overlap");
+  int rc = ensure_derived(s);
+  if (rc) return rc;
+  if (s->nnz < 1) return fail(AFX_ERR_ARG, "empty matrix");
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  if (kernel_ms) {
+    HIPCHK(hipEventCreate(&ev0));
+    if (hipEventCreate(&ev1) != hipSuccess) {
+      (void)hipEventDestroy(ev0);
+      return fail(AFX_ERR_HIP, "hipEventCreate");
+    }
+  }
+  hipError_t e = kernel_ms ? hipEventRecord(ev0, s->stream) : hipSuccess;
+  if (e == hipSuccess)
+    e = afx::launch_spmv_stream(s->n, s->maxrow, s->d_rowptr, s->d_col, s->d_val64, s->omega, (const cx<double> *)dX,
+                                (cx<double> *)dY, s->stream);
+  if (kernel_ms) {
+    if (e == hipSuccess) e = hipEventRecord(ev1, s->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ev1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0, ev1);
+    *kernel_ms = ms;
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+  }
   if (e != hipSuccess) return fail(AFX_ERR_HIP, hipGetErrorString(e));
   return AFX_OK;
 }

@@ -123,6 +123,8 @@ hipError_t launch_cocg32(const CocgParams<float> &p, int lpr, int grid, hipStrea
 int max_coop_grid(int lpr, bool multi, bool fp64, size_t dyn);
 int max_coop_grid_ic();
 hipError_t launch_spmv(const SpmvParams &p, hipStream_t s);
+hipError_t launch_spmv_stream(int n, int maxrow, const int *rowptr, const int *col, const cx<double> *val, double w,
+                              const cx<double> *X, cx<double> *Y, hipStream_t s);
 hipError_t launch_residual_norms(const SpmvParams &p, int grid, double *res_out_dev, hipStream_t s);
 hipError_t launch_precond_setup(int n, int nf, int kind, double sd, double sig, const cx<double> *Dkm,
                                 const int *Dpos_full, const cx<double> *full_val, const int *rowptr,

@@ -11,6 +11,9 @@ on the device at the end) with SSOR-preconditioned COCG in FP64 (SSOR is the pre
 reference's solveCG passes; its 'IC' stagnates on this system). The whole COCG iteration runs in
 one persistent launch per 2000 iterations.
 
+After the solve, the SpMV q = A p runs as a kernel of its own (k_spmv_stream, 50 timed launches
+on the solution vector); the line's "spmv" object gives its algorithmic GB/s against the HBM peak.
+
 Variants, off by default because each adds minutes (their results are documented):
   --mixed    complex-FP32 preconditioner inside FP64 refinement (same tolerance);
   --batched  the 5 tidal constituents (M2, S2, N2, K1, O1) in one launch, per-column omega.
@@ -53,6 +56,12 @@ def bytes_per_iteration(n, nU, nL, k, z_bytes):
     return spmv + sweeps + vec
 
 
+def spmv_bytes(n, nnz):
+    """Compulsory HBM bytes of one SpMV over the full CSR: (K, M) values 16 B and a column 4 B per
+    entry, the row pointers, x read and y written (16 B per row each)."""
+    return nnz * 20 + 4 * (n + 1) + 2 * 16 * n
+
+
 def measured_traffic(workload, iterations_per_launch):
     """HBM bytes per launch of k_cocg from the committed rocprofv3 --pmc summary of this workload
     (FETCH_SIZE and WRITE_SIZE per COCG iteration, scaled by the counters' reading for a copy of
@@ -75,6 +84,7 @@ def main():
     ap.add_argument("--mixed", action="store_true", help="also time the complex-FP32-preconditioner solve")
     ap.add_argument("--batched", action="store_true", help="also time the 5 constituents in one launch")
     ap.add_argument("--maxit", type=int, default=400000, help="iteration cap of each solve")
+    ap.add_argument("--spmv-launches", type=int, default=50, help="timed launches of the standalone SpMV")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="after the solves, copy this many bytes device-to-device (counter calibration)")
     args = ap.parse_args()
@@ -140,6 +150,21 @@ def main():
                 iterations_per_launch=round(ipl, 1), us_per_iteration=round(1e3 * kms / max(iters, 1), 1),
                 traffic_source=tsrc)
 
+    # the SpMV (q = A p) as a kernel of its own, on the solution vector: algorithmic bytes of one
+    # product over the average launch duration (HIP events on the library's stream)
+    dQ = torch.empty_like(dX)
+    for _ in range(5):
+        s.spmv_device(dX, dQ, timed=True)
+    sp_ms = [s.spmv_device(dX, dQ, timed=True) for _ in range(args.spmv_launches)]
+    nnz = n + s.npu + s.npl
+    sp_bytes = spmv_bytes(n, nnz)
+    sp_avg = sum(sp_ms) / len(sp_ms)
+    spmv = dict(kernel="k_spmv_stream", bound="hbm", achieved=round(sp_bytes / (sp_avg / 1e3) / 1e9, 1),
+                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(sp_bytes / (sp_avg / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                bytes_per_launch=sp_bytes, avg_launch_us=round(1e3 * sp_avg, 2),
+                min_launch_us=round(1e3 * min(sp_ms), 2), launches=len(sp_ms))
+    log(f"SpMV: {1e3 * sp_avg:.1f} us per launch, {spmv['achieved']:.0f} GB/s ({100 * spmv['frac']:.1f}% of peak)")
+
     variants = {}
     if args.mixed:
         dXm = torch.zeros(n, dtype=torch.complex128, device=dev)
@@ -197,6 +222,7 @@ def main():
         "iters": its,
         "true_residual": true_res,
         "roofline": roof,
+        "spmv": spmv,
         "cpu_baseline": cpu,
         "variants": variants or None,
     }

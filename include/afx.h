@@ -160,6 +160,12 @@ int afx_autotune(afx_system s, int nrhs, int verbose);
 /* y = A x (complex, host arrays) with the device SpMV -- parity hook. */
 int afx_spmv(afx_system s, const double *x, double *y);
 
+/* q = A p on the device, one column (COCG13_single.f90:270-290: the D, upper and lower entries of
+   each row, partial sums of lane-strided entries added in a fixed tree): dX, dY device arrays of
+   n complex values at the system's frequency, not overlapping. Runs on the system's stream. With kernel_ms non-null it records HIP
+   events around the launch on that stream, waits for them and stores the launch's duration. */
+int afx_spmv_device(afx_system s, const void *dX, void *dY, double *kernel_ms);
+
 /* z = M^-1 r with the device preconditioner (after afx_precond_setup) -- parity hook. */
 int afx_precond_apply(afx_system s, const double *r, double *z);
 

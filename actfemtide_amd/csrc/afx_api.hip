@@ -120,6 +120,10 @@ struct afx_system_s {
   std::vector<double> h_xyz;
   std::vector<int> h_n4;
   bool have_mesh = false;
+  // bucket grid of the tets for point location (built on the first afx_prepobscoeff)
+  afx::PointGrid pgrid{};
+  int *d_bptr = nullptr, *d_btet = nullptr;
+  bool have_pgrid = false;
   int *d_cptr = nullptr, *d_cidx = nullptr, *d_rptr = nullptr, *d_ridx = nullptr, *d_mirror = nullptr;
   bool have_slots = false;
   cx<double> *d_rhs_unit = nullptr;  // tidal RHS of the last assembly
@@ -922,6 +926,7 @@ void release(afx_system s) {
   dfree(s->d_val32); dfree(s->d_Lval32); dfree(s->d_Uval32);
   dfree(s->d_DD); dfree(s->d_DD32);
   dfree(s->d_xyz); dfree(s->d_fxyz); dfree(s->d_n4); dfree(s->d_n6); dfree(s->d_group); dfree(s->d_vxyz);
+  dfree(s->d_bptr); dfree(s->d_btet);
   dfree(s->d_cptr); dfree(s->d_cidx); dfree(s->d_rptr); dfree(s->d_ridx); dfree(s->d_mirror);
   dfree(s->d_rhs_unit);
   free_side(s->fw);
@@ -1213,6 +1218,8 @@ int afx_set_mesh(afx_system s, int node, int ntet, const double *xyz, const int 
   for (long long q = 0; q < (long long)ntet * 4; q++)
     if (n4[q] < 1 || n4[q] > node) return fail(AFX_ERR_ARG, "n4 entry out of range");
   dfree(s->d_xyz); dfree(s->d_fxyz); dfree(s->d_n4); dfree(s->d_n6); dfree(s->d_group); dfree(s->d_vxyz);
+  dfree(s->d_bptr); dfree(s->d_btet);
+  s->have_pgrid = false;
   int rc;
   std::vector<double> zero3((size_t)node * 3, 0.0), zero6((size_t)node * 6, 0.0);
   if ((rc = upload(&s->d_xyz, xyz, (size_t)node * 3))) return rc;
@@ -1780,21 +1787,79 @@ int afx_solve_cg(int doftot, int NPL, int NPU, const double *D, const double *AL
   return rc;
 }
 
+// The point-location grid of the mesh: buckets of about 8 tets (by centroid) over the node
+// bounding box, each bucket's tets in increasing id, and the largest centroid-to-vertex extent
+// per axis (the search radius).
+static int build_point_grid(afx_system s) {
+  const int ntet = s->ntet, node = s->node;
+  afx::PointGrid g{};
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; d++) lo[d] = hi[d] = s->h_xyz[d];
+  for (int v = 0; v < node; v++)
+    for (int d = 0; d < 3; d++) {
+      lo[d] = std::min(lo[d], s->h_xyz[(size_t)v * 3 + d]);
+      hi[d] = std::max(hi[d], s->h_xyz[(size_t)v * 3 + d]);
+    }
+  double ext[3], vol = 1.0;
+  for (int d = 0; d < 3; d++) {
+    ext[d] = hi[d] - lo[d];
+    if (!(ext[d] > 0.0)) ext[d] = 1.0;
+    vol *= ext[d];
+  }
+  const double nb = std::max(1.0, ntet / 8.0);
+  const double h0 = std::cbrt(vol / nb);
+  for (int d = 0; d < 3; d++) {
+    g.n[d] = (int)std::max(1.0, std::min(1024.0, std::ceil(ext[d] / h0)));
+    g.h[d] = ext[d] / g.n[d];
+    g.lo[d] = lo[d];
+    g.R[d] = 0.0;
+  }
+  const long long nbk = (long long)g.n[0] * g.n[1] * g.n[2];
+  std::vector<int> key(ntet), bptr(nbk + 1, 0), btet(ntet);
+  for (int t = 0; t < ntet; t++) {
+    double x[4][3], c[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < 4; j++)
+      for (int d = 0; d < 3; d++) {
+        x[j][d] = s->h_xyz[(size_t)(s->h_n4[(size_t)t * 4 + j] - 1) * 3 + d];
+        c[d] += 0.25 * x[j][d];
+      }
+    int ib[3];
+    for (int d = 0; d < 3; d++) {
+      for (int j = 0; j < 4; j++) g.R[d] = std::max(g.R[d], std::fabs(x[j][d] - c[d]));
+      ib[d] = (int)std::max(0.0, std::min(std::floor((c[d] - g.lo[d]) / g.h[d]), (double)(g.n[d] - 1)));
+    }
+    key[t] = (ib[2] * g.n[1] + ib[1]) * g.n[0] + ib[0];
+    bptr[key[t] + 1]++;
+  }
+  for (long long b = 0; b < nbk; b++) bptr[b + 1] += bptr[b];
+  std::vector<int> fill(bptr.begin(), bptr.end() - 1);
+  for (int t = 0; t < ntet; t++) btet[fill[key[t]]++] = t;  // increasing id within a bucket
+  dfree(s->d_bptr);
+  dfree(s->d_btet);
+  int rc;
+  if ((rc = upload(&s->d_bptr, bptr.data(), (size_t)nbk + 1)) || (rc = upload(&s->d_btet, btet.data(), (size_t)ntet)))
+    return rc;
+  s->pgrid = g;
+  s->have_pgrid = true;
+  return AFX_OK;
+}
+
 int afx_prepobscoeff(afx_system s, int nobs, const double *xyzobs, int *item, double *val_e, double *val_b,
                      int *iele) {
   if (!s || !xyzobs || !item || !val_e || !val_b) return fail(AFX_ERR_ARG, "null argument");
   if (!s->have_mesh) return fail(AFX_ERR_ARG, "afx_set_mesh first");
   if (nobs <= 0) return AFX_OK;
+  int rc;
+  if (!s->have_pgrid && (rc = build_point_grid(s))) return rc;
   double *dp = nullptr;
   int *dfirst = nullptr;
-  int rc;
   if ((rc = upload(&dp, xyzobs, (size_t)nobs * 3)) || (rc = dalloc(&dfirst, nobs))) {
     dfree(dp);
     return rc;
   }
   std::vector<int> first(nobs, 0x7fffffff);
-  hipError_t e = hipMemcpy(dfirst, first.data(), nobs * sizeof(int), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = afx::launch_findelement(nobs, dp, s->ntet, s->d_xyz, s->d_n4, dfirst, s->stream);
+  hipError_t e = afx::launch_findelement_grid(nobs, dp, s->pgrid, s->d_bptr, s->d_btet, s->d_xyz, s->d_n4, dfirst,
+                                              s->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(first.data(), dfirst, nobs * sizeof(int), hipMemcpyDeviceToHost, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   dfree(dp);

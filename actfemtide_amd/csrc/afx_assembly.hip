@@ -234,16 +234,24 @@ hipError_t launch_genbc(int nline, const int *line, const double *xyz, double zm
   return hipGetLastError();
 }
 
-// FINDELEMENT0 (m_mesh_type.f90:697-734): the FIRST tet (lowest id) whose barycentric
-// coordinates (nodebasisfun, m_fem_util.f90:56-83) are all >= 0; first[o] starts at INT_MAX.
-__global__ void k_findelement(int nobs, const double *pts, int ntet, const double *xyz, const int *n4, int *first) {
-  const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
-  const int t = (int)(gid % ntet);
-  const int o = (int)(gid / ntet);
-  if (o >= nobs) return;
+// nodebasisfun (m_fem_util.f90:56-83) at P for tet t and FINDELEMENT0's test (m_mesh_type.f90:715-719):
+// all four barycentric coordinates >= 0. Evaluated without contraction, in the reference's
+// operation order (helpers included), so the decision on a shared face, edge or node is the
+// reference's.
+__device__ __forceinline__ double inner3_nc(const double *a, const double *b) {
+#pragma clang fp contract(off)
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+__device__ __forceinline__ void outer3_nc(const double *a, const double *b, double *c) {
+#pragma clang fp contract(off)
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ bool tet_contains(const double *xyz, const int *n4, int t, const double *P) {
+#pragma clang fp contract(off)
   double x[4][3];
   load_tet(xyz, n4, t, x);
-  const double *P = &pts[o * 3];
   double v0[3], v1[3], x12[3], x13[3], x14[3], x24[3], x23[3], c[3];
   for (int d = 0; d < 3; d++) {
     v0[d] = -(x[0][d] - P[d]);
@@ -254,24 +262,55 @@ __global__ void k_findelement(int nobs, const double *pts, int ntet, const doubl
     x24[d] = x[3][d] - x[1][d];
     x23[d] = x[2][d] - x[1][d];
   }
-  outer3(x12, x13, c);
-  const double a5 = 1.0 / 6.0 * inner3(c, x14);
-  outer3(x12, x13, c);
-  const double a4 = 1.0 / 6.0 * inner3(c, v0) / a5;
-  outer3(x14, x12, c);
-  const double a3 = 1.0 / 6.0 * inner3(c, v0) / a5;
-  outer3(x13, x14, c);
-  const double a2 = 1.0 / 6.0 * inner3(c, v0) / a5;
-  outer3(x24, x23, c);
-  const double a1 = 1.0 / 6.0 * inner3(c, v1) / a5;
-  if (a1 >= 0.0 && a2 >= 0.0 && a3 >= 0.0 && a4 >= 0.0) atomicMin(&first[o], t);
+  outer3_nc(x12, x13, c);
+  const double a5 = 1.0 / 6.0 * inner3_nc(c, x14);
+  const double a4 = 1.0 / 6.0 * inner3_nc(c, v0) / a5;
+  outer3_nc(x14, x12, c);
+  const double a3 = 1.0 / 6.0 * inner3_nc(c, v0) / a5;
+  outer3_nc(x13, x14, c);
+  const double a2 = 1.0 / 6.0 * inner3_nc(c, v0) / a5;
+  outer3_nc(x24, x23, c);
+  const double a1 = 1.0 / 6.0 * inner3_nc(c, v1) / a5;
+  return a1 >= 0.0 && a2 >= 0.0 && a3 >= 0.0 && a4 >= 0.0;
 }
-hipError_t launch_findelement(int nobs, const double *pts, int ntet, const double *xyz, const int *n4, int *first,
-                              hipStream_t s) {
-  if (nobs <= 0 || ntet <= 0) return hipSuccess;
-  const long long m = (long long)nobs * ntet;
-  hipLaunchKernelGGL(k_findelement, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, nobs, pts, ntet, xyz, n4,
-                     first);
+
+// FINDELEMENT0 (m_mesh_type.f90:697-734) over a bucket grid (the reference's own search,
+// findelewithgrid :204-277, lists tets by the bucket of their centroid). One wave per point: it
+// tests every tet whose centroid bucket lies within the largest centroid-to-vertex extent of
+// the point (plus one bucket for rounding), so every tet containing the point is tested, and
+// keeps the lowest containing id -- the tet FINDELEMENT0's scan in id order returns. Tets of a
+// run of buckets along x are contiguous in `btet`, so the lanes stride over each run.
+__global__ void __launch_bounds__(256) k_findelement_grid(int nobs, const double *pts, PointGrid g,
+                                                          const int *bptr, const int *btet, const double *xyz,
+                                                          const int *n4, int *first) {
+  const int o = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  if (o >= nobs) return;  // wave-uniform
+  const double P[3] = {pts[o * 3], pts[o * 3 + 1], pts[o * 3 + 2]};
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; d++) {
+    const double a = floor((P[d] - g.R[d] - g.lo[d]) / g.h[d]) - 1.0;
+    const double b = floor((P[d] + g.R[d] - g.lo[d]) / g.h[d]) + 1.0;
+    lo[d] = (int)fmax(0.0, fmin(a, (double)(g.n[d] - 1)));
+    hi[d] = (int)fmax(0.0, fmin(b, (double)(g.n[d] - 1)));
+  }
+  int best = 0x7fffffff;
+  for (int iz = lo[2]; iz <= hi[2]; iz++)
+    for (int iy = lo[1]; iy <= hi[1]; iy++) {
+      const int b0 = (iz * g.n[1] + iy) * g.n[0];
+      for (int q = bptr[b0 + lo[0]] + lane; q < bptr[b0 + hi[0] + 1]; q += 64) {
+        const int t = btet[q];
+        if (t < best && tet_contains(xyz, n4, t, P)) best = t;
+      }
+    }
+  for (int m = 32; m >= 1; m >>= 1) best = min(best, __shfl_xor(best, m, 64));
+  if (lane == 0) first[o] = best;
+}
+hipError_t launch_findelement_grid(int nobs, const double *pts, const PointGrid &g, const int *bptr, const int *btet,
+                                   const double *xyz, const int *n4, int *first, hipStream_t s) {
+  if (nobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_findelement_grid, dim3((unsigned)((nobs + 3) / 4)), dim3(256), 0, s, nobs, pts, g, bptr, btet,
+                     xyz, n4, first);
   return hipGetLastError();
 }
 

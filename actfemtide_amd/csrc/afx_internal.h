@@ -117,6 +117,13 @@ struct SpmvParams {
   double *part_rr, *part_bb;  // optional [G][k] partial |Y|^2 and |B|^2
 };
 
+// Bucket grid of the tets for point location: n[d] buckets of width h[d] from lo[d]; a tet is
+// listed in the bucket of its centroid; R[d] is the largest |vertex - centroid| along axis d.
+struct PointGrid {
+  double lo[3], h[3], R[3];
+  int n[3];
+};
+
 // launchers (afx_solver.hip)
 hipError_t launch_cocg64(const CocgParams<double> &p, int lpr, int grid, hipStream_t s);
 hipError_t launch_cocg32(const CocgParams<float> &p, int lpr, int grid, hipStream_t s);
@@ -174,8 +181,8 @@ hipError_t launch_bc_values(int n, const int *INU, const int *IAU, const int *IN
                             hipStream_t s);
 hipError_t launch_genbc(int nline, const int *line, const double *xyz, double zmin, double zmax, double xout,
                         double yout, unsigned char *bc, hipStream_t s);
-hipError_t launch_findelement(int nobs, const double *pts, int ntet, const double *xyz, const int *n4,
-                              int *first, hipStream_t s);
+hipError_t launch_findelement_grid(int nobs, const double *pts, const PointGrid &g, const int *bptr, const int *btet,
+                                   const double *xyz, const int *n4, int *first, hipStream_t s);
 hipError_t launch_ixyh(int nobs, const double *zobs, const cx<double> *exy, const int *ptr, const int *item,
                        const double *val, const cx<double> *v, const double *F, double sigma, cx<double> *out,
                        double *hout, hipStream_t s);

@@ -94,6 +94,7 @@ SIGNATURES = {
     "afx_jacobian_device": ([_VP, _I, _VP, _VP, _I, _VP, _VP, _VP, _D, _D, _VP], _I),
     "afx_jacobian_ampphase_device": ([_I, _I, _VP, _VP, _VP], _I),
     "afx_gn_update_device": ([_I, _I, _VP, _VP, _VP, _VP, _VP, _D, _VP, _VP, _VP, _VP, _VP, _VP], _I),
+    "afx_spectral_alpha_device": ([_I, _I, _VP, _VP, _VP, _VP, _VP, _I, _VP, _VP, _VP], _I),
     "afx_dgemm_nt_device": ([_I, _I, _I, _VP, _I, _VP, _I, _VP, _I, _I], _I),
     "afx_inv_last_error": ([], C.c_char_p),
 }
@@ -514,6 +515,20 @@ def gn_update(dJ, Cm, cd, alpha, dobs, dcal, model, model_ref):
                                         float(alpha), _p(dobs), _p(dcal), _p(model), _p(model_ref), _p(out),
                                         _p(beta)))
     return out, beta
+
+
+def spectral_alpha(dJ, Cm, cd, kmax=30):
+    """alphaspectralradius_v2 + LANCZOSMINMAX (src/src_inv_ap/m_spectral.f90:63-179) on the device:
+    alpha = rho(J Cm J^T) / max(cd) from kmax Lanczos steps. dJ a (ndat, nmodel) float64 torch
+    device tensor, Cm scipy sparse symmetric, cd host. Returns (alpha, emin, emax)."""
+    import scipy.sparse as sp
+    ndat, nmodel = dJ.shape
+    Cm = sp.csr_matrix(Cm)
+    al, lo, hi = C.c_double(), C.c_double(), C.c_double()
+    _inv_err(lib().afx_spectral_alpha_device(ndat, nmodel, C.c_void_p(dJ.data_ptr()), _p(_i32(Cm.indptr)),
+                                             _p(_i32(Cm.indices)), _p(_f64(Cm.data)), _p(_f64(cd)), int(kmax),
+                                             C.byref(al), C.byref(lo), C.byref(hi)))
+    return al.value, lo.value, hi.value
 
 
 def forward_system(case, omega: float, sigma_tet=None) -> tuple:

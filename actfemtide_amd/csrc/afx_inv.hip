@@ -7,6 +7,7 @@
 // device and the two triangular solves run in one block each.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -290,6 +291,108 @@ int afx_gn_update_device(int ndat, int nmodel, const double *dJ, const int *cm_p
   if (!ok) return fail(AFX_ERR_HIP, "gn_update: HIP error");
   if (bad) return fail(AFX_ERR_BREAKDOWN, "gn_update: C is not positive definite");
   for (int i = 0; i < nmodel; i++) model_new[i] += model_ref[i];
+  return AFX_OK;
+}
+
+// Extreme eigenvalues of the symmetric tridiagonal matrix (diagonal a[0..m), off-diagonal
+// b[0..m-1)) by bisection on Sturm sequence counts (the number of eigenvalues below x is the
+// number of negative pivots of T - x I), from the Gershgorin interval down to adjacent doubles.
+static int sturm_below(const std::vector<double> &a, const std::vector<double> &b, double x) {
+  int cnt = 0;
+  double d = 1.0;
+  for (size_t i = 0; i < a.size(); i++) {
+    const double bb = i ? b[i - 1] * b[i - 1] : 0.0;
+    d = a[i] - x - (i ? bb / d : 0.0);
+    if (d == 0.0) d = -1e-300;  // a zero pivot counts as negative (x is an eigenvalue: move below it)
+    if (d < 0.0) cnt++;
+  }
+  return cnt;
+}
+static double tridiag_eig(const std::vector<double> &a, const std::vector<double> &b, int k) {
+  // k-th smallest eigenvalue (0-based)
+  const size_t m = a.size();
+  double lo = a[0], hi = a[0];
+  for (size_t i = 0; i < m; i++) {
+    const double r = (i ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(b[i]) : 0.0);
+    lo = std::min(lo, a[i] - r);
+    hi = std::max(hi, a[i] + r);
+  }
+  for (int it = 0; it < 2200; it++) {
+    const double mid = 0.5 * (lo + hi);
+    if (!(mid > lo && mid < hi)) break;
+    if (sturm_below(a, b, mid) > k) hi = mid;
+    else lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+int afx_spectral_alpha_device(int ndat, int nmodel, const double *dJ, const int *cm_ptr, const int *cm_col,
+                              const double *cm_val, const double *cd, int kmax, double *alpha, double *emin,
+                              double *emax) {
+  if (ndat <= 0 || nmodel <= 0 || kmax < 1 || !dJ || !cm_ptr || !cm_col || !cm_val || !cd || !alpha)
+    return fail(AFX_ERR_ARG, "spectral_alpha: bad arguments");
+  const int nnz = cm_ptr[nmodel];
+  for (int q = 0; q < nnz; q++)
+    if (cm_col[q] < 0 || cm_col[q] >= nmodel) return fail(AFX_ERR_ARG, "spectral_alpha: Cm column out of range");
+  int *dptr = nullptr, *dcol = nullptr;
+  double *dval = nullptr, *dG = nullptr, *dC = nullptr, *dq = nullptr, *dr = nullptr;
+  auto cleanup = [&]() { dfree(dptr); dfree(dcol); dfree(dval); dfree(dG); dfree(dC); dfree(dq); dfree(dr); };
+  int rc;
+  if ((rc = upload(&dptr, cm_ptr, (size_t)nmodel + 1)) || (rc = upload(&dcol, cm_col, (size_t)nnz)) ||
+      (rc = upload(&dval, cm_val, (size_t)nnz)) || (rc = dalloc(&dG, (size_t)ndat * nmodel)) ||
+      (rc = dalloc(&dC, (size_t)ndat * ndat)) || (rc = dalloc(&dq, (size_t)ndat)) || (rc = dalloc(&dr, (size_t)ndat))) {
+    cleanup();
+    return rc;
+  }
+  // JCMJT = J (Cm J^T) (m_spectral.f90:77-80), as G = J Cm and C = G J^T on FP64 MFMA
+  const long long ng = (long long)ndat * nmodel;
+  hipLaunchKernelGGL(k_spmm_rows, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, 0, ndat, nmodel, dJ, dptr, dcol,
+                     dval, dG);
+  hipLaunchKernelGGL(k_dgemm_nt, dim3((ndat + 63) / 64, (ndat + 63) / 64), dim3(256), 0, 0, dG, nmodel, dJ, nmodel,
+                     ndat, ndat, nmodel, dC, ndat, 0);
+  bool ok = hipGetLastError() == hipSuccess;
+  // LANCZOSMINMAX (:102-179): q1 = ones / |ones|; res = C q; alpha_j = q.res; res -= beta_{j-1} v + alpha_j q;
+  // beta_j = |res|; stop early when beta_j = 0 (the Krylov space is invariant)
+  const size_t N = (size_t)ndat;
+  std::vector<double> q(N, 1.0), v(N), res(N), al, be;
+  {
+    double qq = 0.0;
+    for (size_t i = 0; i < N; i++) qq += q[i] * q[i];
+    const double sq = std::sqrt(qq);
+    for (size_t i = 0; i < N; i++) q[i] = q[i] / sq;
+  }
+  for (int j = 0; ok && j < kmax; j++) {
+    if (j > 0) {
+      v = q;
+      for (size_t i = 0; i < N; i++) q[i] = res[i] / be[j - 1];
+    }
+    ok = hipMemcpy(dq, q.data(), N * 8, hipMemcpyHostToDevice) == hipSuccess;
+    hipLaunchKernelGGL(k_gemv_rows, dim3(ndat), dim3(256), 0, 0, ndat, ndat, dC, dq, dr);
+    ok = ok && hipGetLastError() == hipSuccess && hipMemcpy(res.data(), dr, N * 8, hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok) break;
+    if (j > 0)
+      for (size_t i = 0; i < N; i++) res[i] = res[i] - be[j - 1] * v[i];
+    double a = 0.0;
+    for (size_t i = 0; i < N; i++) a += q[i] * res[i];
+    for (size_t i = 0; i < N; i++) res[i] = res[i] - a * q[i];
+    double rr = 0.0;
+    for (size_t i = 0; i < N; i++) rr += res[i] * res[i];
+    al.push_back(a);
+    be.push_back(std::sqrt(rr));
+    if (be.back() == 0.0) break;
+  }
+  cleanup();
+  if (!ok) return fail(AFX_ERR_HIP, "spectral_alpha: HIP error");
+  // eigenvalues of the tridiagonal (alpha_1..alpha_m; beta_1..beta_{m-1}) (:153-170)
+  std::vector<double> off(be.begin(), be.end() - 1);
+  const double lo = tridiag_eig(al, off, 0), hi = tridiag_eig(al, off, (int)al.size() - 1);
+  const double numer = std::max(std::fabs(hi), std::fabs(lo));
+  double denom = 0.0;  // (:84-88)
+  for (int i = 0; i < ndat; i++) denom = std::max(denom, cd[i]);
+  if (!(denom > 0.0)) return fail(AFX_ERR_ARG, "spectral_alpha: Cd has no positive entry");
+  *alpha = std::fabs(numer / denom);
+  if (emin) *emin = lo;
+  if (emax) *emax = hi;
   return AFX_OK;
 }
 

@@ -249,6 +249,15 @@ int afx_gn_update_device(int ndat, int nmodel, const double *dJ, const int *cm_p
                          const double *cm_val, const double *cd, double alpha, const double *dobs, const double *dcal,
                          const double *model, const double *model_ref, double *model_new, double *beta_out);
 
+/* alphaspectralradius_v2 (src/src_inv_ap/m_spectral.f90:63-99) with LANCZOSMINMAX (:102-179):
+   alpha = rho(J Cm J^T) / max_i cd_i, rho the largest |eigenvalue| of the tridiagonal that kmax
+   Lanczos steps on J Cm J^T build from the normalized ones vector (fewer when a step's beta is 0).
+   dJ (ndat, nmodel) row-major on the device; Cm CSR (0-based) and cd host. emin/emax (may be NULL)
+   receive the extreme Ritz values. */
+int afx_spectral_alpha_device(int ndat, int nmodel, const double *dJ, const int *cm_ptr, const int *cm_col,
+                              const double *cm_val, const double *cd, int kmax, double *alpha, double *emin,
+                              double *emax);
+
 /* C (M, N) = A (M, K) B (N, K)^T (+ C if accumulate), FP64 on MFMA, device row-major arrays --
    the dense contraction of the update, exposed for its parity test. */
 int afx_dgemm_nt_device(int M, int N, int K, const double *dA, int lda, const double *dB, int ldb, double *dC, int ldc,

@@ -286,3 +286,41 @@ def to_scipy(INU, IAU, INL, IAL, D, AU, AL):
     cols = np.concatenate([np.arange(n), np.asarray(IAU) - 1, np.asarray(IAL) - 1])
     vals = np.concatenate([D, AU, AL])
     return sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+
+
+def lanczos_minmax(A, kmax):
+    """LANCZOSMINMAX (src/src_inv_ap/m_spectral.f90:102-179) on a dense symmetric A: kmax Lanczos
+    steps from the normalized ones vector (stopping when a beta is 0), then the extreme eigenvalues
+    of the tridiagonal (the reference's TQLI; here LAPACK's symmetric eigensolver). Returns
+    (emin, emax, alphas, betas)."""
+    A = np.asarray(A, np.float64)
+    n = A.shape[0]
+    q = np.ones(n)
+    q = q / np.sqrt(np.dot(q, q))
+    res = A @ q                                   # :135
+    al = [float(np.dot(q, res))]                  # :136
+    res = res - al[0] * q                         # :137
+    be = [float(np.sqrt(np.dot(res, res)))]       # :138
+    for j in range(1, kmax):                      # :140-151
+        if be[-1] == 0.0:
+            break
+        v = q
+        q = res / be[j - 1]
+        res = A @ q
+        res = res - be[j - 1] * v
+        al.append(float(np.dot(q, res)))
+        res = res - al[j] * q
+        be.append(float(np.sqrt(np.dot(res, res))))
+    m = len(al)
+    T = np.diag(al) + np.diag(be[:m - 1], 1) + np.diag(be[:m - 1], -1)
+    ev = np.linalg.eigvalsh(T)
+    return float(ev[0]), float(ev[-1]), np.array(al), np.array(be)
+
+
+def spectral_alpha_v2(J, Cm, cd, kmax):
+    """alphaspectralradius_v2 (m_spectral.f90:63-99): alpha = rho(J Cm J^T) / max(Cd)."""
+    JCMJT = J @ (Cm @ J.T)
+    emin, emax, _, _ = lanczos_minmax(JCMJT, kmax)
+    numer = max(abs(emax), abs(emin))
+    denom = max(0.0, float(np.max(cd)))
+    return abs(numer / denom), emin, emax

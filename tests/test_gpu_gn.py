@@ -77,3 +77,30 @@ def test_gn_update_rejects_indefinite_system():
     with pytest.raises(afx.AfxError):
         afx.gn_update(torch.from_numpy(J).cuda(), sp.identity(nmodel, format="csr"), np.zeros(ndat), 0.0,
                       np.ones(ndat), np.zeros(ndat), np.zeros(nmodel), np.zeros(nmodel))
+
+
+def test_spectral_alpha_matches_oracle():
+    """alphaspectralradius_v2 + LANCZOSMINMAX (m_spectral.f90:63-179): J Cm J^T on FP64 MFMA, the
+    Lanczos recurrence and the tridiagonal's extreme eigenvalues (Sturm bisection), against the
+    oracle (numpy, LAPACK eigenvalues), for short and full-length recurrences."""
+    import scipy.sparse as sp
+    import torch
+    from actfemtide_amd import afx
+    from oracle import oracle as O
+    rng = np.random.default_rng(4)
+    ndat, nmodel = 150, 700
+    J = rng.standard_normal((ndat, nmodel)) * 0.1
+    main = np.full(nmodel, 2.0)
+    off = np.full(nmodel - 1, -0.45)
+    Cm = sp.diags([off, main, off], [-1, 0, 1], format="csr")
+    cd = rng.uniform(0.5, 2.0, ndat)
+    dJ = torch.from_numpy(J).cuda()
+    for kmax in (1, 5, 30, ndat):
+        a, lo, hi = afx.spectral_alpha(dJ, Cm, cd, kmax)
+        a_r, lo_r, hi_r = O.spectral_alpha_v2(J, Cm.toarray(), cd, kmax)
+        print(f"kmax {kmax}: alpha {a:.12e} (oracle {a_r:.12e}), emin {lo:.6e} ({lo_r:.6e}), emax {hi:.6e}")
+        assert abs(a - a_r) <= 1e-10 * a_r
+        assert abs(hi - hi_r) <= 1e-10 * hi_r
+        assert abs(lo - lo_r) <= 1e-8 * hi_r
+    ev = np.linalg.eigvalsh(J @ Cm.toarray() @ J.T)
+    assert abs(hi - ev[-1]) <= 1e-9 * ev[-1]

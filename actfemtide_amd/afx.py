@@ -95,6 +95,7 @@ SIGNATURES = {
     "afx_jacobian_ampphase_device": ([_I, _I, _VP, _VP, _VP], _I),
     "afx_gn_update_device": ([_I, _I, _VP, _VP, _VP, _VP, _VP, _D, _VP, _VP, _VP, _VP, _VP, _VP], _I),
     "afx_spectral_alpha_device": ([_I, _I, _VP, _VP, _VP, _VP, _VP, _I, _VP, _VP, _VP], _I),
+    "afx_model_roughening": ([_I, _VP, _I, _VP, _I, _VP, _VP, _VP, _VP, _VP, _VP], _I),
     "afx_dgemm_nt_device": ([_I, _I, _I, _VP, _I, _VP, _I, _VP, _I, _I], _I),
     "afx_inv_last_error": ([], C.c_char_p),
 }

@@ -3,8 +3,9 @@
 // C = J Cm J^T + alpha Cd and the update of getnewmodel (:623-787) with solvebeta's C beta = X.
 //
 // The J Cm J^T product is a real dense contraction (ndat x nmodel by nmodel x ndat) and runs on
-// FP64 MFMA (v_mfma_f64_16x16x4f64); C is factorised by a column-by-column Cholesky on the
-// device and the two triangular solves run in one block each.
+// FP64 MFMA (v_mfma_f64_16x16x4f64); C is factorised on the device by LU with partial pivoting
+// (solvebeta's PARDISO factorisation is the general real one: Cm of the minimum-support-gradient
+// stabiliser is not symmetric) and the two triangular solves run in one block.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -64,7 +65,7 @@ __global__ void __launch_bounds__(256) k_dgemm_nt(const double *__restrict__ A, 
       }
 }
 
-// G[d][i] = sum_j J[d][j] Cm[i][j] (Cm CSR; symmetric Cm gives G = J Cm)
+// G[d][i] = sum_j J[d][j] Cm[i][j]: G = (Cm J^T)^T, the transpose of getnewmodel's CMJT (Cm CSR)
 __global__ void k_spmm_rows(int ndat, int nmodel, const double *J, const int *ptr, const int *col, const double *val,
                             double *G) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -105,65 +106,93 @@ __global__ void k_gemv_cols(int nrow, int ncol, const double *A, const double *x
   y[i] = s;
 }
 
-// Cholesky, column j, step 1: A[i][j] -= sum_{k<j} A[i][k] A[j][k] for i >= j (one block per i)
-__global__ void k_chol_dots(int n, int j, double *A) {
-  __shared__ double red[256];
-  const int i = j + blockIdx.x;
-  if (i >= n) return;
-  double s = 0.0;
-  for (int k = threadIdx.x; k < j; k += 256) s += A[(size_t)i * n + k] * A[(size_t)j * n + k];
-  red[threadIdx.x] = s;
+// solvebeta (src/src_inv/solvebeta.f90) factors C with PARDISO as a general real matrix
+// (mtype 11): here LU with partial pivoting of the row-major n x n A, one column per step.
+// Step 1 (one block): the pivot row p = argmax_{i >= j} |A[i][j]| (the lowest such i), rows j
+// and p swapped, piv[j] = p; a zero pivot sets *bad.
+__global__ void __launch_bounds__(1024) k_lu_pivot(int n, int j, double *A, int *piv, int *bad) {
+  __shared__ double bv[1024];
+  __shared__ int bi[1024];
+  const int t = threadIdx.x;
+  double best = -1.0;
+  int bidx = j;
+  for (int i = j + t; i < n; i += 1024) {
+    const double v = fabs(A[(size_t)i * n + j]);
+    if (v > best) {
+      best = v;
+      bidx = i;
+    }
+  }
+  bv[t] = best;
+  bi[t] = bidx;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+  for (int w = 512; w > 0; w >>= 1) {
+    if (t < w && (bv[t + w] > bv[t] || (bv[t + w] == bv[t] && bi[t + w] < bi[t]))) {
+      bv[t] = bv[t + w];
+      bi[t] = bi[t + w];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) A[(size_t)i * n + j] -= red[0];
-}
-
-// step 2: A[i][j] /= sqrt(A[j][j]) for i > j (the pivot stays in A[j][j] until k_chol_diag: no
-// later column reads a diagonal entry); a pivot <= 0 sets *bad
-__global__ void k_chol_scale(int n, int j, double *A, int *bad) {
-  const int i = j + 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const double p = A[(size_t)j * n + j];
-  if (!(p > 0.0)) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *bad = j + 1;
-    return;
+  const int p = bi[0];
+  if (t == 0) {
+    piv[j] = p;
+    if (!(bv[0] > 0.0)) *bad = j + 1;
   }
-  if (i < n) A[(size_t)i * n + j] /= sqrt(p);
+  if (p != j)
+    for (int k = t; k < n; k += 1024) {
+      const double a = A[(size_t)j * n + k];
+      A[(size_t)j * n + k] = A[(size_t)p * n + k];
+      A[(size_t)p * n + k] = a;
+    }
+}
+// step 2: the multipliers A[i][j] /= A[j][j] (i > j)
+__global__ void k_lu_scale(int n, int j, double *A) {
+  const int i = j + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) A[(size_t)i * n + j] /= A[(size_t)j * n + j];
+}
+// step 3: A[i][k] -= A[i][j] A[j][k] (i, k > j; rows over blockIdx.y, columns over threads)
+__global__ void k_lu_update(int n, int j, double *A) {
+  const int i = j + 1 + blockIdx.y;
+  const int k = j + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && k < n) A[(size_t)i * n + k] -= A[(size_t)i * n + j] * A[(size_t)j * n + k];
 }
 
-__global__ void k_chol_diag(int n, double *A) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) A[(size_t)j * n + j] = sqrt(A[(size_t)j * n + j]);
-}
-
-// L y = b then L^T x = y, L lower (row-major, n x n); one block of 1024 threads
-__global__ void __launch_bounds__(1024) k_chol_solve(int n, const double *L, const double *b, double *x, double *y) {
+// A x = b from the factors: y = P b, L y' = y (unit lower), U x = y'; one block of 1024 threads
+__global__ void __launch_bounds__(1024) k_lu_solve(int n, const double *LU, const int *piv, const double *b, double *x,
+                                                   double *y) {
   __shared__ double red[1024];
   const int t = threadIdx.x;
-  for (int i = 0; i < n; ++i) {
+  if (t == 0) {
+    for (int i = 0; i < n; i++) y[i] = b[i];
+    for (int j = 0; j < n; j++) {
+      const double a = y[j];
+      y[j] = y[piv[j]];
+      y[piv[j]] = a;
+    }
+  }
+  __syncthreads();
+  for (int i = 1; i < n; ++i) {
     double s = 0.0;
-    for (int k = t; k < i; k += 1024) s += L[(size_t)i * n + k] * y[k];
+    for (int k = t; k < i; k += 1024) s += LU[(size_t)i * n + k] * y[k];
     red[t] = s;
     __syncthreads();
     for (int w = 512; w > 0; w >>= 1) {
       if (t < w) red[t] += red[t + w];
       __syncthreads();
     }
-    if (t == 0) y[i] = (b[i] - red[0]) / L[(size_t)i * n + i];
+    if (t == 0) y[i] -= red[0];
     __syncthreads();
   }
   for (int i = n - 1; i >= 0; --i) {
     double s = 0.0;
-    for (int k = i + 1 + t; k < n; k += 1024) s += L[(size_t)k * n + i] * x[k];
+    for (int k = i + 1 + t; k < n; k += 1024) s += LU[(size_t)i * n + k] * x[k];
     red[t] = s;
     __syncthreads();
     for (int w = 512; w > 0; w >>= 1) {
       if (t < w) red[t] += red[t + w];
       __syncthreads();
     }
-    if (t == 0) x[i] = (y[i] - red[0]) / L[(size_t)i * n + i];
+    if (t == 0) x[i] = (y[i] - red[0]) / LU[(size_t)i * n + i];
     __syncthreads();
   }
 }
@@ -237,15 +266,16 @@ int afx_gn_update_device(int ndat, int nmodel, const double *dJ, const int *cm_p
   if (ndat <= 0 || nmodel <= 0 || !dJ || !cm_ptr || !cm_col || !cm_val || !cd || !dobs || !dcal || !model ||
       !model_ref || !model_new)
     return fail(AFX_ERR_ARG, "gn_update: bad arguments");
+  if (ndat > 65535) return fail(AFX_ERR_ARG, "gn_update: ndat above 65535 (the LU update grid's row extent)");
   const int nnz = cm_ptr[nmodel];
   for (int q = 0; q < nnz; q++)
     if (cm_col[q] < 0 || cm_col[q] >= nmodel) return fail(AFX_ERR_ARG, "gn_update: Cm column out of range");
-  int *dptr = nullptr, *dcol = nullptr, *dbad = nullptr;
+  int *dptr = nullptr, *dcol = nullptr, *dbad = nullptr, *dpiv = nullptr;
   double *dval = nullptr, *dG = nullptr, *dC = nullptr, *dcd = nullptr, *ddm = nullptr, *dJM = nullptr;
   double *dX = nullptr, *dy = nullptr, *dbeta = nullptr, *dmod = nullptr;
   int rc;
   auto cleanup = [&]() {
-    dfree(dptr); dfree(dcol); dfree(dbad); dfree(dval); dfree(dG); dfree(dC); dfree(dcd); dfree(ddm);
+    dfree(dptr); dfree(dcol); dfree(dbad); dfree(dpiv); dfree(dval); dfree(dG); dfree(dC); dfree(dcd); dfree(ddm);
     dfree(dJM); dfree(dX); dfree(dy); dfree(dbeta); dfree(dmod);
   };
   std::vector<double> dm(nmodel);
@@ -255,16 +285,16 @@ int afx_gn_update_device(int ndat, int nmodel, const double *dJ, const int *cm_p
       (rc = dalloc(&dC, (size_t)ndat * ndat)) || (rc = upload(&dcd, cd, (size_t)ndat)) ||
       (rc = upload(&ddm, dm.data(), (size_t)nmodel)) || (rc = dalloc(&dJM, (size_t)ndat)) ||
       (rc = dalloc(&dX, (size_t)ndat)) || (rc = dalloc(&dy, (size_t)ndat)) || (rc = dalloc(&dbeta, (size_t)ndat)) ||
-      (rc = dalloc(&dmod, (size_t)nmodel)) || (rc = dalloc(&dbad, 1))) {
+      (rc = dalloc(&dmod, (size_t)nmodel)) || (rc = dalloc(&dbad, 1)) || (rc = dalloc(&dpiv, (size_t)ndat))) {
     cleanup();
     return rc;
   }
   bool ok = hipMemset(dbad, 0, sizeof(int)) == hipSuccess;
-  // G = J Cm (= (Cm J^T)^T, the CMJT of getnewmodel :699-701); C = G J^T + alpha Cd (:704-711)
+  // G = (Cm J^T)^T (getnewmodel's CMJT, :699-701); C = J CMJT + alpha Cd = J G^T + alpha Cd (:704-711)
   const long long ng = (long long)ndat * nmodel;
   hipLaunchKernelGGL(k_spmm_rows, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, 0, ndat, nmodel, dJ, dptr, dcol,
                      dval, dG);
-  hipLaunchKernelGGL(k_dgemm_nt, dim3((ndat + 63) / 64, (ndat + 63) / 64), dim3(256), 0, 0, dG, nmodel, dJ, nmodel,
+  hipLaunchKernelGGL(k_dgemm_nt, dim3((ndat + 63) / 64, (ndat + 63) / 64), dim3(256), 0, 0, dJ, nmodel, dG, nmodel,
                      ndat, ndat, nmodel, dC, ndat, 0);
   hipLaunchKernelGGL(k_add_diag, dim3((ndat + 255) / 256), dim3(256), 0, 0, ndat, dC, ndat, dcd, alpha);
   // X = dobs - dcal + J (m - m_ref) (:713-715)
@@ -273,13 +303,16 @@ int afx_gn_update_device(int ndat, int nmodel, const double *dJ, const int *cm_p
   ok = ok && hipGetLastError() == hipSuccess && hipMemcpy(JM.data(), dJM, ndat * 8, hipMemcpyDeviceToHost) == hipSuccess;
   for (int d = 0; d < ndat; d++) X[d] = dobs[d] - dcal[d] + JM[d];
   ok = ok && hipMemcpy(dX, X.data(), ndat * 8, hipMemcpyHostToDevice) == hipSuccess;
-  // C beta = X (solvebeta): Cholesky C = L L^T in place, then two triangular solves
+  // C beta = X (solvebeta): LU with partial pivoting in place, then the two triangular solves
   for (int j = 0; ok && j < ndat; j++) {
-    hipLaunchKernelGGL(k_chol_dots, dim3(ndat - j), dim3(256), 0, 0, ndat, j, dC);
-    hipLaunchKernelGGL(k_chol_scale, dim3((ndat - j + 255) / 256), dim3(256), 0, 0, ndat, j, dC, dbad);
+    hipLaunchKernelGGL(k_lu_pivot, dim3(1), dim3(1024), 0, 0, ndat, j, dC, dpiv, dbad);
+    const int rest = ndat - j - 1;
+    if (rest > 0) {
+      hipLaunchKernelGGL(k_lu_scale, dim3((rest + 255) / 256), dim3(256), 0, 0, ndat, j, dC);
+      hipLaunchKernelGGL(k_lu_update, dim3((rest + 255) / 256, rest), dim3(256), 0, 0, ndat, j, dC);
+    }
   }
-  hipLaunchKernelGGL(k_chol_diag, dim3((ndat + 255) / 256), dim3(256), 0, 0, ndat, dC);
-  hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(1024), 0, 0, ndat, dC, dX, dbeta, dy);
+  hipLaunchKernelGGL(k_lu_solve, dim3(1), dim3(1024), 0, 0, ndat, dC, dpiv, dX, dbeta, dy);
   // m_new = m_ref + Cm J^T beta = m_ref + G^T beta (:724-733)
   hipLaunchKernelGGL(k_gemv_cols, dim3((nmodel + 255) / 256), dim3(256), 0, 0, ndat, nmodel, dG, dbeta, dmod);
   int bad = 0;
@@ -289,7 +322,7 @@ int afx_gn_update_device(int ndat, int nmodel, const double *dJ, const int *cm_p
        (!beta_out || hipMemcpy(beta_out, dbeta, (size_t)ndat * 8, hipMemcpyDeviceToHost) == hipSuccess);
   cleanup();
   if (!ok) return fail(AFX_ERR_HIP, "gn_update: HIP error");
-  if (bad) return fail(AFX_ERR_BREAKDOWN, "gn_update: C is not positive definite");
+  if (bad) return fail(AFX_ERR_BREAKDOWN, "gn_update: C is singular (zero pivot)");
   for (int i = 0; i < nmodel; i++) model_new[i] += model_ref[i];
   return AFX_OK;
 }
@@ -344,11 +377,11 @@ int afx_spectral_alpha_device(int ndat, int nmodel, const double *dJ, const int 
     cleanup();
     return rc;
   }
-  // JCMJT = J (Cm J^T) (m_spectral.f90:77-80), as G = J Cm and C = G J^T on FP64 MFMA
+  // JCMJT = J (Cm J^T) (m_spectral.f90:77-80), as G = (Cm J^T)^T and C = J G^T on FP64 MFMA
   const long long ng = (long long)ndat * nmodel;
   hipLaunchKernelGGL(k_spmm_rows, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, 0, ndat, nmodel, dJ, dptr, dcol,
                      dval, dG);
-  hipLaunchKernelGGL(k_dgemm_nt, dim3((ndat + 63) / 64, (ndat + 63) / 64), dim3(256), 0, 0, dG, nmodel, dJ, nmodel,
+  hipLaunchKernelGGL(k_dgemm_nt, dim3((ndat + 63) / 64, (ndat + 63) / 64), dim3(256), 0, 0, dJ, nmodel, dG, nmodel,
                      ndat, ndat, nmodel, dC, ndat, 0);
   bool ok = hipGetLastError() == hipSuccess;
   // LANCZOSMINMAX (:102-179): q1 = ones / |ones|; res = C q; alpha_j = q.res; res -= beta_{j-1} v + alpha_j q;
@@ -393,6 +426,89 @@ int afx_spectral_alpha_device(int ndat, int nmodel, const double *dJ, const int 
   *alpha = std::fabs(numer / denom);
   if (emin) *emin = lo;
   if (emax) *emax = hi;
+  return AFX_OK;
+}
+
+int afx_model_roughening(int ntet, const int *n4, int nphys1, const int *ele2model, int nmodel, const int *iactive,
+                         int *nrow, long long *nnz, int *ptr, int *col, double *val) {
+  if (ntet <= 0 || !n4 || nphys1 < 0 || nphys1 >= ntet || !ele2model || nmodel <= 0 || !nrow || !nnz)
+    return fail(AFX_ERR_ARG, "model_roughening: bad arguments");
+  const int nphys2 = ntet - nphys1;
+  for (int i = 0; i < nphys2; i++)
+    if (ele2model[i] < 1 || ele2model[i] > nmodel) return fail(AFX_ERR_ARG, "model_roughening: ele2model out of range");
+  // faces of the land elements as sorted node triples; two land elements listing the same face
+  // are neighbours (GENDQ :652-675 through face2ele)
+  struct Face {
+    int a, b, c, e;
+  };
+  static const int FN[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+  std::vector<Face> faces((size_t)nphys2 * 4);
+  for (int i = 0; i < nphys2; i++) {
+    const int *t = &n4[(size_t)(nphys1 + i) * 4];
+    for (int f = 0; f < 4; f++) {
+      int v[3] = {t[FN[f][0]], t[FN[f][1]], t[FN[f][2]]};
+      std::sort(v, v + 3);
+      faces[(size_t)i * 4 + f] = {v[0], v[1], v[2], i};
+    }
+  }
+  std::sort(faces.begin(), faces.end(), [](const Face &x, const Face &y) {
+    return x.a != y.a ? x.a < y.a : x.b != y.b ? x.b < y.b : x.c != y.c ? x.c < y.c : x.e < y.e;
+  });
+  std::vector<std::pair<int, int>> adj;  // (model, neighbour model), both directions, 0-based
+  for (size_t q = 0; q + 1 < faces.size(); q++) {
+    const Face &x = faces[q], &y = faces[q + 1];
+    if (x.a == y.a && x.b == y.b && x.c == y.c) {
+      const int m1 = ele2model[x.e] - 1, m2 = ele2model[y.e] - 1;
+      if (m1 != m2) {
+        adj.push_back({m1, m2});
+        adj.push_back({m2, m1});
+      }
+    }
+  }
+  std::sort(adj.begin(), adj.end());
+  adj.erase(std::unique(adj.begin(), adj.end()), adj.end());
+  // R = Q^T D Q with every value replaced: -1 off the diagonal, (row length - 1) on it (:770-790);
+  // rows and columns of inactive models dropped, the values kept (:795-830)
+  std::vector<int> newid(nmodel, -1);
+  int na = 0;
+  for (int m = 0; m < nmodel; m++)
+    if (!iactive || iactive[m]) newid[m] = na++;
+  std::vector<int> deg(nmodel, 0);
+  for (const auto &pr : adj) deg[pr.first]++;
+  for (int m = 0; m < nmodel; m++)
+    if (deg[m] == 0 && newid[m] >= 0) return fail(AFX_ERR_ARG, "model_roughening: a model has no neighbour model");
+  long long cnt = 0;
+  for (int m = 0; m < nmodel; m++)
+    if (newid[m] >= 0) cnt++;  // diagonal
+  for (const auto &pr : adj)
+    if (newid[pr.first] >= 0 && newid[pr.second] >= 0) cnt++;
+  *nrow = na;
+  *nnz = cnt;
+  if (!ptr) return AFX_OK;
+  if (!col || !val) return fail(AFX_ERR_ARG, "model_roughening: null output");
+  size_t a = 0;
+  long long pos = 0;
+  for (int m = 0; m < nmodel; m++) {
+    while (a < adj.size() && adj[a].first < m) a++;
+    if (newid[m] < 0) continue;
+    ptr[newid[m]] = (int)pos;
+    bool diag = false;
+    for (size_t b = a; b <= adj.size(); b++) {
+      const bool end = b == adj.size() || adj[b].first != m;
+      const int nb = end ? nmodel : adj[b].second;
+      if (!diag && nb > m) {  // columns ascending, the diagonal in place
+        col[pos] = newid[m];
+        val[pos++] = (double)deg[m];
+        diag = true;
+      }
+      if (end) break;
+      if (newid[nb] >= 0) {
+        col[pos] = newid[nb];
+        val[pos++] = -1.0;
+      }
+    }
+  }
+  ptr[na] = (int)pos;
   return AFX_OK;
 }
 

@@ -243,8 +243,9 @@ int afx_jacobian_ampphase_device(int npar, int nobs, const void *dJ, const void 
    Jacobian (ndat, nmodel) row-major on the device; Cm (nmodel x nmodel, symmetric) as CSR
    (cm_ptr 0-based offsets, cm_col 0-based), cd (ndat) the diagonal of Cd, all host;
      C = J Cm J^T + alpha Cd (FP64 MFMA), X = dobs - dcal + J (model - model_ref),
-     C beta = X (device Cholesky), model_new = model_ref + Cm J^T beta.
-   beta_out (ndat, host) may be NULL. Returns AFX_ERR_BREAKDOWN when C is not positive definite. */
+     C beta = X (device LU with partial pivoting: solvebeta's PARDISO solve is the general real
+     one, mtype 11, src/src_inv/solvebeta.f90:78), model_new = model_ref + Cm J^T beta.
+   beta_out (ndat, host) may be NULL. Returns AFX_ERR_BREAKDOWN when C is singular (a zero pivot). */
 int afx_gn_update_device(int ndat, int nmodel, const double *dJ, const int *cm_ptr, const int *cm_col,
                          const double *cm_val, const double *cd, double alpha, const double *dobs, const double *dcal,
                          const double *model, const double *model_ref, double *model_new, double *beta_out);
@@ -257,6 +258,19 @@ int afx_gn_update_device(int ndat, int nmodel, const double *dJ, const int *cm_p
 int afx_spectral_alpha_device(int ndat, int nmodel, const double *dJ, const int *cm_ptr, const int *cm_col,
                               const double *cm_val, const double *cd, int kmax, double *alpha, double *emin,
                               double *emax);
+
+/* GENDQ (src/src_inv_ap/m_modelroughness.f90:575-862): the model roughening matrix R of the MSG
+   and SM roughness (R^T R for SM). Land elements are tets nphys1+1..ntet (1-based n4, ntet x 4),
+   ele2model their 1-based models (ntet - nphys1 entries). Two models are neighbours when land
+   elements of each share a tet face; R(m, m') = -1 for neighbours, R(m, m) = the number of
+   neighbour models (GENDQ's Q^T D Q with every value replaced, :770-790). iactive (nmodel, 0/1,
+   NULL = all): rows and columns of inactive models are dropped and the rest renumbered in
+   order, the values kept (:795-830; the reference keeps the column ids, the same when the
+   inactive models are the last ones). CSR, 0-based, columns ascending. Call with ptr == NULL to
+   get *nrow and *nnz, then with ptr (nrow + 1), col and val (nnz). AFX_ERR_ARG when an active
+   model has no neighbour model (the reference stops, :786-789). Host code: no GPU needed. */
+int afx_model_roughening(int ntet, const int *n4, int nphys1, const int *ele2model, int nmodel, const int *iactive,
+                         int *nrow, long long *nnz, int *ptr, int *col, double *val);
 
 /* C (M, N) = A (M, K) B (N, K)^T (+ C if accumulate), FP64 on MFMA, device row-major arrays --
    the dense contraction of the update, exposed for its parity test. */

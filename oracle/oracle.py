@@ -324,3 +324,43 @@ def spectral_alpha_v2(J, Cm, cd, kmax):
     numer = max(abs(emax), abs(emin))
     denom = max(0.0, float(np.max(cd)))
     return abs(numer / denom), emin, emax
+
+
+def model_roughening(n4, nphys1, ele2model, nmodel, iactive=None):
+    """GENDQ (src/src_inv_ap/m_modelroughness.f90:575-862) step by step: the land-element
+    connection matrix D (diagonal = number of land neighbours, -1 per neighbour, :634-676), the
+    element-to-model map Q (:732-743), Q^T D Q as a structural product (the reference multiplies
+    with MKL, which keeps every structural entry), every value replaced by -1 and the diagonal by
+    (row length - 1) (:770-790), then the active rows and columns (:795-830). Neighbours are
+    found with a dictionary of the faces' node sets. Returns a dense (nactive, nactive) array."""
+    n4 = np.asarray(n4).reshape(-1, 4)
+    ntet = n4.shape[0]
+    nphys2 = ntet - nphys1
+    owner = {}
+    nbrs = [[] for _ in range(nphys2)]
+    for i in range(nphys2):
+        t = n4[nphys1 + i]
+        for f in range(4):
+            key = frozenset(int(t[k]) for k in range(4) if k != f)
+            if key in owner:
+                j = owner[key]
+                nbrs[i].append(j)
+                nbrs[j].append(i)
+            else:
+                owner[key] = i
+    D = np.zeros((nphys2, nphys2))
+    for i in range(nphys2):
+        D[i, i] = len(nbrs[i])
+        for j in nbrs[i]:
+            D[i, j] = -1.0
+    Q = np.zeros((nphys2, nmodel))
+    Q[np.arange(nphys2), np.asarray(ele2model) - 1] = 1.0
+    S = (Q.T @ (np.abs(D) > 0) @ Q) > 0          # structure of Q^T D Q
+    R = np.where(S, -1.0, 0.0)
+    for m in range(nmodel):
+        j1 = int(S[m].sum())
+        if j1 == 1:
+            raise ValueError(f"model {m + 1} has no neighbour model")
+        R[m, m] = j1 - 1
+    act = np.ones(nmodel, bool) if iactive is None else np.asarray(iactive, bool)
+    return R[np.ix_(act, act)]

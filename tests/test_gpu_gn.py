@@ -68,7 +68,7 @@ def test_gn_update_matches_closed_form():
     assert eb < 1e-10 and em < 1e-10
 
 
-def test_gn_update_rejects_indefinite_system():
+def test_gn_update_rejects_singular_system():
     import scipy.sparse as sp
     import torch
     from actfemtide_amd import afx
@@ -104,3 +104,48 @@ def test_spectral_alpha_matches_oracle():
         assert abs(lo - lo_r) <= 1e-8 * hi_r
     ev = np.linalg.eigvalsh(J @ Cm.toarray() @ J.T)
     assert abs(hi - ev[-1]) <= 1e-9 * ev[-1]
+
+
+@pytest.mark.parametrize("stabiliser", ["MS", "MSG"])
+def test_regularised_gn_step(stabiliser):
+    """One model update as n_inv_ap.f90:380-430 runs it: Cm = BMI of the minimum-support (MS) or
+    minimum-support-gradient (MSG, with GENDQ's R of a box mesh's crust cells) stabiliser,
+    alpha = gamma * rho(J Cm J^T) / max(Cd) on the device, then getnewmodel; checked against the
+    closed form in numpy."""
+    import scipy.sparse as sp
+    import torch
+    from actfemtide_amd import afx, mesh
+    from actfemtide_amd import regularization as G
+    from oracle import oracle as O
+    c = mesh.tide_case(6, 5, 8, seed=2)
+    e2m, nmodel = G.crust_cell_models(c)
+    rng = np.random.default_rng(9)
+    ndat, beta, gamma = 40, 0.5, 0.1
+    J = rng.standard_normal((ndat, nmodel)) * 0.05
+    m_ref = np.full(nmodel, 2.0)
+    m = m_ref + 0.3 * rng.standard_normal(nmodel)
+    dm = m - m_ref
+    if stabiliser == "MS":
+        Cm = G.bmi_ms(dm, beta)
+    else:
+        R = G.model_roughening(c.n4, c.nphys1, e2m, nmodel)
+        Cm = sp.csr_matrix(G.bmi_msg(G.ri_msg(R), dm, beta))
+    cd = rng.uniform(0.5, 2.0, ndat)
+    dJ = torch.from_numpy(J).cuda()
+    a, _, _ = afx.spectral_alpha(dJ, Cm, cd, 30)
+    alpha = gamma * a
+    dobs, dcal = rng.standard_normal(ndat), rng.standard_normal(ndat)
+    new, _ = afx.gn_update(dJ, Cm, cd, alpha, dobs, dcal, m, m_ref)
+    Cd = Cm.toarray()
+    if stabiliser == "MS":  # J Cm J^T symmetric: the Lanczos extreme is its largest eigenvalue
+        ev = np.linalg.eigvalsh(J @ Cd @ J.T)
+        assert a == pytest.approx(ev[-1] / cd.max(), rel=1e-8)
+    # MSG's Cm is not symmetric; the reference runs the same recurrence on J Cm J^T regardless
+    a_ref, _, _ = O.spectral_alpha_v2(J, Cd, cd, 30)
+    assert a == pytest.approx(a_ref, rel=1e-9)
+    Cmat = J @ Cd @ J.T + alpha * np.diag(cd)
+    beta_ref = np.linalg.solve(Cmat, dobs - dcal + J @ dm)
+    new_ref = m_ref + Cd @ J.T @ beta_ref
+    err = np.abs(new - new_ref).max() / np.abs(new_ref).max()
+    print(f"{stabiliser}: nmodel {nmodel}, alpha {alpha:.4e}, model error {err:.1e}")
+    assert err < 1e-10

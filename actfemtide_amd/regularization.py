@@ -3,8 +3,9 @@
   GENDQ                 the model roughening matrix R (library: afx_model_roughening),
                         src/src_inv_ap/m_modelroughness.f90:575-862
   CALROUGHNESS_SM/MS/MSG the roughness of a model against the reference model (:312-404)
-  GENBMI_MS / GENBMI_MSG the model covariance Cm = BMI of the minimum-support and minimum-support-
-                        gradient stabilisers (:72-252); GENRI_MSG's RI = (R + 0.01 I)^-1 (:410-462)
+  GENBMI_SM / MS / MSG  the model covariance Cm = BMI of the smoothest-model, minimum-support and
+                        minimum-support-gradient stabilisers (:72-252, :466-566); GENRI_MSG's
+                        RI = (R + 0.01 I)^-1 (:410-462)
   alphaspectralradius_v2 alpha = rho(J Cm J^T) / max(Cd) (library: afx.spectral_alpha)
 
 Models are the reference's `logrho_model` (log10 resistivity); dm = model - model_ref.
@@ -69,10 +70,43 @@ def bmi_ms(dm, beta: float) -> sp.csr_matrix:
     return sp.diags((m2 + beta ** 2) ** 2 / beta ** 2, format="csr")
 
 
+# PARDISOphase3 (src/src_inv_ap/m_solveCM_ap.f90:133-178) keeps the entries of the inverse above
+# this magnitude (conv_full2ccs, src/common/m_matrix.f90:219-240)
+INVERSE_THRESHOLD = 1e-15
+
+
+def _inverse(A) -> np.ndarray:
+    X = np.linalg.inv(A)
+    X[np.abs(X) <= INVERSE_THRESHOLD] = 0.0
+    return X
+
+
 def ri_msg(R, eps: float = 1e-2) -> np.ndarray:
-    """GENRI_MSG (:410-462): RI = (R + eps I)^-1 (dense; eps = 0.01 on the diagonal)."""
+    """GENRI_MSG (:410-462): RI = (R + eps I)^-1 (eps = 0.01 on the diagonal), dense, entries of
+    magnitude <= 1e-15 dropped as PARDISOphase3 drops them."""
     A = np.asarray(sp.csr_matrix(R).todense(), np.float64) + eps * np.eye(R.shape[0])
-    return np.linalg.inv(A)
+    return _inverse(A)
+
+
+def bm_sm(R) -> sp.csr_matrix:
+    """GENBMI_SM's BM = R^T R (GENDQ :843-848, taken before eps is added)."""
+    R = sp.csr_matrix(R)
+    return (R.T @ R).tocsr()
+
+
+def bmi_sm(R, eps: float = 1e-2) -> np.ndarray:
+    """GENBMI_SM (:466-566): Cm = BMI = (R^T R + eps I)^-1, dense, small entries dropped."""
+    A = np.asarray(bm_sm(R).todense(), np.float64) + eps * np.eye(R.shape[0])
+    return _inverse(A)
+
+
+def bm_msg(R, dm, beta: float) -> sp.csr_matrix:
+    """GENBMI_MSG's BM for the first iteration (:225-238): W1 R W2 with W1 = beta^2 /
+    (dm^2 + beta^2)^(3/2) and W2 = 1 / sqrt(dm^2 + beta^2)."""
+    m2 = np.asarray(dm, np.float64) ** 2
+    w1 = beta ** 2 / (m2 + beta ** 2) ** 1.5
+    w2 = 1.0 / np.sqrt(m2 + beta ** 2)
+    return (sp.diags(w1) @ sp.csr_matrix(R) @ sp.diags(w2)).tocsr()
 
 
 def bmi_msg(RI, dm, beta: float) -> np.ndarray:

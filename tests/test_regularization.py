@@ -13,6 +13,7 @@ What pins it:
 """
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 from actfemtide_amd import afx
 from actfemtide_amd import regularization as G
@@ -107,3 +108,10 @@ def test_roughness_and_cm_closed_forms(box):
     assert np.allclose(G.bmi_msg(np.eye(nmodel), dm, beta), G.bmi_ms(dm, beta).toarray(), rtol=1e-14, atol=0)
     RI = G.ri_msg(R)
     assert np.allclose((R.toarray() + 0.01 * np.eye(nmodel)) @ RI, np.eye(nmodel), atol=1e-10)
+    assert np.all(np.abs(RI[RI != 0]) > G.INVERSE_THRESHOLD)
+    BMI = G.bmi_sm(R)
+    assert np.allclose((G.bm_sm(R).toarray() + 0.01 * np.eye(nmodel)) @ BMI, np.eye(nmodel), atol=1e-8)
+    assert np.array_equal(G.bm_sm(R).toarray(), R.toarray().T @ R.toarray())
+    # MSG's first-iteration BM is the inverse of BMI_MSG's diagonal weights around R
+    W = G.bm_msg(sp.identity(nmodel, format="csr"), dm, beta).diagonal()
+    assert np.allclose(W * G.bmi_ms(dm, beta).diagonal(), 1.0, rtol=1e-14)

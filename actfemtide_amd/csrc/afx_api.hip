@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <string>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -88,6 +89,9 @@ struct afx_system_s {
   int n = 0;
   long long nU = 0, nL = 0, nnz = 0;
   int maxrow = 0;  // longest full-CSR row
+  // right-hand-side block width per column count k (afx_autotune): 1 = the k-column kernel,
+  // 0 = columns one at a time, -1 = not tuned (one at a time)
+  std::vector<int> batch_multi = std::vector<int>(257, -1);
   std::vector<int> INU, IAU, INL, IAL;  // reference layout (IAU/IAL 1-based)
   int nlevL = 0, nlevU = 0;
   hipStream_t stream = nullptr;
@@ -1007,11 +1011,58 @@ __global__ void k_inv_or_zero(int k, const double *norms, double tol, double *sc
 }
 
 int solve_device_impl(afx_system s, int k, const cx<double> *dB, cx<double> *dX, const double *omega_scale,
+                      const afx_opts *o, afx_stats *stats);
+
+// Right-hand-side block width 1: the k columns solved one after another by the single-column
+// kernel, each at its own frequency. Its pipelined sweep keeps a block's whole row slab in LDS,
+// which k columns do not fit; measured per column, one at a time is 1.9x (32x32x24) and 2.3x
+// (64x64x48) faster than the k-column kernel.
+int solve_columns(afx_system s, int k, const cx<double> *dB, cx<double> *dX, const double *omega_scale,
+                  const afx_opts *o, afx_stats *stats) {
+  cx<double> *b1 = nullptr, *x1 = nullptr;
+  int rc;
+  if ((rc = dalloc(&b1, s->n)) || (rc = dalloc(&x1, s->n))) {
+    dfree(b1);
+    return rc;
+  }
+  int ret = AFX_OK;
+  for (int c = 0; c < k && rc == AFX_OK; c++) {
+    const double sc = omega_scale ? omega_scale[c] : 1.0;
+    afx_stats st1{};
+    hipError_t e = afx::launch_col_get(s->n, k, c, dB, b1, s->stream);
+    if (e == hipSuccess) e = afx::launch_col_get(s->n, k, c, dX, x1, s->stream);
+    if (e != hipSuccess) {
+      rc = fail(AFX_ERR_HIP, hipGetErrorString(e));
+      break;
+    }
+    const int r = solve_device_impl(s, 1, b1, x1, &sc, o, &st1);
+    if (r != AFX_OK && r != AFX_ERR_MAXIT && r != AFX_ERR_BREAKDOWN) {
+      rc = r;
+      break;
+    }
+    if (r == AFX_ERR_BREAKDOWN || (r == AFX_ERR_MAXIT && ret == AFX_OK)) ret = r;
+    if (stats) stats[c] = st1;
+    e = afx::launch_col_put(s->n, k, c, x1, dX, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) rc = fail(AFX_ERR_HIP, hipGetErrorString(e));
+  }
+  dfree(b1);
+  dfree(x1);
+  return rc != AFX_OK ? rc : ret;
+}
+
+int solve_device_impl(afx_system s, int k, const cx<double> *dB, cx<double> *dX, const double *omega_scale,
                       const afx_opts *o, afx_stats *stats) {
   if (!s->have_values) return fail(AFX_ERR_ARG, "no matrix values");
   if (k > 256) return fail(AFX_ERR_ARG, "at most 256 right-hand sides per call");
   int rc = ensure_derived(s);
   if (rc) return rc;
+  // block width: columns one at a time unless the k-column kernel was measured faster for this k
+  // (afx_autotune), is asked for (AFX_BATCH=multi; AFX_BATCH=columns forces one at a time), or
+  // the caller reuses a preconditioner set up for exactly these k frequencies (nset = 0)
+  const char *bm = getenv("AFX_BATCH");
+  const bool multi = bm ? std::string(bm) == "multi" : s->batch_multi[k] == 1;
+  if (k > 1 && o->nset != 0 && !multi) return solve_columns(s, k, dB, dX, omega_scale, o, stats);
   std::vector<double> wcol, wf;
   std::vector<int> col2f;
   column_freqs(s, k, omega_scale, wcol, wf, col2f);
@@ -1658,14 +1709,52 @@ int afx_kernel_timing(afx_system s, int reset, long long *launches, long long *i
   return AFX_OK;
 }
 
+// Right-hand-side block width for k columns: 20 FP64 iterations of the k-column kernel against
+// k times 20 iterations of one column with the single-column kernel; the faster is kept.
+static int autotune_width(afx_system s, int k, int kind, int verbose) {
+  const size_t nk = (size_t)s->n * k;
+  std::vector<cx<double>> hb(nk, cx<double>{1.0, 0.5});
+  cx<double> *dB = nullptr, *dX = nullptr;
+  int rc;
+  if ((rc = upload(&dB, hb.data(), nk)) || (rc = dalloc(&dX, nk))) {
+    dfree(dB);
+    return rc;
+  }
+  afx_opts o;
+  afx_default_opts(&o);
+  o.precond = kind;
+  o.sigma_diag = s->pc_sd;
+  o.sigma = s->pc_sig;
+  o.tol = 0.0;
+  o.maxit = 20;
+  o.check_true = 0;
+  double t[2] = {1e300, 1e300};  // [0] one column (times k), [1] k columns
+  for (int multi = 0; multi < 2; multi++) {
+    s->batch_multi[k] = multi;
+    if (hipMemset(dX, 0, nk * sizeof(cx<double>)) != hipSuccess) break;
+    (void)hipDeviceSynchronize();
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<afx_stats> st(k);
+    const int r2 = solve_device_impl(s, multi ? k : 1, dB, dX, nullptr, &o, st.data());
+    (void)hipDeviceSynchronize();
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (r2 == AFX_OK || r2 == AFX_ERR_MAXIT) t[multi] = multi ? dt : dt * k;
+    if (verbose) fprintf(stderr, "afx_autotune k=%d %s: %.3f ms / 20 it\n", k, multi ? "k-column kernel" : "one column x k", t[multi] * 1e3);
+  }
+  s->batch_multi[k] = t[1] < t[0] ? 1 : 0;
+  dfree(dB);
+  dfree(dX);
+  return AFX_OK;
+}
+
 int afx_autotune(afx_system s, int nrhs, int verbose) {
   // Rows per wavefront (lanes per row `lpr`) of the single-column kernel, per preconditioner
   // precision: time 20 iterations of each candidate on this system. The grid is the sweep
   // partition (one block per row range), so only the lane mapping is tuned.
   if (!s || nrhs <= 0 || nrhs > 256) return fail(AFX_ERR_ARG, "bad arguments");
   if (!s->have_values) return fail(AFX_ERR_ARG, "no matrix values");
-  if (nrhs > 1) return AFX_OK;  // the multi-column kernel maps one lane per (row, column)
   const int kind = s->pc_valid ? s->pc_kind : AFX_PC_SSOR;
+  if (nrhs > 1) return autotune_width(s, nrhs, kind, verbose);
   const size_t nk = (size_t)s->n;
   std::vector<cx<double>> hb(nk, cx<double>{1.0, 0.5});
   cx<double> *dB = nullptr, *dX = nullptr;

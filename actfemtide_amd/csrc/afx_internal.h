@@ -152,6 +152,8 @@ hipError_t launch_mb_fill(long long words, unsigned long long *mb, hipStream_t s
 hipError_t launch_warm_dots(long long n, int k, int kp, int grid, const cx<double> *Y, const cx<double> *B,
                             double *part, hipStream_t s);
 hipError_t launch_scale_cols_cx(long long n, int k, const cx<double> *g, cx<double> *X, hipStream_t s);
+hipError_t launch_col_get(long long n, int k, int c, const cx<double> *A, cx<double> *v, hipStream_t s);
+hipError_t launch_col_put(long long n, int k, int c, const cx<double> *v, cx<double> *A, hipStream_t s);
 hipError_t launch_axpy_f32_to_f64(long long m, const cx<float> *d, cx<double> *x, hipStream_t s);
 hipError_t launch_scale_cols(long long n, int k, const cx<double> *in, const double *scale,
                              cx<float> *out, hipStream_t s);

@@ -1326,6 +1326,24 @@ hipError_t launch_scale_cols_cx(long long n, int k, const cx<double> *g, cx<doub
   return hipGetLastError();
 }
 
+// One column of a row-interleaved (n, k) array to and from a contiguous n-vector.
+__global__ void k_col_get(long long n, int k, int c, const cx<double> *A, cx<double> *v) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) v[i] = A[i * k + c];
+}
+__global__ void k_col_put(long long n, int k, int c, const cx<double> *v, cx<double> *A) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) A[i * k + c] = v[i];
+}
+hipError_t launch_col_get(long long n, int k, int c, const cx<double> *A, cx<double> *v, hipStream_t s) {
+  hipLaunchKernelGGL(k_col_get, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, k, c, A, v);
+  return hipGetLastError();
+}
+hipError_t launch_col_put(long long n, int k, int c, const cx<double> *v, cx<double> *A, hipStream_t s) {
+  hipLaunchKernelGGL(k_col_put, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, k, c, v, A);
+  return hipGetLastError();
+}
+
 hipError_t launch_residual_norms(const SpmvParams &p, int grid, double *res_out_dev, hipStream_t s) {
   hipLaunchKernelGGL(k_spmv, dim3(grid), dim3(256), 0, s, p);
   hipError_t e = hipGetLastError();

@@ -16,7 +16,8 @@ on the solution vector); the line's "spmv" object gives its algorithmic GB/s aga
 
 Variants, off by default because each adds minutes (their results are documented):
   --mixed    complex-FP32 preconditioner inside FP64 refinement (same tolerance);
-  --batched  the 5 tidal constituents (M2, S2, N2, K1, O1) in one launch, per-column omega.
+  --batched  the 5 tidal constituents (M2, S2, N2, K1, O1) in one call, per-column omega (the
+             autotuner picks one column at a time or the 5-column kernel).
 
 `python bench.py [--gpus 1] [--steps K] [--warmup W] [--size cfg2|cfg3|small] [--mixed] [--batched]`
 """
@@ -179,6 +180,7 @@ def main():
         log(f"mixed: {tm:.2f}s, {stm[0].iter} iterations")
     if args.batched:
         k = len(CONSTITUENTS)
+        s.autotune(k)  # right-hand-side block width: one column at a time or the k-column kernel
         dBk = dB.reshape(n, 1).repeat(1, k).contiguous()
         dXk = torch.zeros((n, k), dtype=torch.complex128, device=dev)
         torch.cuda.synchronize()

@@ -38,9 +38,16 @@ def _tidal_rhs(case, seed):
     return o["b"]
 
 
-def test_batched_columns(case_small, sysdev):
-    """4 independent tidal right-hand sides in one persistent launch: each column reaches 1e-8."""
+BATCH_MODES = ["columns", "multi"]
+
+
+@pytest.mark.parametrize("mode", BATCH_MODES)
+def test_batched_columns(case_small, sysdev, mode, monkeypatch):
+    """4 independent tidal right-hand sides, one column at a time (the default block width 1) or in
+    one k-column persistent launch (AFX_BATCH=multi): each column reaches 1e-8."""
     import torch
+    if mode == "multi":
+        monkeypatch.setenv("AFX_BATCH", "multi")
     s, b, bc, dev = sysdev
     B = np.stack([b, (1 + 2j) * b, _tidal_rhs(case_small, 5), _tidal_rhs(case_small, 6)], axis=1)
     dB = torch.from_numpy(B).to(dev).contiguous()
@@ -51,13 +58,37 @@ def test_batched_columns(case_small, sysdev):
     for c in range(4):
         assert st[c].error == 0
         assert np.linalg.norm(B[:, c] - A @ X[:, c]) / np.linalg.norm(B[:, c]) <= 1.05e-8
+    monkeypatch.delenv("AFX_BATCH", raising=False)
     x1, st1 = s.solve(b, precond="SSOR", tol=1e-8, maxit=40000)
     assert abs(st[0].iter - st1[0].iter) <= max(3, 0.08 * st1[0].iter)
+    if mode == "columns":  # the same kernel on the same column: the same iterates
+        assert st[0].iter == st1[0].iter and np.array_equal(X[:, 0], x1)
 
 
-def test_multi_frequency_columns(case_small, sysdev):
+def test_autotuned_block_width(case_small, sysdev, monkeypatch):
+    """afx_autotune for 3 columns times both block widths and keeps the faster; the solve that
+    follows (whichever width it uses) reaches 1e-8 in every column."""
+    import torch
+    monkeypatch.delenv("AFX_BATCH", raising=False)
+    s, b, bc, dev = sysdev
+    s.autotune(3)
+    B = np.stack([b, (0.5 - 1j) * b, _tidal_rhs(case_small, 8)], axis=1)
+    dB = torch.from_numpy(B).to(dev).contiguous()
+    dX = torch.zeros_like(dB)
+    st, rc = s.solve_device(dB, dX, precond="SSOR", tol=1e-8, maxit=40000)
+    A, _ = _oracle_matrix(case_small, OMEGA_M2)
+    X = dX.cpu().numpy()
+    for c in range(3):
+        assert st[c].error == 0
+        assert np.linalg.norm(B[:, c] - A @ X[:, c]) / np.linalg.norm(B[:, c]) <= 1.05e-8
+
+
+@pytest.mark.parametrize("mode", BATCH_MODES)
+def test_multi_frequency_columns(case_small, sysdev, mode, monkeypatch):
     """5 tidal constituents as 5 columns with their own omega (K + i w M on the fly)."""
     import torch
+    if mode == "multi":
+        monkeypatch.setenv("AFX_BATCH", "multi")
     s, b, bc, dev = sysdev
     dB = torch.from_numpy(np.repeat(b[:, None], 5, axis=1)).to(dev).contiguous()
     dX = torch.zeros_like(dB)

@@ -18,7 +18,7 @@ from tests.conftest import OMEGA_M2
 
 pytestmark = pytest.mark.gpu
 
-CONFIGS = {
+SWEEP_SETUPS = {
     "default": {},
     "one_block": {"AFX_SWEEP_BLOCKS": "1"},
     "fewer_slabs": {"AFX_SWEEP_PARTS": "7"},
@@ -53,12 +53,12 @@ def _solves(case, env, maxits):
     return _with_env(env, run)
 
 
-@pytest.mark.parametrize("name", list(CONFIGS))
+@pytest.mark.parametrize("name", list(SWEEP_SETUPS))
 def test_first_iterations_match_oracle(name, case_small, oracle_small):
     from oracle import oracle as O
     o = oracle_small
     print(f"[sweep paths] {name}: n={case_small.nline}", flush=True)
-    for (x, st), it in zip(_solves(case_small, CONFIGS[name], (1, 3)), (1, 3)):
+    for (x, st), it in zip(_solves(case_small, SWEEP_SETUPS[name], (1, 3)), (1, 3)):
         ref = O.cocg13(o["INU"], o["IAU"], o["INL"], o["IAL"], o["D"], o["AU"], o["AL"], o["b"], precond="SSOR",
                        resid=1e-30, maxit=it)
         err = np.max(np.abs(x - ref["x"])) / np.max(np.abs(ref["x"]))

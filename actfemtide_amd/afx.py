@@ -86,6 +86,7 @@ SIGNATURES = {
     "afx_get_precond_diag": ([_VP, _VP], _I),
     "afx_cocg13_single": ([_I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.c_char_p, _D, _D,
                            C.POINTER(_D), C.POINTER(_I), C.POINTER(_I), _I], _I),
+    "afx_cocg13_single_release": ([], _I),
     "afx_solve_cg": ([_I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP], _I),
     "afx_prepobscoeff": ([_VP, _I, _VP, _VP, _VP, _VP, _VP], _I),
     "afx_obs_apply_device": ([_VP, _I, _VP, _VP, _I, _VP, _VP], _I),
@@ -401,6 +402,11 @@ def cocg13_single(D, AL, INL, IAL, AU, INU, IAU, B, X, precond="IC", sigma_diag=
     if rc not in (AFX_OK, AFX_ERR_MAXIT, AFX_ERR_BREAKDOWN, AFX_ERR_BADOPT):
         _err(rc)
     return r.value, it.value, err.value
+
+
+def cocg13_single_release():
+    """Free the device system that afx_cocg13_single keeps between calls on one pattern."""
+    _err(lib().afx_cocg13_single_release())
 
 
 def solve_cg(D, AL, INL, IAL, AU, INU, IAU, rhs):

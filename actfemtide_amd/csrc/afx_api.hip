@@ -1799,6 +1799,27 @@ int afx_autotune(afx_system s, int nrhs, int verbose) {
   return AFX_OK;
 }
 
+// The drop-in keeps the system of its last call: a caller that solves one frequency after
+// another on the same mesh (forward_bxyz's solveCG) repeats the pattern analysis only when the
+// index arrays change. Calls are serialised, as the reference's SAVEd routine requires.
+static std::mutex g_drop_mu;
+static afx_system g_drop_sys = nullptr;
+
+static bool same_pattern(afx_system s, int n, const int *INU, const int *IAU, const int *INL, const int *IAL) {
+  if (!s || s->n != n || s->nU != INU[n] || s->nL != INL[n]) return false;
+  return std::memcmp(s->INU.data(), INU, sizeof(int) * ((size_t)n + 1)) == 0 &&
+         std::memcmp(s->INL.data(), INL, sizeof(int) * ((size_t)n + 1)) == 0 &&
+         std::memcmp(s->IAU.data(), IAU, sizeof(int) * (size_t)s->nU) == 0 &&
+         std::memcmp(s->IAL.data(), IAL, sizeof(int) * (size_t)s->nL) == 0;
+}
+
+int afx_cocg13_single_release(void) {
+  std::lock_guard<std::mutex> lk(g_drop_mu);
+  int rc = g_drop_sys ? afx_destroy(g_drop_sys) : AFX_OK;
+  g_drop_sys = nullptr;
+  return rc;
+}
+
 int afx_cocg13_single(int NP, int NPL, int NPU, double *D, double *AL, const int *INL, const int *IAL, double *AU,
                       const int *INU, const int *IAU, double *B, double *X, const char *PRECOND, double SIGMA_DIAG,
                       double SIGMA, double *RESID, int *ITER, int *ERROR, int NSET) {
@@ -1833,10 +1854,21 @@ int afx_cocg13_single(int NP, int NPL, int NPU, double *D, double *AL, const int
     B[2 * i] *= scale[i];
     B[2 * i + 1] *= scale[i];
   }
-  afx_system s = nullptr;
-  int rc = afx_create(NP, INU, IAU, INL, IAL, &s);
-  if (rc) return rc;
-  rc = afx_set_values(s, D, AU, AL);
+  std::lock_guard<std::mutex> lk(g_drop_mu);
+  bool fresh = false;
+  if (!same_pattern(g_drop_sys, NP, INU, IAU, INL, IAL)) {
+    if (g_drop_sys) afx_destroy(g_drop_sys);
+    g_drop_sys = nullptr;
+    int rc0 = afx_create(NP, INU, IAU, INL, IAL, &g_drop_sys);
+    if (rc0) {
+      g_drop_sys = nullptr;
+      return rc0;
+    }
+    fresh = true;
+  }
+  afx_system s = g_drop_sys;
+  int rc = afx_set_values(s, D, AU, AL);
+  if (rc == AFX_OK && fresh) rc = afx_autotune(s, 1, 0);  // lane width for this pattern, once
   afx_opts o;
   afx_default_opts(&o);
   o.precond = pc_from_string(PRECOND);
@@ -1848,7 +1880,6 @@ int afx_cocg13_single(int NP, int NPL, int NPU, double *D, double *AL, const int
   o.check_true = 0;
   afx_stats st{};
   if (rc == AFX_OK) rc = afx_solve(s, 1, B, NP, X, NP, &o, &st);
-  afx_destroy(s);
   if (rc != AFX_OK && rc != AFX_ERR_MAXIT && rc != AFX_ERR_BREAKDOWN) return rc;
   for (int i = 0; i < NP; i++) {  // :326-329
     X[2 * i] *= scale[i];

@@ -177,10 +177,16 @@ int afx_get_precond_diag(afx_system s, double *DD);
 /* Same arguments and meaning as COCG13_single (src/solver/COCG13_single.f90:9-36), by
    pointer for the Fortran caller (scalars by value where intent(in)): D, AU, AL modified for
    NSET = 2; B scaled and restored; RESID, ITER in/out; ERROR out. Unlike the reference it
-   keeps no SAVEd state, so NSET = 0 always returns ERROR = 101. PRECOND is 20 characters. */
+   keeps no SAVEd preconditioner, so NSET = 0 always returns ERROR = 101. PRECOND is 20
+   characters. The device system of the last call is kept (pattern analysis, device arrays) and
+   reused while INU/IAU/INL/IAL are the same; calls are serialised. */
 int afx_cocg13_single(int NP, int NPL, int NPU, double *D, double *AL, const int *INL, const int *IAL,
                       double *AU, const int *INU, const int *IAU, double *B, double *X, const char *PRECOND,
                       double SIGMA_DIAG, double SIGMA, double *RESID, int *ITER, int *ERROR, int NSET);
+
+/* Frees the system that afx_cocg13_single keeps between calls (no reference counterpart: the
+   reference frees nothing either; for callers that want the device memory back). */
+int afx_cocg13_single_release(void);
 
 /* solveCG (src/solver_mpi/forward_bxyz_mpi.f90:70-94): X = 0, 'SSOR', RESID 1e-4, ITER 3000. */
 int afx_solve_cg(int doftot, int NPL, int NPU, const double *D, const double *AL, const int *INL,

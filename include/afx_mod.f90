@@ -94,6 +94,10 @@ module afx
       integer(c_int), intent(out) :: ERROR
     end function
 
+    integer(c_int) function afx_cocg13_single_release() bind(C, name="afx_cocg13_single_release")
+      import :: c_int
+    end function
+
     integer(c_int) function afx_solve_cg(doftot, NPL, NPU, D, AL, INL, IAL, AU, INU, IAU, rhs, X) &
                                          bind(C, name="afx_solve_cg")
       import :: c_int, c_double_complex

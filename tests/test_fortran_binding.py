@@ -1,6 +1,10 @@
 """The Fortran iso_c_binding module (include/afx_mod.f90) compiles with flang, links against
 libafx.so, and -- on the GPU -- runs the COCG13_single drop-in from a Fortran program on the
-reference's own assembled system (the tiny golden fixture)."""
+reference's own assembled system (the tiny golden fixture).
+
+The Fortran programs run as child processes. A child started after this test process has used
+the GPU itself was seen to find no device (its first HIP call fails), so this file's GPU tests
+run before any other GPU test (pytest's default order, by file name, does that)."""
 import os
 import subprocess
 
@@ -113,6 +117,7 @@ def test_fortran_dropin_cocg13_single(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, cwd=str(tmp_path), timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     it, err, resid = r.stdout.split()
+    assert int(err) == 0, r.stdout + r.stderr[-3000:]
     x = np.fromfile(os.path.join(str(tmp_path), "x.bin"), np.complex128)
     it_ref = int(g["iter"][0])
     print("fortran drop-in: iter", it, "error", err, "resid", resid, "reference iter", it_ref)

@@ -144,6 +144,52 @@ def test_dropin_cocg13_single_matches_reference(name):
     assert np.allclose(B, g["b"], rtol=1e-14, atol=0)
 
 
+def test_dropin_keeps_its_system_between_calls():
+    """Repeated drop-in calls as a per-frequency caller makes them: new values on the same pattern
+    (the kept system) give what a fresh system gives, a changed pattern is noticed, and
+    afx_cocg13_single_release frees the kept system."""
+    from actfemtide_amd import afx
+    from tests.test_golden import load
+    g = load("tiny_ssor")
+    INL, IAL, INU, IAU = g["INL"], g["IAL"], g["INU"], g["IAU"]
+
+    def call(scale):
+        D, AU, AL = (np.array(g[k], np.complex128) * scale for k in ("D", "AU", "AL"))
+        B = np.array(g["b"], np.complex128)
+        X = np.zeros_like(B)
+        r, it, err = afx.cocg13_single(D, AL, INL, IAL, AU, INU, IAU, B, X, precond="SSOR", resid=1e-8, iter=20000)
+        return X, it, err
+
+    afx.cocg13_single_release()
+    x1, it1, e1 = call(1.0)
+    x2, it2, e2 = call(2.0)          # same pattern, kept system, new values
+    afx.cocg13_single_release()
+    x2f, it2f, e2f = call(2.0)       # the same values on a fresh system
+    assert e1 == e2 == e2f == 0
+    assert abs(it2 - it2f) <= max(3, 0.08 * it2f)
+    assert np.linalg.norm(x2 - x2f) <= 1e-6 * np.linalg.norm(x2f)
+    assert np.linalg.norm(2.0 * x2 - x1) <= 1e-6 * np.linalg.norm(x1)  # A -> 2A halves x
+    # another mesh (another pattern) through the same entry point, against the oracle
+    from actfemtide_amd import mesh
+    from oracle import oracle as O
+    o = O.forward_system(mesh.tide_case(5, 4, 6, seed=3), 2.0 * np.pi / 44714.0)
+    D, AU, AL, B = (np.array(o[k], np.complex128) for k in ("D", "AU", "AL", "b"))
+    X = np.zeros_like(B)
+    r, it, err = afx.cocg13_single(D, AL, o["INL"], o["IAL"], AU, o["INU"], o["IAU"], B, X, precond="SSOR",
+                                   resid=1e-8, iter=20000)
+    ref = O.cocg13(o["INU"], o["IAU"], o["INL"], o["IAL"], o["D"], o["AU"], o["AL"], o["b"], precond="SSOR",
+                   resid=1e-8, maxit=20000)
+    # iteration counts on this mesh are sensitive to rounding: the oracle itself takes 369 to 406
+    # iterations when the matrix entries are perturbed by 1e-15 (relative), so the solution and its
+    # true residual are compared instead
+    A = O.to_scipy(o["INU"], o["IAU"], o["INL"], o["IAL"], o["D"], o["AU"], o["AL"])
+    b = np.array(o["b"], np.complex128)
+    print("another mesh: drop-in", it, "oracle", ref["iter"], flush=True)
+    assert err == 0 and np.linalg.norm(b - A @ X) / np.linalg.norm(b) <= 1e-8
+    assert np.linalg.norm(X - ref["x"]) <= 1e-5 * np.linalg.norm(ref["x"])
+    afx.cocg13_single_release()
+
+
 def test_dropin_ic_tracks_reference_history():
     """'IC' (D-ILU) stagnates on this system in the reference too: ERROR = -100 after MAXIT,
     ITER = MAXIT + 1, and the first residuals follow the reference's fort.12 history."""

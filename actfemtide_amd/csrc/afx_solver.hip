@@ -880,7 +880,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
       // columns of round g+1 and the row bounds of round g+2 are in flight; each register set
       // is named by a compile-time index of a 2-way unrolled loop (no copies). Entries of a
       // row are summed in the same order as a row-at-a-time loop.
-      constexpr int SU = LPR >= 8 ? 4 : 2, SE = LPR >= 4 ? (24 + LPR - 1) / LPR : 6;
+      // SE covers rows of up to 20 entries (the edge-element rows have at most 19); longer rows
+      // finish in the loop after the unrolled entries
+      constexpr int SU = LPR >= 8 ? 4 : 2, SE = LPR >= 4 ? (20 + LPR - 1) / LPR : 6;
       struct RB {  // row bounds of one round
         int qb[SU], qe[SU];
       };

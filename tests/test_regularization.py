@@ -115,3 +115,16 @@ def test_roughness_and_cm_closed_forms(box):
     # MSG's first-iteration BM is the inverse of BMI_MSG's diagonal weights around R
     W = G.bm_msg(sp.identity(nmodel, format="csr"), dm, beta).diagonal()
     assert np.allclose(W * G.bmi_ms(dm, beta).diagonal(), 1.0, rtol=1e-14)
+
+
+def test_roughening_rejects_bad_input(box):
+    e2m, nmodel = G.crust_cell_models(box)
+    bad = e2m.copy()
+    bad[0] = nmodel + 1
+    with pytest.raises(afx.AfxError):
+        G.model_roughening(box.n4, box.nphys1, bad, nmodel)
+    with pytest.raises(afx.AfxError):
+        G.model_roughening(box.n4, box.ntet, e2m, nmodel)  # no land elements
+    # every model inactive: an empty matrix, not an error
+    R = G.model_roughening(box.n4, box.nphys1, e2m, nmodel, np.zeros(nmodel, np.int32))
+    assert R.shape == (0, 0)

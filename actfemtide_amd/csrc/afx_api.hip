@@ -766,14 +766,20 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   const bool bj = (pc == AFX_PC_SSOR_BJ);
   p.fw = side_view(bj ? s->bjfw : s->fw, fp64);
   p.bw = side_view(bj ? s->bjbw : s->bw, fp64);
-  // sweep mailboxes, every slot empty at the start of the solve (kept empty between iterations);
-  // one row more than n: the single-column sweep's unused loads may read slot n
+  // sweep mailboxes: two sets per direction (iteration it uses set it & 1 and empties the other
+  // for the next iteration), every slot empty at the start of the solve; one row more than n:
+  // the single-column sweep's unused loads may read slot n
   if (pc == AFX_PC_IC || pc == AFX_PC_SSOR || pc == AFX_PC_SSOR_BJ) {
     const size_t slots = ((size_t)s->n + 1) * k;
-    if ((rc = dalloc(&w.mbf, slots)) || (rc = dalloc(&w.mbb, slots))) return rc;
-    const long long words = (long long)slots * (long long)sizeof(cx<TP>) / 8;
+    if ((rc = dalloc(&w.mbf, 2 * slots)) || (rc = dalloc(&w.mbb, 2 * slots))) return rc;
+    const long long words = 2 * (long long)slots * (long long)sizeof(cx<TP>) / 8;
     HIPCHK(afx::launch_mb_fill(words, (unsigned long long *)w.mbf, s->stream));
     HIPCHK(afx::launch_mb_fill(words, (unsigned long long *)w.mbb, s->stream));
+    p.mbf2 = w.mbf + slots;
+    p.mbb2 = w.mbb + slots;
+  } else {
+    p.mbf2 = nullptr;
+    p.mbb2 = nullptr;
   }
   p.mbf = w.mbf;
   p.mbb = w.mbb;
@@ -860,19 +866,19 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
     std::vector<unsigned long long> h((size_t)G * AFX_DBG_SLOTS);
     HIPCHK(hipMemcpy(h.data(), dbg, h.size() * 8, hipMemcpyDeviceToHost));
     const double its = std::max(1, w.h_status[0]);
-    double mean[10] = {0}, mx[10] = {0};
+    double mean[11] = {0}, mx[11] = {0};
     for (int b = 0; b < G; b++)
-      for (int q = 0; q < 10; q++) {
+      for (int q = 0; q < 11; q++) {
         const double v = (double)h[(size_t)b * AFX_DBG_SLOTS + q] / its * (q != 5 ? 0.01 : 1.0);  // 100 MHz -> us
         mean[q] += v / G;
         mx[q] = std::max(mx[q], v);
       }
     fprintf(stderr,
-            "afx profile (us/iteration, mean/max over %d blocks): fw sweep %.1f/%.1f  barrier %.1f/%.1f  "
-            "bw sweep %.1f/%.1f  rest %.1f/%.1f (rho wait %.1f/%.1f, after rho %.1f/%.1f: p update %.1f, "
-            "spmv+c1 %.1f)  total %.1f/%.1f  mailbox polls/iteration/block %.1f/%.0f\n",
-            G, mean[0], mx[0], mean[1], mx[1], mean[2], mx[2], mean[3], mx[3], mean[6], mx[6], mean[7], mx[7],
-            mean[8], mean[9], mean[4], mx[4], mean[5], mx[5]);
+            "afx profile (us/iteration, mean/max over %d blocks): fw sweep %.1f/%.1f  clear after fw %.1f/%.1f  "
+            "bw sweep %.1f/%.1f  clear after bw %.1f/%.1f  rest %.1f/%.1f (rho wait %.1f/%.1f, after rho %.1f/%.1f: "
+            "p update %.1f, spmv+c1 %.1f)  total %.1f/%.1f  mailbox polls/iteration/block %.1f/%.0f\n",
+            G, mean[0], mx[0], mean[1], mx[1], mean[2], mx[2], mean[10], mx[10], mean[3], mx[3], mean[6], mx[6],
+            mean[7], mx[7], mean[8], mean[9], mean[4], mx[4], mean[5], mx[5]);
   }
   int herr = 0;
   HIPCHK(hipMemcpy(&herr, s->d_err, sizeof(int), hipMemcpyDeviceToHost));

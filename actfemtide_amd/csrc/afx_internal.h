@@ -80,6 +80,7 @@ struct CocgParams {
   const int *rstart; // [G+1] row range of each block
   SweepSide fw, bw;
   cx<TP> *mbf, *mbb;   // [n][k] mailboxes of the forward / backward sweep (MB_EMPTY when idle)
+  cx<TP> *mbf2, *mbb2; // a second set: iteration it uses set (it & 1), the other one is emptied meanwhile
   const cx<TP> *DD;    // [n][nf]
   const int *col2f;    // [k]
   const double *wcol;  // [k] omega of each column

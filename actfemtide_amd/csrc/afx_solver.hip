@@ -728,19 +728,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
   bool alldone = block_all(st != 0);
   // optional per-block phase timing (wall clock, 100 MHz) and mailbox poll count
   constexpr bool prof = PROF;
-  unsigned long long tfw = 0, tbw = 0, tbar = 0, trest = 0, titer = 0, trho = 0, tpost = 0, tpu = 0, tsp = 0;
+  unsigned long long tfw = 0, tbw = 0, tbar = 0, trest = 0, titer = 0, trho = 0, tpost = 0, tpu = 0, tsp = 0, tclr = 0;
   unsigned nfb = 0;
 
+  // blocks of the lower half empty the next iteration's mailboxes between their forward and
+  // backward sweeps (the backward sweep reaches them last), the upper half after the backward
+  // sweep (while they wait for rho): the time each block waits anyway
+  const bool clear_after_fw = blockIdx.x < gridDim.x / 2;
   while (!alldone && it < it_end) {
     ++it;
     const unsigned long long ci0 = prof ? wall_clock64() : 0;
+    // mailboxes of this iteration, and the other set, which no block touches in this iteration
+    // (its last reader finished before the previous rho reduction) and which must be empty at
+    // the next one (the grid barriers in between order the clear before any use)
+    cx<TP> *const mbf = (it & 1) ? p.mbf2 : p.mbf, *const mbb = (it & 1) ? p.mbb2 : p.mbb;
+    cx<TP> *const mbf_nx = (it & 1) ? p.mbf : p.mbf2, *const mbb_nx = (it & 1) ? p.mbb : p.mbb2;
     // ---- z = M^-1 r (COCG13_single.f90:186-241) and rho = r^T z (:246-250) ----
     cx<double> rho_loc = {0.0, 0.0};
     if (sweep_pc) {
       // forward then backward sweep with no grid barrier between them: a block's backward sweep
       // reads its own rows' forward z (LDS or Z) and other blocks' backward z (its own mailbox),
-      // never another block's forward result. Both mailboxes are emptied after the rho
-      // reduction, which every block reaches only after both sweeps.
+      // never another block's forward result.
       if (pipe_sweep) {
         // the block's r staged in LDS, 8 loads in flight per thread
         for (int r0 = rs + t; r0 < re; r0 += 8 * 256) {
@@ -754,10 +762,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         if (t == 0) zs[p.lds_rows - 1] = cx<TP>{TP(0), TP(0)};  // the slot padding entries read
         const unsigned long long c0 = prof ? wall_clock64() : 0;
         __syncthreads();
-        sweep_pipe<TP, true>(p, p.fw, sbf, zs, p.mbf, wT, nfb);
+        sweep_pipe<TP, true>(p, p.fw, sbf, zs, mbf, wT, nfb);
         const unsigned long long c1 = prof ? wall_clock64() : 0;
+        if (clear_after_fw) mb_clear_own2(mbf_nx, p.fw.exp, mbb_nx, p.bw.exp, rs, re, 1);
         const unsigned long long c2 = prof ? wall_clock64() : 0;
-        sweep_pipe<TP, false>(p, p.bw, sbb, zs, p.mbb, wT, nfb);
+        sweep_pipe<TP, false>(p, p.bw, sbb, zs, mbb, wT, nfb);
         if (prof) {
           const unsigned long long c3 = wall_clock64();
           tfw += c1 - c0;
@@ -781,10 +790,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         }
       } else if (MULTI && p.ring_w > 0) {
         const unsigned long long c0 = prof ? wall_clock64() : 0;
-        sweep_ring<TP, 2, 8, true>(p, p.fw, zs, p.mbf, nfb);
+        sweep_ring<TP, 2, 8, true>(p, p.fw, zs, mbf, nfb);
         const unsigned long long c1 = prof ? wall_clock64() : 0;
+        if (clear_after_fw) mb_clear_own2(mbf_nx, p.fw.exp, mbb_nx, p.bw.exp, rs, re, k);
         const unsigned long long c2 = prof ? wall_clock64() : 0;
-        sweep_ring<TP, 2, 8, false>(p, p.bw, zs, p.mbb, nfb);
+        sweep_ring<TP, 2, 8, false>(p, p.bw, zs, mbb, nfb);
         if (prof) {
           const unsigned long long c3 = wall_clock64();
           tfw += c1 - c0;
@@ -798,8 +808,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
             rho_loc = addv(rho_loc, cmul(p.R[ix], up(p.Z[ix])));
           }
       } else {
-        sweep<TP, LPR, MULTI, true>(p, p.fw, zs, p.mbf, c, cvalid, wT, f, rho_loc);
-        sweep<TP, LPR, MULTI, false>(p, p.bw, zs, p.mbb, c, cvalid, wT, f, rho_loc);
+        sweep<TP, LPR, MULTI, true>(p, p.fw, zs, mbf, c, cvalid, wT, f, rho_loc);
+        if (clear_after_fw) mb_clear_own2(mbf_nx, p.fw.exp, mbb_nx, p.bw.exp, rs, re, MULTI ? k : 1);
+        sweep<TP, LPR, MULTI, false>(p, p.bw, zs, mbb, c, cvalid, wT, f, rho_loc);
         if (!MULTI && p.lds_rows > 0)
           for (int r = rs + t; r < re; r += 256) p.Z[r] = zs[r - rs];
       }
@@ -822,12 +833,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         }
       }
     }
+    if (sweep_pc && !clear_after_fw) {
+      const unsigned long long cc0 = prof ? wall_clock64() : 0;
+      mb_clear_own2(mbf_nx, p.fw.exp, mbb_nx, p.bw.exp, rs, re, MULTI ? k : 1);
+      if (prof) tclr += wall_clock64() - cc0;
+    }
     const unsigned long long cr0 = prof ? wall_clock64() : 0;
     const cx<double> rho = grid_cols(rho_loc, p.part_rho, p.fin_rho, lds_c, k, kp, c, dist, p.bar, phase);
     const unsigned long long cr1 = prof ? wall_clock64() : 0;
     if (prof) trho += cr1 - cr0;
-    // every block is past both sweeps: empty its mailbox slots for the next iteration
-    if (sweep_pc) mb_clear_own2(p.mbf, p.fw.exp, p.mbb, p.bw.exp, rs, re, MULTI ? k : 1);
 
     // ---- p = z (first iteration) or z + beta p (:258-268) ----
     const bool first = (it == 1);
@@ -1036,6 +1050,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
       d[7] += tpost;
       d[8] += tpu;
       d[9] += tsp;
+      d[10] += tclr;
     }
     atomicAdd(p.dbg + (size_t)blockIdx.x * AFX_DBG_SLOTS + 5, (unsigned long long)nfb);
   }

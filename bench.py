@@ -17,9 +17,14 @@ on the solution vector); the line's "spmv" object gives its algorithmic GB/s aga
 Variants, off by default because each adds minutes (their results are documented):
   --mixed    complex-FP32 preconditioner inside FP64 refinement (same tolerance);
   --batched  the 5 tidal constituents (M2, S2, N2, K1, O1) in one call, per-column omega (the
-             autotuner picks one column at a time or the 5-column kernel).
+             autotuner picks one column at a time or the 5-column kernel);
+  --mesh-batch  16 independent meshes from 8x8x6 to 40x40x30 cells (1,830,720 edges in all), each
+             with its own seeded crust, at M2: one solve per mesh against the stacked
+             block-diagonal system in one launch (actfemtide_amd.meshbatch), every mesh to a 1e-8
+             true residual either way.
 
-`python bench.py [--gpus 1] [--steps K] [--warmup W] [--size cfg2|cfg3|small] [--mixed] [--batched]`
+`python bench.py [--gpus 1] [--steps K] [--warmup W] [--size cfg2|cfg3|small] [--mixed] [--batched]
+[--mesh-batch]`
 """
 from __future__ import annotations
 
@@ -38,6 +43,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 SIZES = {"cfg2": (64, 64, 48), "cfg3": (32, 32, 24), "small": (16, 16, 12)}
 CONSTITUENTS = ["M2", "S2", "N2", "K1", "O1"]
+# the uneven batch: 16 boxes whose side grows from 8 to 40 cells, depth 3/4 of the side
+MESH_BATCH = [(8 + round(32 * i / 15), 8 + round(32 * i / 15), round(0.75 * (8 + round(32 * i / 15))))
+              for i in range(16)]
 
 
 def log(*a):
@@ -84,6 +92,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed", action="store_true", help="also time the complex-FP32-preconditioner solve")
     ap.add_argument("--batched", action="store_true", help="also time the 5 constituents in one launch")
+    ap.add_argument("--mesh-batch", action="store_true",
+                    help="also time 16 meshes of 8x8x6 to 40x40x30 cells one by one and in one launch")
     ap.add_argument("--maxit", type=int, default=400000, help="iteration cap of each solve")
     ap.add_argument("--spmv-launches", type=int, default=50, help="timed launches of the standalone SpMV")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -194,6 +204,9 @@ def main():
             true_resid_max=float(s.true_residual_device(dBk, dXk, omega_scale=omegas / w0).max()))
         log(f"batched: {k} solves in {tb:.2f}s, iterations {[x.iter for x in stb]}")
 
+    if args.mesh_batch:
+        variants["mesh_batch_16"] = mesh_batch_variant(opts, w0)
+
     if args.copy_bytes:
         src = torch.empty(args.copy_bytes // 8, dtype=torch.float64, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
@@ -251,6 +264,69 @@ def main():
         "variants": variants or None,
     }
     print(json.dumps(out), flush=True)
+
+
+def mesh_batch_variant(opts, omega):
+    """The 16 meshes of MESH_BATCH (seeds 3000 + i, crust N(-2, 0.5) per cell, M2): each solved
+    on its own (pattern, SSOR setup and autotuning before the timed loop), then all together as
+    one block-diagonal system. Both timings cover the solves only, with inputs on the device."""
+    import torch
+
+    from actfemtide_amd import afx, mesh
+    from actfemtide_amd.meshbatch import MeshBatch, joint_tol
+
+    parts, bs = [], []
+    t0 = time.time()
+    for i, dims in enumerate(MESH_BATCH):
+        c = mesh.tide_case(*dims, seed=3000 + i)
+        s, b, _ = afx.forward_system(c, omega)
+        s.precond_setup("SSOR")
+        s.autotune(1)
+        parts.append(s)
+        bs.append(b)
+    log(f"mesh batch: {len(parts)} systems, {sum(p.n for p in parts)} edges ({time.time() - t0:.1f}s)")
+    dBs = [torch.from_numpy(b).cuda() for b in bs]
+    dXs = [torch.zeros_like(d) for d in dBs]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    own = []
+    for s, dB, dX in zip(parts, dBs, dXs):
+        st, _ = s.solve_device(dB, dX, **opts)
+        own.append(st[0].iter)
+    torch.cuda.synchronize()
+    t_own = time.perf_counter() - t0
+    own_res = [float(s.true_residual_device(dB, dX)[0]) for s, dB, dX in zip(parts, dBs, dXs)]
+    log(f"mesh batch, one solve per mesh: {t_own:.2f}s, iterations {own}")
+
+    t0 = time.time()
+    mb = MeshBatch(parts)
+    mb.system.precond_setup("SSOR")
+    mb.system.autotune(1)
+    log(f"stacked system: n={mb.n} levels L/U={mb.system.nlev_l}/{mb.system.nlev_u} ({time.time() - t0:.1f}s)")
+    b, norms = mb.joint_rhs(bs)
+    dB = torch.from_numpy(b).cuda()
+    dX = torch.zeros_like(dB)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st, rc = mb.solve_device(dB, dX, norms=norms, **opts)
+    torch.cuda.synchronize()
+    t_joint = time.perf_counter() - t0
+    joint_res = []
+    for i, s in enumerate(parts):
+        xs = (mb.part(dX, i) * float(norms[i])).contiguous()
+        joint_res.append(float(s.true_residual_device(dBs[i], xs)[0]))
+    log(f"mesh batch, one launch: {t_joint:.2f}s, {st[0].iter} iterations, worst part residual {max(joint_res):.2e}")
+    out = dict(meshes=[f"{a}x{b_}x{c}" for a, b_, c in MESH_BATCH], edges=[p.n for p in parts],
+               one_by_one=dict(solves_per_s=round(len(parts) / t_own, 4), seconds=round(t_own, 3), iters=own,
+                               true_resid_max=max(own_res)),
+               one_launch=dict(solves_per_s=round(len(parts) / t_joint, 4), seconds=round(t_joint, 3),
+                               iters=st[0].iter, joint_tol=joint_tol(1e-8, norms), rc=rc,
+                               true_resid_max=max(joint_res)),
+               speedup=round(t_own / t_joint, 3))
+    mb.close()
+    for p in parts:
+        p.close()
+    return out
 
 
 def cpu_baseline(case, omega, gpu_iters):

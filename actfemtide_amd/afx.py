@@ -62,6 +62,8 @@ SIGNATURES = {
     "afx_default_opts": ([C.POINTER(afx_opts)], None),
     "afx_last_error": ([], C.c_char_p),
     "afx_create": ([_I, _VP, _VP, _VP, _VP, C.POINTER(_VP)], _I),
+    "afx_create_stacked": ([_I, _VP, _I, _VP, _VP, _VP, _VP, C.POINTER(_VP)], _I),
+    "afx_stacked_count": ([_VP, C.POINTER(_I)], _I),
     "afx_create_from_n6line": ([_I, _I, _VP, C.POINTER(_VP)], _I),
     "afx_destroy": ([_VP], _I),
     "afx_get_sizes": ([_VP, _VP, _VP, _VP, _VP, _VP], _I),
@@ -76,6 +78,7 @@ SIGNATURES = {
     "afx_precond_setup": ([_VP, _I, _D, _D], _I),
     "afx_solve": ([_VP, _I, _VP, _I, _VP, _I, C.POINTER(afx_opts), _VP], _I),
     "afx_solve_device": ([_VP, _I, _VP, _VP, _VP, C.POINTER(afx_opts), _VP], _I),
+    "afx_solve_stacked_device": ([_VP, _VP, _VP, C.POINTER(afx_opts), _VP], _I),
     "afx_true_residual_device": ([_VP, _I, _VP, _VP, _VP, _VP], _I),
     "afx_scale_initial_guess_device": ([_VP, _I, _VP, _VP, _VP, _VP, _VP, _VP], _I),
     "afx_autotune": ([_VP, _I, _I], _I),
@@ -192,6 +195,39 @@ class System:
         INU, IAU, INL, IAL = _i32(INU), _i32(IAU), _i32(INL), _i32(IAL)
         _err(lib().afx_create(len(INU) - 1, _p(INU), _p(IAU), _p(INL), _p(IAL), C.byref(h)))
         return cls(h.value)
+
+    @classmethod
+    def from_stacked(cls, row_off, INU, IAU, INL, IAL) -> "System":
+        """Several independent systems stacked block-diagonally (afx_create_stacked); system q
+        owns rows [row_off[q], row_off[q+1])."""
+        h = C.c_void_p()
+        ro = _i32(row_off)
+        INU, IAU, INL, IAL = _i32(INU), _i32(IAU), _i32(INL), _i32(IAL)
+        _err(lib().afx_create_stacked(len(ro) - 1, _p(ro), len(INU) - 1, _p(INU), _p(IAU), _p(INL), _p(IAL),
+                                      C.byref(h)))
+        return cls(h.value)
+
+    @property
+    def nsys(self) -> int:
+        m = C.c_int()
+        _err(lib().afx_stacked_count(self.h, C.byref(m)))
+        return m.value
+
+    def solve_stacked_device(self, dB, dX, **opts):
+        """afx_solve_stacked_device: dB, dX torch complex128 GPU tensors of n elements; returns
+        (one Stats per stacked system, return code)."""
+        import torch
+        for t in (dB, dX):
+            if t.dtype != torch.complex128 or not t.is_cuda or not t.is_contiguous() or t.numel() != self.n:
+                raise ValueError("dB, dX: contiguous complex128 GPU tensors of n elements")
+        m = self.nsys
+        o = default_opts(**opts)
+        st = (afx_stats * m)()
+        rc = lib().afx_solve_stacked_device(self.h, C.c_void_p(dB.data_ptr()), C.c_void_p(dX.data_ptr()),
+                                            C.byref(o), st)
+        if rc not in (AFX_OK, AFX_ERR_MAXIT, AFX_ERR_BREAKDOWN):
+            _err(rc)
+        return _stats(st, m), rc
 
     def close(self):
         if self.h:

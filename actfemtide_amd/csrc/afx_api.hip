@@ -134,6 +134,11 @@ struct afx_system_s {
   // pipelined sweeps
   int Gs = 0, rows_per_block = 0;
   int *d_rstart = nullptr;
+  // stacked independent systems (afx_create_stacked): row offsets [nsys + 1]; each block's rows
+  // lie in one system: d_bsys [Gs] its system, d_sblk [nsys + 1] the first block of each system
+  int nsys = 1;
+  std::vector<int> sys_off;
+  int *d_sys_off = nullptr, *d_bsys = nullptr, *d_sblk = nullptr;
   DevSide fw, bw, bjfw, bjbw;
   unsigned *d_bar = nullptr;
   int *d_err = nullptr;
@@ -602,6 +607,42 @@ int build_sweeps(afx_system s) {
   if (const char *e = getenv("AFX_SWEEP_PARTS")) Gp = std::max(1, std::min(atoi(e), G));
   std::vector<int> rstart(G + 1), blk(n);
   for (int b = 0; b <= G; b++) rstart[b] = (int)((long long)std::min(b, Gp) * n / Gp);
+  if (s->nsys > 1) {
+    // stacked systems: every slab inside one system, ceil(rows * G / n) blocks per system (at
+    // most its rows), then the largest counts lowered until they add up to at most G; blocks
+    // left over own no rows and count with the last system
+    const int m = s->nsys;
+    if (m > G) return fail(AFX_ERR_ARG, ("more stacked systems than blocks (" + std::to_string(G) + ")").c_str());
+    std::vector<int> nb(m);
+    long long tot = 0;
+    for (int q = 0; q < m; q++) {
+      const long long rows = s->sys_off[q + 1] - s->sys_off[q];
+      nb[q] = (int)std::max(1LL, std::min(rows, (rows * G + n - 1) / n));
+      tot += nb[q];
+    }
+    while (tot > G) {
+      int big = 0;
+      for (int q = 1; q < m; q++)
+        if (nb[q] > nb[big]) big = q;
+      nb[big]--;
+      tot--;
+    }
+    std::vector<int> bsys(G, m - 1), sblk(m + 1, G);
+    int b = 0;
+    for (int q = 0; q < m; q++) {
+      sblk[q] = b;
+      const long long o = s->sys_off[q], rows = s->sys_off[q + 1] - o;
+      for (int j = 0; j < nb[q]; j++, b++) {
+        rstart[b] = (int)(o + rows * j / nb[q]);
+        bsys[b] = q;
+      }
+    }
+    for (; b <= G; b++) rstart[b] = n;
+    int rc;
+    if ((rc = upload(&s->d_bsys, bsys.data(), bsys.size())) || (rc = upload(&s->d_sblk, sblk.data(), sblk.size())) ||
+        (rc = upload(&s->d_sys_off, s->sys_off.data(), s->sys_off.size())))
+      return rc;
+  }
   for (int b = 0; b < G; b++)
     for (int i = rstart[b]; i < rstart[b + 1]; i++) blk[i] = b;
   SideHost fw, bw;
@@ -631,16 +672,17 @@ struct Work {
   cx<double> *part_rho = nullptr, *part_c1 = nullptr, *fin_rho = nullptr, *fin_c1 = nullptr, *rho1 = nullptr;
   double *part_rr = nullptr, *fin_rr = nullptr, *bnrm2 = nullptr, *resid = nullptr, *norms = nullptr;
   double *part_a = nullptr, *part_b = nullptr, *wcol = nullptr;
-  int *conv = nullptr, *iters = nullptr, *status = nullptr, *col2f = nullptr;
+  int *conv = nullptr, *iters = nullptr, *status = nullptr, *col2f = nullptr, *run = nullptr;
   int *h_status = nullptr;
   ~Work() {
     dfree(R); dfree(Z); dfree(P); dfree(Q); dfree(mbf); dfree(mbb);
     dfree(part_rho); dfree(part_c1); dfree(fin_rho); dfree(fin_c1); dfree(rho1);
     dfree(part_rr); dfree(fin_rr); dfree(bnrm2); dfree(resid); dfree(norms); dfree(part_a); dfree(part_b);
-    dfree(conv); dfree(iters); dfree(status); dfree(col2f); dfree(wcol);
+    dfree(conv); dfree(iters); dfree(status); dfree(col2f); dfree(wcol); dfree(run);
     if (h_status) (void)hipHostFree(h_status);
   }
-  int alloc(long long n, int k, int G) {
+  // ks: entries of the per-column state (the number of stacked systems for a stacked solve)
+  int alloc(long long n, int k, int G, int ks) {
     int rc;
     const size_t nk = (size_t)n * k;
     if ((rc = dalloc(&R, nk)) || (rc = dalloc(&Z, nk)) || (rc = dalloc(&P, nk)) || (rc = dalloc(&Q, nk))) return rc;
@@ -648,9 +690,10 @@ struct Work {
     if ((rc = dalloc(&part_rho, gk)) || (rc = dalloc(&part_c1, gk)) || (rc = dalloc(&part_rr, gk))) return rc;
     if ((rc = dalloc(&part_a, gk)) || (rc = dalloc(&part_b, gk))) return rc;
     if ((rc = dalloc(&fin_rho, k)) || (rc = dalloc(&fin_c1, k)) || (rc = dalloc(&fin_rr, k))) return rc;
-    if ((rc = dalloc(&rho1, k)) || (rc = dalloc(&bnrm2, k)) || (rc = dalloc(&resid, k))) return rc;
-    if ((rc = dalloc(&norms, 2 * (size_t)k)) || (rc = dalloc(&conv, k)) || (rc = dalloc(&iters, k))) return rc;
+    if ((rc = dalloc(&rho1, ks)) || (rc = dalloc(&bnrm2, ks)) || (rc = dalloc(&resid, ks))) return rc;
+    if ((rc = dalloc(&norms, 2 * (size_t)ks)) || (rc = dalloc(&conv, ks)) || (rc = dalloc(&iters, ks))) return rc;
     if ((rc = dalloc(&status, 2)) || (rc = dalloc(&col2f, k)) || (rc = dalloc(&wcol, k))) return rc;
+    if ((rc = dalloc(&run, G))) return rc;
     if (hipHostMalloc((void **)&h_status, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
       return fail(AFX_ERR_HIP, "hipHostMalloc");
     return AFX_OK;
@@ -671,6 +714,42 @@ __global__ void k_init_state(int k, const double *norms, double *bnrm2, int *con
   iters[c] = 0;
   resid[c] = 0.0;
   rho1[c] = cx<double>{0.0, 0.0};
+}
+
+// per stacked system s (one block each): norms[2s] = |R_s|^2, norms[2s+1] = |B_s|^2 over rows
+// [off[s], off[s+1]), in a fixed order
+__global__ void __launch_bounds__(256) k_seg_norms(int m, const int *off, const cx<double> *R, const cx<double> *B,
+                                                   double *norms) {
+  __shared__ double lr[256], lb[256];
+  const int q = blockIdx.x, t = threadIdx.x;
+  if (q >= m) return;
+  double rr = 0.0, bb = 0.0;
+  for (long long i = off[q] + t; i < off[q + 1]; i += 256) {
+    const cx<double> r = R[i], b = B[i];
+    rr += r.r * r.r + r.i * r.i;
+    bb += b.r * b.r + b.i * b.i;
+  }
+  lr[t] = rr;
+  lb[t] = bb;
+  __syncthreads();
+  for (int h = 128; h >= 1; h >>= 1) {
+    if (t < h) {
+      lr[t] += lr[t + h];
+      lb[t] += lb[t + h];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    norms[2 * q] = lr[0];
+    norms[2 * q + 1] = lb[0];
+  }
+}
+
+// X = 0 on the rows of stacked systems whose right-hand side is zero (conv == 2)
+__global__ void k_zero_sys(int m, const int *off, const int *conv, cx<double> *X) {
+  const int q = blockIdx.x;
+  if (q >= m || conv[q] != 2) return;
+  for (long long i = off[q] + threadIdx.x; i < off[q + 1]; i += blockDim.x) X[i] = cx<double>{0.0, 0.0};
 }
 
 __global__ void k_zero_cols(long long n, int k, const int *conv, cx<double> *X) {
@@ -716,14 +795,19 @@ int ring_w_for(afx_system s, int k, int pc) {
 template <typename TP>
 int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::vector<double> &wcol,
            const std::vector<int> &col2f, int nf, int pc, double tol, int maxit, int chunk, afx_stats *stats,
-           int *host_conv) {
+           int *host_conv, bool stacked = false) {
   const bool fp64 = std::is_same<TP, double>::value;
   const bool multi = k > 1;
+  // stacked systems: one column, per-system scalars and convergence; stats and host_conv then
+  // hold one entry per system
+  const bool sys = stacked && s->nsys > 1;
+  if (sys && k != 1) return fail(AFX_ERR_ARG, "stacked systems take one right-hand side");
+  const int ks = sys ? s->nsys : k;
   const int kp = multi ? next_pow2(k) : 1;
   if (kp > 256) return fail(AFX_ERR_ARG, "at most 256 right-hand sides per launch");
   const int G = s->Gs;
   Work<TP> w;
-  int rc = w.alloc(s->n, k, G);
+  int rc = w.alloc(s->n, k, G, ks);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(w.col2f, col2f.data(), k * sizeof(int), hipMemcpyHostToDevice, s->stream));
   HIPCHK(hipMemcpyAsync(w.wcol, wcol.data(), k * sizeof(double), hipMemcpyHostToDevice, s->stream));
@@ -743,11 +827,18 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   sp.part_rr = w.part_a;
   sp.part_bb = w.part_b;
   HIPCHK(afx::launch_residual_norms(sp, G, w.norms, s->stream));
-  hipLaunchKernelGGL(k_init_state, dim3((k + 255) / 256), dim3(256), 0, s->stream, k, w.norms, w.bnrm2, w.conv,
+  if (sys) {
+    hipLaunchKernelGGL(k_seg_norms, dim3(ks), dim3(256), 0, s->stream, ks, s->d_sys_off, w.R, B, w.norms);
+    HIPCHK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_init_state, dim3((ks + 255) / 256), dim3(256), 0, s->stream, ks, w.norms, w.bnrm2, w.conv,
                      w.iters, w.resid, w.rho1, w.status);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_zero_cols, dim3((unsigned)(((long long)s->n * k + 255) / 256)), dim3(256), 0, s->stream,
-                     (long long)s->n, k, w.conv, X);
+  if (sys)
+    hipLaunchKernelGGL(k_zero_sys, dim3(ks), dim3(256), 0, s->stream, ks, s->d_sys_off, w.conv, X);
+  else
+    hipLaunchKernelGGL(k_zero_cols, dim3((unsigned)(((long long)s->n * k + 255) / 256)), dim3(256), 0, s->stream,
+                       (long long)s->n, k, w.conv, X);
   HIPCHK(hipGetLastError());
   afx::CocgParams<TP> p{};
   p.n = s->n;
@@ -755,7 +846,11 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   p.kp = kp;
   p.nf = nf;
   p.pc = pc;
-  p.dist_reduce = ((long long)G * k > 4096) ? 1 : 0;
+  p.dist_reduce = (!sys && (long long)G * k > 4096) ? 1 : 0;
+  p.nsys = sys ? ks : 1;
+  p.bsys = sys ? s->d_bsys : nullptr;
+  p.sblk = sys ? s->d_sblk : nullptr;
+  p.run = w.run;
   p.rowptr = s->d_rowptr;
   p.col = s->d_col;
   p.val = s->d_val64;
@@ -883,13 +978,13 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   int herr = 0;
   HIPCHK(hipMemcpy(&herr, s->d_err, sizeof(int), hipMemcpyDeviceToHost));
   if (herr) return fail(AFX_ERR_HIP, "a bounded device wait gave up (synchronisation error)");
-  std::vector<int> conv(k), iters(k);
-  std::vector<double> resid(k);
-  HIPCHK(hipMemcpy(conv.data(), w.conv, k * sizeof(int), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(iters.data(), w.iters, k * sizeof(int), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(resid.data(), w.resid, k * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<int> conv(ks), iters(ks);
+  std::vector<double> resid(ks);
+  HIPCHK(hipMemcpy(conv.data(), w.conv, ks * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(iters.data(), w.iters, ks * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(resid.data(), w.resid, ks * sizeof(double), hipMemcpyDeviceToHost));
   int ret = AFX_OK;
-  for (int c = 0; c < k; c++) {
+  for (int c = 0; c < ks; c++) {
     int err = 0;
     if (conv[c] == -100) err = AFX_ERR_MAXIT;
     if (conv[c] == -200) err = AFX_ERR_BREAKDOWN;
@@ -944,6 +1039,7 @@ void release(afx_system s) {
   free_side(s->bjfw);
   free_side(s->bjbw);
   dfree(s->d_rstart); dfree(s->d_bar); dfree(s->d_err);
+  dfree(s->d_sys_off); dfree(s->d_bsys); dfree(s->d_sblk);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   s->stream = nullptr;
 }
@@ -1017,7 +1113,7 @@ __global__ void k_inv_or_zero(int k, const double *norms, double tol, double *sc
 }
 
 int solve_device_impl(afx_system s, int k, const cx<double> *dB, cx<double> *dX, const double *omega_scale,
-                      const afx_opts *o, afx_stats *stats);
+                      const afx_opts *o, afx_stats *stats, bool stacked = false);
 
 // Right-hand-side block width 1: the k columns solved one after another by the single-column
 // kernel, each at its own frequency. Its pipelined sweep keeps a block's whole row slab in LDS,
@@ -1057,12 +1153,56 @@ int solve_columns(afx_system s, int k, const cx<double> *dB, cx<double> *dX, con
   return rc != AFX_OK ? rc : ret;
 }
 
+// FP64 true residual |B_s - A X_s| / |B_s| of every stacked system (0 for a zero B_s)
+int true_residual_stacked(afx_system s, const cx<double> *dB, const cx<double> *dX, double *res) {
+  std::vector<double> wcol, wf;
+  std::vector<int> col2f;
+  column_freqs(s, 1, nullptr, wcol, wf, col2f);
+  const int G = 2 * s->ncu, m = s->nsys;
+  cx<double> *dY = nullptr;
+  double *dw = nullptr, *pa = nullptr, *pb = nullptr, *norms = nullptr;
+  int rc;
+  if ((rc = dalloc(&dY, (size_t)s->n)) || (rc = upload(&dw, wcol.data(), 1)) || (rc = dalloc(&pa, (size_t)G)) ||
+      (rc = dalloc(&pb, (size_t)G)) || (rc = dalloc(&norms, 2 * (size_t)std::max(m, 1)))) {
+    dfree(dY); dfree(dw); dfree(pa); dfree(pb); dfree(norms);
+    return rc;
+  }
+  afx::SpmvParams sp{};
+  sp.n = s->n;
+  sp.k = 1;
+  sp.kp = 1;
+  sp.rowptr = s->d_rowptr;
+  sp.col = s->d_col;
+  sp.val = s->d_val64;
+  sp.wcol = dw;
+  sp.X = dX;
+  sp.B = dB;
+  sp.Y = dY;
+  sp.part_rr = pa;
+  sp.part_bb = pb;
+  hipError_t e = afx::launch_residual_norms(sp, G, norms, s->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_seg_norms, dim3(m), dim3(256), 0, s->stream, m, s->d_sys_off, dY, dB, norms);
+    e = hipGetLastError();
+  }
+  std::vector<double> h(2 * (size_t)m);
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), norms, h.size() * 8, hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  dfree(dY); dfree(dw); dfree(pa); dfree(pb); dfree(norms);
+  if (e != hipSuccess) return fail(AFX_ERR_HIP, hipGetErrorString(e));
+  for (int q = 0; q < m; q++) res[q] = h[2 * q + 1] > 0.0 ? std::sqrt(h[2 * q] / h[2 * q + 1]) : 0.0;
+  return AFX_OK;
+}
+
 int solve_device_impl(afx_system s, int k, const cx<double> *dB, cx<double> *dX, const double *omega_scale,
-                      const afx_opts *o, afx_stats *stats) {
+                      const afx_opts *o, afx_stats *stats, bool stacked) {
   if (!s->have_values) return fail(AFX_ERR_ARG, "no matrix values");
   if (k > 256) return fail(AFX_ERR_ARG, "at most 256 right-hand sides per call");
   int rc = ensure_derived(s);
   if (rc) return rc;
+  // stacked systems: one column; the states, stats and true residuals are per system
+  const bool sys = stacked && s->nsys > 1;
+  if (sys && (k != 1 || omega_scale)) return fail(AFX_ERR_ARG, "stacked systems take one right-hand side");
   // block width: columns one at a time unless the k-column kernel was measured faster for this k
   // (afx_autotune), is asked for (AFX_BATCH=multi; AFX_BATCH=columns forces one at a time), or
   // the caller reuses a preconditioner set up for exactly these k frequencies (nset = 0)
@@ -1081,44 +1221,47 @@ int solve_device_impl(afx_system s, int k, const cx<double> *dB, cx<double> *dX,
   }
   const int kind = (o->nset == 0) ? s->pc_kind : pc;
   const int nf = (int)wf.size();
-  std::vector<afx_stats> st(k);
+  const int ks = sys ? s->nsys : k;
+  std::vector<afx_stats> st(ks);
   for (auto &x : st) x = afx_stats{0, 0, 0.0, -1.0, 0};
   int ret = AFX_OK;
   // COCG with an FP64 (AFX_FP64) or complex-FP32 (AFX_MIXED) preconditioner; the recurrence,
   // SpMV and scalars are FP64. With check_true, the FP64 true residual is recomputed and the
   // solve restarts from the current iterate (an FP64 refinement step) while it exceeds tol.
   const int outer_max = o->check_true ? std::max(1, o->max_outer) : 1;
-  std::vector<int> total(k, 0);
+  std::vector<int> total(ks, 0);
   for (int outer = 0; outer < outer_max; ++outer) {
-    std::vector<afx_stats> ist(k);
+    std::vector<afx_stats> ist(ks);
     int budget = o->maxit;
-    for (int c = 0; c < k; c++) budget = std::min(budget, std::max(1, o->maxit - total[c]));
+    for (int c = 0; c < ks; c++) budget = std::min(budget, std::max(1, o->maxit - total[c]));
     ret = (o->precision == AFX_FP64)
-              ? krylov<double>(s, k, dB, dX, wcol, col2f, nf, kind, o->tol, budget, o->chunk, ist.data(), nullptr)
-              : krylov<float>(s, k, dB, dX, wcol, col2f, nf, kind, o->tol, budget, o->chunk, ist.data(), nullptr);
+              ? krylov<double>(s, k, dB, dX, wcol, col2f, nf, kind, o->tol, budget, o->chunk, ist.data(), nullptr, sys)
+              : krylov<float>(s, k, dB, dX, wcol, col2f, nf, kind, o->tol, budget, o->chunk, ist.data(), nullptr, sys);
     if (ret != AFX_OK && ret != AFX_ERR_MAXIT && ret != AFX_ERR_BREAKDOWN) return ret;
-    for (int c = 0; c < k; c++) {
+    for (int c = 0; c < ks; c++) {
       total[c] += (outer == 0 || ist[c].iter > 0) ? std::min(ist[c].iter, budget) : 0;
       st[c] = ist[c];
       st[c].iter = (outer == 0) ? ist[c].iter : total[c];
       st[c].outer = outer;
     }
     if (!o->check_true) break;
-    std::vector<double> tr(k);
-    if ((rc = afx_true_residual_device(s, k, dB, dX, omega_scale, tr.data()))) return rc;
+    std::vector<double> tr(ks);
+    if ((rc = sys ? true_residual_stacked(s, dB, dX, tr.data())
+                  : afx_true_residual_device(s, k, dB, dX, omega_scale, tr.data())))
+      return rc;
     bool ok = true;
-    for (int c = 0; c < k; c++) {
+    for (int c = 0; c < ks; c++) {
       st[c].true_resid = tr[c];
       if (!(tr[c] <= o->tol)) ok = false;
     }
     if (ok || ret == AFX_ERR_BREAKDOWN) break;
     bool budget_left = false;
-    for (int c = 0; c < k; c++)
+    for (int c = 0; c < ks; c++)
       if (total[c] < o->maxit) budget_left = true;
     if (!budget_left) break;
   }
   if (stats)
-    for (int c = 0; c < k; c++) stats[c] = st[c];
+    for (int c = 0; c < ks; c++) stats[c] = st[c];
   return ret;
 }
 
@@ -1154,6 +1297,43 @@ int afx_create(int n, const int *INU, const int *IAU, const int *INL, const int 
   s->INL.assign(INL, INL + n + 1);
   s->IAU.assign(IAU, IAU + INU[n]);
   s->IAL.assign(IAL, IAL + INL[n]);
+  rc = create_common(s);
+  if (rc) {
+    release(s);
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return AFX_OK;
+}
+
+int afx_create_stacked(int m, const int *row_off, int n, const int *INU, const int *IAU, const int *INL,
+                       const int *IAL, afx_system *out) {
+  if (!out || !row_off || !INU || !INL || m < 1) return fail(AFX_ERR_ARG, "bad arguments");
+  *out = nullptr;
+  if (row_off[0] != 0 || row_off[m] != n) return fail(AFX_ERR_ARG, "row offsets must run from 0 to n");
+  for (int q = 0; q < m; q++)
+    if (row_off[q + 1] <= row_off[q]) return fail(AFX_ERR_ARG, "every stacked system needs at least one row");
+  int rc = validate_pattern(n, INU, IAU, INL, IAL);
+  if (rc) return rc;
+  // no entry may couple two systems
+  for (int q = 0; q < m; q++)
+    for (int i = row_off[q]; i < row_off[q + 1]; i++) {
+      for (int j = INU[i]; j < INU[i + 1]; j++)
+        if (IAU[j] - 1 < row_off[q] || IAU[j] - 1 >= row_off[q + 1])
+          return fail(AFX_ERR_ARG, "an upper entry couples two stacked systems");
+      for (int j = INL[i]; j < INL[i + 1]; j++)
+        if (IAL[j] - 1 < row_off[q] || IAL[j] - 1 >= row_off[q + 1])
+          return fail(AFX_ERR_ARG, "a lower entry couples two stacked systems");
+    }
+  afx_system s = new afx_system_s();
+  s->n = n;
+  s->INU.assign(INU, INU + n + 1);
+  s->INL.assign(INL, INL + n + 1);
+  s->IAU.assign(IAU, IAU + INU[n]);
+  s->IAL.assign(IAL, IAL + INL[n]);
+  s->nsys = m;
+  s->sys_off.assign(row_off, row_off + m + 1);
   rc = create_common(s);
   if (rc) {
     release(s);
@@ -1489,6 +1669,20 @@ int afx_solve_device(afx_system s, int nrhs, const void *dB, void *dX, const dou
   if (opts) o = *opts;
   else afx_default_opts(&o);
   return solve_device_impl(s, nrhs, (const cx<double> *)dB, (cx<double> *)dX, omega_scale, &o, stats);
+}
+
+int afx_solve_stacked_device(afx_system s, const void *dB, void *dX, const afx_opts *opts, afx_stats *stats) {
+  if (!s || !dB || !dX) return fail(AFX_ERR_ARG, "bad arguments");
+  afx_opts o;
+  if (opts) o = *opts;
+  else afx_default_opts(&o);
+  return solve_device_impl(s, 1, (const cx<double> *)dB, (cx<double> *)dX, nullptr, &o, stats, true);
+}
+
+int afx_stacked_count(afx_system s, int *nsys) {
+  if (!s || !nsys) return fail(AFX_ERR_ARG, "bad arguments");
+  *nsys = s->nsys;
+  return AFX_OK;
 }
 
 int afx_solve(afx_system s, int nrhs, const double *B, int ldb, double *X, int ldx, const afx_opts *opts,

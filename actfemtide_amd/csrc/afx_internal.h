@@ -105,6 +105,12 @@ struct CocgParams {
   double tol;
   int maxit, chunk;
   unsigned long long *dbg;  // optional [G][AFX_DBG_SLOTS] phase timing (wall clock ticks) and poll counts
+  // stacked independent systems (nsys > 1, one column): block b's rows lie in system bsys[b],
+  // whose blocks are sblk[s] .. sblk[s+1]-1; rho1, bnrm2, conv, iters and resid are per system
+  // and every vector pass runs over the block's own rows
+  int nsys;
+  const int *bsys, *sblk;
+  int *run;  // [G] the block's system still running after the last iteration
 };
 
 struct SpmvParams {

@@ -231,10 +231,11 @@ __device__ V block_cols(V v, V *lds, int kp) {
   return r;
 }
 
-// Grid-wide per-column reduction with identical results in every block.
+// Grid-wide per-column reduction with identical results in every block that sums the same block
+// range [g0, g1) (all blocks, or the blocks of one stacked system; !dist only).
 template <typename V>
 __device__ V grid_cols(V v, V *part, V *fin, V *lds, int k, int kp, int c, bool dist, const GridBar &bar,
-                       unsigned &phase) {
+                       unsigned &phase, int g0, int g1) {
   const int t = threadIdx.x;
   const int G = gridDim.x;
   V bs = block_cols(v, lds, kp);
@@ -243,7 +244,7 @@ __device__ V grid_cols(V v, V *part, V *fin, V *lds, int k, int kp, int c, bool 
   if (!dist) {
     V s = zero_of(v);
     if (c < k)
-      for (int g = t / kp; g < G; g += 256 / kp) s = addv(s, part[(size_t)g * k + c]);
+      for (int g = g0 + t / kp; g < g1; g += 256 / kp) s = addv(s, part[(size_t)g * k + c]);
     return block_cols(s, lds, kp);
   }
   for (int cc = blockIdx.x; cc < k; cc += G) {
@@ -701,9 +702,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
   const TP wT = (TP)w;
   const int f = cvalid ? p.col2f[c] : 0;
   const bool sweep_pc = (p.pc == AFX_PC_IC || p.pc == AFX_PC_SSOR || p.pc == AFX_PC_SSOR_BJ);
-  const bool dist = p.dist_reduce != 0;
   const int pb = blockIdx.x;
   const int rs = p.rstart[pb], re = p.rstart[pb + 1];
+  // stacked systems: this block's system (its state index), the blocks its reductions sum, and
+  // the rows its vector passes cover (its own rows; otherwise all rows, grid-strided)
+  const bool sys = !MULTI && p.nsys > 1;
+  const int cst = sys ? p.bsys[pb] : c;
+  const int g0 = sys ? p.sblk[cst] : 0, g1 = sys ? p.sblk[cst + 1] : (int)gridDim.x;
+  const long long vlo = sys ? rs + t : gtid, vstep = sys ? 256 : nthr, vend = sys ? re : p.n;
+  const bool dist = !sys && p.dist_reduce != 0;
   // single-column sweeps: the block's step bounds of both directions are staged in LDS once
   const bool pipe_sweep = !MULTI && p.lds_rows > 0 && p.fw.psrc != nullptr && p.bw.psrc != nullptr;
   int *sbf = (int *)(dyn + lds_bounds_offset<TP>(p.lds_rows));
@@ -718,14 +725,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
   if (t == 0) phase = __hip_atomic_load(p.bar.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
 
-  int st = cvalid ? p.conv[c] : 1;
-  const double bn = cvalid ? p.bnrm2[c] : 1.0;
-  cx<double> rho1 = cvalid ? p.rho1[c] : cx<double>{0.0, 0.0};
-  double res = cvalid ? p.resid[c] : 0.0;
-  int its = cvalid ? p.iters[c] : 0;
+  int st = cvalid ? p.conv[cst] : 1;
+  const double bn = cvalid ? p.bnrm2[cst] : 1.0;
+  cx<double> rho1 = cvalid ? p.rho1[cst] : cx<double>{0.0, 0.0};
+  double res = cvalid ? p.resid[cst] : 0.0;
+  int its = cvalid ? p.iters[cst] : 0;
   int it = p.status[0];
   const int it_end = min(it + p.chunk, p.maxit);
-  bool alldone = block_all(st != 0);
+  bool alldone = sys ? block_all(t >= p.nsys || p.conv[t] != 0) : block_all(st != 0);
   // optional per-block phase timing (wall clock, 100 MHz) and mailbox poll count
   constexpr bool prof = PROF;
   unsigned long long tfw = 0, tbw = 0, tbar = 0, trest = 0, titer = 0, trho = 0, tpost = 0, tpu = 0, tsp = 0, tclr = 0;
@@ -825,7 +832,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
             rho_loc = addv(rho_loc, cmul(r, up(z)));
           }
       } else {
-        for (long long row = gtid; row < p.n; row += nthr) {
+        for (long long row = vlo; row < vend; row += vstep) {
           const cx<double> r = p.R[row];
           const cx<TP> z = (p.pc == AFX_PC_DIAG) ? cmul(down<TP>(r), p.DD[row]) : down<TP>(r);
           p.Z[row] = z;
@@ -839,7 +846,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
       if (prof) tclr += wall_clock64() - cc0;
     }
     const unsigned long long cr0 = prof ? wall_clock64() : 0;
-    const cx<double> rho = grid_cols(rho_loc, p.part_rho, p.fin_rho, lds_c, k, kp, c, dist, p.bar, phase);
+    const cx<double> rho = grid_cols(rho_loc, p.part_rho, p.fin_rho, lds_c, k, kp, c, dist, p.bar, phase, g0, g1);
     const unsigned long long cr1 = prof ? wall_clock64() : 0;
     if (prof) trho += cr1 - cr0;
 
@@ -855,19 +862,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         }
     } else {
       // 8 rows per thread in flight (one wave per SIMD: the loads must overlap)
-      for (long long r0 = gtid; r0 < p.n; r0 += 8 * nthr) {
+      for (long long r0 = vlo; r0 < vend; r0 += 8 * vstep) {
         cx<TP> zv[8];
         cx<double> pv[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-          const long long r = r0 + u * nthr < p.n ? r0 + u * nthr : r0;
+          const long long r = r0 + u * vstep < vend ? r0 + u * vstep : r0;
           zv[u] = p.Z[r];
           pv[u] = first ? cx<double>{0.0, 0.0} : p.P[r];
         }
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-          const long long r = r0 + u * nthr;
-          if (r < p.n) p.P[r] = first ? up(zv[u]) : cfma(beta, pv[u], up(zv[u]));
+          const long long r = r0 + u * vstep;
+          if (r < vend) p.P[r] = first ? up(zv[u]) : cfma(beta, pv[u], up(zv[u]));
         }
       }
     }
@@ -905,15 +912,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         int cl[SU][SE];
         cx<double> vv[SU][SE];
       };
-      const long long rstep = (long long)SU * nslot;
+      // stacked systems: the block's own rows, LPR lanes per row slot
+      const long long sslot = sys ? rs + t / LPR : slot, snslot = sys ? 256 / LPR : nslot;
+      const long long rstep = (long long)SU * snslot;
       auto load_rb = [&](long long row0, RB &b) {
 #pragma unroll
         for (int u = 0; u < SU; u++) {
-          const long long r = row0 + u * nslot;
+          const long long r = row0 + u * snslot;
           const long long rc = r < p.n ? r : p.n - 1;
           b.qb[u] = p.rowptr[rc];
           b.qe[u] = p.rowptr[rc + 1];
-          if (r >= p.n) b.qe[u] = b.qb[u];
+          if (r >= vend) b.qe[u] = b.qb[u];
         }
       };
       auto load_vc = [&](const RB &b, VC &v) {
@@ -938,7 +947,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         for (int u = 0; u < SU; u++) {
 #pragma unroll
           for (int e = 0; e < SE; e++) pv[u][e] = p.P[max(v.cl[u][e], 0)];
-          const long long r = row0 + u * nslot;
+          const long long r = row0 + u * snslot;
           pr[u] = p.P[r < p.n ? r : 0];
         }
         load_vc(bn, vn);
@@ -951,9 +960,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
             if (v.cl[u][e] >= 0) acc = cfma(mval(v.vv[u][e], w), pv[u][e], acc);
           for (int q = v.qb[u] + lane + SE * LPR; q < v.qe[u]; q += LPR)
             acc = cfma(mval(p.val[q], w), p.P[p.col[q]], acc);
-          const long long r = row0 + u * nslot;
+          const long long r = row0 + u * snslot;
           const cx<double> a = group_sum<LPR>(acc);
-          if (lane == 0 && r < p.n) {
+          if (lane == 0 && r < vend) {
             p.Q[r] = a;
             c1_loc = addv(c1_loc, cmul(pr[u], a));
           }
@@ -961,17 +970,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
       };
       RB b0, b1;
       VC v0, v1;
-      const long long first0 = slot;
+      const long long first0 = sslot;
       load_rb(first0, b0);
       load_rb(first0 + rstep, b1);
       load_vc(b0, v0);
-      for (long long row0 = first0; row0 < p.n; row0 += 2 * rstep) {
+      for (long long row0 = first0; row0 < vend; row0 += 2 * rstep) {
         round(row0, v0, v1, b1, b0);
-        if (row0 + rstep >= p.n) break;
+        if (row0 + rstep >= vend) break;
         round(row0 + rstep, v1, v0, b0, b1);
       }
     }
-    const cx<double> c1 = grid_cols(c1_loc, p.part_c1, p.fin_c1, lds_c, k, kp, c, dist, p.bar, phase);
+    const cx<double> c1 = grid_cols(c1_loc, p.part_c1, p.fin_c1, lds_c, k, kp, c, dist, p.bar, phase, g0, g1);
     if (prof) tsp += wall_clock64() - cp1;
     const cx<double> alpha = cdivd(rho, c1);
     const bool run = (st == 0);
@@ -996,11 +1005,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         }
     } else if (upd) {
       // 4 rows per thread in flight; |r|^2 accumulated in row order
-      for (long long r0 = gtid; r0 < p.n; r0 += 4 * nthr) {
+      for (long long r0 = vlo; r0 < vend; r0 += 4 * vstep) {
         cx<double> xv[4], pv[4], qv[4], rv[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-          const long long r = r0 + u * nthr < p.n ? r0 + u * nthr : r0;
+          const long long r = r0 + u * vstep < vend ? r0 + u * vstep : r0;
           xv[u] = p.X[r];
           pv[u] = p.P[r];
           qv[u] = p.Q[r];
@@ -1008,8 +1017,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-          const long long row = r0 + u * nthr;
-          if (row < p.n) {
+          const long long row = r0 + u * vstep;
+          if (row < vend) {
             p.X[row] = cfma(alpha, pv[u], xv[u]);
             const cx<double> r = cfma(malpha, qv[u], rv[u]);
             p.R[row] = r;
@@ -1018,7 +1027,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         }
       }
     }
-    const double rr = grid_cols(rr_loc, p.part_rr, p.fin_rr, lds_d, k, kp, c, dist, p.bar, phase);
+    const double rr = grid_cols(rr_loc, p.part_rr, p.fin_rr, lds_d, k, kp, c, dist, p.bar, phase, g0, g1);
     if (upd) {
       res = sqrt(rr / bn);  // :315 RESID = sqrt(DNRM2/BNRM2)
       if (res <= p.tol) {
@@ -1030,7 +1039,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
       }
     }
     rho1 = rho;
-    alldone = block_all(st != 0);
+    if (sys) {
+      // every system's state reaches every block through one more grid barrier (all blocks run
+      // the same iterations, a finished system's x and r no longer change)
+      if (t == 0) p.run[pb] = (st == 0) ? 1 : 0;
+      grid_barrier(p.bar, phase);
+      int any = 0;
+      for (int g = t; g < (int)gridDim.x; g += 256) any |= p.run[g];
+      alldone = __syncthreads_or(any) == 0;
+    } else {
+      alldone = block_all(st != 0);
+    }
     if (prof) {
       const unsigned long long ce = wall_clock64();
       titer += ce - ci0;
@@ -1054,17 +1073,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     }
     atomicAdd(p.dbg + (size_t)blockIdx.x * AFX_DBG_SLOTS + 5, (unsigned long long)nfb);
   }
-  if (blockIdx.x == 0) {
+  // the state of each column (or of each stacked system, from its first block)
+  if (sys ? pb == g0 : pb == 0) {
     if (cvalid && t < kp) {
-      p.conv[c] = st;
-      p.rho1[c] = rho1;
-      p.resid[c] = res;
-      p.iters[c] = its;
+      p.conv[cst] = st;
+      p.rho1[cst] = rho1;
+      p.resid[cst] = res;
+      p.iters[cst] = its;
     }
-    if (t == 0) {
-      p.status[0] = it;
-      p.status[1] = alldone ? 1 : 0;
-    }
+  }
+  if (pb == 0 && t == 0) {
+    p.status[0] = it;
+    p.status[1] = alldone ? 1 : 0;
   }
 }
 

@@ -3,18 +3,20 @@ model, frequency and right-hand side -- solved by one persistent COCG launch.
 
 The reference solves one mesh per call (forward_bxyz, src/solver_mpi/forward_bxyz_mpi.f90:3-67,
 one COCG13_single or solvePARDISO call per frequency). Here m assembled systems A_s x_s = b_s are
-stacked into the block-diagonal system
+stacked into the block-diagonal system diag(A_1, ..., A_m) [x_1; ...; x_m] = [b_1; ...; b_m],
+whose split-triangle arrays are the parts' arrays with row and column ids offset. The SpMV and
+the SSOR sweeps never couple two parts, and the sweeps' level sets run the parts' rows side by
+side, so a launch costs the longest part's dependency chain, not the sum.
 
-    diag(A_1, ..., A_m) [x_1; ...; x_m] = [b_1/|b_1|; ...; b_m/|b_m|]
-
-whose split-triangle arrays are the parts' arrays with row and column ids offset. COCG on it is
-COCG on every part at once: the SpMV and the SSOR sweeps never couple two parts (the sweeps'
-level sets run the parts' rows side by side, so a launch costs the longest part's dependency
-chain, not the sum), and the scalars alpha and beta are shared. With every right-hand side
-scaled to unit norm, |b| = sqrt(m), so stopping the joint solve at a relative residual of
-tol / sqrt(m) gives |r|^2 = sum_s |r_s|^2 <= tol^2 and hence |r_s| / |b_s| <= tol for every
-part. The FP64 true residual is recomputed at the end and the solve restarts from the current
-iterate while it is above that bound, so the guarantee holds for the true residual too.
+Two ways to run COCG on the stack:
+- per system (default; afx_create_stacked + afx_solve_stacked_device): the sweep blocks are cut at
+  the part boundaries and every part keeps its own alpha, beta, stopping test and true residual
+  check, so each part follows its own COCG13_single iteration;
+- shared scalars (afx_create + afx_solve_device on the stacked arrays): one COCG over the whole
+  stack. Every right-hand side is then scaled to unit norm, so |b| = sqrt(m), and stopping at a
+  relative residual of tol / sqrt(m) gives sum_s |r_s|^2 <= tol^2, hence |r_s| / |b_s| <= tol for
+  every part; the FP64 true residual is checked at the end and the solve restarts from its
+  iterate while it is above that bound.
 """
 from __future__ import annotations
 
@@ -80,21 +82,27 @@ def joint_tol(tol: float, norms) -> float:
 
 
 class MeshBatch:
-    """The block-diagonal system of several assembled afx.System parts (boundary conditions
+    """The block-diagonal stack of several assembled afx.System parts (boundary conditions
     already applied), on the device as one afx.System.
 
-    The parts' values are copied at construction; the parts themselves are only used again to
-    report each part's true residual."""
+    per_system=True builds the stacked handle whose solve keeps the scalars per part; False
+    builds a plain handle for the shared-scalar COCG. The parts' values are copied at
+    construction; the parts themselves are only used again to report each part's true residual
+    in the shared-scalar solve."""
 
-    def __init__(self, parts):
+    def __init__(self, parts, per_system: bool = True):
         parts = list(parts)
         if not parts:
             raise ValueError("no parts")
         self.parts = parts
+        self.per_system = per_system
         self.sizes = [p.n for p in parts]
         self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
         arrays = block_diag_split([p.pattern() for p in parts], [p.get_values() for p in parts])
-        self.system = afx.System.from_pattern(*arrays[:4])
+        if per_system:
+            self.system = afx.System.from_stacked(self.offsets, *arrays[:4])
+        else:
+            self.system = afx.System.from_pattern(*arrays[:4])
         self.system.set_values(*arrays[4:])
         self.n = self.system.n
 
@@ -106,15 +114,21 @@ class MeshBatch:
         self.system.close()
 
     def joint_rhs(self, bs):
-        """stack_rhs over this batch's parts."""
-        return stack_rhs(self.sizes, bs)
+        """The stacked right-hand side and the parts' norms: as given for the per-system solve,
+        each part scaled to unit norm for the shared-scalar solve (stack_rhs)."""
+        b, norms = stack_rhs(self.sizes, bs)
+        if self.per_system:
+            b = np.concatenate([np.asarray(x, np.complex128).ravel() for x in bs])
+        return b, norms
 
     def solve_device(self, dB, dX, tol=1e-8, norms=None, **opts):
-        """Joint solve of the stacked unit-norm right-hand side dB (from joint_rhs) into dX, both
-        torch complex128 tensors of n elements on the GPU; dX holds the initial guess. Returns
-        (stats of the joint solve, return code)."""
-        nrm = np.ones(self.m) if norms is None else norms
+        """Solve the stacked right-hand side dB (from joint_rhs) into dX, both torch complex128
+        tensors of n elements on the GPU; dX holds the initial guess. Returns (stats, return
+        code): one Stats per part for the per-system solve, one for the shared-scalar solve."""
         opts.setdefault("check_true", 1)
+        if self.per_system:
+            return self.system.solve_stacked_device(dB, dX, tol=tol, **opts)
+        nrm = np.ones(self.m) if norms is None else norms
         return self.system.solve_device(dB, dX, tol=joint_tol(tol, nrm), **opts)
 
     def part(self, v, s: int):
@@ -122,14 +136,17 @@ class MeshBatch:
         return v[int(self.offsets[s]):int(self.offsets[s + 1])]
 
     def solve(self, bs, tol=1e-8, **opts):
-        """Solve every part to tol; bs are host arrays. Returns (xs, joint stats, per-part FP64
-        true residuals |b_s - A_s x_s| / |b_s| recomputed on each part's own system)."""
+        """Solve every part to tol; bs are host arrays. Returns (xs, stats, per-part FP64 true
+        residuals |b_s - A_s x_s| / |b_s|)."""
         import torch
         b, norms = self.joint_rhs(bs)
         dB = torch.from_numpy(b).cuda()
         dX = torch.zeros_like(dB)
         st, rc = self.solve_device(dB, dX, tol=tol, norms=norms, **opts)
         x = dX.cpu().numpy()
+        if self.per_system:
+            xs = [self.part(x, s).copy() for s in range(self.m)]
+            return xs, st, np.array([t.true_resid for t in st])
         xs, tr = [], []
         for s in range(self.m):
             xp = self.part(x, s) * norms[s]
@@ -140,4 +157,4 @@ class MeshBatch:
                 continue
             r = self.parts[s].true_residual_device(torch.from_numpy(bp).cuda(), torch.from_numpy(xp).cuda())
             tr.append(float(r[0]))
-        return xs, st[0], np.array(tr)
+        return xs, st, np.array(tr)

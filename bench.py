@@ -20,8 +20,8 @@ Variants, off by default because each adds minutes (their results are documented
              autotuner picks one column at a time or the 5-column kernel);
   --mesh-batch  16 independent meshes from 8x8x6 to 40x40x30 cells (1,830,720 edges in all), each
              with its own seeded crust, at M2: one solve per mesh against the stacked
-             block-diagonal system in one launch (actfemtide_amd.meshbatch), every mesh to a 1e-8
-             true residual either way.
+             block-diagonal system in one launch (actfemtide_amd.meshbatch), with per-system and
+             with shared COCG scalars, every mesh to a 1e-8 true residual each way.
 
 `python bench.py [--gpus 1] [--steps K] [--warmup W] [--size cfg2|cfg3|small] [--mixed] [--batched]
 [--mesh-batch]`
@@ -298,32 +298,41 @@ def mesh_batch_variant(opts, omega):
     own_res = [float(s.true_residual_device(dB, dX)[0]) for s, dB, dX in zip(parts, dBs, dXs)]
     log(f"mesh batch, one solve per mesh: {t_own:.2f}s, iterations {own}")
 
-    t0 = time.time()
-    mb = MeshBatch(parts)
-    mb.system.precond_setup("SSOR")
-    mb.system.autotune(1)
-    log(f"stacked system: n={mb.n} levels L/U={mb.system.nlev_l}/{mb.system.nlev_u} ({time.time() - t0:.1f}s)")
-    b, norms = mb.joint_rhs(bs)
-    dB = torch.from_numpy(b).cuda()
-    dX = torch.zeros_like(dB)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    st, rc = mb.solve_device(dB, dX, norms=norms, **opts)
-    torch.cuda.synchronize()
-    t_joint = time.perf_counter() - t0
-    joint_res = []
-    for i, s in enumerate(parts):
-        xs = (mb.part(dX, i) * float(norms[i])).contiguous()
-        joint_res.append(float(s.true_residual_device(dBs[i], xs)[0]))
-    log(f"mesh batch, one launch: {t_joint:.2f}s, {st[0].iter} iterations, worst part residual {max(joint_res):.2e}")
+    modes = {}
+    for mode in ("per_system", "shared"):
+        t0 = time.time()
+        mb = MeshBatch(parts, per_system=(mode == "per_system"))
+        mb.system.precond_setup("SSOR")
+        mb.system.autotune(1)
+        log(f"stacked system ({mode}): n={mb.n} levels L/U={mb.system.nlev_l}/{mb.system.nlev_u} "
+            f"({time.time() - t0:.1f}s)")
+        b, norms = mb.joint_rhs(bs)
+        dB = torch.from_numpy(b).cuda()
+        dX = torch.zeros_like(dB)
+        mb.system.kernel_timing(reset=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st, rc = mb.solve_device(dB, dX, norms=norms, **opts)
+        torch.cuda.synchronize()
+        t_joint = time.perf_counter() - t0
+        launches, iters, kms = mb.system.kernel_timing(reset=True)
+        res = []
+        for i, s in enumerate(parts):
+            xs = (mb.part(dX, i) * (1.0 if mb.per_system else float(norms[i]))).contiguous()
+            res.append(float(s.true_residual_device(dBs[i], xs)[0]))
+        its = [x.iter for x in st]
+        log(f"mesh batch, one launch ({mode}): {t_joint:.2f}s, iterations {its}, "
+            f"{1e3 * kms / max(iters, 1):.1f} us/iteration, worst part residual {max(res):.2e}")
+        modes[mode] = dict(solves_per_s=round(len(parts) / t_joint, 4), seconds=round(t_joint, 3),
+                           iters=its if mb.per_system else its[0], us_per_iteration=round(1e3 * kms / max(iters, 1), 1),
+                           rc=rc, true_resid_max=max(res), speedup=round(t_own / t_joint, 3))
+        if not mb.per_system:
+            modes[mode]["joint_tol"] = joint_tol(1e-8, norms)
+        mb.close()
     out = dict(meshes=[f"{a}x{b_}x{c}" for a, b_, c in MESH_BATCH], edges=[p.n for p in parts],
                one_by_one=dict(solves_per_s=round(len(parts) / t_own, 4), seconds=round(t_own, 3), iters=own,
                                true_resid_max=max(own_res)),
-               one_launch=dict(solves_per_s=round(len(parts) / t_joint, 4), seconds=round(t_joint, 3),
-                               iters=st[0].iter, joint_tol=joint_tol(1e-8, norms), rc=rc,
-                               true_resid_max=max(joint_res)),
-               speedup=round(t_own / t_joint, 3))
-    mb.close()
+               one_launch_per_system_scalars=modes["per_system"], one_launch_shared_scalars=modes["shared"])
     for p in parts:
         p.close()
     return out

@@ -79,6 +79,19 @@ int afx_create(int n, const int *INU, const int *IAU, const int *INL, const int 
    (src/solver/m_iccg_var_takuto.f90:277-485, called at src/solver_mpi/n_ebfem_bxyz_mpi.f90:104-107). */
 int afx_create_from_n6line(int nline, int ntet, const int *n6line, afx_system *out);
 
+/* m independent systems stacked into one block-diagonal system (no reference counterpart: the
+   reference solves one mesh per forward_bxyz call, src/solver_mpi/forward_bxyz_mpi.f90:3-67).
+   System q owns rows [row_off[q], row_off[q+1]) (0-based, row_off[0] = 0, row_off[m] = n, every
+   system at least one row); the split CSR arrays are afx_create's, and no entry may couple two
+   systems. The sweep blocks are cut at the system boundaries, so afx_solve_stacked_device can
+   keep alpha, beta and convergence per system. m is at most the number of sweep blocks (one
+   per compute unit on a large system). */
+int afx_create_stacked(int m, const int *row_off, int n, const int *INU, const int *IAU, const int *INL,
+                       const int *IAL, afx_system *out);
+
+/* Number of stacked systems of a handle (1 unless made by afx_create_stacked). */
+int afx_stacked_count(afx_system s, int *nsys);
+
 int afx_destroy(afx_system s);
 int afx_get_sizes(afx_system s, int *n, int *npu, int *npl, int *nlev_l, int *nlev_u);
 int afx_get_pattern(afx_system s, int *INU, int *IAU, int *INL, int *IAL);
@@ -135,6 +148,14 @@ int afx_solve(afx_system s, int nrhs, const double *B, int ldb, double *X, int l
    (K + i*omega_c*M) x = b with omega_c = omega * omega_scale[c] (needs afx_assemble values). */
 int afx_solve_device(afx_system s, int nrhs, const void *dB, void *dX, const double *omega_scale,
                      const afx_opts *opts, afx_stats *stats);
+
+/* One persistent COCG launch over all systems of an afx_create_stacked handle: one device
+   column B, X (n complex doubles; X holds the initial guess), each system with its own alpha,
+   beta, ||b_s||, stopping test ||r_s|| / ||b_s|| <= tol and FP64 true residual check (the
+   COCG13_single semantics per system, COCG13_single.f90:164-169,315); finished systems stay
+   fixed while the others iterate. stats: one entry per system (afx_stacked_count) or NULL.
+   Returns 0, -100 (some system hit maxit), -200, or an AFX_ERR_*. */
+int afx_solve_stacked_device(afx_system s, const void *dB, void *dX, const afx_opts *opts, afx_stats *stats);
 
 /* FP64 true residual ||b - A x|| / ||b|| per column of device, row-interleaved B and X. */
 int afx_true_residual_device(afx_system s, int nrhs, const void *dB, const void *dX,

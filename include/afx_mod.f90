@@ -80,6 +80,23 @@ module afx
       type(afx_stats), intent(out) :: stats(*)
     end function
 
+    ! several independent meshes in one launch: system q owns rows row_off(q+1)+1 .. row_off(q+2)
+    ! (row_off(1) = 0, row_off(m+1) = n); dB, dX are device pointers (n complex(8) each), stats(m)
+    integer(c_int) function afx_create_stacked(m, row_off, n, INU, IAU, INL, IAL, h) &
+                                               bind(C, name="afx_create_stacked")
+      import :: c_int, c_ptr
+      integer(c_int), value :: m, n
+      integer(c_int), intent(in) :: row_off(*), INU(*), IAU(*), INL(*), IAL(*)
+      type(c_ptr), intent(out) :: h
+    end function
+
+    integer(c_int) function afx_solve_stacked_device(h, dB, dX, opts, stats) bind(C, name="afx_solve_stacked_device")
+      import :: c_int, c_ptr, afx_opts, afx_stats
+      type(c_ptr), value :: h, dB, dX
+      type(afx_opts), intent(in) :: opts
+      type(afx_stats), intent(out) :: stats(*)
+    end function
+
     integer(c_int) function afx_cocg13_single(NP, NPL, NPU, D, AL, INL, IAL, AU, INU, IAU, B, X, PRECOND, &
                                               SIGMA_DIAG, SIGMA, RESID, ITER, ERROR, NSET) &
                                               bind(C, name="afx_cocg13_single")

@@ -7,7 +7,9 @@ What pins it:
   right-hand sides and the joint tolerance tol / sqrt(m), leaves every part's FP64 true residual
   <= tol, and a part's solution, scaled back, is as close to the direct solution (scipy SuperLU)
   as the part's own solve to the same tolerance allows;
-- zero right-hand sides and size mismatches.
+- zero right-hand sides and size mismatches;
+- afx_create_stacked refuses offsets that do not cover the rows and entries that couple two
+  systems (checked on the host before any device work).
 """
 import math
 
@@ -98,3 +100,21 @@ def test_zero_and_mismatched_right_hand_sides(parts):
         MB.block_diag_split(pats, vals[:2])
     with pytest.raises(ValueError):
         MB.block_diag_split(pats[:1], [(vals[0][0][:-1], vals[0][1], vals[0][2])])
+
+
+def test_create_stacked_rejects_bad_input(parts):
+    from actfemtide_amd import afx
+    pats, vals = zip(*[_arrays(o) for o in parts])
+    INU, IAU, INL, IAL, D, AU, AL = MB.block_diag_split(pats, vals)
+    sizes = [len(o["D"]) for o in parts]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    bad_offsets = [off[:-1], np.r_[1, off[1:]], np.r_[off[:2], off[1:]]]  # short of n, not from 0, empty system
+    for ro in bad_offsets:
+        with pytest.raises(afx.AfxError) as e:
+            afx.System.from_stacked(ro, INU, IAU, INL, IAL)
+        assert e.value.code == afx.AFX_ERR_ARG
+    # the first two systems as one claimed to be two: their rows' entries cross the cut
+    cut = np.r_[0, sizes[0] // 2, off[1:]]
+    with pytest.raises(afx.AfxError) as e:
+        afx.System.from_stacked(cut, INU, IAU, INL, IAL)
+    assert e.value.code == afx.AFX_ERR_ARG and "couples" in str(e.value)

@@ -181,6 +181,7 @@ class System:
         n, npu, npl, ll, lu = (C.c_int() for _ in range(5))
         _err(lib().afx_get_sizes(self.h, C.byref(n), C.byref(npu), C.byref(npl), C.byref(ll), C.byref(lu)))
         self.n, self.npu, self.npl, self.nlev_l, self.nlev_u = n.value, npu.value, npl.value, ll.value, lu.value
+        self.omega = None  # angular frequency of the values (afx_assemble / afx_set_omega)
 
     @classmethod
     def from_n6line(cls, nline: int, n6line: np.ndarray) -> "System":
@@ -273,10 +274,12 @@ class System:
         b = np.zeros(self.n, np.complex128)
         sig = _f64(sigma_tet)
         _err(lib().afx_assemble(self.h, _p(sig), omega, _p(b)))
+        self.omega = omega
         return b
 
     def set_omega(self, omega: float):
         _err(lib().afx_set_omega(self.h, omega))
+        self.omega = omega
 
     def genbc(self, line, zmin, zmax, xout, yout) -> np.ndarray:
         line = _i32(line)

@@ -90,15 +90,18 @@ class MeshBatch:
     construction; the parts themselves are only used again to report each part's true residual
     in the shared-scalar solve."""
 
-    def __init__(self, parts, per_system: bool = True):
+    def __init__(self, parts, per_system: bool = True, *, patterns=None, values=None):
         parts = list(parts)
-        if not parts:
-            raise ValueError("no parts")
+        if patterns is None:
+            if not parts:
+                raise ValueError("no parts")
+            patterns = [p.pattern() for p in parts]
+            values = [p.get_values() for p in parts]
         self.parts = parts
         self.per_system = per_system
-        self.sizes = [p.n for p in parts]
+        self.sizes = [len(pt[0]) - 1 for pt in patterns]
         self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
-        arrays = block_diag_split([p.pattern() for p in parts], [p.get_values() for p in parts])
+        arrays = block_diag_split(patterns, values)
         if per_system:
             self.system = afx.System.from_stacked(self.offsets, *arrays[:4])
         else:
@@ -106,9 +109,29 @@ class MeshBatch:
         self.system.set_values(*arrays[4:])
         self.n = self.system.n
 
+    @classmethod
+    def frequency_stack(cls, system, omegas) -> "MeshBatch":
+        """One assembled system (afx_assemble values, boundary conditions applied) stacked once
+        per angular frequency in omegas, copy c holding K + i*omegas[c]*M, for the per-system
+        solve: the right-hand sides of several tidal constituents, or several right-hand sides
+        at one frequency, solved as independent systems in one launch. The system's own
+        frequency is restored afterwards."""
+        if system.omega is None:
+            raise ValueError("the system has no assembled frequency (afx_assemble first)")
+        pat = system.pattern()
+        omega0 = system.omega
+        vals = []
+        try:
+            for w in omegas:
+                system.set_omega(float(w))
+                vals.append(system.get_values())
+        finally:
+            system.set_omega(omega0)
+        return cls([], True, patterns=[pat] * len(vals), values=vals)
+
     @property
     def m(self) -> int:
-        return len(self.parts)
+        return len(self.sizes)
 
     def close(self):
         self.system.close()

@@ -18,6 +18,9 @@ Variants, off by default because each adds minutes (their results are documented
   --mixed    complex-FP32 preconditioner inside FP64 refinement (same tolerance);
   --batched  the 5 tidal constituents (M2, S2, N2, K1, O1) in one call, per-column omega (the
              autotuner picks one column at a time or the 5-column kernel);
+  --rhs-stack  the same 5 constituents as 5 copies of the system (K + i omega_c M each) stacked
+             block-diagonally and solved with per-system scalars in one launch, against the
+             autotuned k-column call;
   --mesh-batch  16 independent meshes from 8x8x6 to 40x40x30 cells (1,830,720 edges in all), each
              with its own seeded crust, at M2: one solve per mesh against the stacked
              block-diagonal system in one launch (actfemtide_amd.meshbatch), with per-system and
@@ -92,6 +95,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed", action="store_true", help="also time the complex-FP32-preconditioner solve")
     ap.add_argument("--batched", action="store_true", help="also time the 5 constituents in one launch")
+    ap.add_argument("--rhs-stack", action="store_true",
+                    help="also time the 5 constituents stacked as 5 systems in one launch")
     ap.add_argument("--mesh-batch", action="store_true",
                     help="also time 16 meshes of 8x8x6 to 40x40x30 cells one by one and in one launch")
     ap.add_argument("--maxit", type=int, default=400000, help="iteration cap of each solve")
@@ -204,6 +209,8 @@ def main():
             true_resid_max=float(s.true_residual_device(dBk, dXk, omega_scale=omegas / w0).max()))
         log(f"batched: {k} solves in {tb:.2f}s, iterations {[x.iter for x in stb]}")
 
+    if args.rhs_stack:
+        variants["stacked_5_constituents"] = rhs_stack_variant(s, b, omegas, w0, opts)
     if args.mesh_batch:
         variants["mesh_batch_16"] = mesh_batch_variant(opts, w0)
 
@@ -264,6 +271,51 @@ def main():
         "variants": variants or None,
     }
     print(json.dumps(out), flush=True)
+
+
+def rhs_stack_variant(s, b, omegas, w0, opts):
+    """The 5 constituents of one mesh: the autotuned 5-column call (one column at a time or the
+    5-column kernel) against the system stacked once per constituent (MeshBatch.frequency_stack)
+    in one launch with per-system scalars. Solves only, inputs on the device."""
+    import torch
+
+    from actfemtide_amd.meshbatch import MeshBatch
+
+    k, n = len(omegas), s.n
+    s.autotune(k)
+    dBk = torch.from_numpy(np.ascontiguousarray(np.tile(b[:, None], (1, k)))).cuda()
+    dXk = torch.zeros_like(dBk)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st, rc = s.solve_device(dBk, dXk, omega_scale=omegas / w0, **opts)
+    torch.cuda.synchronize()
+    t_cols = time.perf_counter() - t0
+    tr_cols = float(s.true_residual_device(dBk, dXk, omega_scale=omegas / w0).max())
+    log(f"5 constituents, autotuned call: {t_cols:.2f}s, iterations {[x.iter for x in st]}")
+    t0 = time.time()
+    mb = MeshBatch.frequency_stack(s, omegas)
+    mb.system.precond_setup("SSOR")
+    mb.system.autotune(1)
+    log(f"stacked constituents: n={mb.n} ({time.time() - t0:.1f}s)")
+    dB = torch.from_numpy(np.tile(b, k)).cuda()
+    dX = torch.zeros_like(dB)
+    mb.system.kernel_timing(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st2, rc2 = mb.solve_device(dB, dX, **opts)
+    torch.cuda.synchronize()
+    t_stack = time.perf_counter() - t0
+    launches, iters, kms = mb.system.kernel_timing(reset=True)
+    log(f"5 constituents stacked: {t_stack:.2f}s, iterations {[x.iter for x in st2]}, "
+        f"{1e3 * kms / max(iters, 1):.1f} us/iteration")
+    out = dict(columns=dict(solves_per_s=round(k / t_cols, 4), seconds=round(t_cols, 3), iters=[x.iter for x in st],
+                            true_resid_max=tr_cols, rc=rc),
+               stacked=dict(solves_per_s=round(k / t_stack, 4), seconds=round(t_stack, 3),
+                            iters=[x.iter for x in st2], us_per_iteration=round(1e3 * kms / max(iters, 1), 1),
+                            true_resid_max=max(x.true_resid for x in st2), rc=rc2),
+               speedup=round(t_cols / t_stack, 3))
+    mb.close()
+    return out
 
 
 def mesh_batch_variant(opts, omega):

@@ -108,3 +108,36 @@ def test_zero_right_hand_side_part(batch, mode):
         assert st[1].iter == 0 and st[1].resid == 0.0
     assert not np.any(xs[1])
     assert tr[0] <= 1e-8 and tr[2] <= 1e-8 and tr[1] == 0.0
+
+
+def test_frequency_stack_matches_each_frequency():
+    """One system stacked at three frequencies (MeshBatch.frequency_stack): copy c is the system
+    at omega_c (values equal afx_get_values after afx_set_omega), each copy ends below 1e-8 and
+    its B field agrees with the system solved at omega_c on its own."""
+    from actfemtide_amd import afx, mesh
+    from actfemtide_amd.meshbatch import MeshBatch
+    c = mesh.tide_case(8, 8, 6, seed=70)
+    s, b, _ = afx.forward_system(c, OMEGAS[0])
+    try:
+        mb = MeshBatch.frequency_stack(s, OMEGAS)
+        assert s.omega == OMEGAS[0] and mb.m == 3 and mb.system.nsys == 3
+        D, AU, AL = mb.system.get_values()
+        rng = np.random.default_rng(3)
+        lo, hi = c.xyz.min(axis=0), c.xyz.max(axis=0)
+        pts = lo + (hi - lo) * rng.uniform(0.05, 0.95, (100, 3))
+        xs, st, tr = mb.solve([b, b, b], tol=1e-8, precond="SSOR", maxit=100000)
+        print(f"[meshbatch] frequency stack: iterations {[x.iter for x in st]}, true residuals {tr}", flush=True)
+        for q, w in enumerate(OMEGAS):
+            s.set_omega(w)
+            Dq, AUq, ALq = s.get_values()
+            o0, o1 = mb.offsets[q], mb.offsets[q + 1]
+            assert np.array_equal(D[o0:o1], Dq)
+            A = _scipy(s)
+            r = np.linalg.norm(b - A @ xs[q]) / np.linalg.norm(b)
+            assert st[q].error == 0 and r <= 1e-8
+            x_own, _ = s.solve(b, precond="SSOR", tol=1e-8, maxit=100000)
+            B, B_own = _b_field(s, pts, xs[q]), _b_field(s, pts, x_own)
+            assert np.linalg.norm(B - B_own) <= 1e-6 * np.linalg.norm(B_own)
+        mb.close()
+    finally:
+        s.close()

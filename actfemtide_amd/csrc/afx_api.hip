@@ -139,6 +139,12 @@ struct afx_system_s {
   int nsys = 1;
   std::vector<int> sys_off;
   int *d_sys_off = nullptr, *d_bsys = nullptr, *d_sblk = nullptr;
+  // right-hand-side block width "stack": k columns solved as k stacked copies of this system
+  // (each at its column's omega, per-system scalars); the copies' handle, the omegas its values
+  // were taken at and the values version they copy (val_ver grows with every value change)
+  afx_system stack = nullptr;
+  std::vector<double> stack_w;
+  long long stack_ver = -1, val_ver = 0;
   DevSide fw, bw, bjfw, bjbw;
   unsigned *d_bar = nullptr;
   int *d_err = nullptr;
@@ -359,6 +365,7 @@ int ensure_precond(afx_system s, int kind, double sd, double sig, const std::vec
 void invalidate_values(afx_system s) {
   s->derived_valid = false;
   s->pc_valid = false;
+  s->val_ver++;
 }
 
 // column frequencies -> distinct list + map
@@ -1040,6 +1047,8 @@ void release(afx_system s) {
   free_side(s->bjbw);
   dfree(s->d_rstart); dfree(s->d_bar); dfree(s->d_err);
   dfree(s->d_sys_off); dfree(s->d_bsys); dfree(s->d_sblk);
+  if (s->stack) afx_destroy(s->stack);
+  s->stack = nullptr;
   if (s->stream) (void)hipStreamDestroy(s->stream);
   s->stream = nullptr;
 }
@@ -1114,6 +1123,51 @@ __global__ void k_inv_or_zero(int k, const double *norms, double tol, double *sc
 
 int solve_device_impl(afx_system s, int k, const cx<double> *dB, cx<double> *dX, const double *omega_scale,
                       const afx_opts *o, afx_stats *stats, bool stacked = false);
+
+int ensure_stack(afx_system s, int k, const std::vector<double> &wcol);
+int get_values_at(afx_system s, double w, double *D, double *AU, double *AL);
+
+// Right-hand-side block width "stack": the k columns solved as k independent systems stacked
+// block-diagonally (copy c holds K + i*omega_c*M) in one single-column launch with per-system
+// scalars. The copies are built once per k, omega list and values version.
+int solve_stacked_columns(afx_system s, int k, const cx<double> *dB, cx<double> *dX, const double *omega_scale,
+                          const afx_opts *o, afx_stats *stats) {
+  std::vector<double> wcol, wf;
+  std::vector<int> col2f;
+  column_freqs(s, k, omega_scale, wcol, wf, col2f);
+  int rc = ensure_stack(s, k, wcol);
+  if (rc) return rc;
+  const long long n = s->n;
+  cx<double> *b1 = nullptr, *x1 = nullptr;
+  if ((rc = dalloc(&b1, (size_t)n * k)) || (rc = dalloc(&x1, (size_t)n * k))) {
+    dfree(b1);
+    return rc;
+  }
+  hipError_t e = hipSuccess;
+  for (int c = 0; c < k && e == hipSuccess; c++) {
+    e = afx::launch_col_get(n, k, c, dB, b1 + (size_t)c * n, s->stream);
+    if (e == hipSuccess) e = afx::launch_col_get(n, k, c, dX, x1 + (size_t)c * n, s->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  int ret = AFX_OK;
+  if (e != hipSuccess) {
+    ret = fail(AFX_ERR_HIP, hipGetErrorString(e));
+  } else {
+    afx_opts o2 = *o;
+    o2.nset = 1;
+    ret = solve_device_impl(s->stack, 1, b1, x1, nullptr, &o2, stats, true);
+    if (ret == AFX_OK || ret == AFX_ERR_MAXIT || ret == AFX_ERR_BREAKDOWN) {
+      hipError_t e2 = hipStreamSynchronize(s->stack->stream);
+      for (int c = 0; c < k && e2 == hipSuccess; c++)
+        e2 = afx::launch_col_put(n, k, c, x1 + (size_t)c * n, dX, s->stream);
+      if (e2 == hipSuccess) e2 = hipStreamSynchronize(s->stream);
+      if (e2 != hipSuccess) ret = fail(AFX_ERR_HIP, hipGetErrorString(e2));
+    }
+  }
+  dfree(b1);
+  dfree(x1);
+  return ret;
+}
 
 // Right-hand-side block width 1: the k columns solved one after another by the single-column
 // kernel, each at its own frequency. Its pipelined sweep keeps a block's whole row slab in LDS,
@@ -1207,7 +1261,9 @@ int solve_device_impl(afx_system s, int k, const cx<double> *dB, cx<double> *dX,
   // (afx_autotune), is asked for (AFX_BATCH=multi; AFX_BATCH=columns forces one at a time), or
   // the caller reuses a preconditioner set up for exactly these k frequencies (nset = 0)
   const char *bm = getenv("AFX_BATCH");
-  const bool multi = bm ? std::string(bm) == "multi" : s->batch_multi[k] == 1;
+  const int width = bm ? (std::string(bm) == "multi" ? 1 : std::string(bm) == "stack" ? 2 : 0) : s->batch_multi[k];
+  const bool multi = width == 1;
+  if (k > 1 && o->nset != 0 && width == 2 && !stacked) return solve_stacked_columns(s, k, dB, dX, omega_scale, o, stats);
   if (k > 1 && o->nset != 0 && !multi) return solve_columns(s, k, dB, dX, omega_scale, o, stats);
   std::vector<double> wcol, wf;
   std::vector<int> col2f;
@@ -1430,11 +1486,19 @@ int afx_set_values(afx_system s, const double *D, const double *AU, const double
 
 int afx_get_values(afx_system s, double *D, double *AU, double *AL) {
   if (!s) return fail(AFX_ERR_ARG, "null system");
+  return get_values_at(s, s->omega, D, AU, AL);
+}
+
+}  // extern "C"
+
+namespace {
+
+// the split values D, AU, AL (interleaved complex, host) at angular frequency w
+int get_values_at(afx_system s, double w, double *D, double *AU, double *AL) {
   const size_t n = s->n, nU = s->nU, nL = s->nL;
   std::vector<cx<double>> h(n + nU + nL);
   HIPCHK(hipStreamSynchronize(s->stream));
   HIPCHK(hipMemcpy(h.data(), s->d_canon, h.size() * sizeof(cx<double>), hipMemcpyDeviceToHost));
-  const double w = s->omega;
   auto put = [&](double *dst, size_t off, size_t m) {
     if (!dst) return;
     for (size_t q = 0; q < m; q++) {
@@ -1447,6 +1511,62 @@ int afx_get_values(afx_system s, double *D, double *AU, double *AL) {
   put(AL, n + nU, nL);
   return AFX_OK;
 }
+
+// the k stacked copies of block width "stack" for the column omegas wcol (pattern once per k,
+// values again whenever the omegas or the system's values changed)
+int ensure_stack(afx_system s, int k, const std::vector<double> &wcol) {
+  if (s->stack && s->stack->nsys == k && s->stack_w == wcol && s->stack_ver == s->val_ver) return AFX_OK;
+  const long long n = s->n, nU = s->nU, nL = s->nL;
+  int rc;
+  if (!s->stack || s->stack->nsys != k) {
+    if (s->stack) afx_destroy(s->stack);
+    s->stack = nullptr;
+    if (n * k > 0x7fffffffLL || s->nnz * k > 0x7fffffffLL)
+      return fail(AFX_ERR_ARG, "the stacked copies exceed 32-bit indices");
+    std::vector<int> off(k + 1), INU(n * k + 1), INL(n * k + 1), IAU(nU * k), IAL(nL * k);
+    INU[0] = INL[0] = 0;
+    for (int c = 0; c < k; c++) {
+      off[c] = (int)(c * n);
+      for (long long i = 0; i < n; i++) {
+        INU[c * n + i + 1] = (int)(c * nU + s->INU[i + 1]);
+        INL[c * n + i + 1] = (int)(c * nL + s->INL[i + 1]);
+      }
+      for (long long j = 0; j < nU; j++) IAU[c * nU + j] = (int)(s->IAU[j] + c * n);
+      for (long long j = 0; j < nL; j++) IAL[c * nL + j] = (int)(s->IAL[j] + c * n);
+    }
+    off[k] = (int)(n * k);
+    afx_system t = nullptr;
+    if ((rc = afx_create_stacked(k, off.data(), (int)(n * k), INU.data(), IAU.data(), INL.data(), IAL.data(), &t)))
+      return rc;
+    t->lpr64 = s->lpr64;  // the same rows: the lane mapping tuned for one copy
+    t->lpr32 = s->lpr32;
+    s->stack = t;
+  }
+  std::vector<cx<double>> h(n + nU + nL);
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpy(h.data(), s->d_canon, h.size() * sizeof(cx<double>), hipMemcpyDeviceToHost));
+  std::vector<double> D(2 * n * k), AU(2 * nU * k), AL(2 * nL * k);
+  for (int c = 0; c < k; c++) {
+    const double w = wcol[c];
+    auto put = [&](double *dst, size_t off, size_t m) {
+      for (size_t q = 0; q < m; q++) {
+        dst[2 * q] = h[off + q].r;
+        dst[2 * q + 1] = w * h[off + q].i;
+      }
+    };
+    put(D.data() + 2 * n * c, 0, n);
+    put(AU.data() + 2 * nU * c, n, nU);
+    put(AL.data() + 2 * nL * c, n + nU, nL);
+  }
+  if ((rc = afx_set_values(s->stack, D.data(), AU.data(), AL.data()))) return rc;
+  s->stack_w = wcol;
+  s->stack_ver = s->val_ver;
+  return AFX_OK;
+}
+
+}  // namespace
+
+extern "C" {
 
 int afx_set_mesh(afx_system s, int node, int ntet, const double *xyz, const int *n4, const int *group,
                  const double *vxyz, const double *fxyz) {
@@ -1928,20 +2048,36 @@ static int autotune_width(afx_system s, int k, int kind, int verbose) {
   o.tol = 0.0;
   o.maxit = 20;
   o.check_true = 0;
-  double t[2] = {1e300, 1e300};  // [0] one column (times k), [1] k columns
-  for (int multi = 0; multi < 2; multi++) {
-    s->batch_multi[k] = multi;
+  // [0] one column (times k), [1] k columns, [2] k stacked copies (only where the copies' rows
+  // of a block fit in LDS; built before timing)
+  double t[3] = {1e300, 1e300, 1e300};
+  std::vector<double> wcol, wf;
+  std::vector<int> col2f;
+  column_freqs(s, k, nullptr, wcol, wf, col2f);
+  // the copies are only tried when a block's share of their rows can fit in LDS (checked exactly
+  // once built): a sweep without it is several times slower
+  const std::string keep_err = g_err;
+  const bool small = (double)s->n * k / std::max(1, s->ncu) * sizeof(cx<double>) <= 150000.0;
+  const bool can_stack = small && ensure_stack(s, k, wcol) == AFX_OK && lds_rows_for<double>(s->stack, false) > 0;
+  g_err = keep_err;
+  static const char *const names[3] = {"one column x k", "k-column kernel", "k stacked copies"};
+  for (int mode = 0; mode < 3; mode++) {
+    if (mode == 2 && !can_stack) continue;
+    s->batch_multi[k] = mode;
     if (hipMemset(dX, 0, nk * sizeof(cx<double>)) != hipSuccess) break;
     (void)hipDeviceSynchronize();
     auto t0 = std::chrono::steady_clock::now();
     std::vector<afx_stats> st(k);
-    const int r2 = solve_device_impl(s, multi ? k : 1, dB, dX, nullptr, &o, st.data());
+    const int r2 = solve_device_impl(s, mode == 0 ? 1 : k, dB, dX, nullptr, &o, st.data());
     (void)hipDeviceSynchronize();
     const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    if (r2 == AFX_OK || r2 == AFX_ERR_MAXIT) t[multi] = multi ? dt : dt * k;
-    if (verbose) fprintf(stderr, "afx_autotune k=%d %s: %.3f ms / 20 it\n", k, multi ? "k-column kernel" : "one column x k", t[multi] * 1e3);
+    if (r2 == AFX_OK || r2 == AFX_ERR_MAXIT) t[mode] = mode == 0 ? dt * k : dt;
+    if (verbose) fprintf(stderr, "afx_autotune k=%d %s: %.3f ms / 20 it\n", k, names[mode], t[mode] * 1e3);
   }
-  s->batch_multi[k] = t[1] < t[0] ? 1 : 0;
+  int best = 0;
+  for (int mode = 1; mode < 3; mode++)
+    if (t[mode] < t[best]) best = mode;
+  s->batch_multi[k] = best;
   dfree(dB);
   dfree(dX);
   return AFX_OK;

@@ -175,7 +175,13 @@ int afx_scale_initial_guess_device(afx_system s, int nrhs, const void *dB, void 
    since the last reset: launches, COCG iterations they ran, and total kernel milliseconds. */
 int afx_kernel_timing(afx_system s, int reset, long long *launches, long long *iterations, double *kernel_ms);
 
-/* Autotune rows-per-wavefront and persistent grid size for nrhs columns on this system. */
+/* Autotune for nrhs columns on this system. nrhs = 1: the rows per wavefront of the persistent
+   kernel. nrhs > 1: the right-hand-side block width of afx_solve / afx_solve_device, timing 20
+   iterations of each: one column at a time, the nrhs-column kernel, and nrhs stacked copies of
+   the system (copy c at column c's omega) in one launch with per-system scalars (only where the
+   copies' rows of a sweep block fit in LDS; the copies are kept by the handle and rebuilt when
+   the values or the column omegas change). The environment variable AFX_BATCH=columns|multi|stack
+   overrides the choice. */
 int afx_autotune(afx_system s, int nrhs, int verbose);
 
 /* y = A x (complex, host arrays) with the device SpMV -- parity hook. */

@@ -1127,41 +1127,74 @@ int solve_device_impl(afx_system s, int k, const cx<double> *dB, cx<double> *dX,
 int ensure_stack(afx_system s, int k, const std::vector<double> &wcol);
 int get_values_at(afx_system s, double w, double *D, double *AU, double *AL);
 
-// Right-hand-side block width "stack": the k columns solved as k independent systems stacked
-// block-diagonally (copy c holds K + i*omega_c*M) in one single-column launch with per-system
-// scalars. The copies are built once per k, omega list and values version.
+// Copies per stacked launch for k columns: as many as let a block's share of the copies' rows
+// fit in LDS (about 150 KB of complex doubles per block), in groups of equal size; 0 when fewer
+// than two copies fit.
+int stack_group(afx_system s, int k) {
+  const double per_copy = (double)s->n / std::max(1, s->ncu) * sizeof(cx<double>);
+  int gmax = (int)std::min<double>(256.0, std::floor(150000.0 / std::max(per_copy, 1.0)));
+  if (const char *e = getenv("AFX_STACK_GROUP")) gmax = std::min(gmax, std::max(2, atoi(e)));  // a smaller cap
+  if (gmax < 2 || k < 2) return 0;
+  const int groups = (k + gmax - 1) / gmax;
+  return (k + groups - 1) / groups;
+}
+
+// Right-hand-side block width "stack": the k columns solved as independent systems stacked
+// block-diagonally (copy c holds K + i*omega_c*M) in single-column launches with per-system
+// scalars, stack_group(k) copies per launch; the last group is padded with zero right-hand
+// sides (solved as x = 0 at once), so every launch reuses the kept copies' pattern.
 int solve_stacked_columns(afx_system s, int k, const cx<double> *dB, cx<double> *dX, const double *omega_scale,
                           const afx_opts *o, afx_stats *stats) {
   std::vector<double> wcol, wf;
   std::vector<int> col2f;
   column_freqs(s, k, omega_scale, wcol, wf, col2f);
-  int rc = ensure_stack(s, k, wcol);
-  if (rc) return rc;
+  int g = stack_group(s, k);
+  if (g == 0) g = k;  // asked for explicitly (AFX_BATCH=stack): one launch, whatever fits in LDS
   const long long n = s->n;
   cx<double> *b1 = nullptr, *x1 = nullptr;
-  if ((rc = dalloc(&b1, (size_t)n * k)) || (rc = dalloc(&x1, (size_t)n * k))) {
+  int rc;
+  if ((rc = dalloc(&b1, (size_t)n * g)) || (rc = dalloc(&x1, (size_t)n * g))) {
     dfree(b1);
     return rc;
   }
-  hipError_t e = hipSuccess;
-  for (int c = 0; c < k && e == hipSuccess; c++) {
-    e = afx::launch_col_get(n, k, c, dB, b1 + (size_t)c * n, s->stream);
-    if (e == hipSuccess) e = afx::launch_col_get(n, k, c, dX, x1 + (size_t)c * n, s->stream);
-  }
-  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   int ret = AFX_OK;
-  if (e != hipSuccess) {
-    ret = fail(AFX_ERR_HIP, hipGetErrorString(e));
-  } else {
+  std::vector<afx_stats> st(g);
+  for (int c0 = 0; c0 < k; c0 += g) {
+    const int c1 = std::min(k, c0 + g);
+    std::vector<double> wg(g, wcol[c1 - 1]);
+    for (int c = c0; c < c1; c++) wg[c - c0] = wcol[c];
+    if ((rc = ensure_stack(s, g, wg))) {
+      ret = rc;
+      break;
+    }
+    hipError_t e = hipMemsetAsync(b1, 0, (size_t)n * g * sizeof(cx<double>), s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(x1, 0, (size_t)n * g * sizeof(cx<double>), s->stream);
+    for (int c = c0; c < c1 && e == hipSuccess; c++) {
+      e = afx::launch_col_get(n, k, c, dB, b1 + (size_t)(c - c0) * n, s->stream);
+      if (e == hipSuccess) e = afx::launch_col_get(n, k, c, dX, x1 + (size_t)(c - c0) * n, s->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+      ret = fail(AFX_ERR_HIP, hipGetErrorString(e));
+      break;
+    }
     afx_opts o2 = *o;
     o2.nset = 1;
-    ret = solve_device_impl(s->stack, 1, b1, x1, nullptr, &o2, stats, true);
-    if (ret == AFX_OK || ret == AFX_ERR_MAXIT || ret == AFX_ERR_BREAKDOWN) {
-      hipError_t e2 = hipStreamSynchronize(s->stack->stream);
-      for (int c = 0; c < k && e2 == hipSuccess; c++)
-        e2 = afx::launch_col_put(n, k, c, x1 + (size_t)c * n, dX, s->stream);
-      if (e2 == hipSuccess) e2 = hipStreamSynchronize(s->stream);
-      if (e2 != hipSuccess) ret = fail(AFX_ERR_HIP, hipGetErrorString(e2));
+    const int r = solve_device_impl(s->stack, 1, b1, x1, nullptr, &o2, st.data(), true);
+    if (r != AFX_OK && r != AFX_ERR_MAXIT && r != AFX_ERR_BREAKDOWN) {
+      ret = r;
+      break;
+    }
+    if (r == AFX_ERR_BREAKDOWN || (r == AFX_ERR_MAXIT && ret == AFX_OK)) ret = r;
+    e = hipStreamSynchronize(s->stack->stream);
+    for (int c = c0; c < c1 && e == hipSuccess; c++) {
+      if (stats) stats[c] = st[c - c0];
+      e = afx::launch_col_put(n, k, c, x1 + (size_t)(c - c0) * n, dX, s->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+      ret = fail(AFX_ERR_HIP, hipGetErrorString(e));
+      break;
     }
   }
   dfree(b1);
@@ -2054,11 +2087,13 @@ static int autotune_width(afx_system s, int k, int kind, int verbose) {
   std::vector<double> wcol, wf;
   std::vector<int> col2f;
   column_freqs(s, k, nullptr, wcol, wf, col2f);
-  // the copies are only tried when a block's share of their rows can fit in LDS (checked exactly
-  // once built): a sweep without it is several times slower
+  // the copies are only tried when a block's share of a group's rows can fit in LDS (checked
+  // exactly once built): a sweep without it is several times slower
   const std::string keep_err = g_err;
-  const bool small = (double)s->n * k / std::max(1, s->ncu) * sizeof(cx<double>) <= 150000.0;
-  const bool can_stack = small && ensure_stack(s, k, wcol) == AFX_OK && lds_rows_for<double>(s->stack, false) > 0;
+  const int grp = stack_group(s, k);
+  const bool can_stack = grp >= 2 &&
+                         ensure_stack(s, grp, std::vector<double>(wcol.begin(), wcol.begin() + grp)) == AFX_OK &&
+                         lds_rows_for<double>(s->stack, false) > 0;
   g_err = keep_err;
   static const char *const names[3] = {"one column x k", "k-column kernel", "k stacked copies"};
   for (int mode = 0; mode < 3; mode++) {

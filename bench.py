@@ -21,6 +21,9 @@ Variants, off by default because each adds minutes (their results are documented
   --rhs-stack  the same 5 constituents as 5 copies of the system (K + i omega_c M each) stacked
              block-diagonally and solved with per-system scalars in one launch, against the
              autotuned k-column call;
+  --adjoint-batch SITES  SITES seeded seafloor sites x 3 electric components as adjoint
+             right-hand sides (solvePARDISOinv's P^T columns) at M2: one column at a time against
+             the autotuned block width (stacked copies in groups where they fit LDS);
   --mesh-batch  16 independent meshes from 8x8x6 to 40x40x30 cells (1,830,720 edges in all), each
              with its own seeded crust, at M2: one solve per mesh against the stacked
              block-diagonal system in one launch (actfemtide_amd.meshbatch), with per-system and
@@ -95,6 +98,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed", action="store_true", help="also time the complex-FP32-preconditioner solve")
     ap.add_argument("--batched", action="store_true", help="also time the 5 constituents in one launch")
+    ap.add_argument("--adjoint-batch", type=int, default=0, metavar="SITES",
+                    help="also time SITES seafloor sites x 3 electric adjoint right-hand sides, one column at a "
+                         "time against the autotuned block width")
     ap.add_argument("--rhs-stack", action="store_true",
                     help="also time the 5 constituents stacked as 5 systems in one launch")
     ap.add_argument("--mesh-batch", action="store_true",
@@ -209,6 +215,8 @@ def main():
             true_resid_max=float(s.true_residual_device(dBk, dXk, omega_scale=omegas / w0).max()))
         log(f"batched: {k} solves in {tb:.2f}s, iterations {[x.iter for x in stb]}")
 
+    if args.adjoint_batch:
+        variants[f"adjoint_{3 * args.adjoint_batch}_columns"] = adjoint_batch_variant(s, case, opts, args.adjoint_batch)
     if args.rhs_stack:
         variants["stacked_5_constituents"] = rhs_stack_variant(s, b, omegas, w0, opts)
     if args.mesh_batch:
@@ -271,6 +279,58 @@ def main():
         "variants": variants or None,
     }
     print(json.dumps(out), flush=True)
+
+
+def seafloor_sites(case, n, seed):
+    """Seeded sites 1 m below the sea-bottom plane (z = -h), strictly inside a tet."""
+    rng = np.random.default_rng(seed)
+    h = case.h
+    xout, yout = case.nx * h / 2, case.ny * h / 2
+    pts = []
+    for _ in range(n):
+        i = rng.integers(1, case.nx - 1)
+        j = rng.integers(1, case.ny - 1)
+        fx, fy = rng.uniform(0.13, 0.87, size=2)
+        if abs(fx - fy) < 0.05:
+            fy = min(0.87, fy + 0.1)
+        pts.append([-xout + (i + fx) * h, -yout + (j + fy) * h, -h - 0.001])
+    return np.array(pts)
+
+
+def adjoint_batch_variant(s, case, opts, nsites):
+    """solvePARDISOinv's adjoint right-hand sides (P^T columns of the 3 electric components at
+    nsites seafloor sites, seed 21) solved one column at a time and with the autotuned block
+    width. Solves only, inputs on the device."""
+    import torch
+
+    item, ve, vb, _ = s.prepobscoeff(seafloor_sites(case, nsites, 21))
+    stack = np.arange(0, 6 * nsites + 1, 6, dtype=np.int32)
+    cols = []
+    for comp in range(3):
+        d = torch.zeros((s.n, nsites), dtype=torch.complex128, device="cuda")
+        s.adjoint_rhs_device(stack, item, ve[comp], d)
+        cols.append(d)
+    dB = torch.cat(cols, dim=1).contiguous()
+    k = dB.shape[1]
+    out = {}
+    for width in ("columns", "autotuned"):
+        if width == "columns":
+            os.environ["AFX_BATCH"] = "columns"
+        else:
+            os.environ.pop("AFX_BATCH", None)
+            s.autotune(k)
+        dX = torch.zeros_like(dB)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st, rc = s.solve_device(dB, dX, **opts)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        tr = float(s.true_residual_device(dB, dX).max())
+        log(f"{k} adjoint columns, {width}: {dt:.2f}s, iterations {[x.iter for x in st]}, worst true residual {tr:.2e}")
+        out[width] = dict(solves_per_s=round(k / dt, 4), seconds=round(dt, 3), iters=[x.iter for x in st],
+                          true_resid_max=tr, rc=rc)
+    out["speedup"] = round(out["columns"]["seconds"] / out["autotuned"]["seconds"], 3)
+    return out
 
 
 def rhs_stack_variant(s, b, omegas, w0, opts):

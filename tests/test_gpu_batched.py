@@ -108,6 +108,25 @@ def test_multi_frequency_columns(case_small, sysdev, mode, monkeypatch):
         assert st[c].error == 0 and r <= 1.05e-8
 
 
+def test_stacked_columns_in_groups(case_small, sysdev, monkeypatch):
+    """5 constituents stacked at most 2 copies per launch (AFX_STACK_GROUP=2): groups of 2, 2 and
+    1 padded with a zero right-hand side; every column reaches 1e-8 at its own omega, and the
+    copies are rebuilt for each group's frequencies."""
+    import torch
+    monkeypatch.setenv("AFX_BATCH", "stack")
+    monkeypatch.setenv("AFX_STACK_GROUP", "2")
+    s, b, bc, dev = sysdev
+    dB = torch.from_numpy(np.repeat(b[:, None], 5, axis=1)).to(dev).contiguous()
+    dX = torch.zeros_like(dB)
+    st, rc = s.solve_device(dB, dX, omega_scale=OMEGAS / OMEGA_M2, precond="SSOR", tol=1e-8, maxit=40000)
+    X = dX.cpu().numpy()
+    print("stacked groups: iterations", [x.iter for x in st], flush=True)
+    for c, w in enumerate(OMEGAS):
+        A, _ = _oracle_matrix(case_small, w)
+        r = np.linalg.norm(b - A @ X[:, c]) / np.linalg.norm(b)
+        assert st[c].error == 0 and st[c].iter > 0 and r <= 1.05e-8
+
+
 def test_mixed_precision_refinement(case_small, sysdev):
     """COCG with a complex-FP32 preconditioner (FP64 SpMV, recurrences, true residual) reaches the
     FP64 true residual 1e-8, and its seafloor fields agree with the all-FP64 oracle solution."""

@@ -21,7 +21,8 @@ def _ptr_list(item, ve, vb):
     return [(stack, it, vb[c].reshape(-1)) for c in range(3)] + [(stack, it, ve[c].reshape(-1)) for c in range(2)]
 
 
-def test_solve_pardiso_inv_identities(case_small):
+@pytest.mark.parametrize("autotune", [False, True])
+def test_solve_pardiso_inv_identities(case_small, autotune):
     import scipy.sparse.linalg as spla
     from actfemtide_amd import afx
     from oracle import oracle as O
@@ -32,7 +33,9 @@ def test_solve_pardiso_inv_identities(case_small):
     PTR = _ptr_list(item, ve, vb)
     # a point-like adjoint source needs a tighter residual than the forward solve for u^T b to
     # reach 1e-6 (at 1e-8 the identity holds to ~6e-5 on this mesh)
-    xout, ut, stats = afx.solve_pardiso_inv(s, b, PTR, iflag_comp=[1, 1, 1, 1, 0], tol=1e-11)
+    # autotune: each 12-column adjoint block in the width afx_autotune picks (stacked copies, the
+    # k-column kernel or one column at a time)
+    xout, ut, stats = afx.solve_pardiso_inv(s, b, PTR, iflag_comp=[1, 1, 1, 1, 0], tol=1e-11, autotune=autotune)
     assert ut[4] == 0
     x = xout[:, 0]
     o = O.forward_system(case_small, OMEGA_M2)

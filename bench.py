@@ -14,6 +14,11 @@ one persistent launch per 2000 iterations.
 After the solve, the SpMV q = A p runs as a kernel of its own (k_spmv_stream, 50 timed launches
 on the solution vector); the line's "spmv" object gives its algorithmic GB/s against the HBM peak.
 
+By default the line also carries one batching variant, "adjoint_12_columns_32x32x24": 12
+adjoint right-hand sides (4 seafloor sites x 3 electric components) on the 32x32x24-cell mesh,
+one column at a time against the autotuned block width (stacked copies in one launch); about a
+minute, skipped with --no-variants.
+
 Variants, off by default because each adds minutes (their results are documented):
   --mixed    complex-FP32 preconditioner inside FP64 refinement (same tolerance);
   --batched  the 5 tidal constituents (M2, S2, N2, K1, O1) in one call, per-column omega (the
@@ -98,6 +103,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed", action="store_true", help="also time the complex-FP32-preconditioner solve")
     ap.add_argument("--batched", action="store_true", help="also time the 5 constituents in one launch")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the default batching variant (12 adjoint columns on 32x32x24 cells)")
     ap.add_argument("--adjoint-batch", type=int, default=0, metavar="SITES",
                     help="also time SITES seafloor sites x 3 electric adjoint right-hand sides, one column at a "
                          "time against the autotuned block width")
@@ -217,6 +224,19 @@ def main():
 
     if args.adjoint_batch:
         variants[f"adjoint_{3 * args.adjoint_batch}_columns"] = adjoint_batch_variant(s, case, opts, args.adjoint_batch)
+    if not args.no_variants and (nx, ny, nz) != SIZES["cfg3"]:
+        # the batching gain where stacked copies fit: solvePARDISOinv's adjoint block on the
+        # 32x32x24-cell mesh (a failure here is recorded, it does not end the run)
+        try:
+            c3 = mesh.tide_case(*SIZES["cfg3"], seed=1002, spheres=3, sigma_sea=3.3)
+            s3, _, _ = afx.forward_system(c3, w0)
+            s3.precond_setup("SSOR")
+            s3.autotune(1)
+            variants["adjoint_12_columns_32x32x24"] = adjoint_batch_variant(s3, c3, opts, 4)
+            s3.close()
+        except Exception as e:  # noqa: BLE001
+            variants["adjoint_12_columns_32x32x24"] = {"error": f"{type(e).__name__}: {e}"}
+        os.environ.pop("AFX_BATCH", None)
     if args.rhs_stack:
         variants["stacked_5_constituents"] = rhs_stack_variant(s, b, omegas, w0, opts)
     if args.mesh_batch:

@@ -732,7 +732,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
   int its = cvalid ? p.iters[cst] : 0;
   int it = p.status[0];
   const int it_end = min(it + p.chunk, p.maxit);
-  bool alldone = sys ? block_all(t >= p.nsys || p.conv[t] != 0) : block_all(st != 0);
+  bool alldone;
+  if (sys) {  // every stacked system finished (any count of systems)
+    int running = 0;
+    for (int q = t; q < p.nsys; q += 256) running |= (p.conv[q] == 0) ? 1 : 0;
+    alldone = __syncthreads_or(running) == 0;
+  } else {
+    alldone = block_all(st != 0);
+  }
   // optional per-block phase timing (wall clock, 100 MHz) and mailbox poll count
   constexpr bool prof = PROF;
   unsigned long long tfw = 0, tbw = 0, tbar = 0, trest = 0, titer = 0, trho = 0, tpost = 0, tpu = 0, tsp = 0, tclr = 0;

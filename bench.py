@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed", action="store_true", help="also time the complex-FP32-preconditioner solve")
     ap.add_argument("--batched", action="store_true", help="also time the 5 constituents in one launch")
+    ap.add_argument("--jacobian", type=int, default=0, metavar="NOBS",
+                    help="also time genjacobian1's accumulation over every crust cell for NOBS adjoint columns")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the default batching variant (12 adjoint columns on 32x32x24 cells)")
     ap.add_argument("--adjoint-batch", type=int, default=0, metavar="SITES",
@@ -222,6 +224,8 @@ def main():
             true_resid_max=float(s.true_residual_device(dBk, dXk, omega_scale=omegas / w0).max()))
         log(f"batched: {k} solves in {tb:.2f}s, iterations {[x.iter for x in stb]}")
 
+    if args.jacobian:
+        variants[f"jacobian_{args.jacobian}_columns"] = jacobian_variant(s, case, w0, args.jacobian)
     if args.adjoint_batch:
         variants[f"adjoint_{3 * args.adjoint_batch}_columns"] = adjoint_batch_variant(s, case, opts, args.adjoint_batch)
     if not args.no_variants and (nx, ny, nz) != SIZES["cfg3"]:
@@ -299,6 +303,42 @@ def main():
         "variants": variants or None,
     }
     print(json.dumps(out), flush=True)
+
+
+def jacobian_variant(s, case, omega, nobs, reps=3):
+    """genjacobian1's accumulation (afx_jacobian_device: k_jac_w, then k_jac_acc, one block per
+    parameter) with one parameter per crust cell (6 tets) and nobs adjoint columns of synthetic
+    values (the kernel's work does not depend on them). Wall time per call includes the upload of
+    the parameter map; the kernel's own duration comes from a rocprofv3 trace."""
+    import torch
+
+    crust = np.nonzero(case.group >= 3)[0]
+    cells, inv = np.unique(case.cell_of_tet[crust], return_inverse=True)
+    order = np.argsort(inv, kind="stable")
+    tets = crust[order].astype(np.int32)
+    ptr = np.concatenate([[0], np.cumsum(np.bincount(inv))]).astype(np.int32)
+    npar = len(cells)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    dU = torch.randn((s.n, nobs), dtype=torch.complex128, device="cuda", generator=g)
+    dx = torch.randn((s.n,), dtype=torch.complex128, device="cuda", generator=g)
+    dJ = torch.empty((npar, nobs), dtype=torch.complex128, device="cuda")
+    sig = case.sigma_tet()
+    edges = np.unique(np.abs(np.asarray(case.n6line)[tets]).ravel())
+    # compulsory bytes of k_jac_acc: the U rows of the cells' edges once, J written once, the
+    # per-tet weights (6 complex) and edge ids (6 int) once
+    acc_bytes = len(edges) * nobs * 16 + npar * nobs * 16 + len(tets) * 6 * (16 + 4) + (npar + 1) * 4
+    times = []
+    for _ in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.jacobian_device(dU, dx, ptr, tets, sig, omega, -math.log(10.0), dJ)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    best = min(times[1:])
+    log(f"jacobian: {npar} parameters x {nobs} columns, {1e3 * best:.1f} ms per call (wall), "
+        f"k_jac_acc compulsory bytes {acc_bytes / 1e9:.2f} GB")
+    return dict(parameters=npar, columns=nobs, tets=len(tets), edges=len(edges), wall_ms=round(1e3 * best, 2),
+                acc_bytes=acc_bytes, note="kernel durations: profiles/k_jac_acc_64x64x48.md")
 
 
 def seafloor_sites(case, n, seed):

@@ -141,3 +141,25 @@ def test_frequency_stack_matches_each_frequency():
         mb.close()
     finally:
         s.close()
+
+
+def test_one_stacked_system_is_the_plain_solve(case_tiny):
+    """afx_create_stacked with a single system keeps the plain partition and kernel path: the same
+    iterates, bit for bit, as afx_solve_device on the same arrays."""
+    import torch
+    from actfemtide_amd import afx
+    from tests.conftest import OMEGA_M2
+    s, b, _ = afx.forward_system(case_tiny, OMEGA_M2)
+    t = afx.System.from_stacked([0, s.n], *s.pattern())
+    try:
+        t.set_values(*s.get_values())
+        assert t.nsys == 1 and s.nsys == 1
+        dB = torch.from_numpy(b).cuda()
+        x1, x2 = torch.zeros_like(dB), torch.zeros_like(dB)
+        st1, _ = s.solve_device(dB, x1, precond="SSOR", tol=1e-8, maxit=20000)
+        st2, _ = t.solve_stacked_device(dB, x2, precond="SSOR", tol=1e-8, maxit=20000)
+        assert len(st2) == 1 and st2[0].iter == st1[0].iter and st2[0].error == 0
+        assert torch.equal(x1, x2)
+    finally:
+        t.close()
+        s.close()

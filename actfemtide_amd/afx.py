@@ -16,9 +16,12 @@ when libafx.so is missing.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import dataclasses
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -126,7 +129,25 @@ def lib():
             f.argtypes = args
             f.restype = res
         _lib = L
+        # the drop-in's kept system is freed while the HIP runtime is still up (an exit handler
+        # of the library itself would run after the runtime has been torn down)
+        atexit.register(_release_at_exit)
     return _lib
+
+
+# every open System, so that the exit handler can free them before the runtime goes away
+_open_systems = weakref.WeakSet()
+
+
+def _release_at_exit():
+    if _lib is None:
+        return
+    for s in list(_open_systems):
+        try:
+            s.close()
+        except Exception:
+            pass
+    _lib.afx_cocg13_single_release()
 
 
 def _err(code):
@@ -183,6 +204,7 @@ class System:
         self.n, self.npu, self.npl, self.nlev_l, self.nlev_u = n.value, npu.value, npl.value, ll.value, lu.value
         self.omega = None  # angular frequency of the values (afx_assemble / afx_set_omega)
         self.tuned_widths = set()  # column counts afx_autotune has chosen a block width for
+        _open_systems.add(self)
 
     @classmethod
     def from_n6line(cls, nline: int, n6line: np.ndarray) -> "System":
@@ -237,10 +259,20 @@ class System:
             self.h = None
 
     def __del__(self):
+        # during interpreter shutdown the HIP runtime (torch's) may already be gone: leave the
+        # device memory to the process exit instead of calling into it
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
             pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def pattern(self):
         INU = np.zeros(self.n + 1, np.int32)

@@ -701,7 +701,7 @@ struct Work {
     if ((rc = dalloc(&norms, 2 * (size_t)ks)) || (rc = dalloc(&conv, ks)) || (rc = dalloc(&iters, ks))) return rc;
     if ((rc = dalloc(&status, 2)) || (rc = dalloc(&col2f, k)) || (rc = dalloc(&wcol, k))) return rc;
     if ((rc = dalloc(&run, G))) return rc;
-    if (hipHostMalloc((void **)&h_status, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void **)&h_status, 3 * sizeof(int), hipHostMallocDefault) != hipSuccess)
       return fail(AFX_ERR_HIP, "hipHostMalloc");
     return AFX_OK;
   }
@@ -818,7 +818,16 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(w.col2f, col2f.data(), k * sizeof(int), hipMemcpyHostToDevice, s->stream));
   HIPCHK(hipMemcpyAsync(w.wcol, wcol.data(), k * sizeof(double), hipMemcpyHostToDevice, s->stream));
-  HIPCHK(hipMemsetAsync(s->d_err, 0, sizeof(int), s->stream));
+  // the error flag of the bounded device waits starts clear; AFX_TEST_RAISE_ERR raises it before
+  // the first launch instead (a test of the abort path: every block then leaves at its first grid
+  // barrier and the solve returns AFX_ERR_HIP)
+  {
+    static const int raised = 4;
+    if (getenv("AFX_TEST_RAISE_ERR"))
+      HIPCHK(hipMemcpyAsync(s->d_err, &raised, sizeof(int), hipMemcpyHostToDevice, s->stream));
+    else
+      HIPCHK(hipMemsetAsync(s->d_err, 0, sizeof(int), s->stream));
+  }
   // initial residual r0 = b - A x and ||b||^2 (COCG13_single.f90:139-169)
   afx::SpmvParams sp{};
   sp.n = s->n;
@@ -947,6 +956,7 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
     }
     HIPCHK(hipEventRecord(ev1, s->stream));
     HIPCHK(hipMemcpyAsync(w.h_status, w.status, 2 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(w.h_status + 2, s->d_err, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
@@ -960,7 +970,8 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
               fp64 ? "fp64" : "fp32-pc", k, G, lpr, p.lds_rows, p.ring_w, w.h_status[0], w.h_status[1], dt,
               w.h_status[0] ? 1e6 * dt / w.h_status[0] : 0.0);
     }
-    if (w.h_status[1] || w.h_status[0] >= maxit) break;
+    // a wait that gave up (or a raised flag) ends the solve: results are not trusted
+    if (w.h_status[2] || w.h_status[1] || w.h_status[0] >= maxit) break;
   }
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
@@ -2172,9 +2183,15 @@ int afx_autotune(afx_system s, int nrhs, int verbose) {
 
 // The drop-in keeps the system of its last call: a caller that solves one frequency after
 // another on the same mesh (forward_bxyz's solveCG) repeats the pattern analysis only when the
-// index arrays change. Calls are serialised, as the reference's SAVEd routine requires.
+// index arrays change. Like the reference's SAVEd DD, SCALE and IFLAG (COCG13_single.f90:39-44,
+// 57-62), the kept system's preconditioner and the scaling vector survive between calls: NSET = 0
+// reuses both, NSET = 1 reuses the scaling of the last NSET = 2 call. Calls are serialised, as the
+// reference's SAVEd routine requires.
 static std::mutex g_drop_mu;
 static afx_system g_drop_sys = nullptr;
+static bool g_drop_iflag = false;       // IFLAG: a call with NSET >= 1 has run (DD and SCALE exist)
+static std::vector<double> g_drop_scale;  // SCALE (NP entries, 1 until an NSET = 2 call)
+static int g_drop_pc = -1;              // PRECOND of the setup that DD holds
 
 static bool same_pattern(afx_system s, int n, const int *INU, const int *IAU, const int *INL, const int *IAL) {
   if (!s || s->n != n || s->nU != INU[n] || s->nL != INL[n]) return false;
@@ -2188,20 +2205,41 @@ int afx_cocg13_single_release(void) {
   std::lock_guard<std::mutex> lk(g_drop_mu);
   int rc = g_drop_sys ? afx_destroy(g_drop_sys) : AFX_OK;
   g_drop_sys = nullptr;
+  g_drop_iflag = false;
+  g_drop_scale.clear();
+  g_drop_pc = -1;
   return rc;
 }
 
 int afx_cocg13_single(int NP, int NPL, int NPU, double *D, double *AL, const int *INL, const int *IAL, double *AU,
                       const int *INU, const int *IAU, double *B, double *X, const char *PRECOND, double SIGMA_DIAG,
                       double SIGMA, double *RESID, int *ITER, int *ERROR, int NSET) {
-  if (!D || !B || !X || !RESID || !ITER || !ERROR) return fail(AFX_ERR_ARG, "null argument");
+  if (!D || !B || !X || !RESID || !ITER || !ERROR || !INU || !INL) return fail(AFX_ERR_ARG, "null argument");
+  if (NP <= 0 || NSET < 0 || NSET > 2) return fail(AFX_ERR_ARG, "bad NP or NSET");
   *ERROR = 0;
-  if (NSET == 0) {  // no SAVEd preconditioner to reuse (COCG13_single.f90:49-52)
-    *ERROR = AFX_ERR_BADOPT;
-    return AFX_ERR_BADOPT;
-  }
   if (INU[NP] != NPU || INL[NP] != NPL) return fail(AFX_ERR_ARG, "NPU/NPL do not match INU/INL");
-  std::vector<double> scale(NP, 1.0);
+  std::lock_guard<std::mutex> lk(g_drop_mu);
+  // the reference allocates DD and SCALE once, at its first call, and cannot change NP after it;
+  // a different NP here starts the saved state afresh
+  if (g_drop_iflag && (int)g_drop_scale.size() != NP) {
+    g_drop_iflag = false;
+    g_drop_pc = -1;
+  }
+  const int pc = pc_from_string(PRECOND);
+  if (NSET == 0) {
+    // COCG13_single.f90:49-52: nothing saved yet. Here the kept preconditioner must also belong to
+    // the same pattern and preconditioner kind (the reference would apply its DD to any matrix)
+    if (!g_drop_iflag || g_drop_pc != pc || !same_pattern(g_drop_sys, NP, INU, IAU, INL, IAL) ||
+        !g_drop_sys->pc_valid) {
+      *ERROR = AFX_ERR_BADOPT;
+      return AFX_ERR_BADOPT;
+    }
+  }
+  if (!g_drop_iflag) {  // :57-62
+    g_drop_scale.assign(NP, 1.0);
+    g_drop_iflag = true;
+  }
+  std::vector<double> &scale = g_drop_scale;
   if (NSET == 2) {  // COCG13_single.f90:77-97, in place on the caller's arrays
     for (int i = 0; i < NP; i++) scale[i] = 1.0 / std::sqrt(std::hypot(D[2 * i], D[2 * i + 1]));
     for (int j = 0; j < NP; j++) {
@@ -2221,38 +2259,63 @@ int afx_cocg13_single(int NP, int NPL, int NPU, double *D, double *AL, const int
       }
     }
   }
-  for (int i = 0; i < NP; i++) {
-    B[2 * i] *= scale[i];
-    B[2 * i + 1] *= scale[i];
-  }
-  std::lock_guard<std::mutex> lk(g_drop_mu);
+  // :98-100 B scaled only when NSET >= 1 (the saved SCALE, 1 unless an NSET = 2 call set it)
+  const bool scaled_b = NSET >= 1;
+  if (scaled_b)
+    for (int i = 0; i < NP; i++) {
+      B[2 * i] *= scale[i];
+      B[2 * i + 1] *= scale[i];
+    }
+  auto unscale_b = [&]() {
+    if (scaled_b)
+      for (int i = 0; i < NP; i++) {
+        B[2 * i] /= scale[i];
+        B[2 * i + 1] /= scale[i];
+      }
+  };
   bool fresh = false;
   if (!same_pattern(g_drop_sys, NP, INU, IAU, INL, IAL)) {
     if (g_drop_sys) afx_destroy(g_drop_sys);
     g_drop_sys = nullptr;
+    g_drop_pc = -1;
     int rc0 = afx_create(NP, INU, IAU, INL, IAL, &g_drop_sys);
     if (rc0) {
       g_drop_sys = nullptr;
+      unscale_b();
       return rc0;
     }
     fresh = true;
   }
   afx_system s = g_drop_sys;
-  int rc = afx_set_values(s, D, AU, AL);
-  if (rc == AFX_OK && fresh) rc = afx_autotune(s, 1, 0);  // lane width for this pattern, once
+  int rc = AFX_OK;
+  if (NSET == 0) {
+    // the current values with the saved DD: the sweeps' DD (dpos) and the preconditioner state
+    // are left as they are, only the matrix values are replaced
+    rc = afx_set_values(s, D, AU, AL);
+    if (rc == AFX_OK) rc = ensure_derived(s);
+    if (rc == AFX_OK) s->pc_valid = true;
+  } else {
+    rc = afx_set_values(s, D, AU, AL);
+    if (rc == AFX_OK && fresh) rc = afx_autotune(s, 1, 0);  // lane width for this pattern, once
+  }
   afx_opts o;
   afx_default_opts(&o);
-  o.precond = pc_from_string(PRECOND);
+  o.precond = pc;
   o.sigma_diag = SIGMA_DIAG;
   o.sigma = SIGMA;
   o.tol = *RESID;
   o.maxit = *ITER;
-  o.nset = 1;
+  o.nset = NSET == 0 ? 0 : 1;
   o.check_true = 0;
   afx_stats st{};
   if (rc == AFX_OK) rc = afx_solve(s, 1, B, NP, X, NP, &o, &st);
-  if (rc != AFX_OK && rc != AFX_ERR_MAXIT && rc != AFX_ERR_BREAKDOWN) return rc;
-  for (int i = 0; i < NP; i++) {  // :326-329
+  if (rc != AFX_OK && rc != AFX_ERR_MAXIT && rc != AFX_ERR_BREAKDOWN) {
+    unscale_b();
+    if (rc == AFX_ERR_BADOPT) *ERROR = AFX_ERR_BADOPT;
+    return rc;
+  }
+  if (NSET >= 1) g_drop_pc = pc;
+  for (int i = 0; i < NP; i++) {  // :326-329, also for NSET = 0 (B divided by the saved SCALE)
     X[2 * i] *= scale[i];
     X[2 * i + 1] *= scale[i];
     B[2 * i] /= scale[i];

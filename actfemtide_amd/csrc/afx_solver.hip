@@ -235,12 +235,12 @@ __device__ V block_cols(V v, V *lds, int kp) {
 // range [g0, g1) (all blocks, or the blocks of one stacked system; !dist only).
 template <typename V>
 __device__ V grid_cols(V v, V *part, V *fin, V *lds, int k, int kp, int c, bool dist, const GridBar &bar,
-                       unsigned &phase, int g0, int g1) {
+                       unsigned &phase, int g0, int g1, bool &abort, bool poll = false) {
   const int t = threadIdx.x;
   const int G = gridDim.x;
   V bs = block_cols(v, lds, kp);
   if (t < kp && t < k) part[(size_t)blockIdx.x * k + t] = bs;
-  grid_barrier(bar, phase);
+  abort |= grid_barrier(bar, phase, poll && !dist);
   if (!dist) {
     V s = zero_of(v);
     if (c < k)
@@ -253,7 +253,7 @@ __device__ V grid_cols(V v, V *part, V *fin, V *lds, int k, int kp, int c, bool 
     s = block_cols(s, lds, 1);
     if (t == 0) fin[cc] = s;
   }
-  grid_barrier(bar, phase);
+  abort |= grid_barrier(bar, phase, poll);
   return (c < k) ? fin[c] : zero_of(v);
 }
 
@@ -749,7 +749,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
   // backward sweeps (the backward sweep reaches them last), the upper half after the backward
   // sweep (while they wait for rho): the time each block waits anyway
   const bool clear_after_fw = blockIdx.x < gridDim.x / 2;
-  while (!alldone && it < it_end) {
+  // set when a grid barrier found the error flag raised: every block leaves after the iteration
+  // (the flag never clears during a launch, so every block sees it by its next barrier)
+  bool abort = false;
+  while (!alldone && it < it_end && !abort) {
     ++it;
     const unsigned long long ci0 = prof ? wall_clock64() : 0;
     // mailboxes of this iteration, and the other set, which no block touches in this iteration
@@ -853,7 +856,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
       if (prof) tclr += wall_clock64() - cc0;
     }
     const unsigned long long cr0 = prof ? wall_clock64() : 0;
-    const cx<double> rho = grid_cols(rho_loc, p.part_rho, p.fin_rho, lds_c, k, kp, c, dist, p.bar, phase, g0, g1);
+    const cx<double> rho = grid_cols(rho_loc, p.part_rho, p.fin_rho, lds_c, k, kp, c, dist, p.bar, phase, g0, g1, abort);
     const unsigned long long cr1 = prof ? wall_clock64() : 0;
     if (prof) trho += cr1 - cr0;
 
@@ -885,7 +888,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         }
       }
     }
-    grid_barrier(p.bar, phase);
+    abort |= grid_barrier(p.bar, phase);
     const unsigned long long cp1 = prof ? wall_clock64() : 0;
     if (prof) tpu += cp1 - cr1;
 
@@ -987,7 +990,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         round(row0 + rstep, v1, v0, b0, b1);
       }
     }
-    const cx<double> c1 = grid_cols(c1_loc, p.part_c1, p.fin_c1, lds_c, k, kp, c, dist, p.bar, phase, g0, g1);
+    const cx<double> c1 = grid_cols(c1_loc, p.part_c1, p.fin_c1, lds_c, k, kp, c, dist, p.bar, phase, g0, g1, abort);
     if (prof) tsp += wall_clock64() - cp1;
     const cx<double> alpha = cdivd(rho, c1);
     const bool run = (st == 0);
@@ -1034,7 +1037,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         }
       }
     }
-    const double rr = grid_cols(rr_loc, p.part_rr, p.fin_rr, lds_d, k, kp, c, dist, p.bar, phase, g0, g1);
+    const double rr = grid_cols(rr_loc, p.part_rr, p.fin_rr, lds_d, k, kp, c, dist, p.bar, phase, g0, g1, abort, true);
     if (upd) {
       res = sqrt(rr / bn);  // :315 RESID = sqrt(DNRM2/BNRM2)
       if (res <= p.tol) {
@@ -1050,7 +1053,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
       // every system's state reaches every block through one more grid barrier (all blocks run
       // the same iterations, a finished system's x and r no longer change)
       if (t == 0) p.run[pb] = (st == 0) ? 1 : 0;
-      grid_barrier(p.bar, phase);
+      abort |= grid_barrier(p.bar, phase);
       int any = 0;
       for (int g = t; g < (int)gridDim.x; g += 256) any |= p.run[g];
       alldone = __syncthreads_or(any) == 0;

@@ -37,8 +37,14 @@ struct GridBar {
 __device__ __forceinline__ void wave_drain() { __builtin_amdgcn_s_waitcnt(0); }
 
 // All threads of all blocks must call it the same number of times. `phase` is the block's
-// view of the generation word (kept by thread 0).
-__device__ __forceinline__ void grid_barrier(const GridBar &b, unsigned &phase) {
+// view of the generation word (kept by thread 0). Returns true in every thread of the block when
+// err[0] was found set (by any block's wait that gave up, or by the host before the launch): the
+// caller then leaves its loop, and the waits of blocks still in barriers give up within 1024
+// polls instead of AFX_SPIN_LIMIT, so a launch whose blocks disagree ends promptly. The flag is
+// read after the wait when `poll` is set (once per solver iteration) or when the wait was long;
+// a fast barrier without `poll` does not pay for the extra load.
+__device__ __forceinline__ bool grid_barrier(const GridBar &b, unsigned &phase, bool poll = false) {
+  __shared__ int s_abort;
   wave_drain();
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -56,18 +62,25 @@ __device__ __forceinline__ void grid_barrier(const GridBar &b, unsigned &phase) 
       if (top + 1u == target * NG) __hip_atomic_store(b.gen, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
     unsigned spins = 0;
+    bool long_wait = false;
     while ((int)(__hip_atomic_load(b.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
       __builtin_amdgcn_s_sleep(1);
       if (++spins > AFX_SPIN_LIMIT) {
         atomicOr(b.err, 1);
         break;
       }
+      if ((spins & 1023u) == 0) {
+        long_wait = true;
+        if (__hip_atomic_load(b.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+      }
     }
     // acquire: drop stale L2 lines before the block reads others' data
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     phase = target;
+    s_abort = (poll || long_wait) ? (__hip_atomic_load(b.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) : 0;
   }
   __syncthreads();
+  return s_abort != 0;
 }
 
 }  // namespace afx

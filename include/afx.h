@@ -11,7 +11,8 @@
  *   - return codes mirror COCG13_single (0 ok, -100 max iterations, 101 bad options) and
  *     add AFX_ERR_* for conditions where the reference calls `stop`;
  *   - one afx_system per host thread; calls on distinct systems are independent.
- *     There is no hidden global state.
+ *     The only global state is the drop-in afx_cocg13_single's saved system and scaling,
+ *     the counterpart of the reference routine's SAVEd DD and SCALE.
  */
 #ifndef AFX_H
 #define AFX_H
@@ -40,7 +41,10 @@ extern "C" {
 
 /* arithmetic of the Krylov loop */
 #define AFX_FP64 0  /* all-FP64 COCG */
-#define AFX_MIXED 1 /* complex-FP32 SpMV / sweeps inside FP64 iterative refinement */
+#define AFX_MIXED 1 /* complex-FP32 preconditioner (DD, sweep values and z in FP32); the SpMV,
+                       the recurrence and its scalars stay FP64. With check_true, the FP64 true
+                       residual is recomputed after each COCG run and the run restarts from the
+                       current iterate while it exceeds tol, at most max_outer times */
 
 typedef struct afx_system_s *afx_system;
 
@@ -52,8 +56,9 @@ typedef struct {
   int maxit;         /* ITER in */
   int nset;          /* NSET: 1 preconditioned, 2 scaled + preconditioned, 0 reuse setup */
   int precision;     /* AFX_FP64 or AFX_MIXED */
-  double inner_tol;  /* AFX_MIXED: relative tolerance of each FP32 inner solve */
-  int max_outer;     /* AFX_MIXED: maximum refinement steps */
+  double inner_tol;  /* unused (kept for the layout of the struct) */
+  int max_outer;     /* with check_true: most COCG runs restarted from the iterate while the true
+                        residual exceeds tol */
   int chunk;         /* iterations per persistent launch (0: default) */
   int check_true;    /* 1: recompute the FP64 true residual at the end */
 } afx_opts;
@@ -203,10 +208,16 @@ int afx_get_precond_diag(afx_system s, double *DD);
 
 /* Same arguments and meaning as COCG13_single (src/solver/COCG13_single.f90:9-36), by
    pointer for the Fortran caller (scalars by value where intent(in)): D, AU, AL modified for
-   NSET = 2; B scaled and restored; RESID, ITER in/out; ERROR out. Unlike the reference it
-   keeps no SAVEd preconditioner, so NSET = 0 always returns ERROR = 101. PRECOND is 20
-   characters. The device system of the last call is kept (pattern analysis, device arrays) and
-   reused while INU/IAU/INL/IAL are the same; calls are serialised. */
+   NSET = 2; B scaled and restored; RESID, ITER in/out; ERROR out. PRECOND is 20 characters.
+   Saved state, as the reference's SAVEd DD / SCALE / IFLAG (:39-44,57-62): the device system of
+   the last call (pattern analysis, device arrays, preconditioner) is kept and reused while
+   INU/IAU/INL/IAL are the same, and so is SCALE (all ones until an NSET = 2 call sets it; NSET = 1
+   then scales B by it, as the reference does). NSET = 0 solves with the current D/AU/AL and the
+   preconditioner diagonal of the last NSET >= 1 call, B unscaled and divided by SCALE on exit
+   (:98-100,326-329); it returns ERROR = 101 when there is no such call, and also when the kept
+   system has another pattern or the last setup another PRECOND (where the reference would apply
+   a DD that belongs to another matrix). A different NP starts the saved state afresh (the
+   reference cannot change NP). On an AFX_ERR_* return B is restored. Calls are serialised. */
 int afx_cocg13_single(int NP, int NPL, int NPU, double *D, double *AL, const int *INL, const int *IAL,
                       double *AU, const int *INU, const int *IAU, double *B, double *X, const char *PRECOND,
                       double SIGMA_DIAG, double SIGMA, double *RESID, int *ITER, int *ERROR, int NSET);

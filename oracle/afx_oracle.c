@@ -487,23 +487,28 @@ static int precond_kind(const char *p) {
  *   D, AU, AL modified in place for NSET = 2 (scaling); B scaled and restored; X in/out;
  *   *resid: tolerance in, achieved recurrence residual out; *iter: max in, count out
  *   (on exhaustion the reference's DO variable ends at MAXIT + 1); *error: 0 / -100 / 101.
- *   DD_state: caller-owned length-n array that plays the role of the SAVEd DD; *have_state
- *   plays IFLAG. resid_hist (optional, length >= maxit) receives ||r||/||b|| per iteration.
+ *   DD_state and scale_state: caller-owned length-n arrays that play the role of the SAVEd DD and
+ *   SCALE (:39-40); *have_state plays IFLAG (:44,49-62): on the first call SCALE is set to 1, an
+ *   NSET = 2 call overwrites it, NSET = 1 scales B by it, NSET = 0 leaves B and DD alone and still
+ *   divides B by it at the end (:98-100,326-329). resid_hist (optional, length >= maxit) receives
+ *   ||r||/||b|| per iteration.
  */
 void orc_cocg13(int np, int npl, int npu, cplx *D, cplx *AL, const int *INL, const int *IAL,
                 cplx *AU, const int *INU, const int *IAU, cplx *B, cplx *X, const char *precond,
                 double sigma_diag, double sigma, double *resid, int *iter, int *error, int nset,
-                cplx *DD_state, int *have_state, double *resid_hist) {
+                cplx *DD_state, double *scale_state, int *have_state, double *resid_hist) {
   (void)npl; (void)npu;
   int n = np;
   *error = 0;
   if (!*have_state && nset == 0) { *error = 101; return; }
-  double *scale = (double *)malloc((size_t)n * sizeof(double));
+  double *scale = scale_state;
   cplx *R = (cplx *)malloc((size_t)n * sizeof(cplx));
   cplx *Zq = (cplx *)malloc((size_t)n * sizeof(cplx));
   cplx *P = (cplx *)malloc((size_t)n * sizeof(cplx));
-  for (int i = 0; i < n; i++) scale[i] = 1.0;
-  if (!*have_state) *have_state = 1;
+  if (!*have_state) {
+    for (int i = 0; i < n; i++) scale[i] = 1.0;
+    *have_state = 1;
+  }
   int maxit = *iter;
   double tol = *resid;
   int kind = precond_kind(precond);
@@ -539,7 +544,7 @@ void orc_cocg13(int np, int npl, int npu, cplx *D, cplx *AL, const int *INL, con
     for (int i = 0; i < n; i++) { X[i].re = 0; X[i].im = 0; }
     *resid = 0.0;
     *iter = 0;
-    free(scale); free(R); free(Zq); free(P);
+    free(R); free(Zq); free(P);
     return;
   }
   cplx rho = {0, 0}, rho1 = {0, 0};
@@ -585,7 +590,7 @@ void orc_cocg13(int np, int npl, int npu, cplx *D, cplx *AL, const int *INL, con
     B[i].re = B[i].re / scale[i];  /* B(i) = B(i) / SCALE(i) */
     B[i].im = B[i].im / scale[i];
   }
-  free(scale); free(R); free(Zq); free(P);
+  free(R); free(Zq); free(P);
 }
 
 /* ---------------- observation operator ---------------- */

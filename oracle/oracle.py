@@ -155,17 +155,29 @@ def precond_apply(kind, INU, IAU, AU, INL, IAL, AL, DD, r):
     return z
 
 
+class Cocg13State:
+    """The SAVEd DD and SCALE and the IFLAG of COCG13_single (COCG13_single.f90:39-44), kept
+    between calls of cocg13(..., state=...) as the reference keeps them between its calls."""
+
+    def __init__(self, n):
+        self.DD = np.zeros(n, np.complex128)
+        self.scale = np.ones(n, np.float64)
+        self.have = C.c_int(0)
+
+
 def cocg13(INU, IAU, INL, IAL, D, AU, AL, B, X=None, precond="IC", resid=1e-8, maxit=3000, nset=1,
-           sigma_diag=1.0, sigma=0.0, history=False):
+           sigma_diag=1.0, sigma=0.0, history=False, state=None):
     """COCG13_single (COCG13_single.f90:9-341) on copies of the inputs.
 
     Returns dict(x, resid, iter, error, hist[, D, AU, AL, B]) -- the modified D/AU/AL/B are
-    returned too because the reference modifies them in place for NSET = 2."""
+    returned too because the reference modifies them in place for NSET = 2. Without `state` each
+    call is the routine's first call in a fresh program; with a Cocg13State the saved state
+    carries over, as between successive calls of the reference."""
     n = len(D)
     D, AU, AL, B = _c128(D).copy(), _c128(AU).copy(), _c128(AL).copy(), _c128(B).copy()
     X = np.zeros(n, np.complex128) if X is None else _c128(X).copy()
-    DD = np.zeros(n, np.complex128)
-    have = C.c_int(0)
+    st = Cocg13State(n) if state is None else state
+    DD, scale, have = st.DD, st.scale, st.have
     r = C.c_double(resid)
     it = C.c_int(maxit)
     err = C.c_int(0)
@@ -174,7 +186,7 @@ def cocg13(INU, IAU, INL, IAL, D, AU, AL, B, X=None, precond="IC", resid=1e-8, m
     lib().orc_cocg13(C.c_int(n), C.c_int(int(INL[-1])), C.c_int(int(INU[-1])), _p(D), _p(AL), _p(_i32(INL)),
                      _p(_i32(IAL)), _p(AU), _p(_i32(INU)), _p(_i32(IAU)), _p(B), _p(X), pc,
                      C.c_double(sigma_diag), C.c_double(sigma), C.byref(r), C.byref(it), C.byref(err),
-                     C.c_int(nset), _p(DD), C.byref(have), _p(hist))
+                     C.c_int(nset), _p(DD), _p(scale), C.byref(have), _p(hist))
     out = dict(x=X, resid=r.value, iter=it.value, error=err.value, D=D, AU=AU, AL=AL, B=B)
     if history:
         out["hist"] = hist[: max(0, min(it.value, maxit))].copy()

@@ -25,8 +25,11 @@ type(global_matrix) :: A
 character(70)       :: meshfile, vfile, ffile, condfile
 character(50)       :: linefile
 character(20)       :: precond
-real(8)             :: omega, zmin, zmax, xout, yout, resid, sd, sg, sigma_air, v
+real(8)             :: omega, zmin, zmax, xout, yout, resid, sd, sg, sigma_air, v, g_tol
 integer(4)          :: nodes, maxit, nset, nobs, i, j, ierr, iter, nline, ntet, iele, dorun
+integer(4)          :: nseq, kseq, ios
+real(8)             :: bfac
+character(8)        :: tag
 integer(4), allocatable :: table_dof(:,:)
 complex(8), allocatable :: b_vec(:,:), bs(:,:), Avalue_bc(:), X(:), B(:), vxyz(:,:)
 real(8),    allocatable :: fxyz(:,:)
@@ -47,6 +50,7 @@ read(*,*) zmin, zmax, xout, yout
 read(*,*) sigma_air
 read(*,'(a)') precond
 read(*,*) resid, maxit, nset, sd, sg, dorun
+g_tol = resid
 read(*,*) nobs
 
 !# inputs through the reference readers
@@ -145,6 +149,29 @@ call MKLINE_V2(l2, g_mesh%node, g_mesh%ntet, 4, g_mesh%n4)
 call MKN6(l2, g_mesh%node, g_mesh%ntet, g_mesh%n4)
 call dumpi('mk_line', l2%line, 2*l2%nline)
 call dumpi('mk_n6line', l2%n6line, 6*g_mesh%ntet)
+
+!# optional sequence of further COCG13_single calls on the same arrays (its SAVEd DD, SCALE and
+!# IFLAG carry over, and D/AU/AL keep what earlier calls did to them): per call one line
+!# "nset precond bfac", B = bfac * b, X = 0, the driver's tolerance and iteration cap
+nseq = 0
+read(*,*,iostat=ios) nseq
+if (ios .ne. 0) nseq = 0
+if (nseq .gt. 0 .and. .not. allocated(X)) allocate(X(nline), B(nline))
+do kseq = 1, nseq
+  read(*,*) nset, precond, bfac
+  X(:) = (0.d0, 0.d0)
+  B(:) = b_vec(:,1) * bfac
+  ierr = 0
+  iter = maxit
+  resid = g_tol
+  call COCG13_single(nline, A%ial_tot, A%iau_tot, A%D, A%AL, A%INL, A%IAL, A%AU, A%INU, A%IAU, &
+                   & B, X, precond, sd, sg, resid, iter, ierr, nset)
+  write(tag,'(i0)') kseq
+  call dumpc('seq_x'//trim(tag), X, nline)
+  call dumpc('seq_b'//trim(tag), B, nline)
+  call dumpr('seq_resid'//trim(tag), (/resid/), 1)
+  call dumpi('seq_iter'//trim(tag), (/iter, ierr/), 2)
+end do
 
 contains
   subroutine dumpi(name, a, n)

@@ -32,6 +32,11 @@ CASES = [
     ("tiny_diag_nset2", 4, 4, 6, 11, "DIAG", 1e-8, 20000, 2, 0, False),
     ("small_ssor", 8, 8, 6, 1001, "SSOR", 1e-8, 20000, 1, 8, False),
     ("cfg1_ssor", 16, 16, 12, 1001, "SSOR", 1e-8, 20000, 1, 16, False),
+    # successive calls on the same arrays: the SAVEd DD / SCALE / IFLAG of COCG13_single.f90:39-62
+    # (NSET = 0 reuses DD; NSET = 2 scales D/AU/AL in place and saves SCALE; a later NSET = 1 call
+    # scales B by the saved SCALE; NSET = 0 divides B by it on exit)
+    ("tiny_nset_seq", 4, 4, 6, 11, "SSOR", 1e-8, 20000, 1, 0, False,
+     [(0, "SSOR", 0.5), (2, "SSOR", 1.0), (1, "SSOR", 1.0), (0, "SSOR", 2.0)]),
 ]
 
 
@@ -52,7 +57,7 @@ def sites_for(case, n, seed):
     return np.array(pts)
 
 
-def run_case(name, nx, ny, nz, seed, precond, tol, maxit, nset, n_sites, full):
+def run_case(name, nx, ny, nz, seed, precond, tol, maxit, nset, n_sites, full, seq=()):
     case = mesh.tide_case(nx, ny, nz, seed=seed)
     sites = sites_for(case, n_sites, seed + 17)
     with tempfile.TemporaryDirectory() as d:
@@ -62,7 +67,8 @@ def run_case(name, nx, ny, nz, seed, precond, tol, maxit, nset, n_sites, full):
             "em3d.msh", "lineinfo.dat", "cond.msh", "vxyz_mesh", "fxyz_mesh",
             str(case.nodes_ocean), repr(float(OMEGA_M2)), f"{float(zmin)!r} {float(zmax)!r} {float(xout)!r} {float(yout)!r}",
             repr(float(case.sigma_air)), precond, f"{float(tol)!r} {maxit} {nset} 1.0 0.0 1", str(len(sites)),
-        ] + [f"{float(p[0])!r} {float(p[1])!r} {float(p[2])!r}" for p in sites]) + "\n"
+        ] + [f"{float(p[0])!r} {float(p[1])!r} {float(p[2])!r}" for p in sites]
+            + [str(len(seq))] + [f"{k} {pc} {float(f)!r}" for k, pc, f in seq]) + "\n"
         r = subprocess.run([HARNESS], input=drv, text=True, capture_output=True, cwd=d)
         if r.returncode != 0:
             raise RuntimeError(f"{name}: harness failed\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
@@ -96,6 +102,15 @@ def run_case(name, nx, ny, nz, seed, precond, tol, maxit, nset, n_sites, full):
                 out[nm + "_idx"] = idx
                 out[nm + "_val"] = a[idx]
                 out[nm + "_norm"] = np.linalg.norm(a)
+        if seq:
+            out["seq_nset"] = np.array([k for k, _, _ in seq], np.int32)
+            out["seq_precond"] = np.array([pc for _, pc, _ in seq])
+            out["seq_bfac"] = np.array([f for _, _, f in seq])
+            for q in range(1, len(seq) + 1):
+                out[f"seq_x{q}"] = rd(f"seq_x{q}", np.complex128)
+                out[f"seq_b{q}"] = rd(f"seq_b{q}", np.complex128)
+                out[f"seq_resid{q}"] = rd(f"seq_resid{q}", np.float64)
+                out[f"seq_iter{q}"] = rd(f"seq_iter{q}", np.int32)
         if n_sites:
             out["obs_iele"] = rd("obs_iele", np.int32)
             out["obs_w"] = rd("obs_w", np.float64).reshape(n_sites, 6, 3)

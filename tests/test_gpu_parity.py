@@ -133,8 +133,10 @@ def test_dropin_cocg13_single_matches_reference(name):
     gg = load(name)
     D, AU, AL, B = (np.array(g[k], np.complex128) for k in ("D", "AU", "AL", "b"))
     X = np.zeros_like(B)
+    afx.cocg13_single_release()  # a first call, as in the reference's fresh program
     resid, it, err = afx.cocg13_single(D, AL, g["INL"], g["IAL"], AU, g["INU"], g["IAU"], B, X,
                                        precond=str(gg["precond"]), resid=1e-8, iter=20000, nset=int(gg["nset"]))
+    afx.cocg13_single_release()
     it_ref = int(gg["iter"][0])
     print(name, "dropin iters", it, "reference", it_ref, flush=True)
     assert err == 0
@@ -222,3 +224,62 @@ def test_solve_cg_mirror(oracle_small):
     A = O.to_scipy(o["INU"], o["IAU"], o["INL"], o["IAL"], o["D"], o["AU"], o["AL"])
     r = np.linalg.norm(o["b"] - A @ x) / np.linalg.norm(o["b"])
     assert 1e-6 < r <= 1.0000000474974513e-4 * 1.01
+
+
+def test_dropin_call_sequence_matches_reference():
+    """Successive drop-in calls on the same arrays, NSET 1, 0, 2, 1, 0, as the reference ran them
+    (tests/golden/tiny_nset_seq.npz): NSET = 0 reuses the kept preconditioner, NSET = 2 scales
+    D/AU/AL in place and saves SCALE, the next NSET = 1 call scales B by it, NSET = 0 divides B by it
+    on exit (COCG13_single.f90:39-62,98-100,326-329). Each call's x against the reference's, its
+    iteration count within 8%, and the returned B."""
+    from actfemtide_amd import afx
+    from tests.test_golden import load
+    g0 = load("tiny_ssor")
+    g = load("tiny_nset_seq")
+    D, AU, AL = (np.array(g0[k], np.complex128) for k in ("D", "AU", "AL"))
+    b = np.array(g0["b"], np.complex128)
+    INL, IAL, INU, IAU = g0["INL"], g0["IAL"], g0["INU"], g0["IAU"]
+    afx.cocg13_single_release()
+    calls = [(int(g["nset"]), str(g["precond"]), 1.0, g["x"], b, g["iter"])]
+    calls += [(int(k), str(pc), float(f), g[f"seq_x{q}"], g[f"seq_b{q}"], g[f"seq_iter{q}"])
+              for q, (k, pc, f) in enumerate(zip(g["seq_nset"], g["seq_precond"], g["seq_bfac"]), start=1)]
+    for q, (nset, pc, f, x_ref, b_ref, it_ref) in enumerate(calls):
+        B = b * f
+        X = np.zeros_like(B)
+        resid, it, err = afx.cocg13_single(D, AL, INL, IAL, AU, INU, IAU, B, X, precond=pc, resid=1e-8,
+                                           iter=20000, nset=nset)
+        dx = np.linalg.norm(X - x_ref) / np.linalg.norm(x_ref)
+        db = np.linalg.norm(B - b_ref) / np.linalg.norm(b_ref)
+        print(f"call {q} nset {nset}: iter {it} (reference {int(it_ref[0])}), x {dx:.1e}, B {db:.1e}", flush=True)
+        assert err == 0 and abs(it - int(it_ref[0])) <= max(3, 0.08 * int(it_ref[0]))
+        assert dx <= 1e-5 and db <= 1e-14
+    # NSET = 0 with no saved setup, and with another preconditioner than the saved one: ERROR = 101
+    afx.cocg13_single_release()
+    B = b.copy()
+    X = np.zeros_like(B)
+    assert afx.cocg13_single(D, AL, INL, IAL, AU, INU, IAU, B, X, precond="SSOR", nset=0)[2] == 101
+    assert np.array_equal(B, b)
+    afx.cocg13_single(D, AL, INL, IAL, AU, INU, IAU, B, X, precond="SSOR", resid=1e-8, iter=20000, nset=1)
+    assert afx.cocg13_single(D, AL, INL, IAL, AU, INU, IAU, b.copy(), np.zeros_like(b), precond="DIAG",
+                             nset=0)[2] == 101
+    afx.cocg13_single_release()
+
+
+def test_raised_error_flag_ends_the_solve_promptly(dev_small, monkeypatch):
+    """The abort path of the persistent kernel's bounded waits: with the device error flag raised
+    by the host before the first launch, every block leaves at its first grid barrier and the
+    solve returns AFX_ERR_HIP at once instead of iterating to maxit; the next solve is normal."""
+    import time
+
+    from actfemtide_amd import afx
+    s, b, _ = dev_small
+    monkeypatch.setenv("AFX_TEST_RAISE_ERR", "1")
+    t0 = time.perf_counter()
+    with pytest.raises(afx.AfxError) as e:
+        s.solve(b, precond="SSOR", tol=1e-30, maxit=100000, chunk=100000)
+    dt = time.perf_counter() - t0
+    print(f"raised flag: {e.value} after {dt:.3f}s", flush=True)
+    assert e.value.code == afx.AFX_ERR_HIP and dt < 10.0
+    monkeypatch.delenv("AFX_TEST_RAISE_ERR")
+    x, st = s.solve(b, precond="SSOR", tol=1e-8, maxit=20000)
+    assert st[0].error == 0 and st[0].true_resid <= 1.05e-8

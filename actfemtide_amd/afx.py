@@ -38,7 +38,10 @@ AFX_ERR_GEOM = -500
 AFX_ERR_PATTERN = -600
 AFX_ERR_OUTSIDE = -700
 
-PC = {"NONE": 0, "IC": 1, "ILU": 1, "SSOR": 2, "DIAG": 3, "SSOR_BJ": 4}
+AFX_PC_GRAD = 0x100
+PC = {"NONE": 0, "IC": 1, "ILU": 1, "SSOR": 2, "DIAG": 3, "SSOR_BJ": 4,
+      # edge preconditioner + the nodal gradient-space correction (needs System.set_nodes)
+      "SSOR+GRAD": 2 | AFX_PC_GRAD, "SSOR_BJ+GRAD": 4 | AFX_PC_GRAD, "DIAG+GRAD": 3 | AFX_PC_GRAD}
 FP64, MIXED = 0, 1
 
 
@@ -78,6 +81,7 @@ SIGNATURES = {
     "afx_set_omega": ([_VP, _D], _I),
     "afx_genbc": ([_VP, _VP, _D, _D, _D, _D, _VP], _I),
     "afx_set_bc": ([_VP, _VP, _I, _VP, _I], _I),
+    "afx_set_nodes": ([_VP, _I, _VP, _VP], _I),
     "afx_precond_setup": ([_VP, _I, _D, _D], _I),
     "afx_solve": ([_VP, _I, _VP, _I, _VP, _I, C.POINTER(afx_opts), _VP], _I),
     "afx_solve_device": ([_VP, _I, _VP, _VP, _VP, C.POINTER(afx_opts), _VP], _I),
@@ -319,6 +323,13 @@ class System:
         bc = np.zeros(self.n, np.uint8)
         _err(lib().afx_genbc(self.h, _p(line), zmin, zmax, xout, yout, _p(bc)))
         return bc.astype(bool)
+
+    def set_nodes(self, node, line, dirichlet):
+        """Edge-to-node incidence for the nodal gradient-space correction (afx_set_nodes): line
+        (nline, 2) 1-based end nodes, dirichlet (nline,) mask."""
+        ln = _i32(line).reshape(-1, 2)
+        d = np.ascontiguousarray(dirichlet, dtype=np.uint8)
+        _err(lib().afx_set_nodes(self.h, int(node), _p(ln), _p(d)))
 
     def set_bc(self, dirichlet, rhs=None):
         d = np.ascontiguousarray(dirichlet, dtype=np.uint8)
@@ -625,6 +636,7 @@ def forward_system(case, omega: float, sigma_tet=None) -> tuple:
     xout, yout, zmin, zmax = case.bounds
     bc = s.genbc(case.line, zmin, zmax, xout, yout)
     b = s.set_bc(bc, b)
+    s.set_nodes(case.node, case.line, bc)
     return s, b, bc
 
 

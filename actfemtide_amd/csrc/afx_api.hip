@@ -154,6 +154,11 @@ struct afx_system_s {
   long long prof_launches = 0, prof_iters = 0;
   double prof_ms = 0.0;
   int ncu = 0;
+  // nodal gradient-space correction (afx_set_nodes, AFX_PC_GRAD): interior nodes, incidence,
+  // end nodes of each edge, and diag(G^T M G) of the current values
+  int nn = -1;
+  int *d_nptr = nullptr, *d_nadj = nullptr, *d_en = nullptr;
+  double *d_mnd = nullptr;
 };
 
 namespace {
@@ -321,10 +326,18 @@ int ensure_derived(afx_system s) {
 }
 
 // Preconditioner diagonal for the distinct column frequencies `wf`.
-int ensure_precond(afx_system s, int kind, double sd, double sig, const std::vector<double> &wf) {
-  if (s->pc_valid && s->pc_kind == kind && s->pc_sd == sd && s->pc_sig == sig && s->pc_w == wf) return AFX_OK;
+int ensure_precond(afx_system s, int kind_full, double sd, double sig, const std::vector<double> &wf) {
+  if (s->pc_valid && s->pc_kind == kind_full && s->pc_sd == sd && s->pc_sig == sig && s->pc_w == wf) return AFX_OK;
+  const bool grad = (kind_full & AFX_PC_GRAD) != 0;
+  const int kind = kind_full & 0xff;
+  if (grad && (kind != AFX_PC_SSOR && kind != AFX_PC_SSOR_BJ && kind != AFX_PC_DIAG))
+    return fail(AFX_ERR_BADOPT, "AFX_PC_GRAD goes with SSOR, SSOR_BJ or DIAG");
+  if (grad && s->nn < 0) return fail(AFX_ERR_ARG, "AFX_PC_GRAD needs afx_set_nodes");
   int rc = ensure_derived(s);
   if (rc) return rc;
+  if (grad)
+    HIPCHK(afx::launch_nodal_diag(s->nn, s->d_nptr, s->d_nadj, s->d_en, s->d_rowptr, s->d_col, s->d_val64, s->d_mnd,
+                                  s->stream));
   const int nf = (int)wf.size();
   dfree(s->d_DD);
   dfree(s->d_DD32);
@@ -354,7 +367,7 @@ int ensure_precond(afx_system s, int kind, double sd, double sig, const std::vec
     g_err = std::string("preconditioner setup: ") + hipGetErrorString(e != hipSuccess ? e : e2);
     return AFX_ERR_HIP;
   }
-  s->pc_kind = kind;
+  s->pc_kind = kind_full;
   s->pc_sd = sd;
   s->pc_sig = sig;
   s->pc_w = wf;
@@ -681,8 +694,9 @@ struct Work {
   double *part_a = nullptr, *part_b = nullptr, *wcol = nullptr;
   int *conv = nullptr, *iters = nullptr, *status = nullptr, *col2f = nullptr, *run = nullptr;
   int *h_status = nullptr;
+  cx<double> *Zn = nullptr;
   ~Work() {
-    dfree(R); dfree(Z); dfree(P); dfree(Q); dfree(mbf); dfree(mbb);
+    dfree(R); dfree(Z); dfree(P); dfree(Q); dfree(mbf); dfree(mbb); dfree(Zn);
     dfree(part_rho); dfree(part_c1); dfree(fin_rho); dfree(fin_c1); dfree(rho1);
     dfree(part_rr); dfree(fin_rr); dfree(bnrm2); dfree(resid); dfree(norms); dfree(part_a); dfree(part_b);
     dfree(conv); dfree(iters); dfree(status); dfree(col2f); dfree(wcol); dfree(run);
@@ -786,6 +800,7 @@ int lds_rows_for(afx_system s, bool multi) {
 // needs them two steps back); 0 = not used.
 template <typename TP>
 int ring_w_for(afx_system s, int k, int pc) {
+  pc &= 0xff;
   if (k <= 1 || getenv("AFX_NO_RING")) return 0;
   if (pc != AFX_PC_IC && pc != AFX_PC_SSOR && pc != AFX_PC_SSOR_BJ) return 0;
   const bool bj = (pc == AFX_PC_SSOR_BJ);
@@ -805,6 +820,10 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
            int *host_conv, bool stacked = false) {
   const bool fp64 = std::is_same<TP, double>::value;
   const bool multi = k > 1;
+  const bool grad = (pc & AFX_PC_GRAD) != 0;
+  pc &= 0xff;
+  if (grad && (multi || (stacked && s->nsys > 1) || s->nn < 0))
+    return fail(AFX_ERR_ARG, "AFX_PC_GRAD: one right-hand side of an unstacked system with afx_set_nodes");
   // stacked systems: one column, per-system scalars and convergence; stats and host_conv then
   // hold one entry per system
   const bool sys = stacked && s->nsys > 1;
@@ -874,6 +893,17 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   p.lds_rows = lds_rows_for<TP>(s, multi);
   p.ring_w = ring_w_for<TP>(s, k, pc);
   p.rstart = s->d_rstart;
+  p.grad = grad ? 1 : 0;
+  p.nn = grad ? s->nn : 0;
+  p.nptr = s->d_nptr;
+  p.nadj = s->d_nadj;
+  p.en = s->d_en;
+  p.mnd = s->d_mnd;
+  if (grad) {
+    if ((rc = dalloc(&w.Zn, (size_t)s->nn + 1))) return rc;
+    HIPCHK(hipMemsetAsync(w.Zn, 0, ((size_t)s->nn + 1) * sizeof(cx<double>), s->stream));
+  }
+  p.Zn = w.Zn;
   const bool bj = (pc == AFX_PC_SSOR_BJ);
   p.fw = side_view(bj ? s->bjfw : s->fw, fp64);
   p.bw = side_view(bj ? s->bjbw : s->bw, fp64);
@@ -1058,6 +1088,7 @@ void release(afx_system s) {
   free_side(s->bjbw);
   dfree(s->d_rstart); dfree(s->d_bar); dfree(s->d_err);
   dfree(s->d_sys_off); dfree(s->d_bsys); dfree(s->d_sblk);
+  dfree(s->d_nptr); dfree(s->d_nadj); dfree(s->d_en); dfree(s->d_mnd);
   if (s->stack) afx_destroy(s->stack);
   s->stack = nullptr;
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -1306,9 +1337,11 @@ int solve_device_impl(afx_system s, int k, const cx<double> *dB, cx<double> *dX,
   // the caller reuses a preconditioner set up for exactly these k frequencies (nset = 0)
   const char *bm = getenv("AFX_BATCH");
   const int width = bm ? (std::string(bm) == "multi" ? 1 : std::string(bm) == "stack" ? 2 : 0) : s->batch_multi[k];
-  const bool multi = width == 1;
-  if (k > 1 && o->nset != 0 && width == 2 && !stacked) return solve_stacked_columns(s, k, dB, dX, omega_scale, o, stats);
-  if (k > 1 && o->nset != 0 && !multi) return solve_columns(s, k, dB, dX, omega_scale, o, stats);
+  const bool grad = ((o->nset == 0 ? s->pc_kind : o->precond) & AFX_PC_GRAD) != 0;
+  const bool multi = width == 1 && !grad;
+  if (k > 1 && o->nset != 0 && width == 2 && !stacked && !grad)
+    return solve_stacked_columns(s, k, dB, dX, omega_scale, o, stats);
+  if (k > 1 && (o->nset != 0 || grad) && !multi) return solve_columns(s, k, dB, dX, omega_scale, o, stats);
   std::vector<double> wcol, wf;
   std::vector<int> col2f;
   column_freqs(s, k, omega_scale, wcol, wf, col2f);
@@ -1715,6 +1748,45 @@ int afx_genbc(afx_system s, const int *line, double zmin, double zmax, double xo
   return AFX_OK;
 }
 
+int afx_set_nodes(afx_system s, int node, const int *line, const unsigned char *dirichlet) {
+  if (!s || !line || node <= 0) return fail(AFX_ERR_ARG, "bad arguments");
+  const int n = s->n;
+  // boundary nodes: the end nodes of Dirichlet edges
+  std::vector<unsigned char> bnd(node, 0);
+  for (int e = 0; e < n; e++) {
+    const int a = line[2 * e] - 1, b = line[2 * e + 1] - 1;
+    if (a < 0 || a >= node || b < 0 || b >= node || a == b) return fail(AFX_ERR_ARG, "line: bad node id");
+    if (dirichlet && dirichlet[e]) bnd[a] = bnd[b] = 1;
+  }
+  std::vector<int> id(node, -1);
+  int nn = 0;
+  for (int v = 0; v < node; v++)
+    if (!bnd[v]) id[v] = nn++;
+  std::vector<int> en(2 * (size_t)n), cnt(nn + 1, 0);
+  for (int e = 0; e < n; e++) {
+    const int a = id[line[2 * e] - 1], b = id[line[2 * e + 1] - 1];
+    en[2 * e] = a < 0 ? nn : a;
+    en[2 * e + 1] = b < 0 ? nn : b;
+    if (a >= 0) cnt[a + 1]++;
+    if (b >= 0) cnt[b + 1]++;
+  }
+  for (int a = 0; a < nn; a++) cnt[a + 1] += cnt[a];
+  std::vector<int> adj(cnt[nn]), fill(cnt.begin(), cnt.end() - 1);
+  for (int e = 0; e < n; e++) {  // ascending edge ids per node
+    if (en[2 * e] < nn) adj[fill[en[2 * e]]++] = -(e + 1);
+    if (en[2 * e + 1] < nn) adj[fill[en[2 * e + 1]]++] = e + 1;
+  }
+  dfree(s->d_nptr); dfree(s->d_nadj); dfree(s->d_en); dfree(s->d_mnd);
+  s->nn = -1;
+  int rc;
+  if ((rc = upload(&s->d_nptr, cnt.data(), cnt.size())) || (rc = upload(&s->d_nadj, adj.data(), adj.size())) ||
+      (rc = upload(&s->d_en, en.data(), en.size())) || (rc = dalloc(&s->d_mnd, (size_t)nn + 1)))
+    return rc;
+  s->nn = nn;
+  if (s->pc_kind & AFX_PC_GRAD) s->pc_valid = false;
+  return AFX_OK;
+}
+
 int afx_set_bc(afx_system s, const unsigned char *dirichlet, int nrhs, double *rhs, int ldr) {
   if (!s || !dirichlet) return fail(AFX_ERR_ARG, "null argument");
   unsigned char *d_dir = nullptr;
@@ -1816,12 +1888,18 @@ int afx_precond_apply(afx_system s, const double *r, double *z) {
     dfree(dr); dfree(dz); dfree(dw);
     return rc;
   }
-  hipError_t e = afx::launch_precond_apply_levels(s->n, s->pc_kind, s->nlevL, s->d_lvlL, s->d_rowL, s->d_Lptr,
+  hipError_t e = afx::launch_precond_apply_levels(s->n, s->pc_kind & 0xff, s->nlevL, s->d_lvlL, s->d_rowL, s->d_Lptr,
                                                   s->d_Lcol, s->d_Lval64, s->nlevU, s->d_lvlU, s->d_rowU, s->d_Uptr,
                                                   s->d_Ucol, s->d_Uval64, s->d_DD, dw, dr, dz, s->stream);
+  cx<double> *dzn = nullptr;
+  if (e == hipSuccess && (s->pc_kind & AFX_PC_GRAD)) {
+    if (dalloc(&dzn, (size_t)s->nn + 1) != AFX_OK) e = hipErrorOutOfMemory;
+    if (e == hipSuccess)
+      e = afx::launch_grad_apply(s->n, s->nn, s->d_nptr, s->d_nadj, s->d_en, s->d_mnd, w, dr, dzn, dz, s->stream);
+  }
   if (e == hipSuccess) e = hipMemcpyAsync(z, dz, (size_t)s->n * 16, hipMemcpyDeviceToHost, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-  dfree(dr); dfree(dz); dfree(dw);
+  dfree(dr); dfree(dz); dfree(dw); dfree(dzn);
   if (e != hipSuccess) return fail(AFX_ERR_HIP, hipGetErrorString(e));
   return AFX_OK;
 }
@@ -2136,6 +2214,10 @@ int afx_autotune(afx_system s, int nrhs, int verbose) {
   if (!s || nrhs <= 0 || nrhs > 256) return fail(AFX_ERR_ARG, "bad arguments");
   if (!s->have_values) return fail(AFX_ERR_ARG, "no matrix values");
   const int kind = s->pc_valid ? s->pc_kind : AFX_PC_SSOR;
+  if (nrhs > 1 && (kind & AFX_PC_GRAD)) {  // the corrected preconditioner runs one column at a time
+    s->batch_multi[nrhs] = 0;
+    return AFX_OK;
+  }
   if (nrhs > 1) return autotune_width(s, nrhs, kind, verbose);
   const size_t nk = (size_t)s->n;
   std::vector<cx<double>> hb(nk, cx<double>{1.0, 0.5});

@@ -111,6 +111,15 @@ struct CocgParams {
   int nsys;
   const int *bsys, *sblk;
   int *run;  // [G] the block's system still running after the last iteration
+  // nodal gradient-space correction (AFX_PC_GRAD, single column): nn interior nodes, their
+  // incident edges nadj[nptr[a] .. nptr[a+1]) as +(e+1) when the edge ends at a and -(e+1) when it
+  // starts there, each edge's end nodes en[2e], en[2e+1] (interior index, or nn for a boundary
+  // node), mnd = diag(G^T M G) (> 0), Zn [nn+1] the nodal correction (slot nn stays 0)
+  int grad;
+  int nn;
+  const int *nptr, *nadj, *en;
+  const double *mnd;
+  cx<double> *Zn;
 };
 
 struct SpmvParams {
@@ -151,6 +160,12 @@ hipError_t launch_precond_apply_levels(int n, int kind, int nlevL, const int *lv
                                        const int *lvlU, const int *rowU, const int *Uptr, const int *Ucol,
                                        const cx<double> *Uval, const cx<double> *DD, const double *w,
                                        const cx<double> *r, cx<double> *z, hipStream_t s);
+// nodal gradient-space correction (AFX_PC_GRAD): mnd[a] = sum_{e,f at a} s_ae s_af M_ef over the
+// full CSR (K, M) values; the host-side apply z += G (g / (i w mnd)), g = G^T r
+hipError_t launch_nodal_diag(int nn, const int *nptr, const int *nadj, const int *en, const int *rowptr,
+                             const int *col, const cx<double> *val, double *mnd, hipStream_t s);
+hipError_t launch_grad_apply(int n, int nn, const int *nptr, const int *nadj, const int *en, const double *mnd,
+                             double w, const cx<double> *r, cx<double> *zn, cx<double> *z, hipStream_t s);
 hipError_t launch_gather_vals(int m, const int *map, const cx<double> *canon, cx<double> *out,
                               cx<float> *out32, hipStream_t s);
 hipError_t launch_dpos(int n, const int *row, const cx<double> *DD, cx<double> *d64, cx<float> *d32, hipStream_t s);

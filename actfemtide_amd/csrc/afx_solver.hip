@@ -257,6 +257,33 @@ __device__ V grid_cols(V v, V *part, V *fin, V *lds, int k, int kp, int c, bool 
   return (c < k) ? fin[c] : zero_of(v);
 }
 
+// Nodal gradient-space correction (AFX_PC_GRAD). g = G^T r at interior node a: the incident
+// edges' r with +1 where the edge ends at a and -1 where it starts there; zn = g / (i w mnd_a),
+// i.e. the Jacobi step on the nodal operator G^T A G, whose diagonal is i w diag(G^T M G) because
+// the curl-curl part annihilates gradients.
+__device__ __forceinline__ cx<double> nodal_g(const int *__restrict__ nptr, const int *__restrict__ nadj,
+                                              const cx<double> *R, int a) {
+  const int q0 = nptr[a], q1 = nptr[a + 1];
+  cx<double> g = {0.0, 0.0};
+#pragma unroll 4
+  for (int q = q0; q < q1; q++) {
+    const int v = nadj[q];
+    const cx<double> r = R[(v > 0 ? v : -v) - 1];
+    if (v > 0) {
+      g.r += r.r;
+      g.i += r.i;
+    } else {
+      g.r -= r.r;
+      g.i -= r.i;
+    }
+  }
+  return g;
+}
+__device__ __forceinline__ cx<double> nodal_z(cx<double> g, double w, double mnd) {
+  const double f = 1.0 / (w * mnd);
+  return {g.i * f, -g.r * f};  // g / (i w mnd)
+}
+
 // all columns finished? identical in every block because the states are
 __device__ bool block_all(bool mine) { return __syncthreads_and(mine ? 1 : 0) != 0; }
 
@@ -762,6 +789,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     cx<TP> *const mbf_nx = (it & 1) ? p.mbf : p.mbf2, *const mbb_nx = (it & 1) ? p.mbb : p.mbb2;
     // ---- z = M^-1 r (COCG13_single.f90:186-241) and rho = r^T z (:246-250) ----
     cx<double> rho_loc = {0.0, 0.0};
+    // nodal part of z = M^-1 r (AFX_PC_GRAD): zn on the interior nodes, and its share of
+    // rho = r^T z, r^T G zn = (G^T r)^T zn; the edges add G zn in the p update, after the rho
+    // barrier has published every block's zn
+    if (!MULTI && p.grad) {
+      for (long long a = gtid; a < p.nn; a += nthr) {
+        const cx<double> g = nodal_g(p.nptr, p.nadj, p.R, (int)a);
+        const cx<double> zn = nodal_z(g, w, p.mnd[a]);
+        p.Zn[a] = zn;
+        rho_loc = addv(rho_loc, cmul(g, zn));
+      }
+    }
     if (sweep_pc) {
       // forward then backward sweep with no grid barrier between them: a block's backward sweep
       // reads its own rows' forward z (LDS or Z) and other blocks' backward z (its own mailbox),
@@ -872,19 +910,47 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         }
     } else {
       // 8 rows per thread in flight (one wave per SIMD: the loads must overlap)
-      for (long long r0 = vlo; r0 < vend; r0 += 8 * vstep) {
-        cx<TP> zv[8];
-        cx<double> pv[8];
+      if (p.grad) {
+        // z = z_edge + G zn: the edge's end-node correction minus its start-node correction
+        for (long long r0 = vlo; r0 < vend; r0 += 8 * vstep) {
+          cx<TP> zv[8];
+          cx<double> pv[8], za[8], zb[8];
+          int2 ev[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-          const long long r = r0 + u * vstep < vend ? r0 + u * vstep : r0;
-          zv[u] = p.Z[r];
-          pv[u] = first ? cx<double>{0.0, 0.0} : p.P[r];
+          for (int u = 0; u < 8; u++) {
+            const long long r = r0 + u * vstep < vend ? r0 + u * vstep : r0;
+            zv[u] = p.Z[r];
+            pv[u] = first ? cx<double>{0.0, 0.0} : p.P[r];
+            ev[u] = ((const int2 *)p.en)[r];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            za[u] = p.Zn[ev[u].x];
+            zb[u] = p.Zn[ev[u].y];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const long long r = r0 + u * vstep;
+            const cx<double> z0 = up(zv[u]);
+            const cx<double> z = {z0.r + (zb[u].r - za[u].r), z0.i + (zb[u].i - za[u].i)};
+            if (r < vend) p.P[r] = first ? z : cfma(beta, pv[u], z);
+          }
         }
+      } else {
+        for (long long r0 = vlo; r0 < vend; r0 += 8 * vstep) {
+          cx<TP> zv[8];
+          cx<double> pv[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-          const long long r = r0 + u * vstep;
-          if (r < vend) p.P[r] = first ? up(zv[u]) : cfma(beta, pv[u], up(zv[u]));
+          for (int u = 0; u < 8; u++) {
+            const long long r = r0 + u * vstep < vend ? r0 + u * vstep : r0;
+            zv[u] = p.Z[r];
+            pv[u] = first ? cx<double>{0.0, 0.0} : p.P[r];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const long long r = r0 + u * vstep;
+            if (r < vend) p.P[r] = first ? up(zv[u]) : cfma(beta, pv[u], up(zv[u]));
+          }
         }
       }
     }
@@ -1524,6 +1590,57 @@ hipError_t launch_precond_apply_levels(int n, int kind, int nlevL, const int *lv
     hipLaunchKernelGGL(k_level_bwd, dim3((m + 255) / 256), dim3(256), 0, s, hu[l], hu[l + 1], rowU, Uptr, Ucol, Uval,
                        DD, w, z);
   }
+  return hipGetLastError();
+}
+
+// mnd[a] = diag(G^T M G)[a]: for each edge e at node a (sign s_ae) the entries M_ef of row e whose
+// column f is also an edge at a (sign s_af), from the full CSR (K, M) pairs. One thread per node.
+__global__ void k_nodal_diag(int nn, const int *nptr, const int *nadj, const int *en, const int *rowptr,
+                             const int *col, const cx<double> *val, double *mnd) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nn) return;
+  double acc = 0.0;
+  for (int q = nptr[a]; q < nptr[a + 1]; q++) {
+    const int v = nadj[q];
+    const int e = (v > 0 ? v : -v) - 1;
+    const double se = v > 0 ? 1.0 : -1.0;
+    for (int j = rowptr[e]; j < rowptr[e + 1]; j++) {
+      const int f = col[j];
+      const int f0 = en[2 * f], f1 = en[2 * f + 1];
+      const double sf = (f1 == a) ? 1.0 : (f0 == a) ? -1.0 : 0.0;
+      acc += se * sf * val[j].i;
+    }
+  }
+  mnd[a] = acc;
+}
+
+// host-side apply (parity hook): zn = g / (i w mnd) on the nodes, then z[e] += zn[end] - zn[start]
+__global__ void k_grad_nodes(int nn, const int *nptr, const int *nadj, const double *mnd, double w,
+                             const cx<double> *r, cx<double> *zn) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a > nn) return;
+  zn[a] = (a == nn) ? cx<double>{0.0, 0.0} : nodal_z(nodal_g(nptr, nadj, r, a), w, mnd[a]);
+}
+__global__ void k_grad_edges(int n, const int *en, const cx<double> *zn, cx<double> *z) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const cx<double> a = zn[en[2 * e]], b = zn[en[2 * e + 1]];
+  z[e].r += b.r - a.r;
+  z[e].i += b.i - a.i;
+}
+
+hipError_t launch_nodal_diag(int nn, const int *nptr, const int *nadj, const int *en, const int *rowptr,
+                             const int *col, const cx<double> *val, double *mnd, hipStream_t s) {
+  if (nn <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_nodal_diag, dim3((nn + 255) / 256), dim3(256), 0, s, nn, nptr, nadj, en, rowptr, col, val, mnd);
+  return hipGetLastError();
+}
+hipError_t launch_grad_apply(int n, int nn, const int *nptr, const int *nadj, const int *en, const double *mnd,
+                             double w, const cx<double> *r, cx<double> *zn, cx<double> *z, hipStream_t s) {
+  hipLaunchKernelGGL(k_grad_nodes, dim3((nn + 1 + 255) / 256), dim3(256), 0, s, nn, nptr, nadj, mnd, w, r, zn);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_grad_edges, dim3((n + 255) / 256), dim3(256), 0, s, n, en, zn, z);
   return hipGetLastError();
 }
 

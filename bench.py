@@ -7,19 +7,24 @@ Mesh, pattern, assembly, boundary conditions and the preconditioner setup happen
 timed region; inputs are resident in HBM.
 
 A step is one solve from x0 = 0 to an FP64 true residual ||b - A x||/||b|| <= 1e-8 (recomputed
-on the device at the end) with SSOR-preconditioned COCG in FP64 (SSOR is the preconditioner the
-reference's solveCG passes; its 'IC' stagnates on this system). The whole COCG iteration runs in
-one persistent launch per 2000 iterations.
+on the device at the end) with FP64 COCG (the reference's recurrence, COCG13_single.f90:181-323)
+preconditioned by --precond, by default DIAG+GRAD: the diagonal edge preconditioner plus the nodal
+gradient-space correction G diag(G^T A G)^-1 G^T (afx.h, AFX_PC_GRAD). The whole iteration runs in
+one persistent launch per 2000 iterations. The timed loop runs --steps solves or stops after the
+step that passes --time-budget seconds; `steps` reports the solves timed.
 
-After the solve, the SpMV q = A p runs as a kernel of its own (k_spmv_stream, 50 timed launches
+Beside it, the comparator: plain FP64 SSOR-COCG in the reference's row order (the preconditioner
+the reference's solveCG passes; its 'IC' stagnates on this system), one solve, and the other
+corrected forms (SSOR+GRAD, SSOR_BJ+GRAD), one solve each; "variants" holds their solves/s, the
+speedup of the measured form over plain SSOR, and how far the two solutions' seafloor site fields
+differ.
+
+After the solves, the SpMV q = A p runs as a kernel of its own (k_spmv_stream, 50 timed launches
 on the solution vector); the line's "spmv" object gives its algorithmic GB/s against the HBM peak.
 
-By default the line also carries one batching variant, "adjoint_12_columns_32x32x24": 12
-adjoint right-hand sides (4 seafloor sites x 3 electric components) on the 32x32x24-cell mesh,
-one column at a time against the autotuned block width (stacked copies in one launch); about a
-minute, skipped with --no-variants.
-
-Variants, off by default because each adds minutes (their results are documented):
+Variants, off by default (their results are documented):
+  --adjoint32  12 adjoint right-hand sides (4 seafloor sites x 3 electric components) on the
+             32x32x24-cell mesh, one column at a time against the autotuned block width;
   --mixed    complex-FP32 preconditioner inside FP64 refinement (same tolerance);
   --batched  the 5 tidal constituents (M2, S2, N2, K1, O1) in one call, per-column omega (the
              autotuner picks one column at a time or the 5-column kernel);
@@ -94,19 +99,68 @@ def measured_traffic(workload, iterations_per_launch):
     return int(m["hbm_bytes_per_iteration"] * iterations_per_launch), os.path.relpath(path, ROOT)
 
 
+def bytes_per_iteration_grad(n, nnz, nn, nadj, edge_pc, nU=0, nL=0):
+    """Compulsory HBM bytes of one COCG iteration with the nodal gradient-space correction, every
+    array touched once per pass:
+    SpMV: (K, M) values 16 B + column 4 B per entry, row pointers, p read, q written;
+    edge part of z: DIAG reads r and DD and writes z; SSOR / SSOR_BJ as in bytes_per_iteration;
+    nodal part: the incidence list (4 B per entry), r read, diag(G^T M G) read, zn written;
+    p update: z, p read, p written, the edges' end nodes (8 B per edge), zn read;
+    x, r update: x, p, q, r read, x and r written."""
+    spmv = nnz * 20 + 4 * (n + 1) + 2 * 16 * n
+    if edge_pc == "DIAG":
+        edge = 3 * 16 * n
+    else:
+        edge = (nU + nL) * 20 + 2 * 8 * n + 2 * (16 * n + 16 * n + 16 * n)
+    nodal = 4 * nadj + 16 * n + 8 * nn + 16 * nn
+    pupd = 3 * 16 * n + 8 * n + 16 * nn
+    xr = 6 * 16 * n
+    return spmv + edge + nodal + pupd + xr
+
+
+def timed_solves(s, dB, dX, opts, steps, budget, label):
+    """Solve from x0 = 0 `steps` times (stopping after the step that passes `budget` seconds);
+    returns (steps, seconds, last stats, (launches, iterations, kernel ms))."""
+    import torch
+    s.kernel_timing(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done, st = 0, None
+    for _ in range(steps):
+        dX.zero_()
+        st, rc = s.solve_device(dB, dX, **opts)
+        done += 1
+        log(f"{label} step {done}: {st[0].iter} iterations, rc {rc}, {time.perf_counter() - t0:.2f}s")
+        if time.perf_counter() - t0 > budget:
+            break
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return done, dt, st[0], s.kernel_timing(reset=True)
+
+
+def site_fields(s, item, ve, vb, dX, omega):
+    from actfemtide_amd import afx
+    return afx.calobsebcomp(s, item, ve, vb, dX, omega)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", default="cfg2", choices=sorted(SIZES))
+    ap.add_argument("--precond", default="DIAG+GRAD",
+                    help="preconditioner of the measured solve (afx.PC); the plain FP64 SSOR-COCG of the "
+                         "reference's order is always timed beside it as the comparator")
+    ap.add_argument("--plain-steps", type=int, default=1, help="timed solves of the plain SSOR comparator (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed", action="store_true", help="also time the complex-FP32-preconditioner solve")
     ap.add_argument("--batched", action="store_true", help="also time the 5 constituents in one launch")
     ap.add_argument("--jacobian", type=int, default=0, metavar="NOBS",
                     help="also time genjacobian1's accumulation over every crust cell for NOBS adjoint columns")
-    ap.add_argument("--no-variants", action="store_true",
-                    help="skip the default batching variant (12 adjoint columns on 32x32x24 cells)")
+    ap.add_argument("--adjoint32", action="store_true",
+                    help="also time 12 adjoint columns on 32x32x24 cells, one at a time against the autotuned "
+                         "block width")
     ap.add_argument("--adjoint-batch", type=int, default=0, metavar="SITES",
                     help="also time SITES seafloor sites x 3 electric adjoint right-hand sides, one column at a "
                          "time against the autotuned block width")
@@ -115,7 +169,7 @@ def main():
     ap.add_argument("--mesh-batch", action="store_true",
                     help="also time 16 meshes of 8x8x6 to 40x40x30 cells one by one and in one launch")
     ap.add_argument("--maxit", type=int, default=400000, help="iteration cap of each solve")
-    ap.add_argument("--time-budget", type=float, default=150.0,
+    ap.add_argument("--time-budget", type=float, default=60.0,
                     help="seconds of timed solves: the timed loop stops after the first step that ends past it "
                          "(at least one step); `steps` in the line reports the steps actually timed")
     ap.add_argument("--warmup-iters", type=int, default=50,
@@ -148,52 +202,90 @@ def main():
     t_system = time.time() - t0
     log(f"system: n={s.n} nU={s.npu} levels L/U={s.nlev_l}/{s.nlev_u} ({t_system:.1f}s)")
     n = s.n
+    nnz = n + s.npu + s.npl
     dB = torch.from_numpy(b.astype(np.complex128)).to(dev)
-    s.precond_setup("SSOR")
+    pc = args.precond
+    opts = dict(precond=pc, tol=1e-8, maxit=args.maxit, chunk=2000, precision=afx.FP64)
+    s.precond_setup(pc)
     s.autotune(1)
     torch.cuda.synchronize()
-    opts = dict(precond="SSOR", tol=1e-8, maxit=args.maxit, chunk=2000, precision=afx.FP64)
-
     dX = torch.zeros(n, dtype=torch.complex128, device=dev)
-    # warmup: short solves of one persistent launch each (the kernels, the sweep value gathers and
-    # the work buffers are exercised; a full 1e-8 solve is tens of seconds at this size)
+    # warmup: short solves of one persistent launch each (the kernels, the value gathers and the
+    # work buffers are exercised; a warmup need not converge)
     for i in range(args.warmup):
         dX.zero_()
         s.solve_device(dB, dX, **dict(opts, maxit=args.warmup_iters))
-    s.kernel_timing(reset=True)
-    torch.cuda.synchronize()
-    stats = []
-    t_all = time.perf_counter()
-    steps = 0
-    for i in range(args.steps):
-        dX.zero_()
-        st, rc = s.solve_device(dB, dX, **opts)
-        stats.append(st[0])
-        steps += 1
-        log(f"step {steps}: {st[0].iter} iterations, rc {rc}, {time.perf_counter() - t_all:.1f}s")
-        if time.perf_counter() - t_all > args.time_budget:
-            break
-    torch.cuda.synchronize()
-    t_run = time.perf_counter() - t_all
-    launches, iters, kms = s.kernel_timing(reset=True)
+    steps, t_run, st_main, (launches, iters, kms) = timed_solves(s, dB, dX, opts, args.steps, args.time_budget, pc)
     true_res = float(s.true_residual_device(dB, dX)[0])
     value = steps / t_run
-    its = stats[-1].iter
-    log(f"FP64 SSOR-COCG: {steps} solve(s) in {t_run:.2f}s, {its} iterations, true residual "
-        f"{true_res:.2e}; k_cocg {launches} launches {kms:.0f} ms")
+    its = st_main.iter
+    log(f"{pc}: {steps} solve(s) in {t_run:.3f}s, {its} iterations, true residual {true_res:.2e}; "
+        f"k_cocg {launches} launches {kms:.1f} ms")
 
     # roofline of the persistent COCG kernel: algorithmic bytes of the iterations each launch ran
-    # over the launch's measured duration (HIP events on the library's stream)
-    by = bytes_per_iteration(n, s.npu, s.npl, 1, 16)
+    # over the launches' measured durations (HIP events on the library's stream)
+    base = pc.split("+")[0]
+    if pc.endswith("+GRAD"):
+        ln = np.asarray(case.line) - 1
+        bnd = np.zeros(case.node, bool)
+        bnd[ln[bc].ravel()] = True
+        nadj = int(np.count_nonzero(~bnd[ln]))  # (edge, interior end node) incidences
+        nn = int(np.count_nonzero(~bnd))
+        by = bytes_per_iteration_grad(n, nnz, nn, nadj, base, s.npu, s.npl)
+    else:
+        by = bytes_per_iteration(n, s.npu, s.npl, 1, 16)
     achieved = by * iters / (kms / 1e3) / 1e9
     ipl = iters / max(launches, 1)
-    traffic, tsrc = measured_traffic(f"{nx}x{ny}x{nz}", ipl)
+    traffic, tsrc = measured_traffic(f"{nx}x{ny}x{nz}_{pc.replace('+', '_')}", ipl)
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                kernel="k_cocg (persistent COCG, FP64, 1 column)",
+                kernel=f"k_cocg (persistent COCG, FP64, 1 column, {pc})",
                 bytes_per_iteration=by, avg_launch_ms=round(kms / max(launches, 1), 3),
                 iterations_per_launch=round(ipl, 1), us_per_iteration=round(1e3 * kms / max(iters, 1), 1),
                 traffic_source=tsrc)
+
+    # observation-site fields of the measured solution (96 seafloor sites, B and E)
+    item, ve, vb, _ = s.prepobscoeff(seafloor_sites(case, 96, 31))
+    f_main = site_fields(s, item, ve, vb, dX, w0)
+
+    # the comparator: plain FP64 SSOR-COCG in the reference's order (solveCG's preconditioner)
+    variants = {}
+    plain = None
+    if args.plain_steps > 0 and pc != "SSOR":
+        s.precond_setup("SSOR")
+        s.autotune(1)
+        dXp = torch.zeros_like(dX)
+        popts = dict(opts, precond="SSOR")
+        psteps, pt, pst, (pla, pit, pms) = timed_solves(s, dB, dXp, popts, args.plain_steps, 1.0, "plain SSOR")
+        f_plain = site_fields(s, item, ve, vb, dXp, w0)
+        ferr = float(max(np.abs(f_main[k] - f_plain[k]).max() / np.abs(f_plain[k]).max() for k in range(5)))
+        plain = dict(solves_per_s=round(psteps / pt, 6), seconds_per_solve=round(pt / psteps, 3), iters=pst.iter,
+                     true_resid=float(s.true_residual_device(dB, dXp)[0]),
+                     us_per_iteration=round(1e3 * pms / max(pit, 1), 1),
+                     note="FP64 SSOR-COCG in the reference's row order (COCG13_single with solveCG's 'SSOR')")
+        variants["plain_fp64_ssor"] = plain
+        variants["speedup_vs_plain"] = round(value / plain["solves_per_s"], 2)
+        variants["site_fields_vs_plain"] = dict(worst_rel_diff=ferr, sites=96,
+                                                note="B (3 components) and E (x, y) at seafloor sites of the "
+                                                     "measured solution against the plain solution, both at a "
+                                                     "1e-8 true residual")
+        log(f"plain SSOR: {pt / psteps:.2f}s per solve, {pst.iter} iterations; speedup "
+            f"{variants['speedup_vs_plain']}x; site fields differ by {ferr:.1e}")
+        del dXp
+    # the other forms of the nodal correction, one solve each
+    for other in ("SSOR+GRAD", "SSOR_BJ+GRAD", "DIAG+GRAD"):
+        if other == pc or time.time() - t_start > 300:
+            continue
+        s.precond_setup(other)
+        s.autotune(1)
+        dXo = torch.zeros_like(dX)
+        ostep, ot, ost, (ola, oit, oms) = timed_solves(s, dB, dXo, dict(opts, precond=other), 1, 1.0, other)
+        variants[other] = dict(solves_per_s=round(ostep / ot, 4), seconds=round(ot, 4), iters=ost.iter,
+                               us_per_iteration=round(1e3 * oms / max(oit, 1), 1),
+                               true_resid=float(s.true_residual_device(dB, dXo)[0]))
+        del dXo
+    s.precond_setup(pc)
+    s.autotune(1)
 
     # the SpMV (q = A p) as a kernel of its own, on the solution vector: algorithmic bytes of one
     # product over the average launch duration (HIP events on the library's stream)
@@ -201,7 +293,6 @@ def main():
     for _ in range(5):
         s.spmv_device(dX, dQ, timed=True)
     sp_ms = [s.spmv_device(dX, dQ, timed=True) for _ in range(args.spmv_launches)]
-    nnz = n + s.npu + s.npl
     sp_bytes = spmv_bytes(n, nnz)
     sp_avg = sum(sp_ms) / len(sp_ms)
     spmv = dict(kernel="k_spmv_stream", bound="hbm", achieved=round(sp_bytes / (sp_avg / 1e3) / 1e9, 1),
@@ -210,7 +301,6 @@ def main():
                 min_launch_us=round(1e3 * min(sp_ms), 2), launches=len(sp_ms))
     log(f"SpMV: {1e3 * sp_avg:.1f} us per launch, {spmv['achieved']:.0f} GB/s ({100 * spmv['frac']:.1f}% of peak)")
 
-    variants = {}
     if args.mixed:
         dXm = torch.zeros(n, dtype=torch.complex128, device=dev)
         torch.cuda.synchronize()
@@ -224,7 +314,7 @@ def main():
         log(f"mixed: {tm:.2f}s, {stm[0].iter} iterations")
     if args.batched:
         k = len(CONSTITUENTS)
-        s.autotune(k)  # right-hand-side block width: one column at a time or the k-column kernel
+        s.autotune(k)
         dBk = dB.reshape(n, 1).repeat(1, k).contiguous()
         dXk = torch.zeros((n, k), dtype=torch.complex128, device=dev)
         torch.cuda.synchronize()
@@ -241,13 +331,14 @@ def main():
         variants[f"jacobian_{args.jacobian}_columns"] = jacobian_variant(s, case, w0, args.jacobian)
     if args.adjoint_batch:
         variants[f"adjoint_{3 * args.adjoint_batch}_columns"] = adjoint_batch_variant(s, case, opts, args.adjoint_batch)
-    if not args.no_variants and (nx, ny, nz) != SIZES["cfg3"] and time.time() - t_start < 300:
-        # the batching gain where stacked copies fit: solvePARDISOinv's adjoint block on the
-        # 32x32x24-cell mesh (a failure here is recorded, it does not end the run)
+    if args.adjoint32 and (nx, ny, nz) != SIZES["cfg3"] and time.time() - t_start < 300:
+        # solvePARDISOinv's adjoint block on the 32x32x24-cell mesh: 12 columns one at a time
+        # against the autotuned block width, with the measured preconditioner (a failure here is
+        # recorded, it does not end the run)
         try:
             c3 = mesh.tide_case(*SIZES["cfg3"], seed=1002, spheres=3, sigma_sea=3.3)
             s3, _, _ = afx.forward_system(c3, w0)
-            s3.precond_setup("SSOR")
+            s3.precond_setup(pc)
             s3.autotune(1)
             variants["adjoint_12_columns_32x32x24"] = adjoint_batch_variant(s3, c3, opts, 4)
             s3.close()
@@ -268,29 +359,29 @@ def main():
         log(f"calibration copy of {args.copy_bytes} bytes done")
 
     # end to end for one more frequency (S2) on the same pattern: GENMAT + COPY_UL, GENBCCSEM +
-    # SET_BC_ICCG, the SSOR setup and the value gathers of the sweeps (a 1-iteration solve builds
+    # SET_BC_ICCG, the preconditioner setup and the value gathers (a 1-iteration solve builds
     # them), i.e. the per-frequency work forward_bxyz repeats, plus the measured solve time
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     b2 = s.assemble(case.sigma_tet(), omegas[1])
     b2 = s.set_bc(bc, b2)
-    s.precond_setup("SSOR")
+    s.precond_setup(pc)
     dB2 = torch.from_numpy(b2).to(dev)
     dX2 = torch.zeros_like(dB2)
     s.solve_device(dB2, dX2, **dict(opts, maxit=1, tol=0.0))
     torch.cuda.synchronize()
     t_freq = time.perf_counter() - t0
     end_to_end = dict(solves_per_s=round(1.0 / (t_freq + t_run / steps), 6),
-                      setup_per_frequency_s=round(t_freq, 3), first_system_s=round(t_system, 2),
-                      note="per frequency: assembly, boundary conditions, SSOR setup and sweep value gathers on "
-                           "the device, added to the measured solve; first_system_s (pattern analysis, mesh "
-                           "upload, first assembly) is paid once per mesh")
+                      setup_per_frequency_s=round(t_freq, 4), first_system_s=round(t_system, 2),
+                      note="per frequency: assembly, boundary conditions, preconditioner setup and value "
+                           "gathers on the device, added to the measured solve; first_system_s (pattern "
+                           "analysis, mesh upload, first assembly) is paid once per mesh")
     log(f"end to end: {t_freq * 1e3:.1f} ms setup per frequency, {end_to_end['solves_per_s']} solves/s")
-
     s.close()
+
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(case, w0, its)
+        cpu = cpu_baseline(case, w0, plain["iters"] if plain else its)
 
     out = {
         "metric": "edge-FEM solves/s to 1e-8 rel. residual, warm-start mixed vs plain COCG, 1 GPU",
@@ -299,15 +390,16 @@ def main():
         "n_gpus": 1,
         "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * t_run / steps, 1),
+        "ms_per_step": round(1e3 * t_run / steps, 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c128",
         "data": "synthetic",
         "config": {"workload": f"{nx}x{ny}x{nz} cells ({case.ntet} tets, {case.nline} edges), one M2 right-hand "
-                               f"side, FP64 SSOR-COCG from x0 = 0 to a 1e-8 true residual",
-                   "edges": case.nline, "tets": case.ntet, "rhs_per_step": 1, "parallelism": "1 GPU"},
+                               f"side, FP64 COCG preconditioned by {pc} from x0 = 0 to a 1e-8 FP64 true residual",
+                   "edges": case.nline, "tets": case.ntet, "rhs_per_step": 1, "precond": pc,
+                   "parallelism": "1 GPU"},
         "iters": its,
         "true_residual": true_res,
         "roofline": roof,
@@ -528,23 +620,98 @@ def mesh_batch_variant(opts, omega):
     return out
 
 
-def cpu_baseline(case, omega, gpu_iters):
-    """The oracle's serial COCG13_single restatement (C, 1 core) on the same system: a bounded
-    sample of iterations, scaled to the iteration count the GPU needed for 1e-8."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(case, omega, plain_iters):
+    """The reference's CPU solver, restated: the oracle's serial COCG13_single (C, SSOR, the
+    reference's row order) on the host cores of this machine.
+    - value: one core on the same 1.4M-edge system; the per-iteration time is the median of 5
+      samples of 6 iterations (after a 1-iteration warm-up), scaled to the iterations the plain
+      solve needs (plain_iters, measured on the GPU for the same system) -- extrapolated;
+    - full_solve_16x16x12: one whole SSOR solve (not extrapolated) at 16x16x12 cells, median of 3;
+    - all_cores: the iteration's parallel phases (SpMV, dot products, vector updates) on every core
+      with OpenMP, the sequential sweeps on one, as an all-core COCG13 iteration would run;
+    - superlu_16x16x12: a direct complex LU solve (scipy SuperLU, symmetric-mode minimum degree
+      ordering; the PARDISO-class reference) on one core, median of 3."""
+    import scipy.sparse.linalg as sla
+
+    from actfemtide_amd import mesh
     from oracle import oracle as O
+    ncores = os.cpu_count() or 1
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():  # the process's share of the host
+        ncores = min(ncores, int(os.environ["OMP_NUM_THREADS"]))
     t0 = time.time()
     o = O.forward_system(case, omega)
     t_asm = time.time() - t0
-    it = 30
-    t0 = time.perf_counter()
-    O.cocg13(o["INU"], o["IAU"], o["INL"], o["IAL"], o["D"], o["AU"], o["AL"], o["b"], precond="SSOR",
-             resid=0.0, maxit=it)
-    dt = time.perf_counter() - t0
-    per_solve = dt / it * gpu_iters
+    args = (o["INU"], o["IAU"], o["INL"], o["IAL"], o["D"], o["AU"], o["AL"], o["b"])
+    O.cocg13(*args, precond="SSOR", resid=0.0, maxit=1)
+    per_it = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        O.cocg13(*args, precond="SSOR", resid=0.0, maxit=6)
+        per_it.append((time.perf_counter() - t0) / 6)
+    t_it = float(np.median(per_it))
+    per_solve = t_it * plain_iters
+    # sequential sweeps of one iteration (the SSOR apply) and the parallel phases on all cores
+    DD = O.precond_setup("SSOR", o["D"], o["INU"], o["AU"], o["INL"], o["IAL"], o["AL"])
+    r = np.asarray(o["b"])
+    sw = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        O.precond_apply("SSOR", o["INU"], o["IAU"], o["AU"], o["INL"], o["IAL"], o["AL"], DD, r)
+        sw.append(time.perf_counter() - t0)
+    t_sweep = float(np.median(sw))
+    t_par1 = O.cocg_parallel_phases(o, 1)
+    t_parn = O.cocg_parallel_phases(o, ncores)
+    t_it_n = t_sweep + t_parn
+    # one full solve at 16x16x12 cells and its direct solve
+    c1 = mesh.tide_case(16, 16, 12, seed=1002, spheres=3, sigma_sea=3.3)
+    o1 = O.forward_system(c1, omega)
+    args1 = (o1["INU"], o1["IAU"], o1["INL"], o1["IAL"], o1["D"], o1["AU"], o1["AL"], o1["b"])
+    full = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        r1 = O.cocg13(*args1, precond="SSOR", resid=1e-8, maxit=100000)
+        full.append(time.perf_counter() - t0)
+    A1 = O.to_scipy(*args1[:7]).tocsc()
+    lu_t = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        lu = sla.splu(A1, permc_spec="MMD_AT_PLUS_A", options=dict(SymmetricMode=True))
+        x1 = lu.solve(np.asarray(o1["b"]))
+        lu_t.append(time.perf_counter() - t0)
+    lu_res = float(np.linalg.norm(o1["b"] - A1 @ x1) / np.linalg.norm(o1["b"]))
+    log(f"cpu baseline: {1e3 * t_it:.1f} ms per iteration on 1 core ({1e3 * t_sweep:.1f} ms sweeps, parallel "
+        f"phases {1e3 * t_par1:.1f} ms on 1 core, {1e3 * t_parn:.1f} ms on {ncores}); 16x16x12 full solve "
+        f"{np.median(full):.2f}s ({r1['iter']} iterations), SuperLU {np.median(lu_t):.2f}s")
     return dict(value=round(1.0 / per_solve, 8), unit="solves/s", cores=1, kind="port",
-                sample=f"{it} iterations of the C oracle's COCG13_single restatement (SSOR) on the same "
-                       f"{case.nline}-edge system ({dt:.1f}s), scaled to the {gpu_iters} iterations the "
-                       f"solve needs; oracle assembly ({t_asm:.1f}s) not included")
+                extrapolated=True,
+                sample=f"the C oracle's COCG13_single restatement (SSOR, reference row order) on the same "
+                       f"{case.nline}-edge system: median of 5 samples of 6 iterations ({1e3 * t_it:.1f} ms per "
+                       f"iteration) scaled to the {plain_iters} iterations of the plain solve; oracle assembly "
+                       f"({t_asm:.1f}s) not included",
+                host=dict(nproc=os.cpu_count(), threads_used=ncores, cpu_model=_cpu_model()),
+                all_cores=dict(cores=ncores, value=round(1.0 / (t_it_n * plain_iters), 8), unit="solves/s",
+                               extrapolated=True, ms_per_iteration=round(1e3 * t_it_n, 2),
+                               sweeps_ms=round(1e3 * t_sweep, 2), parallel_phases_ms_1core=round(1e3 * t_par1, 2),
+                               parallel_phases_ms_all=round(1e3 * t_parn, 2),
+                               note="SpMV, dot products and vector updates with OpenMP on every core, the SSOR "
+                                    "sweeps sequential in the reference's row order (medians of 5)"),
+                full_solve_16x16x12=dict(seconds=round(float(np.median(full)), 3), iters=int(r1["iter"]),
+                                         cores=1, edges=c1.nline, extrapolated=False,
+                                         solves_per_s=round(1.0 / float(np.median(full)), 4)),
+                superlu_16x16x12=dict(seconds=round(float(np.median(lu_t)), 3), cores=1, edges=c1.nline,
+                                      residual=lu_res, note="scipy SuperLU complex128 factor + solve, "
+                                                            "MMD_AT_PLUS_A ordering, symmetric mode"))
 
 
 if __name__ == "__main__":

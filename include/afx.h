@@ -38,6 +38,15 @@ extern "C" {
 #define AFX_PC_DIAG 3 /* 'DIAG' */
 #define AFX_PC_SSOR_BJ 4 /* SSOR on the diagonal blocks of the device row partition (no reference
                             counterpart: changes the preconditioner, keeps the COCG iteration) */
+/* Flag OR-ed into AFX_PC_SSOR, AFX_PC_SSOR_BJ or AFX_PC_DIAG (no reference counterpart): adds
+   the nodal gradient-space correction G diag(G^T A G)^-1 G^T to the edge preconditioner, G the
+   discrete gradient of the interior nodes (afx_set_nodes). The curl-curl part annihilates
+   gradients (K G = 0), so A's gradient components are set by the tiny i*omega*mu0*sigma mass
+   alone, and in the air they are what makes edge-only SSOR-COCG need tens of thousands of
+   iterations; the correction solves for them on the nodes (diag(G^T A G) = i*omega*diag(G^T M G)).
+   The preconditioner stays complex symmetric, so COCG is unchanged. Single right-hand-side
+   launches only (several columns are solved one at a time; stacked systems are refused). */
+#define AFX_PC_GRAD 0x100
 
 /* arithmetic of the Krylov loop */
 #define AFX_FP64 0  /* all-FP64 COCG */
@@ -128,6 +137,13 @@ int afx_set_omega(afx_system s, double omega);
    one outer face; line(nline,2) 1-based. line_bc out (nline bytes). */
 int afx_genbc(afx_system s, const int *line, double zmin, double zmax, double xout, double yout,
               unsigned char *line_bc);
+
+/* Edge-to-node incidence for AFX_PC_GRAD: line(nline, 2) the 1-based end nodes of every edge
+   (READLINE's line, m_line_type.f90:28-48; the edge points from line(i,1) to line(i,2), the
+   orientation n6line's signs refer to), node the node count, dirichlet (nline bytes, 1 = Dirichlet
+   edge, as GENBCCSEM marks them). Nodes on a Dirichlet edge are boundary nodes (their gradients
+   have tangential components on the boundary) and are left out of G. */
+int afx_set_nodes(afx_system s, int node, const int *line, const unsigned char *dirichlet);
 
 /* SET_BC_ICCG (forward_bxyz_mpi.f90:567-636) with zero Dirichlet values: zeroes Dirichlet rows
    and columns in place (pattern kept), D = 1, and zeroes rhs(n, nrhs) (host, may be NULL)

@@ -726,3 +726,36 @@ int orc_jacobian(int nobs, int nline, const cplx *u, const cplx *x, int npar, co
   }
   return 0;
 }
+
+/* ---------------- CPU baseline: the parallel phases of one COCG13_single iteration ----------------
+ * For timing only (the CPU baseline's all-cores figure): one iteration's SpMV (:273-288), its
+ * p^T q (:293-296), the x, r update with ||r||^2 (:304-313), r^T z (:246-250) and the p update
+ * (:258-268), each loop split over `nthreads` OpenMP threads (rows in static chunks, the
+ * reductions combined per thread). The SSOR / D-ILU sweeps are sequential in the reference's row
+ * order and are not part of it. Returns p^T q + ||r||^2 + r^T z (so nothing is optimised away). */
+double orc_cocg_parallel_phases(int n, const cplx *D, const int *INU, const int *IAU, const cplx *AU,
+                                const int *INL, const int *IAL, const cplx *AL, cplx *X, cplx *R, const cplx *Z,
+                                cplx *P, cplx *Q, int nthreads) {
+  double c1r = 0.0, rr = 0.0, rzr = 0.0;
+  const cplx alpha = {1e-3, 2e-3}, beta = {0.5, -0.25};
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : rzr)
+  for (int i = 0; i < n; i++) {
+    rzr += cmul(R[i], Z[i]).re;
+    P[i] = cadd(Z[i], cmul(beta, P[i]));
+  }
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : c1r)
+  for (int j = 0; j < n; j++) {
+    cplx w = cmul(D[j], P[j]);
+    for (long p = INU[j]; p < INU[j + 1]; p++) w = cadd(w, cmul(AU[p], P[IAU[p] - 1]));
+    for (long p = INL[j]; p < INL[j + 1]; p++) w = cadd(w, cmul(AL[p], P[IAL[p] - 1]));
+    Q[j] = w;
+    c1r += cmul(P[j], w).re;
+  }
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : rr)
+  for (int i = 0; i < n; i++) {
+    X[i] = cadd(X[i], cmul(alpha, P[i]));
+    R[i] = csub(R[i], cmul(alpha, Q[i]));
+    rr += R[i].re * R[i].re + R[i].im * R[i].im;
+  }
+  return c1r + rr + rzr;
+}

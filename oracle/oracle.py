@@ -376,3 +376,82 @@ def model_roughening(n4, nphys1, ele2model, nmodel, iactive=None):
         R[m, m] = j1 - 1
     act = np.ones(nmodel, bool) if iactive is None else np.asarray(iactive, bool)
     return R[np.ix_(act, act)]
+
+
+# ---------------------------------------------------------------------------------------------
+# Nodal gradient-space correction of the edge preconditioner (no reference counterpart: the
+# reference's COCG13_single offers IC / SSOR / DIAG only, COCG13_single.f90:107-130). Written with
+# scipy sparse products, a route independent of the library's per-node loops.
+
+def discrete_gradient(line, node, dirichlet):
+    """G (nline x nn): column a is the gradient of interior node a's hat function, +1 on edges that
+    end at the node and -1 on edges that start there (an edge points from line(i,1) to
+    line(i,2)). Nodes on a Dirichlet edge are left out. Returns (G csr, interior node ids)."""
+    import scipy.sparse as sp
+    line = np.asarray(line).reshape(-1, 2) - 1
+    n = line.shape[0]
+    bnd = np.zeros(node, bool)
+    d = np.asarray(dirichlet, bool)
+    bnd[line[d, 0]] = True
+    bnd[line[d, 1]] = True
+    inner = np.nonzero(~bnd)[0]
+    Gfull = sp.csr_matrix((np.r_[-np.ones(n), np.ones(n)], (np.r_[np.arange(n), np.arange(n)],
+                                                          np.r_[line[:, 0], line[:, 1]])), shape=(n, node))
+    return Gfull[:, inner].tocsr(), inner
+
+
+def grad_correction(A, G, omega_eff=1.0):
+    """The correction operator r -> G (g / (i w mnd)), g = G^T r, mnd = diag(G^T Im(A) G) / w for A
+    given at angular frequency w (the imaginary part of A is w M). Jacobi on G^T A G, whose real
+    part G^T K G vanishes for a curl-curl K."""
+    mnd = np.asarray((G.T @ (A.imag @ G)).diagonal())
+    dn = 1.0 / (1j * mnd)
+    return lambda r: G @ (dn * (G.T @ r)), mnd
+
+
+def cocg_precond(A, b, M, tol=1e-8, maxit=100000, x0=None):
+    """COCG of COCG13_single.f90:181-323 (same recurrence, unconjugated forms, stop on the
+    recurrence residual over ||b||) with an arbitrary complex-symmetric preconditioner M (a
+    function r -> z). Returns (x, iterations, recurrence residual)."""
+    x = np.zeros_like(b) if x0 is None else np.array(x0, np.complex128)
+    r = b - A @ x
+    bn = np.linalg.norm(b)
+    rho1 = p = None
+    res = np.linalg.norm(r) / bn
+    for it in range(1, maxit + 1):
+        z = M(r)
+        rho = r @ z
+        p = z if it == 1 else z + (rho / rho1) * p
+        q = A @ p
+        al = rho / (p @ q)
+        x += al * p
+        r -= al * q
+        res = np.linalg.norm(r) / bn
+        if res <= tol:
+            return x, it, res
+        rho1 = rho
+    return x, maxit + 1, res
+
+
+def cocg_parallel_phases(o, nthreads, reps=5):
+    """Seconds per call of orc_cocg_parallel_phases (one COCG13_single iteration's SpMV, dot
+    products and vector updates split over nthreads OpenMP threads) on the split system o; the
+    median of reps calls after one warm-up call."""
+    import time
+    n = len(o["D"])
+    rng = np.random.default_rng(0)
+    vec = lambda: (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex128)  # noqa: E731
+    X, R, Z, P, Q = vec(), vec(), vec(), vec(), np.zeros(n, np.complex128)
+    D, AU, AL = _c128(o["D"]), _c128(o["AU"]), _c128(o["AL"])
+    INU, IAU, INL, IAL = _i32(o["INU"]), _i32(o["IAU"]), _i32(o["INL"]), _i32(o["IAL"])
+    f = lib().orc_cocg_parallel_phases
+    f.restype = C.c_double
+    args = (C.c_int(n), _p(D), _p(INU), _p(IAU), _p(AU), _p(INL), _p(IAL), _p(AL), _p(X), _p(R), _p(Z), _p(P),
+            _p(Q), C.c_int(nthreads))
+    f(*args)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        f(*args)
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))

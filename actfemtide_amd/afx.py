@@ -262,10 +262,11 @@ class System:
             lib().afx_destroy(self.h)
             self.h = None
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         # during interpreter shutdown the HIP runtime (torch's) may already be gone: leave the
-        # device memory to the process exit instead of calling into it
-        if sys.is_finalizing():
+        # device memory to the process exit instead of calling into it (the module's globals,
+        # `sys` among them, may already be cleared then, hence the bound default)
+        if _finalizing():
             return
         try:
             self.close()

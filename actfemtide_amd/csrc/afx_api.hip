@@ -822,8 +822,8 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   const bool multi = k > 1;
   const bool grad = (pc & AFX_PC_GRAD) != 0;
   pc &= 0xff;
-  if (grad && (multi || (stacked && s->nsys > 1) || s->nn < 0))
-    return fail(AFX_ERR_ARG, "AFX_PC_GRAD: one right-hand side of an unstacked system with afx_set_nodes");
+  if (grad && ((stacked && s->nsys > 1) || s->nn < 0))
+    return fail(AFX_ERR_ARG, "AFX_PC_GRAD: an unstacked system with afx_set_nodes");
   // stacked systems: one column, per-system scalars and convergence; stats and host_conv then
   // hold one entry per system
   const bool sys = stacked && s->nsys > 1;
@@ -831,9 +831,16 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   const int ks = sys ? s->nsys : k;
   const int kp = multi ? next_pow2(k) : 1;
   if (kp > 256) return fail(AFX_ERR_ARG, "at most 256 right-hand sides per launch");
-  const int G = s->Gs;
+  // the streaming kernel (no sweep: DIAG or NONE edge preconditioner, one column, FP64) runs
+  // at several blocks per CU; every other case runs k_cocg on the sweep partition
+  const bool stream = fp64 && !(stacked && s->nsys > 1) && (pc == AFX_PC_DIAG || pc == AFX_PC_NONE) &&
+                      !getenv("AFX_NO_STREAM");
+  int per_cu = 1;
+  if (const char *e = getenv("AFX_STREAM_PER_CU")) per_cu = std::max(1, std::min(8, atoi(e)));
+  const int G = stream ? afx::max_coop_grid_diag(s->lpr64, per_cu) : s->Gs;
+  if (G <= 0) return fail(AFX_ERR_HIP, "no co-resident grid for the COCG kernel");
   Work<TP> w;
-  int rc = w.alloc(s->n, k, G, ks);
+  int rc = w.alloc(s->n, k, std::max(G, s->Gs), ks);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(w.col2f, col2f.data(), k * sizeof(int), hipMemcpyHostToDevice, s->stream));
   HIPCHK(hipMemcpyAsync(w.wcol, wcol.data(), k * sizeof(double), hipMemcpyHostToDevice, s->stream));
@@ -899,9 +906,9 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   p.nadj = s->d_nadj;
   p.en = s->d_en;
   p.mnd = s->d_mnd;
-  if (grad) {
-    if ((rc = dalloc(&w.Zn, (size_t)s->nn + 1))) return rc;
-    HIPCHK(hipMemsetAsync(w.Zn, 0, ((size_t)s->nn + 1) * sizeof(cx<double>), s->stream));
+  if (grad) {  // [nn + 1][k]: the last row (boundary nodes) stays zero
+    if ((rc = dalloc(&w.Zn, ((size_t)s->nn + 1) * k))) return rc;
+    HIPCHK(hipMemsetAsync(w.Zn, 0, ((size_t)s->nn + 1) * k * sizeof(cx<double>), s->stream));
   }
   p.Zn = w.Zn;
   const bool bj = (pc == AFX_PC_SSOR_BJ);
@@ -976,8 +983,9 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   int it_before = 0;
   for (;;) {
     HIPCHK(hipEventRecord(ev0, s->stream));
-    hipError_t e = fp64 ? afx::launch_cocg64(*(afx::CocgParams<double> *)&p, lpr, G, s->stream)
-                        : afx::launch_cocg32(*(afx::CocgParams<float> *)&p, lpr, G, s->stream);
+    hipError_t e = stream ? afx::launch_cocg_diag(*(afx::CocgParams<double> *)&p, lpr, G, s->stream)
+                   : fp64 ? afx::launch_cocg64(*(afx::CocgParams<double> *)&p, lpr, G, s->stream)
+                          : afx::launch_cocg32(*(afx::CocgParams<float> *)&p, lpr, G, s->stream);
     if (e != hipSuccess) {
       (void)hipEventDestroy(ev0);
       (void)hipEventDestroy(ev1);
@@ -997,7 +1005,8 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
     if (verbose) {
       const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
       fprintf(stderr, "afx: %s k=%d G=%d lpr=%d lds=%d ring=%d it=%d done=%d %.3fs (%.1f us/it)\n",
-              fp64 ? "fp64" : "fp32-pc", k, G, lpr, p.lds_rows, p.ring_w, w.h_status[0], w.h_status[1], dt,
+              stream ? "fp64-stream" : fp64 ? "fp64" : "fp32-pc", k, G, lpr, p.lds_rows, p.ring_w, w.h_status[0],
+              w.h_status[1], dt,
               w.h_status[0] ? 1e6 * dt / w.h_status[0] : 0.0);
     }
     // a wait that gave up (or a raised flag) ends the solve: results are not trusted
@@ -1338,7 +1347,7 @@ int solve_device_impl(afx_system s, int k, const cx<double> *dB, cx<double> *dX,
   const char *bm = getenv("AFX_BATCH");
   const int width = bm ? (std::string(bm) == "multi" ? 1 : std::string(bm) == "stack" ? 2 : 0) : s->batch_multi[k];
   const bool grad = ((o->nset == 0 ? s->pc_kind : o->precond) & AFX_PC_GRAD) != 0;
-  const bool multi = width == 1 && !grad;
+  const bool multi = width == 1;
   if (k > 1 && o->nset != 0 && width == 2 && !stacked && !grad)
     return solve_stacked_columns(s, k, dB, dX, omega_scale, o, stats);
   if (k > 1 && (o->nset != 0 || grad) && !multi) return solve_columns(s, k, dB, dX, omega_scale, o, stats);
@@ -2180,7 +2189,8 @@ static int autotune_width(afx_system s, int k, int kind, int verbose) {
   // exactly once built): a sweep without it is several times slower
   const std::string keep_err = g_err;
   const int grp = stack_group(s, k);
-  const bool can_stack = grp >= 2 &&
+  // stacked copies carry no node incidence: not a width for the nodal correction
+  const bool can_stack = grp >= 2 && !(kind & AFX_PC_GRAD) &&
                          ensure_stack(s, grp, std::vector<double>(wcol.begin(), wcol.begin() + grp)) == AFX_OK &&
                          lds_rows_for<double>(s->stack, false) > 0;
   g_err = keep_err;
@@ -2214,10 +2224,6 @@ int afx_autotune(afx_system s, int nrhs, int verbose) {
   if (!s || nrhs <= 0 || nrhs > 256) return fail(AFX_ERR_ARG, "bad arguments");
   if (!s->have_values) return fail(AFX_ERR_ARG, "no matrix values");
   const int kind = s->pc_valid ? s->pc_kind : AFX_PC_SSOR;
-  if (nrhs > 1 && (kind & AFX_PC_GRAD)) {  // the corrected preconditioner runs one column at a time
-    s->batch_multi[nrhs] = 0;
-    return AFX_OK;
-  }
   if (nrhs > 1) return autotune_width(s, nrhs, kind, verbose);
   const size_t nk = (size_t)s->n;
   std::vector<cx<double>> hb(nk, cx<double>{1.0, 0.5});

@@ -144,6 +144,10 @@ struct PointGrid {
 hipError_t launch_cocg64(const CocgParams<double> &p, int lpr, int grid, hipStream_t s);
 hipError_t launch_cocg32(const CocgParams<float> &p, int lpr, int grid, hipStream_t s);
 int max_coop_grid(int lpr, bool multi, bool fp64, size_t dyn);
+// the COCG iteration without sweeps (DIAG / NONE edge preconditioner, optional nodal
+// correction, one column, FP64) at two waves per SIMD
+int max_coop_grid_diag(int lpr, int per_cu);
+hipError_t launch_cocg_diag(const CocgParams<double> &p, int lpr, int grid, hipStream_t s);
 int max_coop_grid_ic();
 hipError_t launch_spmv(const SpmvParams &p, hipStream_t s);
 hipError_t launch_spmv_stream(int n, int maxrow, const int *rowptr, const int *col, const cx<double> *val, double w,

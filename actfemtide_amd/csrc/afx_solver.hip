@@ -704,13 +704,10 @@ __host__ __device__ constexpr size_t lds_bounds_offset(int lds_rows) {
   return ((size_t)lds_rows * sizeof(cx<TP>) + 15) & ~(size_t)15;
 }
 
-// One block of 256 threads per CU (the cooperative grid never exceeds the CU count), so each
-// SIMD runs a single wave of this kernel: let it use the whole register file instead of
-// spilling the software-pipelined sweep registers to scratch. PROF selects the instantiation
-// that records per-phase timing (its timers would otherwise hold scalar registers that the
-// sweep loops need).
-template <typename TP, int LPR, bool MULTI, bool PROF>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) k_cocg(CocgParams<TP> p) {
+// The persistent COCG iteration (COCG13_single.f90:181-323). SWEEP = false compiles the
+// preconditioner sweeps out (the DIAG / NONE kernel).
+template <typename TP, int LPR, bool MULTI, bool PROF, bool SWEEP>
+__device__ __forceinline__ void cocg_body(const CocgParams<TP> &p) {
   __shared__ cx<double> lds_c[256];
   __shared__ double lds_d[256];
   extern __shared__ unsigned char dyn[];
@@ -728,9 +725,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
   const double w = cvalid ? p.wcol[c] : 0.0;
   const TP wT = (TP)w;
   const int f = cvalid ? p.col2f[c] : 0;
-  const bool sweep_pc = (p.pc == AFX_PC_IC || p.pc == AFX_PC_SSOR || p.pc == AFX_PC_SSOR_BJ);
+  const bool sweep_pc = SWEEP && (p.pc == AFX_PC_IC || p.pc == AFX_PC_SSOR || p.pc == AFX_PC_SSOR_BJ);
   const int pb = blockIdx.x;
-  const int rs = p.rstart[pb], re = p.rstart[pb + 1];
+  // the row range of the block (sweep partition; without sweeps only stacked systems use it, and
+  // the grid may then be larger than the partition)
+  const bool use_rows = SWEEP || (!MULTI && p.nsys > 1);
+  const int rs = use_rows ? p.rstart[pb] : 0, re = use_rows ? p.rstart[pb + 1] : 0;
   // stacked systems: this block's system (its state index), the blocks its reductions sum, and
   // the rows its vector passes cover (its own rows; otherwise all rows, grid-strided)
   const bool sys = !MULTI && p.nsys > 1;
@@ -739,7 +739,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
   const long long vlo = sys ? rs + t : gtid, vstep = sys ? 256 : nthr, vend = sys ? re : p.n;
   const bool dist = !sys && p.dist_reduce != 0;
   // single-column sweeps: the block's step bounds of both directions are staged in LDS once
-  const bool pipe_sweep = !MULTI && p.lds_rows > 0 && p.fw.psrc != nullptr && p.bw.psrc != nullptr;
+  const bool pipe_sweep = SWEEP && !MULTI && p.lds_rows > 0 && p.fw.psrc != nullptr && p.bw.psrc != nullptr;
   int *sbf = (int *)(dyn + lds_bounds_offset<TP>(p.lds_rows));
   int *sbb = sbf + p.fw.maxsteps + 1;
   if (pipe_sweep) {
@@ -797,6 +797,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         const cx<double> g = nodal_g(p.nptr, p.nadj, p.R, (int)a);
         const cx<double> zn = nodal_z(g, w, p.mnd[a]);
         p.Zn[a] = zn;
+        rho_loc = addv(rho_loc, cmul(g, zn));
+      }
+    }
+    if (MULTI && p.grad && cvalid) {  // one lane per (node, column), zn at the column's omega
+      for (long long a = slot; a < p.nn; a += nslot) {
+        const int q0 = p.nptr[a], q1 = p.nptr[a + 1];
+        cx<double> g = {0.0, 0.0};
+        for (int q = q0; q < q1; q++) {
+          const int v = p.nadj[q];
+          const cx<double> r = p.R[(size_t)((v > 0 ? v : -v) - 1) * k + c];
+          g.r += v > 0 ? r.r : -r.r;
+          g.i += v > 0 ? r.i : -r.i;
+        }
+        const cx<double> zn = nodal_z(g, w, p.mnd[a]);
+        p.Zn[(size_t)a * k + c] = zn;
         rho_loc = addv(rho_loc, cmul(g, zn));
       }
     }
@@ -905,7 +920,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
       if (cvalid)
         for (long long row = slot; row < p.n; row += nslot) {
           const size_t ix = (size_t)row * k + c;
-          const cx<double> z = up(p.Z[ix]);
+          cx<double> z = up(p.Z[ix]);
+          if (p.grad) {  // + G zn
+            const int2 e = ((const int2 *)p.en)[row];
+            const cx<double> za = p.Zn[(size_t)e.x * k + c], zb = p.Zn[(size_t)e.y * k + c];
+            z.r += zb.r - za.r;
+            z.i += zb.i - za.i;
+          }
           p.P[ix] = first ? z : cfma(beta, p.P[ix], z);
         }
     } else {
@@ -961,14 +982,34 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     // ---- q = A p (:273-288) and c1 = p^T q (:293-296) ----
     cx<double> c1_loc = {0.0, 0.0};
     if (MULTI) {
+      // one lane per (row, column): the row's first ME values and columns are loaded before the
+      // first gather, and all its gathers before the first product (entries summed in row order)
+      constexpr int ME = 20;
       for (long long row = slot; row < p.n; row += nslot) {
         const int qb = p.rowptr[row], qe = p.rowptr[row + 1];
         if (cvalid) {
-          cx<double> acc = {0.0, 0.0};
-          for (int q = qb; q < qe; ++q) acc = cfma(mval(p.val[q], w), p.P[(size_t)p.col[q] * k + c], acc);
+          cx<double> vv[ME], pv[ME];
+          int cl[ME];
+#pragma unroll
+          for (int e = 0; e < ME; e++) {
+            const int q = qb + e;
+            const bool in = q < qe;
+            const int qc = in ? q : qb;
+            vv[e] = p.val[qc];
+            const int cq = p.col[qc];
+            cl[e] = in ? cq : -1;
+          }
+#pragma unroll
+          for (int e = 0; e < ME; e++) pv[e] = p.P[(size_t)max(cl[e], 0) * k + c];
           const size_t ix = (size_t)row * k + c;
+          const cx<double> prow = p.P[ix];
+          cx<double> acc = {0.0, 0.0};
+#pragma unroll
+          for (int e = 0; e < ME; e++)
+            if (cl[e] >= 0) acc = cfma(mval(vv[e], w), pv[e], acc);
+          for (int q = qb + ME; q < qe; ++q) acc = cfma(mval(p.val[q], w), p.P[(size_t)p.col[q] * k + c], acc);
           p.Q[ix] = acc;
-          c1_loc = addv(c1_loc, cmul(p.P[ix], acc));
+          c1_loc = addv(c1_loc, cmul(prow, acc));
         }
       }
     } else {
@@ -1162,6 +1203,58 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     p.status[0] = it;
     p.status[1] = alldone ? 1 : 0;
   }
+}
+
+// One block of 256 threads per CU (the cooperative grid never exceeds the CU count), so each
+// SIMD runs a single wave of this kernel: let it use the whole register file instead of
+// spilling the software-pipelined sweep registers to scratch. PROF selects the instantiation
+// that records per-phase timing (its timers would otherwise hold scalar registers that the
+// sweep loops need).
+template <typename TP, int LPR, bool MULTI, bool PROF>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) k_cocg(CocgParams<TP> p) {
+  cocg_body<TP, LPR, MULTI, PROF, true>(p);
+}
+
+// The same iteration without the sweeps (DIAG or NONE edge preconditioner, with or without the
+// nodal correction, one column, FP64): every pass is a bandwidth-bound stream with no dependency
+// chain, so the launch runs two waves per SIMD (two blocks per CU) to keep more loads in flight.
+template <int LPR>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_cocg_flat(CocgParams<double> p) {
+  cocg_body<double, LPR, false, false, false>(p);
+}
+
+// several columns without sweeps (one lane per row and column)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) k_cocg_flat_multi(CocgParams<double> p) {
+  cocg_body<double, 1, true, false, false>(p);
+}
+
+template <int LPR>
+static const void *flat_kernel() {
+  return (const void *)k_cocg_flat<LPR>;
+}
+static const void *flat_kernel_for(int lpr) {
+  switch (lpr) {
+    case 1: return flat_kernel<1>();
+    case 2: return flat_kernel<2>();
+    case 4: return flat_kernel<4>();
+    case 16: return flat_kernel<16>();
+    default: return flat_kernel<8>();
+  }
+}
+
+// Largest co-resident grid of the flat kernel (blocks), at most `per_cu` blocks per CU.
+int max_coop_grid_diag(int lpr, int per_cu) {
+  int dev = 0, ncu = 0, nb = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, flat_kernel_for(lpr), 256, 0) != hipSuccess) return 0;
+  return std::min(nb, per_cu) * ncu;
+}
+
+hipError_t launch_cocg_diag(const CocgParams<double> &p, int lpr, int grid, hipStream_t s) {
+  void *args[] = {(void *)&p};
+  const void *f = p.k > 1 ? (const void *)k_cocg_flat_multi : flat_kernel_for(lpr);
+  return hipLaunchCooperativeKernel(f, dim3(grid), dim3(256), args, 0, s);
 }
 
 template <typename TP>

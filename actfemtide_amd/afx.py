@@ -87,6 +87,7 @@ SIGNATURES = {
     "afx_solve_device": ([_VP, _I, _VP, _VP, _VP, C.POINTER(afx_opts), _VP], _I),
     "afx_solve_stacked_device": ([_VP, _VP, _VP, C.POINTER(afx_opts), _VP], _I),
     "afx_true_residual_device": ([_VP, _I, _VP, _VP, _VP, _VP], _I),
+    "afx_true_residual_device_noalloc": ([_VP, _I, _VP, _VP, _VP, _VP], _I),
     "afx_scale_initial_guess_device": ([_VP, _I, _VP, _VP, _VP, _VP, _VP, _VP], _I),
     "afx_autotune": ([_VP, _I, _I], _I),
     "afx_kernel_timing": ([_VP, _I, _VP, _VP, _VP], _I),
@@ -415,6 +416,15 @@ class System:
         sc = None if omega_scale is None else _f64(omega_scale)
         _err(lib().afx_true_residual_device(self.h, k, C.c_void_p(dB.data_ptr()), C.c_void_p(dX.data_ptr()),
                                             _p(sc), _p(res)))
+        return res
+
+    def true_residual_device_noalloc(self, dB, dX, omega_scale=None):
+        """afx_true_residual_device_noalloc: the same norms with the library's static scratch."""
+        k = 1 if dB.dim() == 1 else dB.shape[1]
+        res = np.zeros(k)
+        sc = None if omega_scale is None else _f64(omega_scale)
+        _err(lib().afx_true_residual_device_noalloc(self.h, k, C.c_void_p(dB.data_ptr()), C.c_void_p(dX.data_ptr()),
+                                                    _p(sc), _p(res)))
         return res
 
     def scale_initial_guess_device(self, dB, dX, omega_scale=None):

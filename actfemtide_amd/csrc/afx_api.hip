@@ -1970,6 +1970,86 @@ int afx_solve(afx_system s, int nrhs, const double *B, int ldb, double *X, int l
   return rc;
 }
 
+}  // extern "C"
+
+// ---- the residual norms without allocation --------------------------------------------------
+// Device scratch of afx_true_residual_device_noalloc: the column omegas [AFX_NOALLOC_MAX_COLS],
+// then the per-block partial sums of ||b - A x||^2 and ||b||^2 [AFX_NOALLOC_MAX_GRID][nrhs] each,
+// then the finished norms [2 nrhs]. One copy per device, held until the process ends.
+#define AFX_NOALLOC_MAX_COLS 64
+#define AFX_NOALLOC_MAX_GRID 1024
+#define AFX_NOALLOC_SCRATCH_DOUBLES \
+  (AFX_NOALLOC_MAX_COLS + 2 * AFX_NOALLOC_MAX_GRID * AFX_NOALLOC_MAX_COLS + 2 * AFX_NOALLOC_MAX_COLS)
+static __device__ __align__(256) double scratch[AFX_NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+// the device address of `scratch` on each device, looked up once per device (the symbol's host
+// name is a stand-in, not a device address)
+std::mutex g_scratch_mu;
+double *g_scratch_dev[64] = {nullptr};
+
+int scratch_for_current_device(double **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(AFX_ERR_ARG, "device id beyond 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_dev[dev] = (double *)ptr;
+  }
+  *out = g_scratch_dev[dev];
+  return AFX_OK;
+}
+
+struct OmegaArg {
+  double w[AFX_NOALLOC_MAX_COLS];
+};
+__global__ void k_put_omegas(int k, OmegaArg a, double *out) {
+  const int c = threadIdx.x;
+  if (c < k) out[c] = a.w[c];
+}
+}  // namespace
+
+extern "C" {
+
+int afx_true_residual_device_noalloc(afx_system s, int nrhs, const void *dB, const void *dX,
+                                     const double *omega_scale, double *res) {
+  if (!s || !dB || !dX || !res || nrhs <= 0) return fail(AFX_ERR_ARG, "bad arguments");
+  if (nrhs > AFX_NOALLOC_MAX_COLS) return fail(AFX_ERR_ARG, "afx_true_residual_device_noalloc: more than 64 columns");
+  if (!s->have_values) return fail(AFX_ERR_ARG, "no matrix values");
+  const int G = std::min(2 * s->ncu, AFX_NOALLOC_MAX_GRID);
+  if (G <= 0) return fail(AFX_ERR_ARG, "no compute units");
+  double *sc = nullptr;
+  int rc = scratch_for_current_device(&sc);
+  if (rc) return rc;
+  if ((rc = ensure_derived(s))) return rc;  // value gathers only (kernels, no allocation)
+  OmegaArg a{};
+  for (int c = 0; c < nrhs; c++) a.w[c] = s->omega * (omega_scale ? omega_scale[c] : 1.0);
+  double *dw = sc, *pa = sc + AFX_NOALLOC_MAX_COLS, *pb = pa + (size_t)G * nrhs, *norms = pb + (size_t)G * nrhs;
+  hipLaunchKernelGGL(k_put_omegas, dim3(1), dim3(AFX_NOALLOC_MAX_COLS), 0, s->stream, nrhs, a, dw);
+  HIPCHK(hipGetLastError());
+  afx::SpmvParams sp{};
+  sp.n = s->n;
+  sp.k = nrhs;
+  sp.kp = nrhs > 1 ? next_pow2(nrhs) : 1;
+  sp.rowptr = s->d_rowptr;
+  sp.col = s->d_col;
+  sp.val = s->d_val64;
+  sp.wcol = dw;
+  sp.X = (const cx<double> *)dX;
+  sp.B = (const cx<double> *)dB;
+  sp.Y = nullptr;  // only the norms are kept
+  sp.part_rr = pa;
+  sp.part_bb = pb;
+  HIPCHK(afx::launch_residual_norms(sp, G, norms, s->stream));
+  double h[2 * AFX_NOALLOC_MAX_COLS];
+  HIPCHK(hipMemcpyAsync(h, norms, 2 * (size_t)nrhs * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  for (int c = 0; c < nrhs; c++) res[c] = (h[2 * c + 1] > 0) ? std::sqrt(h[2 * c] / h[2 * c + 1]) : 0.0;
+  return AFX_OK;
+}
+
 int afx_true_residual_device(afx_system s, int nrhs, const void *dB, const void *dX, const double *omega_scale,
                              double *res) {
   if (!s || !dB || !dX || !res || nrhs <= 0) return fail(AFX_ERR_ARG, "bad arguments");

@@ -1369,7 +1369,7 @@ __global__ void __launch_bounds__(256) k_spmv(SpmvParams p) {
         y = b - acc;
         bb += b.r * b.r + b.i * b.i;
       }
-      p.Y[ix] = y;
+      if (p.Y) p.Y[ix] = y;
       rr += y.r * y.r + y.i * y.i;
     }
   }

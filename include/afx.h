@@ -182,6 +182,15 @@ int afx_solve_stacked_device(afx_system s, const void *dB, void *dX, const afx_o
 int afx_true_residual_device(afx_system s, int nrhs, const void *dB, const void *dX,
                              const double *omega_scale, double *res);
 
+/* afx_true_residual_device without any allocation (no counterpart in the reference: for callers
+   that cannot allocate device memory -- code under HIP graph capture, tight serving loops,
+   harnesses that forbid hipMalloc). Same arguments and results; the per-block partial sums and
+   the column omegas live in a static device array of the library (about 1 MB, one per device,
+   kept until the process ends). At most 64 columns (AFX_ERR_ARG beyond). Every call on a device
+   shares that array: calls that overlap on one device race, so serialise them (one stream). */
+int afx_true_residual_device_noalloc(afx_system s, int nrhs, const void *dB, const void *dX,
+                                     const double *omega_scale, double *res);
+
 /* Minimal-residual scaling of device initial guesses X (row-interleaved, nrhs columns), for the
    learned warm start (no reference counterpart; the reference always starts COCG13_single from
    the X it is given, COCG13_single.f90:139-145). Per column: y = A x, gamma = y^H b / y^H y;

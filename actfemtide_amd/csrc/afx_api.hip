@@ -158,6 +158,7 @@ struct afx_system_s {
   // end nodes of each edge, and diag(G^T M G) of the current values
   int nn = -1;
   int *d_nptr = nullptr, *d_nadj = nullptr, *d_en = nullptr;
+  int *d_noff = nullptr;  // stacked systems: first interior node of each system [nsys + 1]
   double *d_mnd = nullptr;
 };
 
@@ -822,8 +823,8 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   const bool multi = k > 1;
   const bool grad = (pc & AFX_PC_GRAD) != 0;
   pc &= 0xff;
-  if (grad && ((stacked && s->nsys > 1) || s->nn < 0))
-    return fail(AFX_ERR_ARG, "AFX_PC_GRAD: an unstacked system with afx_set_nodes");
+  if (grad && (s->nn < 0 || (stacked && s->nsys > 1 && !s->d_noff)))
+    return fail(AFX_ERR_ARG, "AFX_PC_GRAD needs afx_set_nodes (on a stacked system, nodes that do not couple two systems)");
   // stacked systems: one column, per-system scalars and convergence; stats and host_conv then
   // hold one entry per system
   const bool sys = stacked && s->nsys > 1;
@@ -905,6 +906,7 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   p.nptr = s->d_nptr;
   p.nadj = s->d_nadj;
   p.en = s->d_en;
+  p.noff = s->d_noff;
   p.mnd = s->d_mnd;
   if (grad) {  // [nn + 1][k]: the last row (boundary nodes) stays zero
     if ((rc = dalloc(&w.Zn, ((size_t)s->nn + 1) * k))) return rc;
@@ -1097,7 +1099,7 @@ void release(afx_system s) {
   free_side(s->bjbw);
   dfree(s->d_rstart); dfree(s->d_bar); dfree(s->d_err);
   dfree(s->d_sys_off); dfree(s->d_bsys); dfree(s->d_sblk);
-  dfree(s->d_nptr); dfree(s->d_nadj); dfree(s->d_en); dfree(s->d_mnd);
+  dfree(s->d_nptr); dfree(s->d_nadj); dfree(s->d_en); dfree(s->d_mnd); dfree(s->d_noff);
   if (s->stack) afx_destroy(s->stack);
   s->stack = nullptr;
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -1785,9 +1787,36 @@ int afx_set_nodes(afx_system s, int node, const int *line, const unsigned char *
     if (en[2 * e] < nn) adj[fill[en[2 * e]]++] = -(e + 1);
     if (en[2 * e + 1] < nn) adj[fill[en[2 * e + 1]]++] = e + 1;
   }
-  dfree(s->d_nptr); dfree(s->d_nadj); dfree(s->d_en); dfree(s->d_mnd);
+  // stacked systems: each system's interior nodes must form one contiguous range that no other
+  // system's edges touch (the parts' node ids concatenated in system order)
+  std::vector<int> noff;
+  if (s->nsys > 1) {
+    noff.assign(s->nsys + 1, 0);
+    std::vector<int> lo(s->nsys, nn), hi(s->nsys, -1);
+    for (int q = 0; q < s->nsys; q++)
+      for (int e = s->sys_off[q]; e < s->sys_off[q + 1]; e++)
+        for (int j = 0; j < 2; j++)
+          if (en[2 * e + j] < nn) {
+            lo[q] = std::min(lo[q], en[2 * e + j]);
+            hi[q] = std::max(hi[q], en[2 * e + j]);
+          }
+    int next = 0;
+    for (int q = 0; q < s->nsys; q++) {
+      if (hi[q] < 0) {  // no interior node
+        noff[q] = next;
+        continue;
+      }
+      if (lo[q] != next) return fail(AFX_ERR_ARG, "afx_set_nodes: a stacked system's nodes are not contiguous");
+      noff[q] = lo[q];
+      next = hi[q] + 1;
+    }
+    noff[s->nsys] = next;
+    if (next != nn) return fail(AFX_ERR_ARG, "afx_set_nodes: interior nodes outside the stacked systems");
+  }
+  dfree(s->d_nptr); dfree(s->d_nadj); dfree(s->d_en); dfree(s->d_mnd); dfree(s->d_noff);
   s->nn = -1;
   int rc;
+  if (!noff.empty() && (rc = upload(&s->d_noff, noff.data(), noff.size()))) return rc;
   if ((rc = upload(&s->d_nptr, cnt.data(), cnt.size())) || (rc = upload(&s->d_nadj, adj.data(), adj.size())) ||
       (rc = upload(&s->d_en, en.data(), en.size())) || (rc = dalloc(&s->d_mnd, (size_t)nn + 1)))
     return rc;

@@ -118,6 +118,7 @@ struct CocgParams {
   int grad;
   int nn;
   const int *nptr, *nadj, *en;
+  const int *noff;  // stacked systems: system q's interior nodes are [noff[q], noff[q+1])
   const double *mnd;
   cx<double> *Zn;
 };

@@ -793,7 +793,12 @@ __device__ __forceinline__ void cocg_body(const CocgParams<TP> &p) {
     // rho = r^T z, r^T G zn = (G^T r)^T zn; the edges add G zn in the p update, after the rho
     // barrier has published every block's zn
     if (!MULTI && p.grad) {
-      for (long long a = gtid; a < p.nn; a += nthr) {
+      // stacked systems: the nodes of the block's own system, split over that system's blocks (its
+      // rho sums only its own blocks' partials)
+      const long long a0 = sys ? p.noff[cst] + (long long)(pb - g0) * 256 + t : gtid;
+      const long long a1 = sys ? p.noff[cst + 1] : p.nn;
+      const long long astep = sys ? (long long)(g1 - g0) * 256 : nthr;
+      for (long long a = a0; a < a1; a += astep) {
         const cx<double> g = nodal_g(p.nptr, p.nadj, p.R, (int)a);
         const cx<double> zn = nodal_z(g, w, p.mnd[a]);
         p.Zn[a] = zn;

@@ -129,6 +129,18 @@ class MeshBatch:
             system.set_omega(omega0)
         return cls([], True, patterns=[pat] * len(vals), values=vals)
 
+    def set_nodes(self, node_counts, lines, dirichlets):
+        """Edge-to-node incidence of the stack for the nodal gradient-space correction
+        (afx_set_nodes): each part's line (nline_s, 2) 1-based and Dirichlet mask, its node ids
+        offset by the nodes of the parts before it, so the stacked G is block diagonal too."""
+        if not (len(node_counts) == len(lines) == len(dirichlets) == self.m):
+            raise ValueError("one node count, line array and Dirichlet mask per part")
+        off, ls = 0, []
+        for nc, ln in zip(node_counts, lines):
+            ls.append(np.asarray(ln, np.int64).reshape(-1, 2) + off)
+            off += int(nc)
+        self.system.set_nodes(off, np.concatenate(ls), np.concatenate([np.asarray(d, bool) for d in dirichlets]))
+
     @property
     def m(self) -> int:
         return len(self.sizes)

@@ -73,3 +73,24 @@ def seafloor_sites(case, n, seed):
             fy = min(0.87, fy + 0.1)
         pts.append([-xout + (i + fx) * h, -yout + (j + fy) * h, -h - 0.001])
     return np.array(pts)
+
+
+def near_site_params(case, sites, nsel, seed, radius=1.0, layers=1):
+    """Parameters = crust cells (6 tets each) of the top `layers` crust layers whose centre lies
+    within `radius` cells (horizontally) of a site, nsel of them seeded: the cells whose
+    conductivity the seafloor fields are most sensitive to. Returns (par_ptr, par_tet, cell ids)."""
+    crust = np.nonzero(case.group >= 3)[0]
+    cells, inv = np.unique(case.cell_of_tet[crust], return_inverse=True)
+    ci, cj, ck = cells % case.nx, (cells // case.nx) % case.ny, cells // (case.nx * case.ny)
+    xc = -case.nx * case.h / 2 + (ci + 0.5) * case.h
+    yc = -case.ny * case.h / 2 + (cj + 0.5) * case.h
+    d = np.min(np.hypot(xc[:, None] - sites[None, :, 0], yc[:, None] - sites[None, :, 1]), axis=1)
+    ok = np.nonzero((d <= radius * case.h) & (ck >= ck.max() - layers + 1))[0]
+    rng = np.random.default_rng(seed)
+    pick = np.sort(rng.choice(ok, min(nsel, len(ok)), replace=False))
+    order = np.argsort(inv, kind="stable")
+    tets = crust[order]
+    ptr = np.concatenate([[0], np.cumsum(np.bincount(inv))])
+    sel = [tets[ptr[m]:ptr[m + 1]] for m in pick]
+    return (np.concatenate([[0], np.cumsum([len(x) for x in sel])]).astype(np.int32),
+            np.concatenate(sel).astype(np.int32), cells[pick])

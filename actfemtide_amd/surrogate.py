@@ -58,6 +58,7 @@ class Dataset:
         xout, yout, zmin, zmax = base.bounds
         self.s.assemble(base.sigma_tet(), omega)
         self.bc = self.s.genbc(base.line, zmin, zmax, xout, yout)
+        self.s.set_nodes(base.node, base.line, self.bc)  # the nodal correction of DIAG+GRAD
         self.bc_dev = torch.from_numpy(np.ascontiguousarray(self.bc, np.uint8)).to(self.dev)
         self.maps = nn.GridMaps(base, device=device)
         self.scale = None
@@ -74,7 +75,7 @@ class Dataset:
         f = nn.cell_features(case, self.cin)
         return torch.from_numpy(f).to(self.dev).to(torch.bfloat16)
 
-    def add(self, seed, tol=1e-6, maxit=200000, precond="SSOR"):
+    def add(self, seed, tol=1e-6, maxit=200000, precond="DIAG+GRAD"):
         import torch
         case = grf_case(self.grid, seed)
         b = self.system_for(case)
@@ -142,7 +143,7 @@ def predict_x0(net: nn.Surrogate, data: Dataset, feat, air=False):
     return data.maps.cells_to_edges(out, data.scale, ldf=32, zero=zero)
 
 
-def warm_solve(net, data: Dataset, case, tol=1e-8, precond="SSOR", maxit=400000, accept=0.5, air=False):
+def warm_solve(net, data: Dataset, case, tol=1e-8, precond="DIAG+GRAD", maxit=400000, accept=0.5, air=False):
     """Solve `case` cold (X = 0) and warm (surrogate X, minimal-residual scaled) and return a dict
     with iterations, times, initial residuals and the FP64 true residuals of both. The warm start
     is used only when its scaled relative residual is at most `accept`; otherwise the solve

@@ -40,3 +40,62 @@ def test_surrogate_train_and_warm_start():
         tot_w += r["iters_warm"]
     print(f"iterations cold {tot_c} warm {tot_w}: gain {1 - tot_w / tot_c:+.3f}")
     assert tot_w <= tot_c * 1.02 + 5
+
+
+def test_surrogate_at_config4_shape():
+    """The surrogate of the learned-warm-start configuration: 32x32x24 cells, 4 levels, 32 base
+    channels, BF16 weights with FP32 accumulation.
+    - forward and backward of the whole network against the torch-CPU model on the same bf16
+      weights (one sample of the real grid);
+    - a short training on models solved by this library (16 training models instead of the
+      configuration's 2,000: the test budget), then warm-started solves of 2 held-out models
+      against cold ones, every solve to a 1e-8 FP64 true residual; the gain is printed."""
+    import torch
+    from actfemtide_amd import nn, surrogate as S
+    from tests.test_gpu_nn import _RefNet, _bf, _ncdhw, _rel
+    grid = (32, 32, 24)
+    dims = (1, grid[2], grid[1], grid[0])
+    V = grid[0] * grid[1] * grid[2]
+    net = nn.Surrogate(base=32, cin=8, seed=2)
+    g = torch.Generator().manual_seed(3)
+    x = _bf(torch.randn(V, 8, generator=g))
+    out = net.forward(x.to(torch.bfloat16).cuda(), dims)
+    tgt = torch.randn(V, 6, generator=g)
+    loss, grad = net.loss(out, tgt.cuda())
+    net.backward(grad)
+    torch.cuda.synchronize()
+    ref = _RefNet(net)
+    ro = ref(_ncdhw(x, dims))
+    rl = ((ro - tgt) ** 2).mean()
+    rl.backward()
+    err_o = _rel(out.cpu()[:, :6], ro.detach())
+    worst, cos_min = 0.0, 1.0
+    for name, P in net.named_params():
+        if not name.endswith(".b"):
+            continue
+        gg = P.g.cpu()[:6] if name.startswith("head") else P.g.cpu()
+        gr = ref.P[name].grad
+        worst = max(worst, _rel(gg, gr))
+        cos_min = min(cos_min, float((gg.double() * gr.double()).sum() / (gg.double().norm() * gr.double().norm())))
+    print(f"32x32x24 base 32: output rel err {err_o:.2e}, loss {float(loss.cpu()[0]):.5f} vs {float(rl):.5f}, "
+          f"bias gradients worst rel {worst:.2e}, cos >= {cos_min:.5f}", flush=True)
+    assert err_o < 3e-2 and abs(float(loss.cpu()[0]) - float(rl)) < 3e-2 * float(rl)
+    assert worst < 8e-2 and cos_min > 0.996
+    # training on 16 solved models, warm starts on 2 held-out ones
+    w = 2 * math.pi / 44714.0
+    d = S.Dataset(grid, w)
+    for i in range(16):
+        d.add(5000 + i, tol=1e-6)
+    net = nn.Surrogate(base=32, cin=8, seed=0)
+    log = S.train(net, d, range(16), steps=150, batch=2, lr=2e-3, log_every=50)
+    print("loss", log.losses, f"{1e3 * log.seconds / log.steps:.1f} ms per Adam step", flush=True)
+    assert log.losses[-1][1] < log.losses[0][1]
+    tot_c = tot_w = 0
+    for k in range(2):
+        r = S.warm_solve(net, d, S.grf_case(grid, 9100 + k), tol=1e-8)
+        print({key: (round(v, 6) if isinstance(v, float) else v) for key, v in r.items()}, flush=True)
+        assert r["true_cold"] <= 1.05e-8 and r["true_warm"] <= 1.05e-8
+        tot_c += r["iters_cold"]
+        tot_w += r["iters_warm"]
+    print(f"held-out: iterations cold {tot_c} warm {tot_w}, gain {1 - tot_w / tot_c:+.3f}")
+    assert tot_w <= tot_c * 1.02 + 5

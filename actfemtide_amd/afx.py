@@ -99,6 +99,10 @@ SIGNATURES = {
                            C.POINTER(_D), C.POINTER(_I), C.POINTER(_I), _I], _I),
     "afx_cocg13_single_release": ([], _I),
     "afx_solve_cg": ([_I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP], _I),
+    "afx_solve_pardiso": ([_I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, _D], _I),
+    "afx_solve_pardiso_inv": ([_I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, _I, _VP, _VP, _VP,
+                               _VP, _VP, _I, _VP, _VP, _VP, _VP, _D], _I),
+    "afx_solve_pardiso_release": ([], _I),
     "afx_prepobscoeff": ([_VP, _I, _VP, _VP, _VP, _VP, _VP], _I),
     "afx_obs_apply_device": ([_VP, _I, _VP, _VP, _I, _VP, _VP], _I),
     "afx_ixyh": ([_I, _VP, _VP, _VP, _VP, _VP, _I, _VP, _VP, _D, _VP, _VP], _I),
@@ -153,6 +157,7 @@ def _release_at_exit():
         except Exception:
             pass
     _lib.afx_cocg13_single_release()
+    _lib.afx_solve_pardiso_release()
 
 
 def _err(code):
@@ -513,6 +518,83 @@ def solve_cg(D, AL, INL, IAL, AU, INU, IAU, rhs):
     if rc not in (AFX_OK, AFX_ERR_MAXIT):
         _err(rc)
     return X
+
+
+def solve_pardiso(D, AU, AL, INU, IAU, INL, IAL, rhs, node=0, line=None, line_bc=None, tol=1e-12):
+    """solvePARDISO (src/solver/solvePARDISO.f90:6-16) through afx_solve_pardiso on host arrays:
+    rhs (n, nsr) or (n,) complex; returns xout of the same shape. With the edges' end nodes
+    (line (nline, 2) 1-based), the node count and the Dirichlet mask, DIAG + the nodal correction,
+    else SSOR; FP64 COCG to the true residual tol. Raises on AFX_ERR_*; -100 leaves xout as reached
+    and is returned as the second value."""
+    D, AU, AL = _c128(D), _c128(AU), _c128(AL)
+    INU, IAU, INL, IAL = _i32(INU), _i32(IAU), _i32(INL), _i32(IAL)
+    r = np.asfortranarray(np.asarray(rhs, np.complex128).reshape(len(D), -1, order="F"))
+    x = np.zeros_like(r, order="F")
+    ln = None if line is None else _i32(line).reshape(-1, 2)
+    bc = None if line_bc is None else np.ascontiguousarray(line_bc, dtype=np.uint8)
+    rc = lib().afx_solve_pardiso(len(D), r.shape[1], _p(INU), _p(IAU), _p(INL), _p(IAL), _p(D), _p(AU), _p(AL),
+                                 int(node), _p(ln), _p(bc), C.c_void_p(r.ctypes.data), C.c_void_p(x.ctypes.data),
+                                 float(tol))
+    if rc not in (AFX_OK, AFX_ERR_MAXIT):
+        _err(rc)
+    return (x[:, 0] if np.ndim(rhs) == 1 else x), rc
+
+
+def solve_pardiso_inv_host(D, AU, AL, INU, IAU, INL, IAL, rhs, PTR, iflag_comp=None, node=0, line=None,
+                           line_bc=None, tol=1e-12, ut_cap=None):
+    """solvePARDISOinv (src/src_inv/solvePARDISOinv.f90:5-23) through afx_solve_pardiso_inv on host
+    arrays: PTR a list of 5 observation matrices (stack 0-based, item 1-based, val) or None, all
+    with the same row count. Returns (xout, ut, rc) with ut[c] a scipy CSR [nobs x n] (0 when not
+    flagged)."""
+    import scipy.sparse as sp
+    D, AU, AL = _c128(D), _c128(AU), _c128(AL)
+    INU, IAU, INL, IAL = _i32(INU), _i32(IAU), _i32(INL), _i32(IAL)
+    n = len(D)
+    r = np.asfortranarray(np.asarray(rhs, np.complex128).reshape(n, -1, order="F"))
+    x = np.zeros_like(r, order="F")
+    flags = np.array([0 if (P is None or (iflag_comp is not None and not iflag_comp[c])) else 1
+                      for c, P in enumerate(list(PTR) + [None] * (5 - len(PTR)))], np.int32)
+    Ps = list(PTR) + [None] * (5 - len(PTR))
+    nobs = max((len(P[0]) - 1 for P in Ps if P is not None), default=0)
+    stacks = np.zeros((5, nobs + 1), np.int32)
+    base = np.zeros(5, np.int32)
+    items, vals, off = [], [], 0
+    for c, P in enumerate(Ps):
+        if P is None:
+            continue
+        st, it, va = _i32(P[0]), _i32(P[1]), _f64(P[2])
+        stacks[c] = st
+        base[c] = off
+        items.append(it)
+        vals.append(va)
+        off += len(it)
+    item = _i32(np.concatenate(items)) if items else np.zeros(1, np.int32)
+    val = _f64(np.concatenate(vals)) if vals else np.zeros(1)
+    ln = None if line is None else _i32(line).reshape(-1, 2)
+    bc = None if line_bc is None else np.ascontiguousarray(line_bc, dtype=np.uint8)
+    cap = int(ut_cap) if ut_cap is not None else nobs * n
+    ust = np.zeros((5, nobs + 1), np.int32)
+    ntot = np.zeros(5, np.int32)
+    uit = np.zeros(5 * max(cap, 1), np.int32)
+    uva = np.zeros(5 * max(cap, 1), np.complex128)
+    rc = lib().afx_solve_pardiso_inv(n, r.shape[1], _p(INU), _p(IAU), _p(INL), _p(IAL), _p(D), _p(AU), _p(AL),
+                                     int(node), _p(ln), _p(bc), C.c_void_p(r.ctypes.data), C.c_void_p(x.ctypes.data),
+                                     nobs, _p(flags), _p(stacks), _p(base), _p(item), _p(val), cap, _p(ust), _p(ntot),
+                                     _p(uit), _p(uva), float(tol))
+    if rc not in (AFX_OK, AFX_ERR_MAXIT):
+        _err(rc)
+    ut = []
+    for c in range(5):
+        if not flags[c]:
+            ut.append(0)
+            continue
+        k = int(ntot[c])
+        ut.append(sp.csr_matrix((uva[c * cap:c * cap + k], uit[c * cap:c * cap + k] - 1, ust[c]), shape=(nobs, n)))
+    return x, ut, rc
+
+
+def solve_pardiso_release():
+    _err(lib().afx_solve_pardiso_release())
 
 
 def solve_pardiso_inv(s: System, rhs, PTR, iflag_comp=None, threshold=1e-10, max_cols=256, autotune=False,

@@ -2524,6 +2524,171 @@ int afx_cocg13_single(int NP, int NPL, int NPU, double *D, double *AL, const int
   return rc;
 }
 
+// ---- solvePARDISO / solvePARDISOinv shaped entries -------------------------------------------
+// The system of the last call is kept and reused while the pattern is the same (a per-frequency
+// caller repeats the pattern analysis only when the mesh changes); calls are serialised.
+static std::mutex g_pds_mu;
+static afx_system g_pds_sys = nullptr;
+
+// Pattern, values, node incidence and preconditioner of the kept system; *pc receives the
+// preconditioner: DIAG with the nodal correction when the edges' end nodes are given, else SSOR.
+static int pardiso_system(int doftot, const int *INU, const int *IAU, const int *INL, const int *IAL, const double *D,
+                          const double *AU, const double *AL, int node, const int *line,
+                          const unsigned char *line_bc, int *pc) {
+  if (!INU || !INL || !D || doftot <= 0) return fail(AFX_ERR_ARG, "bad arguments");
+  bool fresh = false;
+  if (!same_pattern(g_pds_sys, doftot, INU, IAU, INL, IAL)) {
+    if (g_pds_sys) afx_destroy(g_pds_sys);
+    g_pds_sys = nullptr;
+    int rc = afx_create(doftot, INU, IAU, INL, IAL, &g_pds_sys);
+    if (rc) {
+      g_pds_sys = nullptr;
+      return rc;
+    }
+    fresh = true;
+  }
+  afx_system s = g_pds_sys;
+  int rc = afx_set_values(s, D, AU, AL);
+  if (rc) return rc;
+  *pc = AFX_PC_SSOR;
+  if (line && node > 0) {
+    if ((rc = afx_set_nodes(s, node, line, line_bc))) return rc;
+    *pc = AFX_PC_DIAG | AFX_PC_GRAD;
+  }
+  if ((rc = afx_precond_setup(s, *pc, 1.0, 0.0))) return rc;
+  if (fresh) rc = afx_autotune(s, 1, 0);
+  return rc;
+}
+
+int afx_solve_pardiso_release(void) {
+  std::lock_guard<std::mutex> lk(g_pds_mu);
+  int rc = g_pds_sys ? afx_destroy(g_pds_sys) : AFX_OK;
+  g_pds_sys = nullptr;
+  return rc;
+}
+
+int afx_solve_pardiso(int doftot, int nsr, const int *INU, const int *IAU, const int *INL, const int *IAL,
+                      const double *D, const double *AU, const double *AL, int node, const int *line,
+                      const unsigned char *line_bc, const double *rhs, double *xout, double tol) {
+  if (nsr <= 0 || !rhs || !xout || !(tol > 0.0)) return fail(AFX_ERR_ARG, "bad arguments");
+  std::lock_guard<std::mutex> lk(g_pds_mu);
+  int pc = 0;
+  int rc = pardiso_system(doftot, INU, IAU, INL, IAL, D, AU, AL, node, line, line_bc, &pc);
+  if (rc) return rc;
+  afx_opts o;
+  afx_default_opts(&o);
+  o.precond = pc;
+  o.tol = tol;
+  o.maxit = 400000;
+  o.check_true = 1;
+  o.max_outer = 4;
+  std::fill(xout, xout + 2 * (size_t)doftot * nsr, 0.0);
+  std::vector<afx_stats> st(nsr);
+  return afx_solve(g_pds_sys, nsr, rhs, doftot, xout, doftot, &o, st.data());
+}
+
+int afx_solve_pardiso_inv(int doftot, int nsr, const int *INU, const int *IAU, const int *INL, const int *IAL,
+                          const double *D, const double *AU, const double *AL, int node, const int *line,
+                          const unsigned char *line_bc, const double *rhs, double *xout, int nobs,
+                          const int *iflag_comp, const int *ptr_stack, const int *ptr_base, const int *ptr_item,
+                          const double *ptr_val, int ut_cap, int *ut_stack, int *ut_ntot, int *ut_item, double *ut_val,
+                          double tol) {
+  if (nsr < 0 || nobs < 0 || !iflag_comp || !ut_ntot || !(tol > 0.0) || ut_cap < 0)
+    return fail(AFX_ERR_ARG, "bad arguments");
+  std::lock_guard<std::mutex> lk(g_pds_mu);
+  int pc = 0;
+  int rc = pardiso_system(doftot, INU, IAU, INL, IAL, D, AU, AL, node, line, line_bc, &pc);
+  if (rc) return rc;
+  afx_system s = g_pds_sys;
+  afx_opts o;
+  afx_default_opts(&o);
+  o.precond = pc;
+  o.tol = tol;
+  o.maxit = 400000;
+  o.check_true = 1;
+  o.max_outer = 4;
+  int ret = AFX_OK;
+  // the forward solve (solvePARDISOinv.f90:157-164)
+  if (nsr > 0) {
+    if (!rhs || !xout) return fail(AFX_ERR_ARG, "bad arguments");
+    std::fill(xout, xout + 2 * (size_t)doftot * nsr, 0.0);
+    std::vector<afx_stats> st(nsr);
+    rc = afx_solve(s, nsr, rhs, doftot, xout, doftot, &o, st.data());
+    if (rc != AFX_OK && rc != AFX_ERR_MAXIT) return rc;
+    ret = rc;
+  }
+  for (int c = 0; c < 5; c++) ut_ntot[c] = 0;
+  if (nobs == 0) return ret;
+  if (!ptr_stack || !ptr_base || !ptr_item || !ptr_val || !ut_stack) return fail(AFX_ERR_ARG, "bad arguments");
+  // per flagged component: u_o = A^-1 P_o^T for the nobs rows of PTR(c) (:176-191) in blocks of
+  // at most 64 columns, kept where |u| > 1e-10 as the CRS of u^T (convcomp_full2crs and its
+  // transposition, :194-196; m_matrix.f90:405-451)
+  const double threshold = 1.0e-10;
+  const int blk = std::min(nobs, 64);
+  cx<double> *dB = nullptr, *dX = nullptr;
+  if ((rc = dalloc(&dB, (size_t)doftot * blk)) || (rc = dalloc(&dX, (size_t)doftot * blk))) {
+    dfree(dB);
+    return rc;
+  }
+  std::vector<cx<double>> U((size_t)doftot * blk);
+  bool short_cap = false;
+  for (int c = 0; c < 5 && rc == AFX_OK; c++) {
+    if (!iflag_comp[c]) continue;
+    const int *stk = ptr_stack + (size_t)c * (nobs + 1);
+    int *ust = ut_stack + (size_t)c * (nobs + 1);
+    int *uit = ut_item + (size_t)c * ut_cap;
+    double *uva = ut_val + 2 * (size_t)c * ut_cap;
+    long long cnt = 0;
+    ust[0] = 0;
+    for (int o0 = 0; o0 < nobs && rc == AFX_OK; o0 += blk) {
+      const int k = std::min(blk, nobs - o0);
+      // the rows o0 .. o0 + k of PTR(c) as a CRS of their own
+      std::vector<int> st2(k + 1);
+      for (int q = 0; q <= k; q++) st2[q] = stk[o0 + q] - stk[o0];
+      const int *it2 = ptr_item + ptr_base[c] + stk[o0];
+      const double *va2 = ptr_val + ptr_base[c] + stk[o0];
+      if ((rc = afx_adjoint_rhs_device(s, k, st2.data(), it2, va2, dB))) break;
+      if (hipMemsetAsync(dX, 0, (size_t)doftot * k * sizeof(cx<double>), s->stream) != hipSuccess) {
+        rc = fail(AFX_ERR_HIP, "hipMemsetAsync");
+        break;
+      }
+      std::vector<afx_stats> st(k);
+      const int r = afx_solve_device(s, k, dB, dX, nullptr, &o, st.data());
+      if (r != AFX_OK && r != AFX_ERR_MAXIT) {
+        rc = r;
+        break;
+      }
+      if (r == AFX_ERR_MAXIT) ret = AFX_ERR_MAXIT;
+      if (hipMemcpy(U.data(), dX, (size_t)doftot * k * sizeof(cx<double>), hipMemcpyDeviceToHost) != hipSuccess) {
+        rc = fail(AFX_ERR_HIP, "hipMemcpy");
+        break;
+      }
+      for (int q = 0; q < k; q++) {  // row o0 + q of u^T: edges in ascending order
+        for (int e = 0; e < doftot; e++) {
+          const cx<double> v = U[(size_t)e * k + q];
+          if (std::hypot(v.r, v.i) > threshold) {
+            if (cnt < ut_cap) {
+              uit[cnt] = e + 1;
+              uva[2 * cnt] = v.r;
+              uva[2 * cnt + 1] = v.i;
+            }
+            cnt++;
+          }
+        }
+        ust[o0 + q + 1] = (int)std::min<long long>(cnt, 0x7fffffffLL);
+      }
+    }
+    if (cnt > 0x7fffffffLL) rc = fail(AFX_ERR_ARG, "ut: more than 2^31 entries in one component");
+    ut_ntot[c] = (int)std::min<long long>(cnt, 0x7fffffffLL);
+    if (cnt > ut_cap) short_cap = true;
+  }
+  dfree(dB);
+  dfree(dX);
+  if (rc) return rc;
+  if (short_cap) return fail(AFX_ERR_ARG, "ut_cap too small: ut_ntot holds the entries each component needs");
+  return ret;
+}
+
 int afx_solve_cg(int doftot, int NPL, int NPU, const double *D, const double *AL, const int *INL, const int *IAL,
                  const double *AU, const int *INU, const int *IAU, const double *rhs, double *X) {
   // solveCG (forward_bxyz_mpi.f90:70-94)

@@ -251,6 +251,41 @@ int afx_cocg13_single(int NP, int NPL, int NPU, double *D, double *AL, const int
    reference frees nothing either; for callers that want the device memory back). */
 int afx_cocg13_single_release(void);
 
+/* solvePARDISO(doftot, nsr, A, rhs, xout, ip) (src/solver/solvePARDISO.f90:6-16): A's split arrays
+   (type(global_matrix), m_iccg_var_takuto.f90:9-18), rhs(doftot, nsr) and xout(doftot, nsr)
+   column-major complex. The direct factorisation is replaced by FP64 COCG to the relative FP64
+   true residual tol (||b - A x|| / ||b|| per column, recomputed; PARDISO's own accuracy is about
+   1e-12); with the edges' end nodes line(2, nline) (READLINE's g_line%line, 1-based) and node count,
+   and the Dirichlet mask line_bc (GENBCCSEM's line_bc as one byte per edge), the preconditioner is
+   DIAG + AFX_PC_GRAD, otherwise SSOR (line = NULL). Returns 0, -100 (tol not reached) or an
+   AFX_ERR_* where the reference stops. The system of the last call is kept while the pattern is
+   the same (afx_solve_pardiso_release frees it); calls are serialised. ip is not needed. */
+int afx_solve_pardiso(int doftot, int nsr, const int *INU, const int *IAU, const int *INL, const int *IAL,
+                      const double *D, const double *AU, const double *AL, int node, const int *line,
+                      const unsigned char *line_bc, const double *rhs, double *xout, double tol);
+
+/* solvePARDISOinv(doftot, nsr, A, rhs, xout, PTR, ut, iflag_comp, ip, np)
+   (src/src_inv/solvePARDISOinv.f90:5-23): the forward solve as afx_solve_pardiso (nsr may be 0),
+   then for every component c < 5 with iflag_comp[c] != 0 the adjoint solutions u_o = A^-1 P_o^T of
+   the nobs rows of PTR(c) (:176-191), returned as ut(c), the CRS of u^T [nobs x doftot] keeping
+   |u| > 1e-10 (:43,194-196; convcomp_full2crs, m_matrix.f90:405-451). Every flagged component has
+   nobs rows (as the reference assumes). PTR(c): stack ptr_stack[c * (nobs + 1) + i] (0-based
+   offsets), item (1-based edge ids) and val at ptr_item / ptr_val + ptr_base[c]. ut(c): stack
+   ut_stack[c * (nobs + 1) + i], item (1-based) and complex val at ut_item + c * ut_cap and
+   ut_val + 2 * c * ut_cap, entry count ut_ntot[c]. When some component needs more than ut_cap
+   entries it returns AFX_ERR_ARG with ut_ntot holding the counts needed (call again with a larger
+   ut_cap). Same kept system and serialisation as afx_solve_pardiso. */
+int afx_solve_pardiso_inv(int doftot, int nsr, const int *INU, const int *IAU, const int *INL, const int *IAL,
+                          const double *D, const double *AU, const double *AL, int node, const int *line,
+                          const unsigned char *line_bc, const double *rhs, double *xout, int nobs,
+                          const int *iflag_comp, const int *ptr_stack, const int *ptr_base, const int *ptr_item,
+                          const double *ptr_val, int ut_cap, int *ut_stack, int *ut_ntot, int *ut_item, double *ut_val,
+                          double tol);
+
+/* Frees the system afx_solve_pardiso / afx_solve_pardiso_inv keep between calls (no reference
+   counterpart: PARDISO frees its factors at the end of each call, solvePARDISO.f90:165-166). */
+int afx_solve_pardiso_release(void);
+
 /* solveCG (src/solver_mpi/forward_bxyz_mpi.f90:70-94): X = 0, 'SSOR', RESID 1e-4, ITER 3000. */
 int afx_solve_cg(int doftot, int NPL, int NPU, const double *D, const double *AL, const int *INL,
                  const int *IAL, const double *AU, const int *INU, const int *IAU, const double *rhs, double *X);

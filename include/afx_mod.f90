@@ -6,7 +6,9 @@
 !
 !   call afx_cocg13_single_f(...)   replaces   call COCG13_single(...)   (COCG13_single.f90:9-36)
 !   call afx_solve_cg_f(...)        replaces   call solveCG(...)         (forward_bxyz_mpi.f90:70-94)
-!   afx_solve(...) on a handle      replaces   call solvePARDISO(...)    (solvePARDISO.f90:6-16)
+!   call afx_solvepardiso_f(...)    replaces   call solvePARDISO(...)    (solvePARDISO.f90:6-16)
+!   afx_solve_pardiso_inv(...)      replaces   call solvePARDISOinv(...) (solvePARDISOinv.f90:5-23)
+!   afx_solve(...) on a handle      keeps the system on the device across calls
 !   afx_ixyh(...)                   computes   IXYHOUT's seafloor currents (n_ebfem_bxyz_mpi.f90:192-268)
 module afx
   use iso_c_binding
@@ -16,6 +18,7 @@ module afx
   integer(c_int), parameter :: AFX_ERR_BREAKDOWN = -200, AFX_ERR_HIP = -300, AFX_ERR_ARG = -400
   integer(c_int), parameter :: AFX_PC_NONE = 0, AFX_PC_IC = 1, AFX_PC_SSOR = 2, AFX_PC_DIAG = 3
   integer(c_int), parameter :: AFX_FP64 = 0, AFX_MIXED = 1
+  integer(c_int), parameter :: AFX_PC_SSOR_BJ = 4, AFX_PC_GRAD = 256
 
   type, bind(C) :: afx_opts
     integer(c_int) :: precond
@@ -124,6 +127,50 @@ module afx
       complex(c_double_complex), intent(out) :: X(*)
     end function
 
+    ! edge end nodes line(2,nline) (g_line%line) and a one-byte Dirichlet mask per edge, for the
+    ! nodal correction (AFX_PC_GRAD)
+    integer(c_int) function afx_set_nodes(h, node, line, line_bc) bind(C, name="afx_set_nodes")
+      import :: c_int, c_ptr, c_signed_char
+      type(c_ptr), value :: h
+      integer(c_int), value :: node
+      integer(c_int), intent(in) :: line(*)
+      integer(c_signed_char), intent(in) :: line_bc(*)
+    end function
+
+    ! solvePARDISO(doftot, nsr, A, rhs, xout, ip) on A's arrays (see afx_solvepardiso_f)
+    integer(c_int) function afx_solve_pardiso(doftot, nsr, INU, IAU, INL, IAL, D, AU, AL, node, line, line_bc, &
+                                              rhs, xout, tol) bind(C, name="afx_solve_pardiso")
+      import :: c_int, c_double, c_double_complex, c_signed_char
+      integer(c_int), value :: doftot, nsr, node
+      integer(c_int), intent(in) :: INU(*), IAU(*), INL(*), IAL(*), line(*)
+      complex(c_double_complex), intent(in) :: D(*), AU(*), AL(*), rhs(*)
+      integer(c_signed_char), intent(in) :: line_bc(*)
+      complex(c_double_complex), intent(out) :: xout(*)
+      real(c_double), value :: tol
+    end function
+
+    ! solvePARDISOinv(doftot, nsr, A, rhs, xout, PTR, ut, iflag_comp, ip, np): PTR(c) and ut(c) as
+    ! plain CRS arrays (afx.h); ut_cap entries per component
+    integer(c_int) function afx_solve_pardiso_inv(doftot, nsr, INU, IAU, INL, IAL, D, AU, AL, node, line, line_bc, &
+                                                  rhs, xout, nobs, iflag_comp, ptr_stack, ptr_base, ptr_item, ptr_val, &
+                                                  ut_cap, ut_stack, ut_ntot, ut_item, ut_val, tol) &
+                                                  bind(C, name="afx_solve_pardiso_inv")
+      import :: c_int, c_double, c_double_complex, c_signed_char
+      integer(c_int), value :: doftot, nsr, node, nobs, ut_cap
+      integer(c_int), intent(in) :: INU(*), IAU(*), INL(*), IAL(*), line(*), iflag_comp(*)
+      integer(c_int), intent(in) :: ptr_stack(*), ptr_base(*), ptr_item(*)
+      real(c_double), intent(in) :: ptr_val(*)
+      complex(c_double_complex), intent(in) :: D(*), AU(*), AL(*), rhs(*)
+      integer(c_signed_char), intent(in) :: line_bc(*)
+      complex(c_double_complex), intent(out) :: xout(*), ut_val(*)
+      integer(c_int), intent(out) :: ut_stack(*), ut_ntot(*), ut_item(*)
+      real(c_double), value :: tol
+    end function
+
+    integer(c_int) function afx_solve_pardiso_release() bind(C, name="afx_solve_pardiso_release")
+      import :: c_int
+    end function
+
     ! IXYHOUT's arrays (n_ebfem_bxyz_mpi.f90:192-268) in the reference's own shapes:
     ! xyzobs(3,nobs), exy(2,nobs), vxyz(3,node), fxyz(3,node); coeff_vF as stack(0:nobs) from 0,
     ! item (1-based nodes), val. out(2,4,nobs) holds ixyh_e, ixyh_vF, ixyh, vh_obs for each point.
@@ -161,5 +208,24 @@ contains
                            RESID, ITER, ERROR, NSET)
     if (rc /= AFX_OK .and. rc /= AFX_ERR_MAXIT .and. rc /= AFX_ERR_BADOPT) ERROR = rc
   end subroutine afx_cocg13_single_f
+
+  ! The reference's solvePARDISO call on A's arrays: rhs(doftot,nsr) and xout(doftot,nsr) as
+  ! PARDISO takes them, line(2,nline) and line_bc(nline) (GENBCCSEM's logical mask) for the nodal
+  ! correction, FP64 COCG to a 1e-12 true residual. ierr = 0, -100 (not reached) or AFX_ERR_*.
+  subroutine afx_solvepardiso_f(doftot, nsr, D, AU, AL, INU, IAU, INL, IAL, node, line, line_bc, rhs, xout, ierr)
+    integer(4), intent(in) :: doftot, nsr, node
+    complex(8), intent(in) :: D(*), AU(*), AL(*)
+    integer(4), intent(in) :: INU(0:doftot), IAU(*), INL(0:doftot), IAL(*), line(2,doftot)
+    logical, intent(in) :: line_bc(doftot)
+    complex(8), intent(in) :: rhs(doftot,nsr)
+    complex(8), intent(out) :: xout(doftot,nsr)
+    integer(4), intent(out) :: ierr
+    integer(c_signed_char), allocatable :: bc(:)
+    allocate(bc(doftot))
+    bc = 0_c_signed_char
+    where (line_bc) bc = 1_c_signed_char
+    ierr = afx_solve_pardiso(doftot, nsr, INU, IAU, INL, IAL, D, AU, AL, node, line, bc, rhs, xout, 1.d-12)
+    deallocate(bc)
+  end subroutine afx_solvepardiso_f
 
 end module afx

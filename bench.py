@@ -22,9 +22,14 @@ differ.
 After the solves, the SpMV q = A p runs as a kernel of its own (k_spmv_stream, 50 timed launches
 on the solution vector); the line's "spmv" object gives its algorithmic GB/s against the HBM peak.
 
+By default the line also carries the batched forms (skipped with --no-variants): the 5 tidal
+constituents of this mesh (one column at a time, the k-column launch, and a warm chain where each
+constituent starts from the previous one's solution); solvePARDISOinv's 48 adjoint columns
+(16 seafloor sites x 3 electric components) on the 32x32x24-cell mesh, one column at a time
+against the autotuned block width; the 16-mesh uneven batch one mesh at a time against one
+per-system launch.
+
 Variants, off by default (their results are documented):
-  --adjoint32  12 adjoint right-hand sides (4 seafloor sites x 3 electric components) on the
-             32x32x24-cell mesh, one column at a time against the autotuned block width;
   --mixed    complex-FP32 preconditioner inside FP64 refinement (same tolerance);
   --batched  the 5 tidal constituents (M2, S2, N2, K1, O1) in one call, per-column omega (the
              autotuner picks one column at a time or the 5-column kernel);
@@ -34,7 +39,8 @@ Variants, off by default (their results are documented):
   --adjoint-batch SITES  SITES seeded seafloor sites x 3 electric components as adjoint
              right-hand sides (solvePARDISOinv's P^T columns) at M2: one column at a time against
              the autotuned block width (stacked copies in groups where they fit LDS);
-  --mesh-batch  16 independent meshes from 8x8x6 to 40x40x30 cells (1,830,720 edges in all), each
+  --mesh-batch  (also in the default line, per-system only) 16 independent meshes from 8x8x6 to
+             40x40x30 cells (1,830,720 edges in all), each
              with its own seeded crust, at M2: one solve per mesh against the stacked
              block-diagonal system in one launch (actfemtide_amd.meshbatch), with per-system and
              with shared COCG scalars, every mesh to a 1e-8 true residual each way.
@@ -158,9 +164,9 @@ def main():
     ap.add_argument("--batched", action="store_true", help="also time the 5 constituents in one launch")
     ap.add_argument("--jacobian", type=int, default=0, metavar="NOBS",
                     help="also time genjacobian1's accumulation over every crust cell for NOBS adjoint columns")
-    ap.add_argument("--adjoint32", action="store_true",
-                    help="also time 12 adjoint columns on 32x32x24 cells, one at a time against the autotuned "
-                         "block width")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the batched forms (5 constituents; 48 adjoint columns on 32x32x24 cells; the "
+                         "16-mesh batch)")
     ap.add_argument("--adjoint-batch", type=int, default=0, metavar="SITES",
                     help="also time SITES seafloor sites x 3 electric adjoint right-hand sides, one column at a "
                          "time against the autotuned block width")
@@ -239,7 +245,8 @@ def main():
     traffic, tsrc = measured_traffic(f"{nx}x{ny}x{nz}_{pc.replace('+', '_')}", ipl)
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                kernel=f"k_cocg (persistent COCG, FP64, 1 column, {pc})",
+                kernel=(f"k_cocg_flat (persistent COCG without sweeps, FP64, 1 column, {pc})"
+                        if base in ("DIAG", "NONE") else f"k_cocg (persistent COCG, FP64, 1 column, {pc})"),
                 bytes_per_iteration=by, avg_launch_ms=round(kms / max(launches, 1), 3),
                 iterations_per_launch=round(ipl, 1), us_per_iteration=round(1e3 * kms / max(iters, 1), 1),
                 traffic_source=tsrc)
@@ -331,24 +338,22 @@ def main():
         variants[f"jacobian_{args.jacobian}_columns"] = jacobian_variant(s, case, w0, args.jacobian)
     if args.adjoint_batch:
         variants[f"adjoint_{3 * args.adjoint_batch}_columns"] = adjoint_batch_variant(s, case, opts, args.adjoint_batch)
-    if args.adjoint32 and (nx, ny, nz) != SIZES["cfg3"] and time.time() - t_start < 300:
-        # solvePARDISOinv's adjoint block on the 32x32x24-cell mesh: 12 columns one at a time
-        # against the autotuned block width, with the measured preconditioner (a failure here is
-        # recorded, it does not end the run)
-        try:
-            c3 = mesh.tide_case(*SIZES["cfg3"], seed=1002, spheres=3, sigma_sea=3.3)
-            s3, _, _ = afx.forward_system(c3, w0)
-            s3.precond_setup(pc)
-            s3.autotune(1)
-            variants["adjoint_12_columns_32x32x24"] = adjoint_batch_variant(s3, c3, opts, 4)
-            s3.close()
-        except Exception as e:  # noqa: BLE001
-            variants["adjoint_12_columns_32x32x24"] = {"error": f"{type(e).__name__}: {e}"}
-        os.environ.pop("AFX_BATCH", None)
+    if not args.no_variants and time.time() - t_start < 300:
+        # the batched forms (a failure here is recorded, it does not end the run):
+        # the 5 constituents of this mesh; solvePARDISOinv's 48 adjoint columns (16 seafloor sites
+        # x 3 electric components) on the 32x32x24-cell mesh; the 16-mesh uneven batch
+        for name, fn in (("constituents_5", lambda: constituents_variant(s, b, omegas, w0, opts)),
+                         ("adjoint_48_columns_32x32x24", lambda: adjoint32(pc, opts, w0)),
+                         ("mesh_batch_16", lambda: mesh_batch_variant(opts, w0, grad_only=True))):
+            try:
+                variants[name] = fn()
+            except Exception as e:  # noqa: BLE001
+                variants[name] = {"error": f"{type(e).__name__}: {e}"}
+            os.environ.pop("AFX_BATCH", None)
     if args.rhs_stack:
         variants["stacked_5_constituents"] = rhs_stack_variant(s, b, omegas, w0, opts)
     if args.mesh_batch:
-        variants["mesh_batch_16"] = mesh_batch_variant(opts, w0)
+        variants["mesh_batch_16_all_modes"] = mesh_batch_variant(opts, w0)
 
     if args.copy_bytes:
         src = torch.empty(args.copy_bytes // 8, dtype=torch.float64, device=dev).fill_(1.0)
@@ -463,6 +468,18 @@ def seafloor_sites(case, n, seed):
     return np.array(pts)
 
 
+def adjoint32(pc, opts, omega):
+    """adjoint_batch_variant with 16 sites (48 columns) on the 32x32x24-cell mesh."""
+    from actfemtide_amd import afx, mesh
+    c3 = mesh.tide_case(*SIZES["cfg3"], seed=1002, spheres=3, sigma_sea=3.3)
+    s3, _, _ = afx.forward_system(c3, omega)
+    s3.precond_setup(pc)
+    s3.autotune(1)
+    out = adjoint_batch_variant(s3, c3, opts, 16)
+    s3.close()
+    return out
+
+
 def adjoint_batch_variant(s, case, opts, nsites):
     """solvePARDISOinv's adjoint right-hand sides (P^T columns of the 3 electric components at
     nsites seafloor sites, seed 21) solved one column at a time and with the autotuned block
@@ -495,11 +512,58 @@ def adjoint_batch_variant(s, case, opts, nsites):
         # recomputed over the k-column layout (another summation order: within ~1e-11 of it)
         tr = max(x.true_resid for x in st)
         tr2 = float(s.true_residual_device(dB, dX).max())
-        log(f"{k} adjoint columns, {width}: {dt:.2f}s, iterations {[x.iter for x in st]}, worst true residual "
+        log(f"{k} adjoint columns, {width}: {dt:.3f}s, iterations {min(x.iter for x in st)}..{max(x.iter for x in st)}, worst true residual "
             f"{tr:.3e} (recomputed {tr2:.3e})")
-        out[width] = dict(solves_per_s=round(k / dt, 4), seconds=round(dt, 3), iters=[x.iter for x in st],
+        its = [x.iter for x in st]
+        out[width] = dict(solves_per_s=round(k / dt, 4), seconds=round(dt, 3), iters_min=min(its), iters_max=max(its),
                           true_resid_max=tr, true_resid_max_recomputed=tr2, rc=rc)
     out["speedup"] = round(out["columns"]["seconds"] / out["autotuned"]["seconds"], 3)
+    return out
+
+
+def constituents_variant(s, b, omegas, w0, opts):
+    """The 5 tidal constituents (M2, S2, N2, K1, O1) of one mesh, each column at its own omega:
+    one column at a time from x0 = 0; the k-column launch (AFX_BATCH=multi); and one at a time with
+    each constituent warm-started from the previous constituent's solution (the right-hand side is
+    the same source, so the fields of neighbouring frequencies are close). Solves only, inputs on
+    the device; every column to a 1e-8 FP64 true residual."""
+    import torch
+
+    k = len(omegas)
+    sc = omegas / w0
+    dB = torch.from_numpy(np.ascontiguousarray(np.tile(b[:, None], (1, k)))).cuda()
+    out = {}
+    for width in ("columns", "multi"):
+        os.environ["AFX_BATCH"] = width
+        dX = torch.zeros_like(dB)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st, rc = s.solve_device(dB, dX, omega_scale=sc, **opts)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        tr = float(s.true_residual_device(dB, dX, omega_scale=sc).max())
+        out[width] = dict(solves_per_s=round(k / dt, 4), seconds=round(dt, 4), iters=[x.iter for x in st],
+                          true_resid_max=tr, rc=rc)
+        log(f"5 constituents, {width}: {dt:.3f}s, iterations {[x.iter for x in st]}, worst {tr:.2e}")
+    os.environ.pop("AFX_BATCH", None)
+    # warm chain: column c starts from column c - 1's solution
+    dX = torch.zeros_like(dB)
+    its = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c in range(k):
+        x = (dX[:, c - 1] if c else torch.zeros_like(dB[:, 0])).contiguous()
+        bc_ = dB[:, c].contiguous()
+        st, rc = s.solve_device(bc_, x, omega_scale=np.array([sc[c]]), **opts)
+        dX[:, c] = x
+        its.append(st[0].iter)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    tr = float(s.true_residual_device(dB, dX.contiguous(), omega_scale=sc).max())
+    out["warm_chain"] = dict(solves_per_s=round(k / dt, 4), seconds=round(dt, 4), iters=its, true_resid_max=tr)
+    log(f"5 constituents, warm chain: {dt:.3f}s, iterations {its}, worst {tr:.2e}")
+    best = min(out[m]["seconds"] for m in ("multi", "warm_chain"))
+    out["speedup_vs_columns"] = round(out["columns"]["seconds"] / best, 3)
     return out
 
 
@@ -548,24 +612,29 @@ def rhs_stack_variant(s, b, omegas, w0, opts):
     return out
 
 
-def mesh_batch_variant(opts, omega):
+def mesh_batch_variant(opts, omega, grad_only=False):
     """The 16 meshes of MESH_BATCH (seeds 3000 + i, crust N(-2, 0.5) per cell, M2): each solved
-    on its own (pattern, SSOR setup and autotuning before the timed loop), then all together as
-    one block-diagonal system. Both timings cover the solves only, with inputs on the device."""
+    on its own (pattern, preconditioner setup and autotuning before the timed loop), then all
+    together as one block-diagonal system (per-system scalars; and shared scalars unless
+    grad_only). The preconditioner is opts["precond"]; with the nodal correction the stack carries
+    the parts' node incidence. Both timings cover the solves only, with inputs on the device."""
     import torch
 
     from actfemtide_amd import afx, mesh
     from actfemtide_amd.meshbatch import MeshBatch, joint_tol
 
-    parts, bs = [], []
+    pc = opts["precond"]
+    parts, bs, cases, bcs = [], [], [], []
     t0 = time.time()
     for i, dims in enumerate(MESH_BATCH):
         c = mesh.tide_case(*dims, seed=3000 + i)
-        s, b, _ = afx.forward_system(c, omega)
-        s.precond_setup("SSOR")
+        s, b, bc = afx.forward_system(c, omega)
+        s.precond_setup(pc)
         s.autotune(1)
         parts.append(s)
         bs.append(b)
+        cases.append(c)
+        bcs.append(bc)
     log(f"mesh batch: {len(parts)} systems, {sum(p.n for p in parts)} edges ({time.time() - t0:.1f}s)")
     dBs = [torch.from_numpy(b).cuda() for b in bs]
     dXs = [torch.zeros_like(d) for d in dBs]
@@ -581,10 +650,12 @@ def mesh_batch_variant(opts, omega):
     log(f"mesh batch, one solve per mesh: {t_own:.2f}s, iterations {own}")
 
     modes = {}
-    for mode in ("per_system", "shared"):
+    for mode in ("per_system",) if grad_only else ("per_system", "shared"):
         t0 = time.time()
         mb = MeshBatch(parts, per_system=(mode == "per_system"))
-        mb.system.precond_setup("SSOR")
+        if pc.endswith("+GRAD"):
+            mb.set_nodes([c.node for c in cases], [c.line for c in cases], bcs)
+        mb.system.precond_setup(pc)
         mb.system.autotune(1)
         log(f"stacked system ({mode}): n={mb.n} levels L/U={mb.system.nlev_l}/{mb.system.nlev_u} "
             f"({time.time() - t0:.1f}s)")
@@ -614,7 +685,7 @@ def mesh_batch_variant(opts, omega):
     out = dict(meshes=[f"{a}x{b_}x{c}" for a, b_, c in MESH_BATCH], edges=[p.n for p in parts],
                one_by_one=dict(solves_per_s=round(len(parts) / t_own, 4), seconds=round(t_own, 3), iters=own,
                                true_resid_max=max(own_res)),
-               one_launch_per_system_scalars=modes["per_system"], one_launch_shared_scalars=modes["shared"])
+               one_launch_per_system_scalars=modes["per_system"], one_launch_shared_scalars=modes.get("shared"))
     for p in parts:
         p.close()
     return out

@@ -709,7 +709,8 @@ struct Work {
     const size_t nk = (size_t)n * k;
     if ((rc = dalloc(&R, nk)) || (rc = dalloc(&Z, nk)) || (rc = dalloc(&P, nk)) || (rc = dalloc(&Q, nk))) return rc;
     const size_t gk = (size_t)G * k;
-    if ((rc = dalloc(&part_rho, gk)) || (rc = dalloc(&part_c1, gk)) || (rc = dalloc(&part_rr, gk))) return rc;
+    // part_rr holds complex partials in the k-column kernel without sweeps: twice the doubles
+    if ((rc = dalloc(&part_rho, gk)) || (rc = dalloc(&part_c1, gk)) || (rc = dalloc(&part_rr, 2 * gk))) return rc;
     if ((rc = dalloc(&part_a, gk)) || (rc = dalloc(&part_b, gk))) return rc;
     if ((rc = dalloc(&fin_rho, k)) || (rc = dalloc(&fin_c1, k)) || (rc = dalloc(&fin_rr, k))) return rc;
     if ((rc = dalloc(&rho1, ks)) || (rc = dalloc(&bnrm2, ks)) || (rc = dalloc(&resid, ks))) return rc;
@@ -835,7 +836,7 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   // the streaming kernel (no sweep: DIAG or NONE edge preconditioner, one column, FP64) runs
   // at several blocks per CU; every other case runs k_cocg on the sweep partition
   const bool stream = fp64 && !(stacked && s->nsys > 1) && (pc == AFX_PC_DIAG || pc == AFX_PC_NONE) &&
-                      !getenv("AFX_NO_STREAM");
+                      (!multi || k <= 8) && !getenv("AFX_NO_STREAM");
   int per_cu = 1;
   if (const char *e = getenv("AFX_STREAM_PER_CU")) per_cu = std::max(1, std::min(8, atoi(e)));
   const int G = stream ? afx::max_coop_grid_diag(s->lpr64, per_cu) : s->Gs;
@@ -985,7 +986,8 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   int it_before = 0;
   for (;;) {
     HIPCHK(hipEventRecord(ev0, s->stream));
-    hipError_t e = stream ? afx::launch_cocg_diag(*(afx::CocgParams<double> *)&p, lpr, G, s->stream)
+    hipError_t e = (stream && multi) ? afx::launch_cocg_blk(*(afx::CocgParams<double> *)&p, G, s->stream)
+                   : stream ? afx::launch_cocg_diag(*(afx::CocgParams<double> *)&p, lpr, G, s->stream)
                    : fp64 ? afx::launch_cocg64(*(afx::CocgParams<double> *)&p, lpr, G, s->stream)
                           : afx::launch_cocg32(*(afx::CocgParams<float> *)&p, lpr, G, s->stream);
     if (e != hipSuccess) {
@@ -1023,10 +1025,17 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
     double mean[11] = {0}, mx[11] = {0};
     for (int b = 0; b < G; b++)
       for (int q = 0; q < 11; q++) {
-        const double v = (double)h[(size_t)b * AFX_DBG_SLOTS + q] / its * (q != 5 ? 0.01 : 1.0);  // 100 MHz -> us
+        const double v = (double)h[(size_t)b * AFX_DBG_SLOTS + q] / its * (q != 5 || (stream && multi) ? 0.01 : 1.0);  // 100 MHz -> us
         mean[q] += v / G;
         mx[q] = std::max(mx[q], v);
       }
+    if (stream && multi)
+      fprintf(stderr,
+              "afx profile k-column (us/iteration, mean/max over %d blocks): z pass %.1f/%.1f  rho sum %.1f/%.1f  "
+              "p update %.1f/%.1f  spmv %.1f/%.1f  c1 sum %.1f/%.1f  x,r pass %.1f/%.1f  rr sum %.1f/%.1f\n",
+              G, mean[0], mx[0], mean[1], mx[1], mean[2], mx[2], mean[3], mx[3], mean[4], mx[4], mean[5], mx[5],
+              mean[6], mx[6]);
+    else
     fprintf(stderr,
             "afx profile (us/iteration, mean/max over %d blocks): fw sweep %.1f/%.1f  clear after fw %.1f/%.1f  "
             "bw sweep %.1f/%.1f  clear after bw %.1f/%.1f  rest %.1f/%.1f (rho wait %.1f/%.1f, after rho %.1f/%.1f: "
@@ -1175,7 +1184,7 @@ __global__ void k_inv_or_zero(int k, const double *norms, double tol, double *sc
 }
 
 int solve_device_impl(afx_system s, int k, const cx<double> *dB, cx<double> *dX, const double *omega_scale,
-                      const afx_opts *o, afx_stats *stats, bool stacked = false);
+                      const afx_opts *o, afx_stats *stats, bool stacked = false, int force_width = -1);
 
 int ensure_stack(afx_system s, int k, const std::vector<double> &wcol);
 int get_values_at(afx_system s, double w, double *D, double *AU, double *AL);
@@ -1253,6 +1262,42 @@ int solve_stacked_columns(afx_system s, int k, const cx<double> *dB, cx<double> 
   dfree(b1);
   dfree(x1);
   return ret;
+}
+
+// Blocks of g columns (the last one smaller) solved one block after another, each by one k-column
+// launch at the columns' own frequencies.
+int solve_groups(afx_system s, int k, int g, const cx<double> *dB, cx<double> *dX, const double *omega_scale,
+                 const afx_opts *o, afx_stats *stats) {
+  cx<double> *bg = nullptr, *xg = nullptr;
+  int rc;
+  if ((rc = dalloc(&bg, (size_t)s->n * g)) || (rc = dalloc(&xg, (size_t)s->n * g))) {
+    dfree(bg);
+    return rc;
+  }
+  int ret = AFX_OK;
+  std::vector<double> sc(g);
+  for (int c0 = 0; c0 < k && rc == AFX_OK; c0 += g) {
+    const int gg = std::min(g, k - c0);
+    for (int j = 0; j < gg; j++) sc[j] = omega_scale ? omega_scale[c0 + j] : 1.0;
+    hipError_t e = afx::launch_cols_get(s->n, k, c0, gg, dB, bg, s->stream);
+    if (e == hipSuccess) e = afx::launch_cols_get(s->n, k, c0, gg, dX, xg, s->stream);
+    if (e != hipSuccess) {
+      rc = fail(AFX_ERR_HIP, hipGetErrorString(e));
+      break;
+    }
+    const int r = solve_device_impl(s, gg, bg, xg, sc.data(), o, stats ? stats + c0 : nullptr, false, gg > 1 ? 1 : 0);
+    if (r != AFX_OK && r != AFX_ERR_MAXIT && r != AFX_ERR_BREAKDOWN) {
+      rc = r;
+      break;
+    }
+    if (r == AFX_ERR_BREAKDOWN || (r == AFX_ERR_MAXIT && ret == AFX_OK)) ret = r;
+    e = afx::launch_cols_put(s->n, k, c0, gg, xg, dX, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) rc = fail(AFX_ERR_HIP, hipGetErrorString(e));
+  }
+  dfree(bg);
+  dfree(xg);
+  return rc != AFX_OK ? rc : ret;
 }
 
 // Right-hand-side block width 1: the k columns solved one after another by the single-column
@@ -1335,7 +1380,7 @@ int true_residual_stacked(afx_system s, const cx<double> *dB, const cx<double> *
 }
 
 int solve_device_impl(afx_system s, int k, const cx<double> *dB, cx<double> *dX, const double *omega_scale,
-                      const afx_opts *o, afx_stats *stats, bool stacked) {
+                      const afx_opts *o, afx_stats *stats, bool stacked, int force_width) {
   if (!s->have_values) return fail(AFX_ERR_ARG, "no matrix values");
   if (k > 256) return fail(AFX_ERR_ARG, "at most 256 right-hand sides per call");
   int rc = ensure_derived(s);
@@ -1347,12 +1392,18 @@ int solve_device_impl(afx_system s, int k, const cx<double> *dB, cx<double> *dX,
   // (afx_autotune), is asked for (AFX_BATCH=multi; AFX_BATCH=columns forces one at a time), or
   // the caller reuses a preconditioner set up for exactly these k frequencies (nset = 0)
   const char *bm = getenv("AFX_BATCH");
-  const int width = bm ? (std::string(bm) == "multi" ? 1 : std::string(bm) == "stack" ? 2 : 0) : s->batch_multi[k];
+  const int width = force_width >= 0 ? force_width
+                    : bm ? (std::string(bm) == "multi" ? 1 : std::string(bm) == "stack" ? 2 : 0) : s->batch_multi[k];
   const bool grad = ((o->nset == 0 ? s->pc_kind : o->precond) & AFX_PC_GRAD) != 0;
   const bool multi = width == 1;
   if (k > 1 && o->nset != 0 && width == 2 && !stacked && !grad)
     return solve_stacked_columns(s, k, dB, dX, omega_scale, o, stats);
   if (k > 1 && (o->nset != 0 || grad) && !multi) return solve_columns(s, k, dB, dX, omega_scale, o, stats);
+  // the k-column kernel without sweeps takes at most 8 columns: larger blocks in groups of 8
+  const int base_pc = (o->nset == 0 ? s->pc_kind : o->precond) & 0xff;
+  if (multi && k > 8 && o->precision == AFX_FP64 && (base_pc == AFX_PC_DIAG || base_pc == AFX_PC_NONE) &&
+      !getenv("AFX_NO_STREAM"))
+    return solve_groups(s, k, 8, dB, dX, omega_scale, o, stats);
   std::vector<double> wcol, wf;
   std::vector<int> col2f;
   column_freqs(s, k, omega_scale, wcol, wf, col2f);

@@ -181,6 +181,10 @@ hipError_t launch_warm_dots(long long n, int k, int kp, int grid, const cx<doubl
 hipError_t launch_scale_cols_cx(long long n, int k, const cx<double> *g, cx<double> *X, hipStream_t s);
 hipError_t launch_col_get(long long n, int k, int c, const cx<double> *A, cx<double> *v, hipStream_t s);
 hipError_t launch_col_put(long long n, int k, int c, const cx<double> *v, cx<double> *A, hipStream_t s);
+hipError_t launch_cols_get(long long n, int k, int c0, int g, const cx<double> *A, cx<double> *B, hipStream_t s);
+hipError_t launch_cols_put(long long n, int k, int c0, int g, const cx<double> *B, cx<double> *A, hipStream_t s);
+// k <= 8 columns of the iteration without sweeps, a row's matrix entries read once for all columns
+hipError_t launch_cocg_blk(const CocgParams<double> &p, int grid, hipStream_t s);
 hipError_t launch_axpy_f32_to_f64(long long m, const cx<float> *d, cx<double> *x, hipStream_t s);
 hipError_t launch_scale_cols(long long n, int k, const cx<double> *in, const double *scale,
                              cx<float> *out, hipStream_t s);

@@ -1233,6 +1233,335 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
   cocg_body<double, 1, true, false, false>(p);
 }
 
+// ---------------------------------------------------------------------------------------
+// Several columns (k <= K) of the iteration without sweeps, the columns of a row kept by the
+// same lanes: the SpMV reads each matrix entry once for all k columns (K + i w_c M per column c)
+// and the vector passes stream whole rows of the [n][k] vectors. Each column keeps its own
+// alpha, beta, stopping test and state, as k independent COCG13_single iterations (same
+// recurrence and unconjugated forms). Reductions: per-wave XOR-shuffle sums, per-block sums in
+// LDS, per-block partials [G][k] added in block order by every block (deterministic).
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+// v[c] summed over the grid for c < k; every thread returns the totals. lds: 4*K complex.
+template <int K>
+__device__ __forceinline__ void grid_sum_blk(cx<double> (&v)[K], int k, cx<double> *part, cx<double> *lds,
+                                             const GridBar &bar, unsigned &phase, bool &abort, bool poll) {
+  const int t = threadIdx.x, wid = t >> 6, lane = t & 63;
+#pragma unroll
+  for (int c = 0; c < K; c++) {
+    v[c].r = wave_sum(v[c].r);
+    v[c].i = wave_sum(v[c].i);
+  }
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < K; c++) lds[wid * K + c] = v[c];
+  __syncthreads();
+  if (t < k) {
+    cx<double> b = lds[t];
+    for (int w = 1; w < 4; w++) b = addv(b, lds[w * K + t]);
+    part[(size_t)blockIdx.x * k + t] = b;
+  }
+  abort |= grid_barrier(bar, phase, poll);
+  // totals: thread t sums column t % K over the blocks t / K, t / K + 256 / K, ... in order
+  constexpr int J = 256 / K;
+  const int c = t % K, j = t / K;
+  cx<double> sum = {0.0, 0.0};
+  if (c < k)
+    for (int g = j; g < (int)gridDim.x; g += J) sum = addv(sum, part[(size_t)g * k + c]);
+  __shared__ cx<double> red[256];
+  red[t] = sum;
+  __syncthreads();
+  for (int h = J >> 1; h >= 1; h >>= 1) {
+    if (j < h) red[t] = addv(red[t], red[t + h * K]);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int cc = 0; cc < K; cc++) v[cc] = red[cc];
+  __syncthreads();
+}
+
+// v (this thread's column c) added into acc[c] without indexing a register array at run time
+template <int K, typename V>
+__device__ __forceinline__ void add_col(V (&acc)[K], int c, V v) {
+#pragma unroll
+  for (int cc = 0; cc < K; cc++)
+    if (cc == c) acc[cc] = addv(acc[cc], v);
+}
+template <int K, typename V>
+__device__ __forceinline__ V pick_col(const V (&a)[K], int c) {
+  V r = a[0];
+#pragma unroll
+  for (int cc = 1; cc < K; cc++)
+    if (cc == c) r = a[cc];
+  return r;
+}
+
+template <int K>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) k_cocg_blk(CocgParams<double> p) {
+  constexpr int LPR = 4, SE = 5;
+  __shared__ cx<double> lds_k[4 * K];
+  const int t = threadIdx.x;
+  const int gtid = blockIdx.x * 256 + t;
+  const int nthr = gridDim.x * 256;
+  const int n = p.n, k = p.k;
+  const bool diag = p.pc == AFX_PC_DIAG;
+  // element passes over the [n][k] vectors: element i = row * k + c, a stride that is a multiple
+  // of k so every thread keeps one column (consecutive threads, consecutive 16-byte elements)
+  const int estride = (nthr / k) * k;
+  const int ec = gtid % k;
+  const bool eact = gtid < estride;
+  const long long nk = (long long)n * k, nnk = (long long)p.nn * k;
+  unsigned phase = 0;
+  if (t == 0) phase = __hip_atomic_load(p.bar.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  double w[K], bn[K], res[K];
+  int st[K], its[K];
+  cx<double> rho1[K];
+#pragma unroll
+  for (int c = 0; c < K; c++) {
+    const bool v = c < k;
+    w[c] = v ? p.wcol[c] : 0.0;
+    st[c] = v ? p.conv[c] : 1;
+    bn[c] = v ? p.bnrm2[c] : 1.0;
+    rho1[c] = v ? p.rho1[c] : cx<double>{0.0, 0.0};
+    res[c] = v ? p.resid[c] : 0.0;
+    its[c] = v ? p.iters[c] : 0;
+  }
+  const double we = p.wcol[ec];
+  const int fe = p.col2f[ec];
+  int it = p.status[0];
+  const int it_end = min(it + p.chunk, p.maxit);
+  bool alldone = true;
+#pragma unroll
+  for (int c = 0; c < K; c++) alldone = alldone && st[c] != 0;
+  bool abort = false;
+  // optional per-block phase timing (wall clock, 100 MHz): z pass, rho sum, p update, SpMV, c1 sum,
+  // x/r pass, rr sum
+  unsigned long long tph[7] = {0, 0, 0, 0, 0, 0, 0}, tmark = 0;
+  const bool prof = p.dbg != nullptr;
+#define AFX_BLK_MARK(q)                                 \
+  if (prof) {                                           \
+    const unsigned long long now_ = wall_clock64();     \
+    tph[q] += now_ - tmark;                             \
+    tmark = now_;                                       \
+  }
+  while (!alldone && it < it_end && !abort) {
+    ++it;
+    if (prof) tmark = wall_clock64();
+    // ---- z = M^-1 r and rho = r^T z per column ----
+    cx<double> acc[K];
+#pragma unroll
+    for (int c = 0; c < K; c++) acc[c] = cx<double>{0.0, 0.0};
+    cx<double> mine = {0.0, 0.0};
+    if (p.grad && eact)
+      for (long long i = gtid; i < nnk; i += estride) {  // (node, column) elements
+        const int a = (int)(i / k);
+        cx<double> g = {0.0, 0.0};
+        for (int q = p.nptr[a]; q < p.nptr[a + 1]; q++) {
+          const int v = p.nadj[q];
+          const cx<double> r = p.R[(size_t)((v > 0 ? v : -v) - 1) * k + ec];
+          g.r += v > 0 ? r.r : -r.r;
+          g.i += v > 0 ? r.i : -r.i;
+        }
+        const cx<double> zn = nodal_z(g, we, p.mnd[a]);
+        p.Zn[i] = zn;
+        mine = addv(mine, cmul(g, zn));
+      }
+    if (eact)
+      for (long long i0 = gtid; i0 < nk; i0 += 4LL * estride) {
+        cx<double> rv[4], dv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const long long i = i0 + u * (long long)estride < nk ? i0 + u * (long long)estride : i0;
+          rv[u] = p.R[i];
+          dv[u] = diag ? p.DD[(size_t)(i / k) * p.nf + fe] : cx<double>{1.0, 0.0};
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const long long i = i0 + u * (long long)estride;
+          if (i < nk) {
+            const cx<double> z = cmul(rv[u], dv[u]);
+            p.Z[i] = z;
+            mine = addv(mine, cmul(rv[u], z));
+          }
+        }
+      }
+    add_col<K>(acc, ec, mine);
+    AFX_BLK_MARK(0)
+    grid_sum_blk<K>(acc, k, p.part_rho, lds_k, p.bar, phase, abort, false);
+    AFX_BLK_MARK(1)
+    cx<double> rho[K], beta[K];
+    const bool first = (it == 1);
+#pragma unroll
+    for (int c = 0; c < K; c++) {
+      rho[c] = acc[c];
+      beta[c] = first ? cx<double>{0.0, 0.0} : cdivd(rho[c], rho1[c]);
+    }
+    // ---- p = z + G zn (+ beta p) ----
+    {
+      const cx<double> be = pick_col<K>(beta, ec);
+      if (eact)
+        for (long long i0 = gtid; i0 < nk; i0 += 4LL * estride) {
+          cx<double> zv[4], pv[4], za[4], zb[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const long long i = i0 + u * (long long)estride < nk ? i0 + u * (long long)estride : i0;
+            zv[u] = p.Z[i];
+            pv[u] = first ? cx<double>{0.0, 0.0} : p.P[i];
+            if (p.grad) {
+              const int2 e = ((const int2 *)p.en)[i / k];
+              za[u] = p.Zn[(size_t)e.x * k + ec];
+              zb[u] = p.Zn[(size_t)e.y * k + ec];
+            } else {
+              za[u] = zb[u] = cx<double>{0.0, 0.0};
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const long long i = i0 + u * (long long)estride;
+            const cx<double> z = {zv[u].r + (zb[u].r - za[u].r), zv[u].i + (zb[u].i - za[u].i)};
+            if (i < nk) p.P[i] = first ? z : cfma(be, pv[u], z);
+          }
+        }
+    }
+    abort |= grid_barrier(p.bar, phase);
+    AFX_BLK_MARK(2)
+    // ---- q = A p, c1 = p^T q: LPR lanes per row, a row's entries read once for all columns ----
+#pragma unroll
+    for (int c = 0; c < K; c++) acc[c] = cx<double>{0.0, 0.0};
+    {
+      const int lane = t % LPR;
+      for (int row = gtid / LPR; row < n; row += nthr / LPR) {
+        const int qb = p.rowptr[row], qe = p.rowptr[row + 1];
+        cx<double> vv[SE];
+        int cl[SE];
+#pragma unroll
+        for (int e = 0; e < SE; e++) {
+          const int q = qb + lane + e * LPR;
+          const bool in = q < qe;
+          const int qc = in ? q : qb;
+          vv[e] = p.val[qc];
+          const int cq = p.col[qc];
+          cl[e] = in ? cq : -1;
+        }
+#pragma unroll
+        for (int c = 0; c < K; c++)
+          if (c < k) {
+            cx<double> sum = {0.0, 0.0};
+#pragma unroll
+            for (int e = 0; e < SE; e++) {
+              const cx<double> pv = p.P[(size_t)max(cl[e], 0) * k + c];
+              if (cl[e] >= 0) sum = cfma(mval(vv[e], w[c]), pv, sum);
+            }
+            for (int q = qb + lane + SE * LPR; q < qe; q += LPR)
+              sum = cfma(mval(p.val[q], w[c]), p.P[(size_t)p.col[q] * k + c], sum);
+            sum = group_sum<LPR>(sum);
+            if (lane == 0) {
+              const size_t ix = (size_t)row * k + c;
+              p.Q[ix] = sum;
+              acc[c] = addv(acc[c], cmul(p.P[ix], sum));
+            }
+          }
+      }
+    }
+    AFX_BLK_MARK(3)
+    grid_sum_blk<K>(acc, k, p.part_c1, lds_k, p.bar, phase, abort, false);
+    AFX_BLK_MARK(4)
+    cx<double> alpha[K];
+    bool upd[K];
+#pragma unroll
+    for (int c = 0; c < K; c++) {
+      const cx<double> c1 = acc[c];
+      alpha[c] = cdivd(rho[c], c1);
+      if (st[c] == 0 && (!(c1.r != 0.0 || c1.i != 0.0) || !isfinite(alpha[c].r) || !isfinite(alpha[c].i) ||
+                         !(rho[c].r != 0.0 || rho[c].i != 0.0))) {
+        st[c] = -200;  // breakdown
+        its[c] = it;
+      }
+      upd[c] = st[c] == 0;
+    }
+    // ---- x += alpha p, r -= alpha q, ||r||^2 per column ----
+    cx<double> rr[K];
+#pragma unroll
+    for (int c = 0; c < K; c++) rr[c] = cx<double>{0.0, 0.0};
+    {
+      const cx<double> al = pick_col<K>(alpha, ec);
+      const bool ue = pick_col<K>(upd, ec);
+      double rre = 0.0;
+      if (eact && ue)
+        for (long long i0 = gtid; i0 < nk; i0 += 4LL * estride) {
+          cx<double> xv[4], pv[4], qv[4], rv[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const long long i = i0 + u * (long long)estride < nk ? i0 + u * (long long)estride : i0;
+            xv[u] = p.X[i];
+            pv[u] = p.P[i];
+            qv[u] = p.Q[i];
+            rv[u] = p.R[i];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const long long i = i0 + u * (long long)estride;
+            if (i < nk) {
+              p.X[i] = cfma(al, pv[u], xv[u]);
+              const cx<double> r = cfma(cx<double>{-al.r, -al.i}, qv[u], rv[u]);
+              p.R[i] = r;
+              rre += r.r * r.r + r.i * r.i;
+            }
+          }
+        }
+      add_col<K>(rr, ec, cx<double>{rre, 0.0});
+    }
+    AFX_BLK_MARK(5)
+    grid_sum_blk<K>(rr, k, (cx<double> *)p.part_rr, lds_k, p.bar, phase, abort, true);
+    AFX_BLK_MARK(6)
+    alldone = true;
+#pragma unroll
+    for (int c = 0; c < K; c++) {
+      if (upd[c]) {
+        res[c] = sqrt(rr[c].r / bn[c]);  // :315 RESID = sqrt(DNRM2/BNRM2)
+        if (res[c] <= p.tol) {
+          st[c] = 1;
+          its[c] = it;
+        } else if (it == p.maxit) {
+          st[c] = -100;
+          its[c] = it + 1;
+        }
+      }
+      rho1[c] = rho[c];
+      alldone = alldone && st[c] != 0;
+    }
+  }
+#undef AFX_BLK_MARK
+  if (prof && t == 0)
+    for (int q = 0; q < 7; q++) p.dbg[(size_t)blockIdx.x * AFX_DBG_SLOTS + q] += tph[q];
+  if (blockIdx.x == 0 && t == 0) {
+#pragma unroll
+    for (int c = 0; c < K; c++)
+      if (c < k) {
+        p.conv[c] = st[c];
+        p.rho1[c] = rho1[c];
+        p.resid[c] = res[c];
+        p.iters[c] = its[c];
+      }
+    p.status[0] = it;
+    p.status[1] = alldone ? 1 : 0;
+  }
+}
+
+hipError_t launch_cocg_blk(const CocgParams<double> &p, int grid, hipStream_t s) {
+  void *args[] = {(void *)&p};
+  const void *f = p.k <= 2 ? (const void *)k_cocg_blk<2> : p.k <= 4 ? (const void *)k_cocg_blk<4>
+                                                                    : (const void *)k_cocg_blk<8>;
+  if (p.k > 8) return hipErrorInvalidValue;
+  return hipLaunchCooperativeKernel(f, dim3(grid), dim3(256), args, 0, s);
+}
+
 template <int LPR>
 static const void *flat_kernel() {
   return (const void *)k_cocg_flat<LPR>;
@@ -1552,6 +1881,23 @@ hipError_t launch_col_get(long long n, int k, int c, const cx<double> *A, cx<dou
 }
 hipError_t launch_col_put(long long n, int k, int c, const cx<double> *v, cx<double> *A, hipStream_t s) {
   hipLaunchKernelGGL(k_col_put, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, k, c, v, A);
+  return hipGetLastError();
+}
+// columns c0 .. c0+g-1 of A [n][k] to / from a [n][g] block
+__global__ void k_cols_get(long long n, int k, int c0, int g, const cx<double> *A, cx<double> *B) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n * g) B[i] = A[(i / g) * k + c0 + i % g];
+}
+__global__ void k_cols_put(long long n, int k, int c0, int g, const cx<double> *B, cx<double> *A) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n * g) A[(i / g) * k + c0 + i % g] = B[i];
+}
+hipError_t launch_cols_get(long long n, int k, int c0, int g, const cx<double> *A, cx<double> *B, hipStream_t s) {
+  hipLaunchKernelGGL(k_cols_get, dim3((unsigned)((n * g + 255) / 256)), dim3(256), 0, s, n, k, c0, g, A, B);
+  return hipGetLastError();
+}
+hipError_t launch_cols_put(long long n, int k, int c0, int g, const cx<double> *B, cx<double> *A, hipStream_t s) {
+  hipLaunchKernelGGL(k_cols_put, dim3((unsigned)((n * g + 255) / 256)), dim3(256), 0, s, n, k, c0, g, B, A);
   return hipGetLastError();
 }
 

@@ -914,6 +914,7 @@ __device__ __forceinline__ void cocg_body(const CocgParams<TP> &p) {
       if (prof) tclr += wall_clock64() - cc0;
     }
     const unsigned long long cr0 = prof ? wall_clock64() : 0;
+    if (prof && !sweep_pc) tfw += cr0 - ci0;  // without sweeps: the z pass (edge and nodal parts)
     const cx<double> rho = grid_cols(rho_loc, p.part_rho, p.fin_rho, lds_c, k, kp, c, dist, p.bar, phase, g0, g1, abort);
     const unsigned long long cr1 = prof ? wall_clock64() : 0;
     if (prof) trho += cr1 - cr0;
@@ -1223,9 +1224,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
 // The same iteration without the sweeps (DIAG or NONE edge preconditioner, with or without the
 // nodal correction, one column, FP64): every pass is a bandwidth-bound stream with no dependency
 // chain, so the launch runs two waves per SIMD (two blocks per CU) to keep more loads in flight.
-template <int LPR>
+template <int LPR, bool PROF>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_cocg_flat(CocgParams<double> p) {
-  cocg_body<double, LPR, false, false, false>(p);
+  cocg_body<double, LPR, false, PROF, false>(p);
 }
 
 // several columns without sweeps (one lane per row and column)
@@ -1563,16 +1564,16 @@ hipError_t launch_cocg_blk(const CocgParams<double> &p, int grid, hipStream_t s)
 }
 
 template <int LPR>
-static const void *flat_kernel() {
-  return (const void *)k_cocg_flat<LPR>;
+static const void *flat_kernel(bool prof) {
+  return prof ? (const void *)k_cocg_flat<LPR, true> : (const void *)k_cocg_flat<LPR, false>;
 }
-static const void *flat_kernel_for(int lpr) {
+static const void *flat_kernel_for(int lpr, bool prof = false) {
   switch (lpr) {
-    case 1: return flat_kernel<1>();
-    case 2: return flat_kernel<2>();
-    case 4: return flat_kernel<4>();
-    case 16: return flat_kernel<16>();
-    default: return flat_kernel<8>();
+    case 1: return flat_kernel<1>(prof);
+    case 2: return flat_kernel<2>(prof);
+    case 4: return flat_kernel<4>(prof);
+    case 16: return flat_kernel<16>(prof);
+    default: return flat_kernel<8>(prof);
   }
 }
 
@@ -1587,7 +1588,7 @@ int max_coop_grid_diag(int lpr, int per_cu) {
 
 hipError_t launch_cocg_diag(const CocgParams<double> &p, int lpr, int grid, hipStream_t s) {
   void *args[] = {(void *)&p};
-  const void *f = p.k > 1 ? (const void *)k_cocg_flat_multi : flat_kernel_for(lpr);
+  const void *f = p.k > 1 ? (const void *)k_cocg_flat_multi : flat_kernel_for(lpr, p.dbg != nullptr);
   return hipLaunchCooperativeKernel(f, dim3(grid), dim3(256), args, 0, s);
 }
 

@@ -692,7 +692,7 @@ struct Work {
   cx<TP> *Z = nullptr, *mbf = nullptr, *mbb = nullptr;
   cx<double> *part_rho = nullptr, *part_c1 = nullptr, *fin_rho = nullptr, *fin_c1 = nullptr, *rho1 = nullptr;
   double *part_rr = nullptr, *fin_rr = nullptr, *bnrm2 = nullptr, *resid = nullptr, *norms = nullptr;
-  double *part_a = nullptr, *part_b = nullptr, *wcol = nullptr;
+  double *part_a = nullptr, *part_b = nullptr, *wcol = nullptr, *part4 = nullptr;
   int *conv = nullptr, *iters = nullptr, *status = nullptr, *col2f = nullptr, *run = nullptr;
   int *h_status = nullptr;
   cx<double> *Zn = nullptr;
@@ -700,6 +700,7 @@ struct Work {
     dfree(R); dfree(Z); dfree(P); dfree(Q); dfree(mbf); dfree(mbb); dfree(Zn);
     dfree(part_rho); dfree(part_c1); dfree(fin_rho); dfree(fin_c1); dfree(rho1);
     dfree(part_rr); dfree(fin_rr); dfree(bnrm2); dfree(resid); dfree(norms); dfree(part_a); dfree(part_b);
+    dfree(part4);
     dfree(conv); dfree(iters); dfree(status); dfree(col2f); dfree(wcol); dfree(run);
     if (h_status) (void)hipHostFree(h_status);
   }
@@ -711,7 +712,7 @@ struct Work {
     const size_t gk = (size_t)G * k;
     // part_rr holds complex partials in the k-column kernel without sweeps: twice the doubles
     if ((rc = dalloc(&part_rho, gk)) || (rc = dalloc(&part_c1, gk)) || (rc = dalloc(&part_rr, 2 * gk))) return rc;
-    if ((rc = dalloc(&part_a, gk)) || (rc = dalloc(&part_b, gk))) return rc;
+    if ((rc = dalloc(&part_a, gk)) || (rc = dalloc(&part_b, gk)) || (rc = dalloc(&part4, 5 * (size_t)G))) return rc;
     if ((rc = dalloc(&fin_rho, k)) || (rc = dalloc(&fin_c1, k)) || (rc = dalloc(&fin_rr, k))) return rc;
     if ((rc = dalloc(&rho1, ks)) || (rc = dalloc(&bnrm2, ks)) || (rc = dalloc(&resid, ks))) return rc;
     if ((rc = dalloc(&norms, 2 * (size_t)ks)) || (rc = dalloc(&conv, ks)) || (rc = dalloc(&iters, ks))) return rc;
@@ -839,7 +840,12 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
                       (!multi || k <= 8) && !getenv("AFX_NO_STREAM");
   int per_cu = 1;
   if (const char *e = getenv("AFX_STREAM_PER_CU")) per_cu = std::max(1, std::min(8, atoi(e)));
-  const int G = stream ? afx::max_coop_grid_diag(s->lpr64, per_cu) : s->Gs;
+  // one column: the fused kernel (three grid synchronisations per iteration); AFX_FUSED=0 selects
+  // the four-pass kernel
+  const char *fe = getenv("AFX_FUSED");
+  const bool fused = stream && !multi && !(fe && fe[0] == '0');
+  const int G = fused ? afx::max_coop_grid_cocg1(s->lpr64)
+                : stream ? afx::max_coop_grid_diag(s->lpr64, per_cu) : s->Gs;
   if (G <= 0) return fail(AFX_ERR_HIP, "no co-resident grid for the COCG kernel");
   Work<TP> w;
   int rc = w.alloc(s->n, k, std::max(G, s->Gs), ks);
@@ -914,6 +920,7 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
     HIPCHK(hipMemsetAsync(w.Zn, 0, ((size_t)s->nn + 1) * k * sizeof(cx<double>), s->stream));
   }
   p.Zn = w.Zn;
+  p.part4 = w.part4;
   const bool bj = (pc == AFX_PC_SSOR_BJ);
   p.fw = side_view(bj ? s->bjfw : s->fw, fp64);
   p.bw = side_view(bj ? s->bjbw : s->bw, fp64);
@@ -986,7 +993,8 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   int it_before = 0;
   for (;;) {
     HIPCHK(hipEventRecord(ev0, s->stream));
-    hipError_t e = (stream && multi) ? afx::launch_cocg_blk(*(afx::CocgParams<double> *)&p, G, s->stream)
+    hipError_t e = fused ? afx::launch_cocg1(*(afx::CocgParams<double> *)&p, lpr, G, s->stream)
+                   : (stream && multi) ? afx::launch_cocg_blk(*(afx::CocgParams<double> *)&p, G, s->stream)
                    : stream ? afx::launch_cocg_diag(*(afx::CocgParams<double> *)&p, lpr, G, s->stream)
                    : fp64 ? afx::launch_cocg64(*(afx::CocgParams<double> *)&p, lpr, G, s->stream)
                           : afx::launch_cocg32(*(afx::CocgParams<float> *)&p, lpr, G, s->stream);
@@ -1009,7 +1017,7 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
     if (verbose) {
       const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
       fprintf(stderr, "afx: %s k=%d G=%d lpr=%d lds=%d ring=%d it=%d done=%d %.3fs (%.1f us/it)\n",
-              stream ? "fp64-stream" : fp64 ? "fp64" : "fp32-pc", k, G, lpr, p.lds_rows, p.ring_w, w.h_status[0],
+              fused ? "fp64-fused" : stream ? "fp64-stream" : fp64 ? "fp64" : "fp32-pc", k, G, lpr, p.lds_rows, p.ring_w, w.h_status[0],
               w.h_status[1], dt,
               w.h_status[0] ? 1e6 * dt / w.h_status[0] : 0.0);
     }
@@ -1029,7 +1037,12 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
         mean[q] += v / G;
         mx[q] = std::max(mx[q], v);
       }
-    if (stream && multi)
+    if (fused)
+      fprintf(stderr,
+              "afx profile fused (us/iteration, mean/max over %d blocks): p pass + barrier %.1f/%.1f  spmv + c1 sum "
+              "%.1f/%.1f  x,r,z pass + sums %.1f/%.1f  total %.1f/%.1f\n",
+              G, mean[0], mx[0], mean[2], mx[2], mean[3], mx[3], mean[4], mx[4]);
+    else if (stream && multi)
       fprintf(stderr,
               "afx profile k-column (us/iteration, mean/max over %d blocks): z pass %.1f/%.1f  rho sum %.1f/%.1f  "
               "p update %.1f/%.1f  spmv %.1f/%.1f  c1 sum %.1f/%.1f  x,r pass %.1f/%.1f  rr sum %.1f/%.1f\n",

@@ -121,6 +121,7 @@ struct CocgParams {
   const int *noff;  // stacked systems: system q's interior nodes are [noff[q], noff[q+1])
   const double *mnd;
   cx<double> *Zn;
+  double *part4;  // [G][5] per-block partials of the fused single-column kernel (c1: 2, rho and |r|^2: 3)
 };
 
 struct SpmvParams {
@@ -149,6 +150,9 @@ int max_coop_grid(int lpr, bool multi, bool fp64, size_t dyn);
 // correction, one column, FP64) at two waves per SIMD
 int max_coop_grid_diag(int lpr, int per_cu);
 hipError_t launch_cocg_diag(const CocgParams<double> &p, int lpr, int grid, hipStream_t s);
+// one column, DIAG / NONE (+ nodal correction), FP64: the fused kernel, one block of 256 threads per CU
+int max_coop_grid_cocg1(int lpr);
+hipError_t launch_cocg1(const CocgParams<double> &p, int lpr, int grid, hipStream_t s);
 int max_coop_grid_ic();
 hipError_t launch_spmv(const SpmvParams &p, hipStream_t s);
 hipError_t launch_spmv_stream(int n, int maxrow, const int *rowptr, const int *col, const cx<double> *val, double w,

@@ -697,6 +697,92 @@ __device__ void sweep_ring(const CocgParams<TP> &p, const SweepSide &sd, cx<TP> 
   }
 }
 
+// q = A p over the rows sslot, sslot + snslot, ... below vend, LPR lanes per row, and
+// c1 += p_r q_r (row order within a row slot). Rows are taken in rounds of SU rows per row slot.
+// A software pipeline over rounds: while round g is computed, the P gathers of round g, the
+// values and columns of round g+1 and the row bounds of round g+2 are in flight; each register
+// set is named by a compile-time index of a 2-way unrolled loop (no copies). Entries of a row are
+// summed in the same order as a row-at-a-time loop. SE covers rows of up to 20 entries (the
+// edge-element rows have at most 19); longer rows finish in the loop after the unrolled entries.
+template <int LPR>
+__device__ __forceinline__ void spmv_pipe(const int *__restrict__ rowptr, const int *__restrict__ col,
+                                          const cx<double> *__restrict__ val, double w, const cx<double> *P,
+                                          cx<double> *Q, long long n, long long sslot, long long snslot,
+                                          long long vend, int lane, cx<double> &c1_loc) {
+  constexpr int SU = LPR >= 8 ? 4 : 2, SE = LPR >= 4 ? (20 + LPR - 1) / LPR : 6;
+  struct RB {  // row bounds of one round
+    int qb[SU], qe[SU];
+  };
+  struct VC {  // values and columns of one round (column -1: no entry)
+    int qb[SU], qe[SU];
+    int cl[SU][SE];
+    cx<double> vv[SU][SE];
+  };
+  const long long rstep = (long long)SU * snslot;
+  auto load_rb = [&](long long row0, RB &b) {
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const long long r = row0 + u * snslot;
+      const long long rc = r < n ? r : n - 1;
+      b.qb[u] = rowptr[rc];
+      b.qe[u] = rowptr[rc + 1];
+      if (r >= vend) b.qe[u] = b.qb[u];
+    }
+  };
+  auto load_vc = [&](const RB &b, VC &v) {
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      v.qb[u] = b.qb[u];
+      v.qe[u] = b.qe[u];
+#pragma unroll
+      for (int e = 0; e < SE; e++) {
+        const int q = b.qb[u] + lane + e * LPR;
+        const bool in = q < b.qe[u];
+        const int qc = in ? q : 0;
+        v.vv[u][e] = val[qc];
+        const int cq = col[qc];  // loaded unconditionally (a fixed count of loads per round)
+        v.cl[u][e] = in ? cq : -1;
+      }
+    }
+  };
+  auto round = [&](long long row0, const VC &v, VC &vn, const RB &bn, RB &bnn) {
+    cx<double> pv[SU][SE], pr[SU];
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+#pragma unroll
+      for (int e = 0; e < SE; e++) pv[u][e] = P[max(v.cl[u][e], 0)];
+      const long long r = row0 + u * snslot;
+      pr[u] = P[r < n ? r : 0];
+    }
+    load_vc(bn, vn);
+    load_rb(row0 + 2 * rstep, bnn);
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      cx<double> acc = {0.0, 0.0};
+#pragma unroll
+      for (int e = 0; e < SE; e++)
+        if (v.cl[u][e] >= 0) acc = cfma(mval(v.vv[u][e], w), pv[u][e], acc);
+      for (int q = v.qb[u] + lane + SE * LPR; q < v.qe[u]; q += LPR) acc = cfma(mval(val[q], w), P[col[q]], acc);
+      const long long r = row0 + u * snslot;
+      const cx<double> a = group_sum<LPR>(acc);
+      if (lane == 0 && r < vend) {
+        Q[r] = a;
+        c1_loc = addv(c1_loc, cmul(pr[u], a));
+      }
+    }
+  };
+  RB b0, b1;
+  VC v0, v1;
+  load_rb(sslot, b0);
+  load_rb(sslot + rstep, b1);
+  load_vc(b0, v0);
+  for (long long row0 = sslot; row0 < vend; row0 += 2 * rstep) {
+    round(row0, v0, v1, b1, b0);
+    if (row0 + rstep >= vend) break;
+    round(row0 + rstep, v1, v0, b0, b1);
+  }
+}
+
 // dynamic LDS of the single-column path: z of the block's rows, then the step bounds of the
 // forward and backward sweeps (16-byte aligned)
 template <typename TP>
@@ -1019,89 +1105,9 @@ __device__ __forceinline__ void cocg_body(const CocgParams<TP> &p) {
         }
       }
     } else {
-      // Rows are taken in rounds of SU rows per row slot (row0 + u * nslot). A software pipeline
-      // over rounds: while round g is computed, the P gathers of round g, the values and
-      // columns of round g+1 and the row bounds of round g+2 are in flight; each register set
-      // is named by a compile-time index of a 2-way unrolled loop (no copies). Entries of a
-      // row are summed in the same order as a row-at-a-time loop.
-      // SE covers rows of up to 20 entries (the edge-element rows have at most 19); longer rows
-      // finish in the loop after the unrolled entries
-      constexpr int SU = LPR >= 8 ? 4 : 2, SE = LPR >= 4 ? (20 + LPR - 1) / LPR : 6;
-      struct RB {  // row bounds of one round
-        int qb[SU], qe[SU];
-      };
-      struct VC {  // values and columns of one round (column -1: no entry)
-        int qb[SU], qe[SU];
-        int cl[SU][SE];
-        cx<double> vv[SU][SE];
-      };
       // stacked systems: the block's own rows, LPR lanes per row slot
       const long long sslot = sys ? rs + t / LPR : slot, snslot = sys ? 256 / LPR : nslot;
-      const long long rstep = (long long)SU * snslot;
-      auto load_rb = [&](long long row0, RB &b) {
-#pragma unroll
-        for (int u = 0; u < SU; u++) {
-          const long long r = row0 + u * snslot;
-          const long long rc = r < p.n ? r : p.n - 1;
-          b.qb[u] = p.rowptr[rc];
-          b.qe[u] = p.rowptr[rc + 1];
-          if (r >= vend) b.qe[u] = b.qb[u];
-        }
-      };
-      auto load_vc = [&](const RB &b, VC &v) {
-#pragma unroll
-        for (int u = 0; u < SU; u++) {
-          v.qb[u] = b.qb[u];
-          v.qe[u] = b.qe[u];
-#pragma unroll
-          for (int e = 0; e < SE; e++) {
-            const int q = b.qb[u] + lane + e * LPR;
-            const bool in = q < b.qe[u];
-            const int qc = in ? q : 0;
-            v.vv[u][e] = p.val[qc];
-            const int cq = p.col[qc];  // loaded unconditionally (a fixed count of loads per round)
-            v.cl[u][e] = in ? cq : -1;
-          }
-        }
-      };
-      auto round = [&](long long row0, const VC &v, VC &vn, const RB &bn, RB &bnn) {
-        cx<double> pv[SU][SE], pr[SU];
-#pragma unroll
-        for (int u = 0; u < SU; u++) {
-#pragma unroll
-          for (int e = 0; e < SE; e++) pv[u][e] = p.P[max(v.cl[u][e], 0)];
-          const long long r = row0 + u * snslot;
-          pr[u] = p.P[r < p.n ? r : 0];
-        }
-        load_vc(bn, vn);
-        load_rb(row0 + 2 * rstep, bnn);
-#pragma unroll
-        for (int u = 0; u < SU; u++) {
-          cx<double> acc = {0.0, 0.0};
-#pragma unroll
-          for (int e = 0; e < SE; e++)
-            if (v.cl[u][e] >= 0) acc = cfma(mval(v.vv[u][e], w), pv[u][e], acc);
-          for (int q = v.qb[u] + lane + SE * LPR; q < v.qe[u]; q += LPR)
-            acc = cfma(mval(p.val[q], w), p.P[p.col[q]], acc);
-          const long long r = row0 + u * snslot;
-          const cx<double> a = group_sum<LPR>(acc);
-          if (lane == 0 && r < vend) {
-            p.Q[r] = a;
-            c1_loc = addv(c1_loc, cmul(pr[u], a));
-          }
-        }
-      };
-      RB b0, b1;
-      VC v0, v1;
-      const long long first0 = sslot;
-      load_rb(first0, b0);
-      load_rb(first0 + rstep, b1);
-      load_vc(b0, v0);
-      for (long long row0 = first0; row0 < vend; row0 += 2 * rstep) {
-        round(row0, v0, v1, b1, b0);
-        if (row0 + rstep >= vend) break;
-        round(row0 + rstep, v1, v0, b0, b1);
-      }
+      spmv_pipe<LPR>(p.rowptr, p.col, p.val, w, p.P, p.Q, p.n, sslot, snslot, vend, lane, c1_loc);
     }
     const cx<double> c1 = grid_cols(c1_loc, p.part_c1, p.fin_c1, lds_c, k, kp, c, dist, p.bar, phase, g0, g1, abort);
     if (prof) tsp += wall_clock64() - cp1;
@@ -1553,6 +1559,317 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     p.status[0] = it;
     p.status[1] = alldone ? 1 : 0;
   }
+}
+
+// ---------------------------------------------------------------------------------------
+// One column without sweeps (edge preconditioner DIAG or NONE, with or without the nodal
+// correction), FP64: one block of 256 threads per CU, three grid synchronisations per iteration
+// instead of four. The recurrence is COCG13_single's (:181-323); the preconditioner apply of
+// iteration it+1 moves into the x, r pass of iteration it:
+//   P   p = z + beta p (:258-268), z = D r + G zn formed on the fly (z is never stored);
+//   S   q = A p (:273-288) and c1 = p^T q (:293-296)                      -> reduction of c1;
+//   XN  x += alpha p, r' = r - alpha q (:304-307) into the other residual buffer, |r'|^2
+//       (:310-313) and the edge part r'^T D r' of the next rho (:246-250); on the interior
+//       nodes g' = G^T r', each r'_e recomputed from r and q by the same operation as the edge
+//       pass (so g' is G^T of the stored r' bit for bit), zn' = g' / (i w mnd) and the nodal
+//       part g'^T zn' of the next rho                                     -> one reduction of
+//       (|r'|^2, rho').
+// The residual alternates between R (before odd iterations) and Z (before even ones). A launch
+// that starts the solve computes z and rho of the initial residual first; a launch that resumes
+// takes rho from fin_rho. All reductions add per-block partials in block order in every block
+// (identical scalars everywhere, deterministic). The c1 partials and the (rho, |r|^2) partials
+// live in separate halves of part4: a block may write the next reduction's partials while a
+// slower block still reads the last one's, and only the c1 -> x, r step has no other grid
+// barrier in between.
+template <int NV>
+__device__ __forceinline__ void block_sum_n(double (&v)[NV], double *lds) {
+  constexpr int NW = 256 / 64;
+#pragma unroll
+  for (int i = 0; i < NV; i++) v[i] = wave_sum(v[i]);
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int i = 0; i < NV; i++) lds[wv * NV + i] = v[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    double s = lds[i];
+#pragma unroll
+    for (int q = 1; q < NW; q++) s += lds[q * NV + i];
+    v[i] = s;
+  }
+  __syncthreads();
+}
+
+template <int NV>
+__device__ __forceinline__ void grid_sum_n(double (&v)[NV], double *part, double *lds, const GridBar &bar,
+                                           unsigned &phase, bool &abort) {
+  block_sum_n<NV>(v, lds);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < NV; i++) part[(size_t)blockIdx.x * NV + i] = v[i];
+  abort |= grid_barrier(bar, phase);
+  double s[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) s[i] = 0.0;
+  for (int g = threadIdx.x; g < (int)gridDim.x; g += 256)
+#pragma unroll
+    for (int i = 0; i < NV; i++) s[i] += part[(size_t)g * NV + i];
+  block_sum_n<NV>(s, lds);
+#pragma unroll
+  for (int i = 0; i < NV; i++) v[i] = s[i];
+}
+
+// g = G^T r' at interior node a with r' = r - alpha q recomputed per incident edge (the x, r
+// pass's operation), incidences added in list order. The first NI incidences, their r and their
+// q are all requested before the first use (a node of a tetrahedral mesh has about 14 edges).
+template <int NI>
+__device__ __forceinline__ cx<double> nodal_g_next(const int *__restrict__ nptr, const int *__restrict__ nadj,
+                                                   const cx<double> *R, const cx<double> *Q, cx<double> malpha,
+                                                   long long a) {
+  const int q0 = nptr[a], q1 = nptr[a + 1];
+  const int ql = q1 > 0 ? q1 - 1 : 0;
+  int v[NI];
+  cx<double> rr[NI], qq[NI];
+#pragma unroll
+  for (int j = 0; j < NI; j++) {
+    const int q = q0 + j;
+    const int vq = nadj[q < q1 ? q : ql];  // loaded unconditionally: a fixed count of loads
+    v[j] = q < q1 ? vq : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < NI; j++) {
+    const int e = v[j] > 0 ? v[j] - 1 : (v[j] < 0 ? -v[j] - 1 : 0);
+    rr[j] = R[e];
+    qq[j] = Q[e];
+  }
+  cx<double> g = {0.0, 0.0};
+#pragma unroll
+  for (int j = 0; j < NI; j++) {
+    const cx<double> rn = cfma(malpha, qq[j], rr[j]);
+    if (v[j] > 0) {
+      g.r += rn.r;
+      g.i += rn.i;
+    } else if (v[j] < 0) {
+      g.r -= rn.r;
+      g.i -= rn.i;
+    }
+  }
+  for (int q = q0 + NI; q < q1; q++) {
+    const int vq = nadj[q];
+    const int e = (vq > 0 ? vq : -vq) - 1;
+    const cx<double> rn = cfma(malpha, Q[e], R[e]);
+    if (vq > 0) {
+      g.r += rn.r;
+      g.i += rn.i;
+    } else {
+      g.r -= rn.r;
+      g.i -= rn.i;
+    }
+  }
+  return g;
+}
+
+// One block per CU, so each SIMD runs a single wave: the whole register file holds the
+// software-pipelined SpMV rounds and 8 (p pass) or 4 (x, r pass) rows per thread in flight.
+template <int LPR, bool PROF>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) k_cocg1(CocgParams<double> p) {
+  __shared__ double lds[4 * 4];
+  constexpr int UP = 8, UX = 4;  // rows per thread in flight in the p pass and the x, r pass
+  const int t = threadIdx.x;
+  const long long gtid = (long long)blockIdx.x * 256 + t;
+  const long long nthr = (long long)gridDim.x * 256;
+  const long long n = p.n;
+  const double w = p.wcol[0];
+  const bool diag = (p.pc == AFX_PC_DIAG);
+  const bool grad = p.grad != 0;
+  const cx<double> *__restrict__ DD = p.DD;
+  cx<double> *const buf[2] = {p.R, p.Z};
+  double *const part_c1 = p.part4, *const part_xn = p.part4 + 2 * (size_t)gridDim.x;  // [G][2], [G][3]
+  unsigned phase = 0;
+  if (t == 0) phase = __hip_atomic_load(p.bar.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  int st = p.conv[0];
+  const double bn = p.bnrm2[0];
+  cx<double> rho1 = p.rho1[0], rho = p.fin_rho[0];
+  double res = p.resid[0];
+  int its = p.iters[0];
+  int it = p.status[0];
+  const int it_end = min(it + p.chunk, p.maxit);
+  bool abort = false;
+  unsigned long long tp = 0, ts = 0, tx = 0, ttot = 0;
+
+  // edge part of rho = r^T z for the residual r (z_e = D r, or r without a preconditioner)
+  auto edge_rho = [&](cx<double> r, cx<double> d, double (&acc)[3]) {
+    const cx<double> z = diag ? cmul(r, d) : r;
+    const cx<double> rz = cmul(r, z);
+    acc[0] += rz.r;
+    acc[1] += rz.i;
+  };
+  // nodal part: zn = g / (i w mnd) stored, g^T zn added
+  auto nodal_put = [&](long long a, cx<double> g, double (&acc)[3]) {
+    const cx<double> zn = nodal_z(g, w, p.mnd[a]);
+    p.Zn[a] = zn;
+    const cx<double> gz = cmul(g, zn);
+    acc[0] += gz.r;
+    acc[1] += gz.i;
+  };
+
+  if (st == 0 && it == 0) {  // prologue: z and rho of the initial residual (in R)
+    double acc[3] = {0.0, 0.0, 0.0};
+    const cx<double> *R = p.R;
+    for (long long r0 = gtid; r0 < n; r0 += nthr) edge_rho(R[r0], diag ? DD[r0] : cx<double>{0.0, 0.0}, acc);
+    if (grad)
+      for (long long a = gtid; a < p.nn; a += nthr) nodal_put(a, nodal_g(p.nptr, p.nadj, R, (int)a), acc);
+    double v[2] = {acc[0], acc[1]};
+    grid_sum_n<2>(v, part_xn, lds, p.bar, phase, abort);
+    rho = {v[0], v[1]};
+  }
+  bool alldone = (st != 0);
+  while (!alldone && it < it_end && !abort) {
+    ++it;
+    const unsigned long long c0 = PROF ? wall_clock64() : 0;
+    const cx<double> *R = buf[(it - 1) & 1];
+    cx<double> *Rn = buf[it & 1];
+    const bool first = (it == 1);
+    const cx<double> beta = first ? cx<double>{0.0, 0.0} : cdivd(rho, rho1);
+
+    // ---- P: p = z + beta p ----
+    for (long long r0 = gtid; r0 < n; r0 += UP * nthr) {
+      cx<double> rv[UP], dv[UP], pv[UP], za[UP], zb[UP];
+      int2 ev[UP];
+#pragma unroll
+      for (int u = 0; u < UP; u++) {
+        const long long r = r0 + u * nthr < n ? r0 + u * nthr : r0;
+        rv[u] = R[r];
+        dv[u] = diag ? DD[r] : cx<double>{0.0, 0.0};
+        pv[u] = first ? cx<double>{0.0, 0.0} : p.P[r];
+        ev[u] = grad ? ((const int2 *)p.en)[r] : int2{0, 0};
+      }
+#pragma unroll
+      for (int u = 0; u < UP; u++) {
+        za[u] = grad ? p.Zn[ev[u].x] : cx<double>{0.0, 0.0};
+        zb[u] = grad ? p.Zn[ev[u].y] : cx<double>{0.0, 0.0};
+      }
+#pragma unroll
+      for (int u = 0; u < UP; u++) {
+        const long long r = r0 + u * nthr;
+        const cx<double> ze = diag ? cmul(rv[u], dv[u]) : rv[u];
+        const cx<double> z = {ze.r + (zb[u].r - za[u].r), ze.i + (zb[u].i - za[u].i)};
+        if (r < n) p.P[r] = first ? z : cfma(beta, pv[u], z);
+      }
+    }
+    abort |= grid_barrier(p.bar, phase);
+    const unsigned long long c1t = PROF ? wall_clock64() : 0;
+
+    // ---- S: q = A p, c1 = p^T q ----
+    cx<double> c1l = {0.0, 0.0};
+    spmv_pipe<LPR>(p.rowptr, p.col, p.val, w, p.P, p.Q, n, gtid / LPR, nthr / LPR, n, (int)(gtid % LPR), c1l);
+    double cacc[2] = {c1l.r, c1l.i};
+    grid_sum_n<2>(cacc, part_c1, lds, p.bar, phase, abort);
+    const unsigned long long c2t = PROF ? wall_clock64() : 0;
+    const cx<double> c1 = {cacc[0], cacc[1]};
+    const cx<double> alpha = cdivd(rho, c1);
+    if (!(c1.r != 0.0 || c1.i != 0.0) || !isfinite(alpha.r) || !isfinite(alpha.i) || !(rho.r != 0.0 || rho.i != 0.0)) {
+      st = -200;  // breakdown: x and r stay as they are
+      its = it;
+      alldone = true;
+      if (PROF) ts += c2t - c1t;
+      break;
+    }
+    const cx<double> malpha = {-alpha.r, -alpha.i};
+
+    // ---- XN: x += alpha p, r' = r - alpha q, |r'|^2, and z, rho of r' ----
+    double acc[3] = {0.0, 0.0, 0.0};  // rho' (re, im), |r'|^2
+    for (long long r0 = gtid; r0 < n; r0 += UX * nthr) {
+      cx<double> xv[UX], pv[UX], qv[UX], rv[UX], dv[UX];
+#pragma unroll
+      for (int u = 0; u < UX; u++) {
+        const long long r = r0 + u * nthr < n ? r0 + u * nthr : r0;
+        xv[u] = p.X[r];
+        pv[u] = p.P[r];
+        qv[u] = p.Q[r];
+        rv[u] = R[r];
+        dv[u] = diag ? DD[r] : cx<double>{0.0, 0.0};
+      }
+#pragma unroll
+      for (int u = 0; u < UX; u++) {
+        const long long r = r0 + u * nthr;
+        if (r < n) {
+          p.X[r] = cfma(alpha, pv[u], xv[u]);
+          const cx<double> rn = cfma(malpha, qv[u], rv[u]);
+          Rn[r] = rn;
+          acc[2] += rn.r * rn.r + rn.i * rn.i;
+          edge_rho(rn, dv[u], acc);
+        }
+      }
+    }
+    if (grad)
+      for (long long a = gtid; a < p.nn; a += nthr) nodal_put(a, nodal_g_next<16>(p.nptr, p.nadj, R, p.Q, malpha, a), acc);
+    grid_sum_n<3>(acc, part_xn, lds, p.bar, phase, abort);
+    const double rr = acc[2];
+    res = sqrt(rr / bn);  // :315 RESID = sqrt(DNRM2/BNRM2)
+    if (res <= p.tol) {
+      st = 1;
+      its = it;
+    } else if (it == p.maxit) {
+      st = -100;
+      its = it + 1;  // the Fortran DO variable ends at MAXIT + 1
+    }
+    rho1 = rho;
+    rho = {acc[0], acc[1]};
+    alldone = (st != 0);
+    if (PROF) {
+      const unsigned long long c3t = wall_clock64();
+      tp += c1t - c0;
+      ts += c2t - c1t;
+      tx += c3t - c2t;
+      ttot += c3t - c0;
+    }
+  }
+  if (PROF && t == 0) {
+    unsigned long long *d = p.dbg + (size_t)blockIdx.x * AFX_DBG_SLOTS;
+    d[0] += tp;
+    d[2] += ts;
+    d[3] += tx;
+    d[4] += ttot;
+  }
+  if (blockIdx.x == 0 && t == 0) {
+    p.conv[0] = st;
+    p.rho1[0] = rho1;
+    p.fin_rho[0] = rho;
+    p.resid[0] = res;
+    p.iters[0] = its;
+    p.status[0] = it;
+    p.status[1] = alldone ? 1 : 0;
+  }
+}
+
+template <bool PROF>
+static const void *cocg1_kernel(int lpr) {
+  switch (lpr) {
+    case 1:
+    case 2: return (const void *)k_cocg1<2, PROF>;
+    case 4: return (const void *)k_cocg1<4, PROF>;
+    case 16: return (const void *)k_cocg1<16, PROF>;
+    default: return (const void *)k_cocg1<8, PROF>;
+  }
+}
+
+// Blocks of the fused single-column kernel that are co-resident, at most one per CU.
+int max_coop_grid_cocg1(int lpr) {
+  int dev = 0, ncu = 0, nb = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, cocg1_kernel<false>(lpr), 256, 0) != hipSuccess) return 0;
+  return std::min(nb, 1) * ncu;
+}
+
+hipError_t launch_cocg1(const CocgParams<double> &p, int lpr, int grid, hipStream_t s) {
+  if (p.k != 1 || p.nsys > 1 || (p.pc != AFX_PC_DIAG && p.pc != AFX_PC_NONE) || !p.part4) return hipErrorInvalidValue;
+  void *args[] = {(void *)&p};
+  const void *f = p.dbg ? cocg1_kernel<true>(lpr) : cocg1_kernel<false>(lpr);
+  return hipLaunchCooperativeKernel(f, dim3(grid), dim3(256), args, 0, s);
 }
 
 hipError_t launch_cocg_blk(const CocgParams<double> &p, int grid, hipStream_t s) {

@@ -124,6 +124,21 @@ def bytes_per_iteration_grad(n, nnz, nn, nadj, edge_pc, nU=0, nL=0):
     return spmv + edge + nodal + pupd + xr
 
 
+def bytes_per_iteration_fused(n, nnz, nn, nadj, edge_pc):
+    """Compulsory HBM bytes of one iteration of the fused single-column kernel (k_cocg1: DIAG or
+    NONE edge preconditioner with the nodal correction), every array touched once per pass:
+    p pass: r, D (DIAG), p read, p written, the edges' end nodes (8 B per edge), zn read;
+    SpMV: (K, M) values 16 B + column 4 B per entry, row pointers, p read, q written;
+    x, r, z pass: x, p, q, r, D (DIAG) read, x and the next r written; on the nodes the incidence
+    list (4 B per entry), the row pointers, diag(G^T M G) read and zn written (r and q gathered
+    there are the arrays the edge part reads in the same pass)."""
+    d = 16 * n if edge_pc == "DIAG" else 0
+    ppass = 3 * 16 * n + d + 8 * n + 16 * nn
+    spmv = nnz * 20 + 4 * (n + 1) + 2 * 16 * n
+    xn = 6 * 16 * n + d + 4 * nadj + 4 * (nn + 1) + 8 * nn + 16 * nn
+    return ppass + spmv + xn
+
+
 def timed_solves(s, dB, dX, opts, steps, budget, label):
     """Solve from x0 = 0 `steps` times (stopping after the step that passes `budget` seconds);
     returns (steps, seconds, last stats, (launches, iterations, kernel ms))."""
@@ -237,7 +252,10 @@ def main():
         bnd[ln[bc].ravel()] = True
         nadj = int(np.count_nonzero(~bnd[ln]))  # (edge, interior end node) incidences
         nn = int(np.count_nonzero(~bnd))
-        by = bytes_per_iteration_grad(n, nnz, nn, nadj, base, s.npu, s.npl)
+        if base in ("DIAG", "NONE"):
+            by = bytes_per_iteration_fused(n, nnz, nn, nadj, base)
+        else:
+            by = bytes_per_iteration_grad(n, nnz, nn, nadj, base, s.npu, s.npl)
     else:
         by = bytes_per_iteration(n, s.npu, s.npl, 1, 16)
     achieved = by * iters / (kms / 1e3) / 1e9
@@ -245,7 +263,7 @@ def main():
     traffic, tsrc = measured_traffic(f"{nx}x{ny}x{nz}_{pc.replace('+', '_')}", ipl)
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                kernel=(f"k_cocg_flat (persistent COCG without sweeps, FP64, 1 column, {pc})"
+                kernel=(f"k_cocg1 (fused persistent COCG without sweeps, FP64, 1 column, {pc})"
                         if base in ("DIAG", "NONE") else f"k_cocg (persistent COCG, FP64, 1 column, {pc})"),
                 bytes_per_iteration=by, avg_launch_ms=round(kms / max(launches, 1), 3),
                 iterations_per_launch=round(ipl, 1), us_per_iteration=round(1e3 * kms / max(iters, 1), 1),

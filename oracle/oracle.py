@@ -24,9 +24,11 @@ def build(force: bool = False) -> str:
 
 
 def lib():
+    """The oracle library; ORC_LIB names another build of it (the sanitizer build of
+    `make -C oracle sanitize`)."""
     global _LIB
     if _LIB is None:
-        _LIB = C.CDLL(build())
+        _LIB = C.CDLL(os.environ.get("ORC_LIB") or build())
     return _LIB
 
 

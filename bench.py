@@ -404,7 +404,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(case, w0, plain["iters"] if plain else its)
+        cpu = cpu_baseline(case, w0, plain["iters"] if plain else its, bc=bc, grad_iters=its if pc == "DIAG+GRAD" else 0)
 
     out = {
         "metric": "edge-FEM solves/s to 1e-8 rel. residual, warm-start mixed vs plain COCG, 1 GPU",
@@ -720,7 +720,7 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(case, omega, plain_iters):
+def cpu_baseline(case, omega, plain_iters, bc=None, grad_iters=0):
     """The reference's CPU solver, restated: the oracle's serial COCG13_single (C, SSOR, the
     reference's row order) on the host cores of this machine.
     - value: one core on the same 1.4M-edge system; the per-iteration time is the median of 5
@@ -730,7 +730,10 @@ def cpu_baseline(case, omega, plain_iters):
     - all_cores: the iteration's parallel phases (SpMV, dot products, vector updates) on every core
       with OpenMP, the sequential sweeps on one, as an all-core COCG13 iteration would run;
     - superlu_16x16x12: a direct complex LU solve (scipy SuperLU, symmetric-mode minimum degree
-      ordering; the PARDISO-class reference) on one core, median of 3."""
+      ordering; the PARDISO-class reference) on one core, median of 3;
+    - same_algorithm (when grad_iters): the oracle's restatement of the measured form (COCG with
+      the diagonal edge preconditioner and the nodal correction, scipy, one core) on the same
+      system, the median of 3 samples of 5 iterations scaled to grad_iters -- extrapolated."""
     import scipy.sparse.linalg as sla
 
     from actfemtide_amd import mesh
@@ -779,6 +782,26 @@ def cpu_baseline(case, omega, plain_iters):
         x1 = lu.solve(np.asarray(o1["b"]))
         lu_t.append(time.perf_counter() - t0)
     lu_res = float(np.linalg.norm(o1["b"] - A1 @ x1) / np.linalg.norm(o1["b"]))
+    same = None
+    if grad_iters and bc is not None:
+        A = O.to_scipy(o["INU"], o["IAU"], o["INL"], o["IAL"], o["D"], o["AU"], o["AL"]).tocsr()
+        G, _ = O.discrete_gradient(case.line, case.node, bc)
+        corr, _ = O.grad_correction(A, G)
+        DDg = O.precond_setup("DIAG", o["D"], o["INU"], o["AU"], o["INL"], o["IAL"], o["AL"])
+        bb = np.asarray(o["b"])
+        O.cocg_precond(A, bb, lambda r: DDg * r + corr(r), tol=0.0, maxit=1)
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            O.cocg_precond(A, bb, lambda r: DDg * r + corr(r), tol=0.0, maxit=5)
+            ts.append((time.perf_counter() - t0) / 5)
+        t_g = float(np.median(ts))
+        same = dict(value=round(1.0 / (t_g * grad_iters), 6), unit="solves/s", cores=1, kind="port",
+                    extrapolated=True, ms_per_iteration=round(1e3 * t_g, 2), iters=grad_iters,
+                    note="the oracle's scipy restatement of the measured form (COCG, diagonal edge "
+                         "preconditioner + nodal gradient-space correction) on the same system, median of "
+                         "3 samples of 5 iterations scaled to the GPU solve's iteration count")
+        log(f"cpu baseline, same algorithm: {1e3 * t_g:.1f} ms per iteration, {same['value']} solves/s")
     log(f"cpu baseline: {1e3 * t_it:.1f} ms per iteration on 1 core ({1e3 * t_sweep:.1f} ms sweeps, parallel "
         f"phases {1e3 * t_par1:.1f} ms on 1 core, {1e3 * t_parn:.1f} ms on {ncores}); 16x16x12 full solve "
         f"{np.median(full):.2f}s ({r1['iter']} iterations), SuperLU {np.median(lu_t):.2f}s")
@@ -800,7 +823,8 @@ def cpu_baseline(case, omega, plain_iters):
                                          solves_per_s=round(1.0 / float(np.median(full)), 4)),
                 superlu_16x16x12=dict(seconds=round(float(np.median(lu_t)), 3), cores=1, edges=c1.nline,
                                       residual=lu_res, note="scipy SuperLU complex128 factor + solve, "
-                                                            "MMD_AT_PLUS_A ordering, symmetric mode"))
+                                                            "MMD_AT_PLUS_A ordering, symmetric mode"),
+                same_algorithm=same)
 
 
 if __name__ == "__main__":

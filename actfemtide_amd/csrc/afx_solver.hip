@@ -1248,6 +1248,46 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
 // recurrence and unconjugated forms). Reductions: per-wave XOR-shuffle sums, per-block sums in
 // LDS, per-block partials [G][k] added in block order by every block (deterministic).
 
+// g = G^T r at interior node a for column c of the [n][k] vector R, incidences added in list order;
+// the first NI incidences and their r are all requested before the first use.
+template <int NI>
+__device__ __forceinline__ cx<double> nodal_g_cols(const int *__restrict__ nptr, const int *__restrict__ nadj,
+                                                   const cx<double> *R, int k, int c, int a) {
+  const int q0 = nptr[a], q1 = nptr[a + 1];
+  const int ql = q1 > 0 ? q1 - 1 : 0;
+  int v[NI];
+  cx<double> rr[NI];
+#pragma unroll
+  for (int j = 0; j < NI; j++) {
+    const int q = q0 + j;
+    const int vq = nadj[q < q1 ? q : ql];  // loaded unconditionally: a fixed count of loads
+    v[j] = q < q1 ? vq : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < NI; j++) {
+    const int e = v[j] > 0 ? v[j] - 1 : (v[j] < 0 ? -v[j] - 1 : 0);
+    rr[j] = R[(size_t)e * k + c];
+  }
+  cx<double> g = {0.0, 0.0};
+#pragma unroll
+  for (int j = 0; j < NI; j++) {
+    if (v[j] > 0) {
+      g.r += rr[j].r;
+      g.i += rr[j].i;
+    } else if (v[j] < 0) {
+      g.r -= rr[j].r;
+      g.i -= rr[j].i;
+    }
+  }
+  for (int q = q0 + NI; q < q1; q++) {
+    const int vq = nadj[q];
+    const cx<double> r = R[(size_t)((vq > 0 ? vq : -vq) - 1) * k + c];
+    g.r += vq > 0 ? r.r : -r.r;
+    g.i += vq > 0 ? r.i : -r.i;
+  }
+  return g;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
@@ -1368,13 +1408,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     if (p.grad && eact)
       for (long long i = gtid; i < nnk; i += estride) {  // (node, column) elements
         const int a = (int)(i / k);
-        cx<double> g = {0.0, 0.0};
-        for (int q = p.nptr[a]; q < p.nptr[a + 1]; q++) {
-          const int v = p.nadj[q];
-          const cx<double> r = p.R[(size_t)((v > 0 ? v : -v) - 1) * k + ec];
-          g.r += v > 0 ? r.r : -r.r;
-          g.i += v > 0 ? r.i : -r.i;
-        }
+        const cx<double> g = nodal_g_cols<16>(p.nptr, p.nadj, p.R, k, ec, a);
         const cx<double> zn = nodal_z(g, we, p.mnd[a]);
         p.Zn[i] = zn;
         mine = addv(mine, cmul(g, zn));
@@ -1438,14 +1472,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     }
     abort |= grid_barrier(p.bar, phase);
     AFX_BLK_MARK(2)
-    // ---- q = A p, c1 = p^T q: LPR lanes per row, a row's entries read once for all columns ----
+    // ---- q = A p, c1 = p^T q: LPR lanes per row, a row's entries read once for all columns;
+    // every gather of a row (its SE entries x all K columns, a column block being one contiguous
+    // run of the [n][k] vector) is requested before the first product. The row's first entry is
+    // its diagonal, so lane 0's first gathers are the row's own p, which c1 needs. (32 lanes per
+    // row, a lane per (entry slot, column), gathered each column block with one instruction but
+    // kept 8x fewer rows in flight: 4x slower at 64x64x48.) ----
 #pragma unroll
     for (int c = 0; c < K; c++) acc[c] = cx<double>{0.0, 0.0};
     {
       const int lane = t % LPR;
       for (int row = gtid / LPR; row < n; row += nthr / LPR) {
         const int qb = p.rowptr[row], qe = p.rowptr[row + 1];
-        cx<double> vv[SE];
+        cx<double> vv[SE], pv[SE][K];
         int cl[SE];
 #pragma unroll
         for (int e = 0; e < SE; e++) {
@@ -1457,23 +1496,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
           cl[e] = in ? cq : -1;
         }
 #pragma unroll
-        for (int c = 0; c < K; c++)
-          if (c < k) {
-            cx<double> sum = {0.0, 0.0};
+        for (int e = 0; e < SE; e++)
 #pragma unroll
-            for (int e = 0; e < SE; e++) {
-              const cx<double> pv = p.P[(size_t)max(cl[e], 0) * k + c];
-              if (cl[e] >= 0) sum = cfma(mval(vv[e], w[c]), pv, sum);
-            }
-            for (int q = qb + lane + SE * LPR; q < qe; q += LPR)
-              sum = cfma(mval(p.val[q], w[c]), p.P[(size_t)p.col[q] * k + c], sum);
-            sum = group_sum<LPR>(sum);
-            if (lane == 0) {
-              const size_t ix = (size_t)row * k + c;
-              p.Q[ix] = sum;
-              acc[c] = addv(acc[c], cmul(p.P[ix], sum));
-            }
+          for (int c = 0; c < K; c++) pv[e][c] = p.P[(size_t)max(cl[e], 0) * k + (c < k ? c : k - 1)];
+        cx<double> sum[K];
+#pragma unroll
+        for (int c = 0; c < K; c++) {
+          sum[c] = cx<double>{0.0, 0.0};
+#pragma unroll
+          for (int e = 0; e < SE; e++)
+            if (cl[e] >= 0) sum[c] = cfma(mval(vv[e], w[c]), pv[e][c], sum[c]);
+        }
+        for (int q = qb + lane + SE * LPR; q < qe; q += LPR)
+#pragma unroll
+          for (int c = 0; c < K; c++)
+            if (c < k) sum[c] = cfma(mval(p.val[q], w[c]), p.P[(size_t)p.col[q] * k + c], sum[c]);
+#pragma unroll
+        for (int c = 0; c < K; c++) {
+          sum[c] = group_sum<LPR>(sum[c]);
+          if (lane == 0 && c < k) {
+            p.Q[(size_t)row * k + c] = sum[c];
+            acc[c] = addv(acc[c], cmul(pv[0][c], sum[c]));
           }
+        }
       }
     }
     AFX_BLK_MARK(3)

@@ -61,8 +61,16 @@ def test_batched_columns(case_small, sysdev, mode, monkeypatch):
         assert np.linalg.norm(B[:, c] - A @ X[:, c]) / np.linalg.norm(B[:, c]) <= 1.05e-8
     monkeypatch.delenv("AFX_BATCH", raising=False)
     x1, st1 = s.solve(b, precond="SSOR", tol=1e-8, maxit=40000)
-    if mode == "stack":  # the copies' reductions sum in another order: the counts move with rounding
-        assert all(x.iter > 0 and x.resid <= 1e-8 for x in st)
+    if mode == "stack":
+        # every copy runs its own COCG13_single recurrence with per-system scalars: each column's
+        # count is the oracle's (the reference's order) for that right-hand side, up to rounding
+        from oracle import oracle as O
+        o = O.forward_system(case_small, OMEGA_M2)
+        for c in range(4):
+            r = O.cocg13(o["INU"], o["IAU"], o["INL"], o["IAL"], o["D"], o["AU"], o["AL"], B[:, c], precond="SSOR",
+                         resid=1e-8, maxit=40000)
+            print(f"stacked copy {c}: {st[c].iter} iterations, oracle {r['iter']}", flush=True)
+            assert st[c].resid <= 1e-8 and abs(st[c].iter - r["iter"]) <= max(3, 0.08 * r["iter"])
     else:
         assert abs(st[0].iter - st1[0].iter) <= max(3, 0.08 * st1[0].iter)
     if mode == "columns":  # the same kernel on the same column: the same iterates

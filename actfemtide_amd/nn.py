@@ -45,7 +45,7 @@ SIGNATURES = {
     "afx_nn_transpose": ([_VP, _I, _I, _I, _I, _VP, _I, _VP], _I),
     "afx_nn_cast_f2b": ([_VP, _VP, _LL, _VP], _I),
     "afx_nn_relu_bwd": ([_VP, _VP, _VP, _VP, _LL, _VP], _I),
-    "afx_nn_colsum": ([_VP, _I, _I, _VP, _I, _VP], _I),
+    "afx_nn_colsum": ([_VP, _I, _I, _VP, _I, _VP, _VP], _I),
     "afx_nn_add_bf16": ([_VP, _VP, _VP, _LL, _VP], _I),
     "afx_nn_add_f32": ([_VP, _VP, _VP, _LL, _VP], _I),
     "afx_nn_mse": ([_VP, _I, _VP, _VP, _LL, _I, _F, _VP, _VP, _VP, _VP], _I),
@@ -100,18 +100,31 @@ def _stream():
 # ----------------------------------------------------------------------------------------------
 # primitive wrappers (device tensors in, device tensors out)
 
-def gemm_nt(A, B, M, N, K, C_out=None, Cb=None, bias=None, relu=False, splits=1, lda=None, ldb=None, ldc=None,
+def gemm_nt(A, B, M, N, K, C_out=None, Cb=None, bias=None, relu=False, splits=None, lda=None, ldb=None, ldc=None,
             ldcb=None, bias_period=0):
-    """C[m][n] = sum_k A[m][k] B[n][k] (+bias) (ReLU) on the BF16 MFMA GEMM."""
+    """C[m][n] = sum_k A[m][k] B[n][k] (+bias) (ReLU) on the BF16 MFMA GEMM. splits=None splits K
+    when the output has fewer 128-wide tiles than the GPU has CUs and K is long (the deep levels),
+    so that every CU has work; the partial sums are added in split order (deterministic)."""
     torch = _torch()
     lda = lda or A.shape[-1]
     ldb = ldb or B.shape[-1]
+    if splits is None:
+        nw = 4 if N > 64 else (2 if N > 32 else 1)
+        tiles = ((M + 127) // 128) * ((N + 32 * nw - 1) // (32 * nw))
+        splits = max(1, min(K // 256, -(-1024 // tiles))) if (tiles < 256 and K >= 1024) else 1
     part = None
     if splits > 1:
         part = torch.empty(splits * M * N, dtype=torch.float32, device=A.device)
     _call("afx_nn_gemm_nt", _p(A), lda, _p(B), ldb, M, N, K, _p(C_out), ldc or (C_out.shape[-1] if C_out is not None else 0),
           _p(Cb), ldcb or (Cb.shape[-1] if Cb is not None else 0), _p(bias), bias_period, 1 if relu else 0, splits,
           _p(part), _stream())
+
+
+def colsum(X, R, Cc, out, accumulate=False):
+    """out[c] (+)= sum_r X[r][c] for an f32 [R][Cc] device array (two deterministic passes)."""
+    torch = _torch()
+    part = torch.empty(((R + 255) // 256) * Cc, dtype=torch.float32, device=out.device)
+    _call("afx_nn_colsum", _p(X), R, Cc, _p(out), 1 if accumulate else 0, _p(part), _stream())
 
 
 def _splits(M, N, K):
@@ -187,7 +200,7 @@ class Conv3:
         dP = torch.empty((V, co), dtype=torch.float32, device=dy.device)
         dPb = torch.empty((V, co), dtype=torch.bfloat16, device=dy.device)
         _call("afx_nn_relu_bwd", _p(dy), _p(y), _p(dP), _p(dPb), V * co, _stream())
-        _call("afx_nn_colsum", _p(dP), V, co, _p(self.b.g), 0, _stream())
+        colsum(dP, V, co, self.b.g)
         Vp = _ru(V, 32)
         dPt = transpose(dP, V, co, Vp)
         colsT = torch.empty((self.kp, Vp), dtype=torch.bfloat16, device=dy.device)
@@ -235,7 +248,7 @@ class Down:
         dP = torch.empty((Vo, co), dtype=torch.float32, device=dy.device)
         dPb = torch.empty((Vo, co), dtype=torch.bfloat16, device=dy.device)
         _call("afx_nn_relu_bwd", _p(dy), _p(y), _p(dP), _p(dPb), Vo * co, _stream())
-        _call("afx_nn_colsum", _p(dP), Vo, co, _p(self.b.g), 0, _stream())
+        colsum(dP, Vo, co, self.b.g)
         Vp = _ru(Vo, 32)
         dPt = transpose(dP, Vo, co, Vp)
         St = transpose(S, Vo, 8 * ci, Vp)
@@ -283,7 +296,7 @@ class Up:
         dP = torch.empty((V, 8 * co), dtype=torch.float32, device=dy.device)
         dPb = torch.empty((V, 8 * co), dtype=torch.bfloat16, device=dy.device)
         _call("afx_nn_relu_bwd", _p(dyc), _p(yc), _p(dP), _p(dPb), V * 8 * co, _stream())
-        _call("afx_nn_colsum", _p(dP), 8 * V, co, _p(self.b.g), 0, _stream())
+        colsum(dP, 8 * V, co, self.b.g)
         Vp = _ru(V, 32)
         dPt = transpose(dP, V, 8 * co, Vp)
         xt = transpose(x, V, ci, Vp)
@@ -320,7 +333,7 @@ class Head:
         torch = _torch()
         (x,) = self.saved
         V = x.shape[0]
-        _call("afx_nn_colsum", _p(dout), V, 32, _p(self.b.g), 0, _stream())
+        colsum(dout, V, 32, self.b.g)
         Vp = _ru(V, 32)
         dt = transpose(dout, V, 32, Vp)
         xt = transpose(x, V, self.ci, Vp)

@@ -57,7 +57,8 @@ int afx_nn_cast_f2b(const float *x, uint16_t *y, long long n, void *stream);
 /* dP = dY * (Y > 0) with Y the bf16 ReLU output; dPb (optional) its bf16 copy. */
 int afx_nn_relu_bwd(const float *dY, const uint16_t *Y, float *dP, uint16_t *dPb, long long n, void *stream);
 /* out[c] (+)= sum_r X[r][c], X [R][Cc] f32. */
-int afx_nn_colsum(const float *X, int R, int Cc, float *out, int accumulate, void *stream);
+/* out[c] (+)= sum_r X[r][c], X [R][Cc] f32; part: workspace of ceil(R / 256) * Cc floats */
+int afx_nn_colsum(const float *X, int R, int Cc, float *out, int accumulate, float *part, void *stream);
 int afx_nn_add_bf16(const uint16_t *a, const uint16_t *b, uint16_t *y, long long n, void *stream);
 int afx_nn_add_f32(const float *a, const float *b, float *y, long long n, void *stream);
 /* loss = sum over rows with mask != 0 (mask may be NULL) of |out[r][0:Cc] - tgt[r]|^2 * inv_count,

@@ -55,6 +55,26 @@ def test_gemm_nt_matches_cpu():
         assert err < 1e-5
 
 
+def test_gemm_nt_automatic_split_k_with_epilogue():
+    """A deep-level shape (few output tiles, long K): the wrapper splits K by itself, and the
+    split-order sum then applies the bias, the ReLU and both outputs (FP32 and bf16)."""
+    from actfemtide_amd import nn
+    g = torch.Generator().manual_seed(1)
+    M, N, K = 192, 256, 6912
+    A = _bf(torch.randn(M, K, generator=g))
+    B = _bf(torch.randn(N, K, generator=g))
+    bias = torch.randn(N, generator=g)
+    ref = torch.relu(A @ B.T + bias)
+    C = torch.zeros(M, N, dtype=torch.float32, device=_dev())
+    Cb = torch.zeros(M, N, dtype=torch.bfloat16, device=_dev())
+    nn.gemm_nt(A.to(torch.bfloat16).to(_dev()), B.to(torch.bfloat16).to(_dev()), M, N, K, C_out=C,
+               Cb=Cb.view(torch.int16), bias=bias.to(_dev()), relu=True)
+    torch.cuda.synchronize()
+    err, errb = _rel(C.cpu(), ref), _rel(Cb.float().cpu(), ref)
+    print(f"gemm M={M} N={N} K={K} automatic split: rel err {err:.2e}, bf16 output {errb:.2e}")
+    assert err < 1e-5 and errb < 1e-2
+
+
 def _conv_w(layer):
     """torch conv3d weight [co][ci][3][3][3] from Conv3's W [co][27*ci padded] (tap = (dz*3+dy)*3+dx)."""
     co, ci = layer.co, layer.ci

@@ -174,3 +174,77 @@ def warm_solve(net, data: Dataset, case, tol=1e-8, precond="DIAG+GRAD", maxit=40
     return dict(iters_cold=st_c[0].iter, iters_warm=st_w[0].iter, t_cold=t_cold, t_warm=t_warm,
                 r0_given=float(r_given[0]), r0_scaled=float(r_scaled[0]), gamma=complex(g[0]),
                 true_cold=float(res_c[0]), true_warm=float(res[0]), fallback=fallback)
+
+
+def time_training_step(grid=(32, 32, 24), base=32, batch=4, steps=10, warmup=3, seed=0, device="cuda"):
+    """Wall time of one Adam step (forward, MSE loss, backward, update) of nn.Surrogate on
+    synthetic features and targets of the grid's shape (the kernels' work does not depend on
+    the values). Returns ms per step, the GEMM calls of one step as (M, N, K, splits) (splits None:
+    chosen by nn.gemm_nt) and their flops."""
+    import torch
+    nx, ny, nz = grid
+    net = nn.Surrogate(base=base, cin=8, seed=seed)
+    g = torch.Generator(device=device).manual_seed(seed)
+    x = torch.randn(batch * nx * ny * nz, 8, device=device, generator=g).to(torch.bfloat16)
+    tgt = torch.randn(batch * nx * ny * nz, 6, device=device, generator=g)
+
+    def step():
+        out = net.forward(x, (batch, nz, ny, nx))
+        _, grad = net.loss(out, tgt)
+        net.backward(grad)
+        net.step(lr=1e-3)
+
+    for _ in range(warmup):
+        step()
+    calls = []
+    gemm = nn.gemm_nt
+
+    def recording(A, B, M, N, K, *a, **kw):
+        calls.append((M, N, K, kw.get("splits")))
+        return gemm(A, B, M, N, K, *a, **kw)
+
+    nn.gemm_nt = recording
+    try:
+        step()
+    finally:
+        nn.gemm_nt = gemm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    return ms, calls, sum(2.0 * M * N * K for M, N, K, _ in calls)
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="time surrogate training steps")
+    ap.add_argument("--grid", type=int, nargs=3, default=(32, 32, 24), metavar=("NX", "NY", "NZ"))
+    ap.add_argument("--base", type=int, default=32)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--gemm-shapes", type=int, default=0, metavar="N",
+                    help="also time the step's N largest GEMM shapes on their own (10 launches each)")
+    a = ap.parse_args()
+    ms, calls, flops = time_training_step(tuple(a.grid), a.base, a.batch, a.steps)
+    print(f"{len(calls)} GEMM calls per step, {flops / 1e9:.1f} GFLOP; {ms:.2f} ms per training step, "
+          f"{flops / ms / 1e9:.1f} TFLOP/s of GEMM work over the whole step", flush=True)
+    if a.gemm_shapes:
+        import torch
+        for M, N, K, splits in sorted(set(calls), key=lambda c: -c[0] * c[1] * c[2])[:a.gemm_shapes]:
+            A = torch.randn(M, K, device="cuda").to(torch.bfloat16).view(torch.int16)
+            B = torch.randn(N, K, device="cuda").to(torch.bfloat16).view(torch.int16)
+            C = torch.empty(M, N, device="cuda")
+            for _ in range(3):
+                nn.gemm_nt(A, B, M, N, K, C_out=C, splits=splits)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(10):
+                nn.gemm_nt(A, B, M, N, K, C_out=C, splits=splits)
+            e1.record()
+            torch.cuda.synchronize()
+            us = 1e3 * e0.elapsed_time(e1) / 10
+            io = 2.0 * (M * K + N * K) + 4.0 * M * N
+            print(f"  M={M} N={N} K={K} (splits {splits or 'auto'}): {us:.1f} us, {2.0 * M * N * K / us / 1e6:.1f} TFLOP/s, "
+                  f"{io / us / 1e3:.0f} GB/s of operand + output bytes", flush=True)

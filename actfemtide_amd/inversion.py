@@ -52,6 +52,34 @@ def sigma_of_model(case, par_ptr, par_tet, m):
     return s
 
 
+def central_sites(case, n, seed, half=3):
+    """Seeded synthetic sites 1 m above the sea bottom over the central (2 half)^2 cells, each
+    strictly inside a tet (away from the cell's diagonal faces)."""
+    rng = np.random.default_rng(seed)
+    h = case.h
+    pts = []
+    for _ in range(n):
+        i = case.nx // 2 + rng.integers(-half, half)
+        j = case.ny // 2 + rng.integers(-half, half)
+        fx, fy = rng.uniform(0.13, 0.87, size=2)
+        if abs(fx - fy) < 0.05:
+            fy = min(0.87, fy + 0.1)
+        pts.append([-case.nx * h / 2 + (i + fx) * h, -case.ny * h / 2 + (j + fy) * h, -h + 0.001])
+    return np.array(pts)
+
+
+def block_model(case, cells, value, background, half=2, layers=3):
+    """A synthetic model: log10 sigma `value` in the crust cells within `half` cells of the centre
+    horizontally and in the top `layers` crust layers, `background` elsewhere. Returns (model,
+    mask of the block)."""
+    nx, ny = case.nx, case.ny
+    i, j, k = cells % nx, (cells // nx) % ny, cells // (nx * ny)
+    inside = (abs(i - nx // 2) <= half) & (abs(j - ny // 2) <= half) & (k > k.max() - layers)
+    m = np.full(cells.size, float(background))
+    m[inside] = value
+    return m, inside
+
+
 def wrap_degrees(d):
     return (np.asarray(d) + 180.0) % 360.0 - 180.0
 
@@ -140,7 +168,9 @@ class GaussNewton:
         import torch
         if not self.warm:
             dX.zero_()
-        st, rc = self.s.solve_device(dB, dX, omega_scale=scale, precond=self.precond, tol=self.tol, maxit=self.maxit)
+        # up to two FP64 restarts from the iterate when the true residual is still above tol
+        st, rc = self.s.solve_device(dB, dX, omega_scale=scale, precond=self.precond, tol=self.tol, maxit=self.maxit,
+                                     max_outer=3)
         tr = self.s.true_residual_device(dB, dX, omega_scale=scale)
         if rc != 0 or not np.all(tr <= self.tol * 1.05):
             raise afx.AfxError(rc if rc != 0 else afx.AFX_ERR_MAXIT,

@@ -27,7 +27,10 @@ constituents of this mesh (one column at a time, the k-column launch, and a warm
 constituent starts from the previous one's solution); solvePARDISOinv's 48 adjoint columns
 (16 seafloor sites x 3 electric components) on the 32x32x24-cell mesh, one column at a time
 against the autotuned block width; the 16-mesh uneven batch one mesh at a time against one
-per-system launch.
+per-system launch; 3 Gauss-Newton updates on the 32x32x24-cell mesh (5 constituents, 16 sites x
+3 magnetic components: 5 forward and 240 adjoint solves, the Jacobian of every crust cell and the
+model update per update), with every solve warm-started from the previous update's solution
+against every solve started from zero (--gn-updates N with --no-variants runs only that).
 
 Variants, off by default (their results are documented):
   --mixed    complex-FP32 preconditioner inside FP64 refinement (same tolerance);
@@ -189,6 +192,8 @@ def main():
                     help="also time the 5 constituents stacked as 5 systems in one launch")
     ap.add_argument("--mesh-batch", action="store_true",
                     help="also time 16 meshes of 8x8x6 to 40x40x30 cells one by one and in one launch")
+    ap.add_argument("--gn-updates", type=int, default=0, metavar="N",
+                    help="with --no-variants: time N warm- and cold-started Gauss-Newton updates on 32x32x24")
     ap.add_argument("--maxit", type=int, default=400000, help="iteration cap of each solve")
     ap.add_argument("--time-budget", type=float, default=60.0,
                     help="seconds of timed solves: the timed loop stops after the first step that ends past it "
@@ -362,12 +367,15 @@ def main():
         # x 3 electric components) on the 32x32x24-cell mesh; the 16-mesh uneven batch
         for name, fn in (("constituents_5", lambda: constituents_variant(s, b, omegas, w0, opts)),
                          ("adjoint_48_columns_32x32x24", lambda: adjoint32(pc, opts, w0)),
-                         ("mesh_batch_16", lambda: mesh_batch_variant(opts, w0, grad_only=True))):
+                         ("mesh_batch_16", lambda: mesh_batch_variant(opts, w0, grad_only=True)),
+                         ("gauss_newton_warm_starts_32x32x24", lambda: gn_variant())):
             try:
                 variants[name] = fn()
             except Exception as e:  # noqa: BLE001
                 variants[name] = {"error": f"{type(e).__name__}: {e}"}
             os.environ.pop("AFX_BATCH", None)
+    if args.gn_updates and args.no_variants:
+        variants["gauss_newton_warm_starts_32x32x24"] = gn_variant(updates=args.gn_updates)
     if args.rhs_stack:
         variants["stacked_5_constituents"] = rhs_stack_variant(s, b, omegas, w0, opts)
     if args.mesh_batch:
@@ -484,6 +492,48 @@ def seafloor_sites(case, n, seed):
             fy = min(0.87, fy + 0.1)
         pts.append([-xout + (i + fx) * h, -yout + (j + fy) * h, -h - 0.001])
     return np.array(pts)
+
+
+def gn_variant(nsite=16, updates=3, grid=None):
+    """Gauss-Newton updates (actfemtide_amd.inversion) on the 32x32x24-cell mesh with the 5 tidal
+    constituents: per update 5 forward solves and 5 x 3 nsite adjoint solves (Bx, By, Bz rows at
+    nsite sites 1 m above the sea bottom), the Jacobian of every crust cell, the model update.
+    Run twice on the same synthetic data (a conductive block): every solve warm-started from the
+    previous update's solution, and every solve from zero. Reports per update the solve seconds
+    (synchronised), iterations and nRMS, and the warm run's speedup on the solves of updates 2..N."""
+    import torch
+
+    from actfemtide_amd import inversion, mesh
+    case = mesh.tide_case(*(grid or SIZES["cfg3"]), seed=1003, sigma_sea=3.3)
+    sites = inversion.central_sites(case, nsite, 5, half=4)
+    omegas = [2 * math.pi / mesh.PERIODS[c] for c in CONSTITUENTS]
+    ncell = len(inversion.crust_parameters(case)[2])
+    out = dict(mesh="x".join(map(str, grid or SIZES["cfg3"])), sites=nsite, constituents=len(omegas),
+               adjoint_columns_per_update=3 * nsite * len(omegas), parameters=ncell, updates=updates)
+    for warm in (True, False):
+        gn = inversion.GaussNewton(case, sites, omegas, np.full(ncell, -2.0), beta=10.0, gamma=0.1, warm=warm)
+        m_true, _ = inversion.block_model(case, gn.cells, -1.5, -2.0, half=3)
+        F, _, _ = gn.forward(m_true)
+        gn.d_obs = gn.data(F)
+        gn.dX.zero_()
+        ups, _, nr = gn.run(gn.m_ref, updates=updates)
+        torch.cuda.synchronize()
+        out["warm" if warm else "cold"] = dict(
+            nrms=[round(u.nrms, 4) for u in ups] + [round(nr, 4)],
+            solve_seconds=[round(u.solve_seconds, 3) for u in ups],
+            update_seconds=[round(u.seconds, 3) for u in ups],
+            forward_iters=[sum(u.forward_iters) for u in ups],
+            adjoint_iters=[int(sum(map(sum, u.adjoint_iters))) for u in ups],
+            worst_true_resid=max(u.worst_true_resid for u in ups))
+        gn.close()
+        log(f"gauss-newton ({'warm' if warm else 'cold'}): {out['warm' if warm else 'cold']}")
+    w, c = out["warm"], out["cold"]
+    out["speedup_solves_after_first"] = round(sum(c["solve_seconds"][1:]) / sum(w["solve_seconds"][1:]), 3)
+    out["note"] = ("synthetic inversion of a conductive block (log10 sigma -1.5 in -2, 7x7x3 cells) from Bx, By, "
+                   "Bz amplitude and phase; warm: each "
+                   "update's solves start from the previous update's solutions; cold: from zero; both reach "
+                   "a 1e-8 FP64 true residual in every solve")
+    return out
 
 
 def adjoint32(pc, opts, omega):

@@ -11,6 +11,9 @@ two constituents (M2, K1) on the 16x16x12 mesh, from the uniform reference model
 - the same updates run with cold starts give the same models (the solves differ only within
   the tolerance);
 - the warm-started solves of updates 2 and 3 need fewer iterations than the cold ones.
+The Jacobian rows the updates use (log10 amplitude and phase in degrees of the B components,
+genjacobian1 :338-345, from the adjoint columns) are checked against central differences of
+tight forward solves.
 """
 import math
 
@@ -22,41 +25,16 @@ pytestmark = pytest.mark.gpu
 OMEGAS = [2 * math.pi / 44714.0, 2 * math.pi / 86164.0]
 
 
-def _central_sites(case, n, seed):
-    """Seeded sites 1 m above the sea bottom over the central 6x6 cells, strictly inside a tet."""
-    rng = np.random.default_rng(seed)
-    h = case.h
-    pts = []
-    for _ in range(n):
-        i = case.nx // 2 + rng.integers(-3, 3)
-        j = case.ny // 2 + rng.integers(-3, 3)
-        fx, fy = rng.uniform(0.13, 0.87, size=2)
-        if abs(fx - fy) < 0.05:
-            fy = min(0.87, fy + 0.1)
-        pts.append([-case.nx * h / 2 + (i + fx) * h, -case.ny * h / 2 + (j + fy) * h, -h + 0.001])
-    return np.array(pts)
-
-
-def _block_model(case, par_cells, value, background):
-    nx, ny = case.nx, case.ny
-    i, j, k = par_cells % nx, (par_cells // nx) % ny, par_cells // (nx * ny)
-    ktop = k.max()
-    m = np.full(par_cells.size, background)
-    inside = (abs(i - nx // 2) <= 2) & (abs(j - ny // 2) <= 2) & (k >= ktop - 2)
-    m[inside] = value
-    return m, inside
-
-
 @pytest.fixture(scope="module")
 def runs():
     from actfemtide_amd import inversion, mesh
     case = mesh.tide_case(16, 16, 12, seed=1001)
-    sites = _central_sites(case, 8, 77)
+    sites = inversion.central_sites(case, 8, 77)
     out = {}
     for warm in (True, False):
         gn = inversion.GaussNewton(case, sites, OMEGAS, np.full(len(inversion.crust_parameters(case)[2]), -2.0),
                                    beta=10.0, gamma=0.1, warm=warm)
-        m_true, inside = _block_model(case, gn.cells, -0.7, -2.0)
+        m_true, inside = inversion.block_model(case, gn.cells, -0.7, -2.0)
         F, _, _ = gn.forward(m_true)
         gn.d_obs = gn.data(F)
         gn.dX.zero_()
@@ -97,3 +75,49 @@ def test_warm_starts_save_iterations(runs):
         aw, ac = sum(map(sum, uw[k].adjoint_iters)), sum(map(sum, uc[k].adjoint_iters))
         print(f"update {k + 1}: forward {fw} warm / {fc} cold, adjoint {aw} warm / {ac} cold", flush=True)
         assert fw < fc and aw < ac
+
+
+def test_ampphase_jacobian_rows_against_central_differences():
+    """The rows of the first update's Jacobian for the 4 most sensitive crust cells and one weak
+    one, against central differences (step 1e-4 in log10 sigma) of forward solves at a 2e-11 true
+    residual: within 2e-4 relative for the sensitive cells (the step's truncation error is about
+    5e-5 there), 1e-2 for the weak one (whose difference quotient carries the solves' noise).
+    FP64 leaves the true residual of these solves near 1e-11, so they are asked for 2e-11."""
+    from actfemtide_amd import afx, inversion, mesh
+    case = mesh.tide_case(16, 16, 12, seed=1001)
+    sites = inversion.central_sites(case, 8, 77)
+    gn = inversion.GaussNewton(case, sites, OMEGAS, np.full(len(inversion.crust_parameters(case)[2]), -2.0),
+                               tol=2e-11)
+    m0 = gn.m_ref.copy()
+    F0, _, _ = gn.forward(m0)
+    gn.d_obs = gn.data(F0)
+    captured = {}
+    real_update = afx.gn_update
+
+    def capture(dJ, *a, **k):
+        captured["J"] = dJ.cpu().numpy().copy()
+        return real_update(dJ, *a, **k)
+
+    afx.gn_update = capture
+    try:
+        gn.update(m0)
+    finally:
+        afx.gn_update = real_update
+    J = captured["J"]
+    order = np.argsort(-np.abs(J).sum(axis=0))
+    h, nob = 1e-4, gn.nobs
+    for rank, j in enumerate(list(order[:4]) + [order[200]]):
+        dp, dm = m0.copy(), m0.copy()
+        dp[j] += h
+        dm[j] -= h
+        Fp, _, _ = gn.forward(dp)
+        Fm, _, _ = gn.forward(dm)
+        diff = gn.data(Fp) - gn.data(Fm)
+        for f in range(gn.nf):
+            k0 = 2 * nob * f + nob
+            diff[k0:k0 + nob] = inversion.wrap_degrees(diff[k0:k0 + nob])
+        fd = diff / (2 * h)
+        err = np.abs(J[:, j] - fd).max() / np.abs(fd).max()
+        print(f"cell {gn.cells[j]}: max |row| {np.abs(fd).max():.3e}, adjoint vs central difference {err:.1e}", flush=True)
+        assert err <= (2e-4 if rank < 4 else 1e-2)
+    gn.close()

@@ -192,15 +192,22 @@ def tide_case(nx: int, ny: int, nz: int, seed: int, h: float = 2.5, *, spheres: 
               sigma_sea: float = SIGMA_SEA_REFERENCE, sigma_air: float = SIGMA_AIR_DEFAULT,
               crust_mean: float = -2.0, crust_std: float = 0.5,
               log10_sigma_field: np.ndarray | None = None,
-              F=(0.0, 3.0e4, -3.5e4)) -> TideCase:
+              F=(0.0, 3.0e4, -3.5e4), like: TideCase | None = None) -> TideCase:
     """Build a complete synthetic case.
 
     Crust log10(sigma) is drawn per cell from N(crust_mean, crust_std) with
     numpy.random.default_rng(seed) (all 6 tets of a cell share it), unless a full
     (nz, ny, nx) `log10_sigma_field` is given. `spheres` adds that many 1 S/m balls of
-    radius 3 cells with seeded centres inside the crust."""
-    xyz, n4, group, nodes_ocean, cell_of_tet = box_mesh(nx, ny, nz, h)
-    line, n6line = _lines.mkline_mkn6(xyz.shape[0], n4)
+    radius 3 cells with seeded centres inside the crust. `like`: a case of the same box whose
+    mesh arrays (nodes, tets, groups, edges) are shared instead of built again."""
+    if like is not None:
+        if (like.nx, like.ny, like.nz, like.h) != (nx, ny, nz, h):
+            raise ValueError("`like` is a case of another box")
+        xyz, n4, group, nodes_ocean, cell_of_tet = like.xyz, like.n4, like.group, like.nodes_ocean, like.cell_of_tet
+        line, n6line = like.line, like.n6line
+    else:
+        xyz, n4, group, nodes_ocean, cell_of_tet = box_mesh(nx, ny, nz, h)
+        line, n6line = _lines.mkline_mkn6(xyz.shape[0], n4)
     rng = np.random.default_rng(seed)
     n_crust = nz - 4
     if log10_sigma_field is None:

@@ -24,11 +24,12 @@ import numpy as np
 from . import afx, mesh, nn
 
 
-def grf_case(grid, seed, std=0.7, corr_cells=4.0, sigma_sea=3.3):
-    """A box case whose crust log10 sigma is a Gaussian random field (mean -2)."""
+def grf_case(grid, seed, std=0.7, corr_cells=4.0, sigma_sea=3.3, like=None):
+    """A box case whose crust log10 sigma is a Gaussian random field (mean -2); `like` (a case of
+    the same grid) shares its mesh arrays instead of building them again."""
     nx, ny, nz = grid
     f = mesh.gaussian_random_field(nz, ny, nx, seed, mean=-2.0, std=std, corr_cells=corr_cells)
-    return mesh.tide_case(nx, ny, nz, seed=seed, log10_sigma_field=f, sigma_sea=sigma_sea)
+    return mesh.tide_case(nx, ny, nz, seed=seed, log10_sigma_field=f, sigma_sea=sigma_sea, like=like)
 
 
 @dataclass
@@ -53,6 +54,7 @@ class Dataset:
         self.cin = cin
         self.dev = torch.device(device)
         base = grf_case(grid, 0)
+        self.base = base
         self.s = afx.System.from_n6line(base.nline, base.n6line)
         self.s.set_mesh(base)
         xout, yout, zmin, zmax = base.bounds
@@ -75,9 +77,13 @@ class Dataset:
         f = nn.cell_features(case, self.cin)
         return torch.from_numpy(f).to(self.dev).to(torch.bfloat16)
 
+    def case(self, seed):
+        """The model of `seed` on this grid (grf_case on the shared mesh)."""
+        return grf_case(self.grid, seed, like=self.base)
+
     def add(self, seed, tol=1e-6, maxit=200000, precond="DIAG+GRAD"):
         import torch
-        case = grf_case(self.grid, seed)
+        case = self.case(seed)
         b = self.system_for(case)
         self.s.precond_setup(precond)
         db = torch.from_numpy(np.ascontiguousarray(b, np.complex128)).to(self.dev)
@@ -176,6 +182,59 @@ def warm_solve(net, data: Dataset, case, tol=1e-8, precond="DIAG+GRAD", maxit=40
                 true_cold=float(res_c[0]), true_warm=float(res[0]), fallback=fallback)
 
 
+def config4(ntrain=2000, ntest=200, grid=(32, 32, 24), base=32, steps=4000, batch=4, lr=2e-3, tol_train=1e-6,
+            accept=0.5, seed0=10000, test_seed0=900000, printer=print):
+    """The learned warm start at its configuration's own size: ntrain Gaussian-random-field models
+    (std 0.7, correlation 4 cells, M2) on the grid, solved by this library (DIAG+GRAD to tol_train)
+    as training data; the 4-level encoder-decoder with `base` channels trained on them (Adam,
+    `steps` steps of `batch` models); ntest held-out models each solved cold (X = 0) and
+    warm-started, both to a 1e-8 FP64 true residual. Returns the measurements as a dict."""
+    import torch
+    t0 = time.perf_counter()
+    d = Dataset(grid, 2 * math.pi / mesh.PERIODS["M2"])
+    its = []
+    for i in range(ntrain):
+        its.append(d.add(seed0 + i, tol=tol_train).iters)
+        if (i + 1) % 100 == 0:
+            printer(f"[config4] {i + 1}/{ntrain} training models solved ({time.perf_counter() - t0:.0f}s)")
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    net = nn.Surrogate(base=base, cin=8, seed=0)
+    log = train(net, d, range(ntrain), steps=steps, batch=batch, lr=lr, log_every=max(1, steps // 10),
+                printer=lambda m: printer(f"[config4] {m}"))
+    rows, kept = [], []
+    for k in range(ntest):
+        case = d.case(test_seed0 + k)
+        rows.append(warm_solve(net, d, case, tol=1e-8, accept=accept))
+        # the same model with the scaled prediction always kept (no fallback)
+        kept.append(warm_solve(net, d, case, tol=1e-8, accept=math.inf))
+        if (k + 1) % 25 == 0:
+            printer(f"[config4] {k + 1}/{ntest} held-out models solved cold and warm")
+    ic = np.array([r["iters_cold"] for r in rows])
+    iw = np.array([r["iters_warm"] for r in rows])
+    ik = np.array([r["iters_warm"] for r in kept])
+    tc = sum(r["t_cold"] for r in rows)
+    tw = sum(r["t_warm"] for r in rows)
+    r0 = np.array([r["r0_scaled"] for r in rows])
+    return dict(
+        no_fallback=dict(iters_mean=round(float(ik.mean()), 1), iteration_gain=round(float(1.0 - ik.sum() / ic.sum()), 4),
+                         models_fewer_iters=int(np.sum(ik < ic)), models_more_iters=int(np.sum(ik > ic)),
+                         solves_per_s=round(ntest / sum(r["t_warm"] for r in kept), 3),
+                         true_resid_max=max(r["true_warm"] for r in kept)),
+        grid="x".join(map(str, grid)), base_channels=base, levels=4, train_models=ntrain, heldout_models=ntest,
+        data_seconds=round(t_data, 1), train_iters_mean=round(float(np.mean(its)), 1), train_tol=tol_train,
+        train_steps=steps, batch=batch, train_seconds=round(log.seconds, 2),
+        ms_per_train_step=round(1e3 * log.seconds / max(log.steps, 1), 3),
+        loss_first=log.losses[0][1], loss_last=log.losses[-1][1],
+        heldout_iters_cold_mean=round(float(ic.mean()), 1), heldout_iters_warm_mean=round(float(iw.mean()), 1),
+        iteration_gain=round(float(1.0 - iw.sum() / ic.sum()), 4),
+        solves_per_s_cold=round(ntest / tc, 3), solves_per_s_warm=round(ntest / tw, 3),
+        fallbacks=int(sum(r["fallback"] for r in rows)),
+        r0_scaled_min=round(float(r0.min()), 4), r0_scaled_median=round(float(np.median(r0)), 4),
+        r0_given_median=round(float(np.median([r["r0_given"] for r in rows])), 2),
+        true_resid_max=max(max(r["true_cold"], r["true_warm"]) for r in rows))
+
+
 def time_training_step(grid=(32, 32, 24), base=32, batch=4, steps=10, warmup=3, seed=0, device="cuda"):
     """Wall time of one Adam step (forward, MSE loss, backward, update) of nn.Surrogate on
     synthetic features and targets of the grid's shape (the kernels' work does not depend on
@@ -224,9 +283,18 @@ if __name__ == "__main__":
     ap.add_argument("--base", type=int, default=32)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--config4", type=int, nargs=2, default=None, metavar=("NTRAIN", "NTEST"),
+                    help="train on NTRAIN solved models and compare warm and cold solves of NTEST held-out ones")
+    ap.add_argument("--train-steps", type=int, default=4000)
     ap.add_argument("--gemm-shapes", type=int, default=0, metavar="N",
                     help="also time the step's N largest GEMM shapes on their own (10 launches each)")
     a = ap.parse_args()
+    if a.config4:
+        import json
+        out = config4(a.config4[0], a.config4[1], tuple(a.grid), a.base, steps=a.train_steps, batch=a.batch,
+                      printer=lambda m: print(m, flush=True))
+        print(json.dumps(out), flush=True)
+        raise SystemExit(0)
     ms, calls, flops = time_training_step(tuple(a.grid), a.base, a.batch, a.steps)
     print(f"{len(calls)} GEMM calls per step, {flops / 1e9:.1f} GFLOP; {ms:.2f} ms per training step, "
           f"{flops / ms / 1e9:.1f} TFLOP/s of GEMM work over the whole step", flush=True)

@@ -41,7 +41,9 @@ AFX_ERR_OUTSIDE = -700
 AFX_PC_GRAD = 0x100
 PC = {"NONE": 0, "IC": 1, "ILU": 1, "SSOR": 2, "DIAG": 3, "SSOR_BJ": 4,
       # edge preconditioner + the nodal gradient-space correction (needs System.set_nodes)
-      "SSOR+GRAD": 2 | AFX_PC_GRAD, "SSOR_BJ+GRAD": 4 | AFX_PC_GRAD, "DIAG+GRAD": 3 | AFX_PC_GRAD}
+      "SSOR+GRAD": 2 | AFX_PC_GRAD, "SSOR_BJ+GRAD": 4 | AFX_PC_GRAD, "DIAG+GRAD": 3 | AFX_PC_GRAD,
+      # node-block Jacobi (needs System.set_nodes), alone or with the nodal correction
+      "NODEBLK": 5, "NODEBLK+GRAD": 5 | AFX_PC_GRAD}
 FP64, MIXED = 0, 1
 
 

@@ -160,6 +160,14 @@ struct afx_system_s {
   int *d_nptr = nullptr, *d_nadj = nullptr, *d_en = nullptr;
   int *d_noff = nullptr;  // stacked systems: first interior node of each system [nsys + 1]
   double *d_mnd = nullptr;
+  // node-block Jacobi (AFX_PC_NODEBLK, afx_set_nodes): blocks of consecutive rows whose edges
+  // start at the same node (at most AFX_NB_W rows; longer runs are split), packed whole into
+  // tiles of at most 256 rows; per row its place in its block (i | (size - 1) << 4), and the
+  // block inverses as AFX_NB_W complex-FP32 coefficients per row
+  int ntile = 0;
+  int *d_tptr = nullptr;
+  unsigned char *d_bm = nullptr;
+  cx<float> *d_bi = nullptr;
 };
 
 namespace {
@@ -331,9 +339,11 @@ int ensure_precond(afx_system s, int kind_full, double sd, double sig, const std
   if (s->pc_valid && s->pc_kind == kind_full && s->pc_sd == sd && s->pc_sig == sig && s->pc_w == wf) return AFX_OK;
   const bool grad = (kind_full & AFX_PC_GRAD) != 0;
   const int kind = kind_full & 0xff;
-  if (grad && (kind != AFX_PC_SSOR && kind != AFX_PC_SSOR_BJ && kind != AFX_PC_DIAG))
-    return fail(AFX_ERR_BADOPT, "AFX_PC_GRAD goes with SSOR, SSOR_BJ or DIAG");
+  if (grad && (kind != AFX_PC_SSOR && kind != AFX_PC_SSOR_BJ && kind != AFX_PC_DIAG && kind != AFX_PC_NODEBLK))
+    return fail(AFX_ERR_BADOPT, "AFX_PC_GRAD goes with SSOR, SSOR_BJ, DIAG or NODEBLK");
   if (grad && s->nn < 0) return fail(AFX_ERR_ARG, "AFX_PC_GRAD needs afx_set_nodes");
+  if (kind == AFX_PC_NODEBLK && (s->ntile <= 0 || wf.size() != 1 || s->nsys > 1))
+    return fail(AFX_ERR_ARG, "AFX_PC_NODEBLK needs afx_set_nodes, one frequency and one system");
   int rc = ensure_derived(s);
   if (rc) return rc;
   if (grad)
@@ -352,9 +362,17 @@ int ensure_precond(afx_system s, int kind_full, double sd, double sig, const std
     grid = std::min(afx::max_coop_grid_ic(), std::max(8, s->ncu / 8));
     if (grid <= 0) return fail(AFX_ERR_HIP, "no cooperative grid for the D-ILU setup");
   }
-  hipError_t e = afx::launch_precond_setup(s->n, nf, kind, sd, sig, s->d_canon, nullptr, nullptr, nullptr, s->nlevL,
+  // node blocks: the diagonal as for DIAG (the k-column and parity paths read it) and the block
+  // inverses at the one frequency
+  const int dkind = (kind == AFX_PC_NODEBLK) ? AFX_PC_DIAG : kind;
+  hipError_t e = afx::launch_precond_setup(s->n, nf, dkind, sd, sig, s->d_canon, nullptr, nullptr, nullptr, s->nlevL,
                                            s->d_lvlL, s->d_rowL, s->d_Lptr, s->d_Lcol, s->d_Lval64, s->d_INU,
                                            s->d_canon + s->n, d_wf, s->d_DD, grid, s->stream);
+  if (e == hipSuccess && kind == AFX_PC_NODEBLK) {
+    if (!s->d_bi && dalloc(&s->d_bi, (size_t)s->n * AFX_NB_W) != AFX_OK) e = hipErrorOutOfMemory;
+    if (e == hipSuccess)
+      e = afx::launch_nodeblk_setup(s->n, s->d_bm, s->d_rowptr, s->d_col, s->d_val64, wf[0], s->d_bi, s->stream);
+  }
   if (e == hipSuccess) e = afx::launch_cast_vec((long long)s->n * nf, s->d_DD, s->d_DD32, s->stream);
   if (e == hipSuccess && nf == 1) e = afx::launch_dpos(s->n, s->fw.row, s->d_DD, s->fw.dpos64, s->fw.dpos32, s->stream);
   if (e == hipSuccess && nf == 1) e = afx::launch_dpos(s->n, s->bw.row, s->d_DD, s->bw.dpos64, s->bw.dpos32, s->stream);
@@ -695,9 +713,9 @@ struct Work {
   double *part_a = nullptr, *part_b = nullptr, *wcol = nullptr, *part4 = nullptr;
   int *conv = nullptr, *iters = nullptr, *status = nullptr, *col2f = nullptr, *run = nullptr;
   int *h_status = nullptr;
-  cx<double> *Zn = nullptr;
+  cx<double> *Zn = nullptr, *Zb = nullptr;
   ~Work() {
-    dfree(R); dfree(Z); dfree(P); dfree(Q); dfree(mbf); dfree(mbb); dfree(Zn);
+    dfree(R); dfree(Z); dfree(P); dfree(Q); dfree(mbf); dfree(mbb); dfree(Zn); dfree(Zb);
     dfree(part_rho); dfree(part_c1); dfree(fin_rho); dfree(fin_c1); dfree(rho1);
     dfree(part_rr); dfree(fin_rr); dfree(bnrm2); dfree(resid); dfree(norms); dfree(part_a); dfree(part_b);
     dfree(part4);
@@ -836,14 +854,17 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   if (kp > 256) return fail(AFX_ERR_ARG, "at most 256 right-hand sides per launch");
   // the streaming kernel (no sweep: DIAG or NONE edge preconditioner, one column, FP64) runs
   // at several blocks per CU; every other case runs k_cocg on the sweep partition
-  const bool stream = fp64 && !(stacked && s->nsys > 1) && (pc == AFX_PC_DIAG || pc == AFX_PC_NONE) &&
-                      (!multi || k <= 8) && !getenv("AFX_NO_STREAM");
+  const bool nblk = (pc == AFX_PC_NODEBLK);
+  if (nblk && (multi || !fp64 || (stacked && s->nsys > 1)))
+    return fail(AFX_ERR_ARG, "AFX_PC_NODEBLK runs one FP64 column of one system per launch");
+  const bool stream = fp64 && !(stacked && s->nsys > 1) && (pc == AFX_PC_DIAG || pc == AFX_PC_NONE || nblk) &&
+                      (!multi || k <= 8) && (nblk || !getenv("AFX_NO_STREAM"));
   int per_cu = 1;
   if (const char *e = getenv("AFX_STREAM_PER_CU")) per_cu = std::max(1, std::min(8, atoi(e)));
   // one column: the fused kernel (three grid synchronisations per iteration); AFX_FUSED=0 selects
   // the four-pass kernel
   const char *fe = getenv("AFX_FUSED");
-  const bool fused = stream && !multi && !(fe && fe[0] == '0');
+  const bool fused = stream && !multi && (nblk || !(fe && fe[0] == '0'));
   const int G = fused ? afx::max_coop_grid_cocg1(s->lpr64)
                 : stream ? afx::max_coop_grid_diag(s->lpr64, per_cu) : s->Gs;
   if (G <= 0) return fail(AFX_ERR_HIP, "no co-resident grid for the COCG kernel");
@@ -921,6 +942,12 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   }
   p.Zn = w.Zn;
   p.part4 = w.part4;
+  if (nblk && (rc = dalloc(&w.Zb, (size_t)s->n))) return rc;
+  p.ntile = s->ntile;
+  p.tptr = s->d_tptr;
+  p.bm = s->d_bm;
+  p.bi = s->d_bi;
+  p.Zb = w.Zb;
   const bool bj = (pc == AFX_PC_SSOR_BJ);
   p.fw = side_view(bj ? s->bjfw : s->fw, fp64);
   p.bw = side_view(bj ? s->bjbw : s->bw, fp64);
@@ -1122,6 +1149,7 @@ void release(afx_system s) {
   dfree(s->d_rstart); dfree(s->d_bar); dfree(s->d_err);
   dfree(s->d_sys_off); dfree(s->d_bsys); dfree(s->d_sblk);
   dfree(s->d_nptr); dfree(s->d_nadj); dfree(s->d_en); dfree(s->d_mnd); dfree(s->d_noff);
+  dfree(s->d_tptr); dfree(s->d_bm); dfree(s->d_bi);
   if (s->stack) afx_destroy(s->stack);
   s->stack = nullptr;
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -1407,8 +1435,9 @@ int solve_device_impl(afx_system s, int k, const cx<double> *dB, cx<double> *dX,
   const char *bm = getenv("AFX_BATCH");
   const int width = force_width >= 0 ? force_width
                     : bm ? (std::string(bm) == "multi" ? 1 : std::string(bm) == "stack" ? 2 : 0) : s->batch_multi[k];
-  const bool grad = ((o->nset == 0 ? s->pc_kind : o->precond) & AFX_PC_GRAD) != 0;
-  const bool multi = width == 1;
+  const bool grad = ((o->nset == 0 ? s->pc_kind : o->precond) & AFX_PC_GRAD) != 0 ||
+                    ((o->nset == 0 ? s->pc_kind : o->precond) & 0xff) == AFX_PC_NODEBLK;
+  const bool multi = width == 1 && ((o->nset == 0 ? s->pc_kind : o->precond) & 0xff) != AFX_PC_NODEBLK;
   if (k > 1 && o->nset != 0 && width == 2 && !stacked && !grad)
     return solve_stacked_columns(s, k, dB, dX, omega_scale, o, stats);
   if (k > 1 && (o->nset != 0 || grad) && !multi) return solve_columns(s, k, dB, dX, omega_scale, o, stats);
@@ -1877,15 +1906,32 @@ int afx_set_nodes(afx_system s, int node, const int *line, const unsigned char *
     noff[s->nsys] = next;
     if (next != nn) return fail(AFX_ERR_ARG, "afx_set_nodes: interior nodes outside the stacked systems");
   }
+  // node blocks: maximal runs of consecutive rows with the same first node (MKLINE numbers edges
+  // by their first node, so a run is the node's edges towards higher-numbered nodes), cut into
+  // pieces of at most AFX_NB_W rows; tiles of whole blocks, at most 256 rows each
+  std::vector<unsigned char> bm(n);
+  std::vector<int> tptr(1, 0);
+  for (int e0 = 0; e0 < n;) {
+    int e1 = e0 + 1;
+    while (e1 < n && e1 - e0 < AFX_NB_W && line[2 * e1] == line[2 * e0]) e1++;
+    if (e1 - tptr.back() > 256) tptr.push_back(e0);
+    for (int e = e0; e < e1; e++) bm[e] = (unsigned char)((e - e0) | ((e1 - e0 - 1) << 4));
+    e0 = e1;
+  }
+  tptr.push_back(n);
   dfree(s->d_nptr); dfree(s->d_nadj); dfree(s->d_en); dfree(s->d_mnd); dfree(s->d_noff);
+  dfree(s->d_tptr); dfree(s->d_bm); dfree(s->d_bi);
   s->nn = -1;
+  s->ntile = 0;
   int rc;
   if (!noff.empty() && (rc = upload(&s->d_noff, noff.data(), noff.size()))) return rc;
   if ((rc = upload(&s->d_nptr, cnt.data(), cnt.size())) || (rc = upload(&s->d_nadj, adj.data(), adj.size())) ||
-      (rc = upload(&s->d_en, en.data(), en.size())) || (rc = dalloc(&s->d_mnd, (size_t)nn + 1)))
+      (rc = upload(&s->d_en, en.data(), en.size())) || (rc = dalloc(&s->d_mnd, (size_t)nn + 1)) ||
+      (rc = upload(&s->d_tptr, tptr.data(), tptr.size())) || (rc = upload(&s->d_bm, bm.data(), bm.size())))
     return rc;
   s->nn = nn;
-  if (s->pc_kind & AFX_PC_GRAD) s->pc_valid = false;
+  s->ntile = (int)tptr.size() - 1;
+  if (s->pc_kind & AFX_PC_GRAD || (s->pc_kind & 0xff) == AFX_PC_NODEBLK) s->pc_valid = false;
   return AFX_OK;
 }
 
@@ -1990,9 +2036,12 @@ int afx_precond_apply(afx_system s, const double *r, double *z) {
     dfree(dr); dfree(dz); dfree(dw);
     return rc;
   }
-  hipError_t e = afx::launch_precond_apply_levels(s->n, s->pc_kind & 0xff, s->nlevL, s->d_lvlL, s->d_rowL, s->d_Lptr,
-                                                  s->d_Lcol, s->d_Lval64, s->nlevU, s->d_lvlU, s->d_rowU, s->d_Uptr,
-                                                  s->d_Ucol, s->d_Uval64, s->d_DD, dw, dr, dz, s->stream);
+  hipError_t e = (s->pc_kind & 0xff) == AFX_PC_NODEBLK
+                     ? afx::launch_nodeblk_apply(s->ntile, s->d_tptr, s->d_bm, s->d_bi, dr, dz, s->stream)
+                     : afx::launch_precond_apply_levels(s->n, s->pc_kind & 0xff, s->nlevL, s->d_lvlL, s->d_rowL,
+                                                        s->d_Lptr, s->d_Lcol, s->d_Lval64, s->nlevU, s->d_lvlU,
+                                                        s->d_rowU, s->d_Uptr, s->d_Ucol, s->d_Uval64, s->d_DD, dw, dr,
+                                                        dz, s->stream);
   cx<double> *dzn = nullptr;
   if (e == hipSuccess && (s->pc_kind & AFX_PC_GRAD)) {
     if (dalloc(&dzn, (size_t)s->nn + 1) != AFX_OK) e = hipErrorOutOfMemory;

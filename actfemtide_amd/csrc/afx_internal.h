@@ -15,6 +15,8 @@ namespace afx {
 #define AFX_STEP_ROWS 32
 // entries per position of the single-column sweep layout (8 lanes x 2)
 #define AFX_PIPE_W 16
+// rows of one node block of AFX_PC_NODEBLK (the coefficients kept per row)
+#define AFX_NB_W 8
 // words of the grid barrier: top counter, generation, AFX_BAR_GROUPS group counters, each on
 // its own 128-byte line
 #define AFX_BAR_WORDS (32 * (2 + AFX_BAR_GROUPS))
@@ -122,6 +124,13 @@ struct CocgParams {
   const double *mnd;
   cx<double> *Zn;
   double *part4;  // [G][5] per-block partials of the fused single-column kernel (c1: 2, rho and |r|^2: 3)
+  // node-block Jacobi (pc == AFX_PC_NODEBLK, fused single-column kernel): tiles of whole blocks
+  // [ntile + 1], each row's place in its block, the coefficients, and Zb = Bi r of the residual
+  int ntile;
+  const int *tptr;
+  const unsigned char *bm;
+  const cx<float> *bi;
+  cx<double> *Zb;
 };
 
 struct SpmvParams {
@@ -152,6 +161,11 @@ int max_coop_grid_diag(int lpr, int per_cu);
 hipError_t launch_cocg_diag(const CocgParams<double> &p, int lpr, int grid, hipStream_t s);
 // one column, DIAG / NONE (+ nodal correction), FP64: the fused kernel, one block of 256 threads per CU
 int max_coop_grid_cocg1(int lpr);
+// node-block Jacobi (AFX_PC_NODEBLK): block inverses from the CSR values at omega w, and z = Bi r
+hipError_t launch_nodeblk_setup(int n, const unsigned char *bm, const int *rowptr, const int *col,
+                                const cx<double> *val, double w, cx<float> *bi, hipStream_t s);
+hipError_t launch_nodeblk_apply(int ntile, const int *tptr, const unsigned char *bm, const cx<float> *bi,
+                                const cx<double> *r, cx<double> *z, hipStream_t s);
 hipError_t launch_cocg1(const CocgParams<double> &p, int lpr, int grid, hipStream_t s);
 int max_coop_grid_ic();
 hipError_t launch_spmv(const SpmvParams &p, hipStream_t s);

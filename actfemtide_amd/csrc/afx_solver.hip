@@ -1720,6 +1720,7 @@ __device__ __forceinline__ cx<double> nodal_g_next(const int *__restrict__ nptr,
 template <int LPR, bool PROF>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) k_cocg1(CocgParams<double> p) {
   __shared__ double lds[4 * 4];
+  __shared__ cx<double> tile_r[2][256];  // node-block pass: r' of the round's tiles
   constexpr int UP = 8, UX = 4;  // rows per thread in flight in the p pass and the x, r pass
   const int t = threadIdx.x;
   const long long gtid = (long long)blockIdx.x * 256 + t;
@@ -1727,6 +1728,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
   const long long n = p.n;
   const double w = p.wcol[0];
   const bool diag = (p.pc == AFX_PC_DIAG);
+  const bool nblk = (p.pc == AFX_PC_NODEBLK);
   const bool grad = p.grad != 0;
   const cx<double> *__restrict__ DD = p.DD;
   cx<double> *const buf[2] = {p.R, p.Z};
@@ -1760,10 +1762,65 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     acc[1] += gz.i;
   };
 
+  // node-block pass: over tiles of whole blocks (UT tiles per round, every load of a round issued
+  // first), r' = r - alpha q (q null: r' = r) into Rn when Rn is given, |r'|^2 into acc[2],
+  // Zb = Bi r' with r' staged in LDS, and the edge part r'^T Zb of rho into acc[0..1]
+  auto nblk_pass = [&](const cx<double> *Rr, const cx<double> *Qq, cx<double> ma, cx<double> *Rn, double (&acc)[3]) {
+    constexpr int UT = 2;
+    for (long long t0 = blockIdx.x; t0 < p.ntile; t0 += UT * (long long)gridDim.x) {
+      cx<double> rv[UT], qv[UT];
+      cx<float> cv[UT][AFX_NB_W];
+      int row[UT], m[UT];
+      bool ok[UT];
+#pragma unroll
+      for (int u = 0; u < UT; u++) {
+        const long long tl = t0 + u * (long long)gridDim.x;
+        const int rs = tl < p.ntile ? p.tptr[tl] : 0, re = tl < p.ntile ? p.tptr[tl + 1] : 0;
+        row[u] = rs + t;
+        ok[u] = row[u] < re;
+        const int rc = ok[u] ? row[u] : 0;
+        rv[u] = Rr[rc];
+        qv[u] = Qq ? Qq[rc] : cx<double>{0.0, 0.0};
+        m[u] = ok[u] ? (int)p.bm[rc] : 0;
+#pragma unroll
+        for (int j = 0; j < AFX_NB_W; j++) cv[u][j] = p.bi[(size_t)rc * AFX_NB_W + j];
+      }
+#pragma unroll
+      for (int u = 0; u < UT; u++) {
+        const cx<double> rn = ok[u] ? cfma(ma, qv[u], rv[u]) : cx<double>{0.0, 0.0};
+        if (ok[u]) {
+          if (Rn) Rn[row[u]] = rn;
+          acc[2] += rn.r * rn.r + rn.i * rn.i;
+        }
+        rv[u] = rn;
+        tile_r[u][t] = rn;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < UT; u++) {
+        const int i = m[u] & 15, sz = (m[u] >> 4) + 1;
+        cx<double> z = {0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < AFX_NB_W; j++)
+          z = cfma(cx<double>{(double)cv[u][j].r, (double)cv[u][j].i}, tile_r[u][t - i + min(j, sz - 1)], z);
+        if (ok[u]) {
+          p.Zb[row[u]] = z;
+          const cx<double> rz = cmul(rv[u], z);
+          acc[0] += rz.r;
+          acc[1] += rz.i;
+        }
+      }
+      __syncthreads();
+    }
+  };
+
   if (st == 0 && it == 0) {  // prologue: z and rho of the initial residual (in R)
     double acc[3] = {0.0, 0.0, 0.0};
     const cx<double> *R = p.R;
-    for (long long r0 = gtid; r0 < n; r0 += nthr) edge_rho(R[r0], diag ? DD[r0] : cx<double>{0.0, 0.0}, acc);
+    if (nblk)
+      nblk_pass(R, nullptr, cx<double>{0.0, 0.0}, nullptr, acc);
+    else
+      for (long long r0 = gtid; r0 < n; r0 += nthr) edge_rho(R[r0], diag ? DD[r0] : cx<double>{0.0, 0.0}, acc);
     if (grad)
       for (long long a = gtid; a < p.nn; a += nthr) nodal_put(a, nodal_g(p.nptr, p.nadj, R, (int)a), acc);
     double v[2] = {acc[0], acc[1]};
@@ -1786,7 +1843,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
 #pragma unroll
       for (int u = 0; u < UP; u++) {
         const long long r = r0 + u * nthr < n ? r0 + u * nthr : r0;
-        rv[u] = R[r];
+        rv[u] = nblk ? p.Zb[r] : R[r];  // node blocks: Bi r, formed by the last r pass
         dv[u] = diag ? DD[r] : cx<double>{0.0, 0.0};
         pv[u] = first ? cx<double>{0.0, 0.0} : p.P[r];
         ev[u] = grad ? ((const int2 *)p.en)[r] : int2{0, 0};
@@ -1826,6 +1883,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
 
     // ---- XN: x += alpha p, r' = r - alpha q, |r'|^2, and z, rho of r' ----
     double acc[3] = {0.0, 0.0, 0.0};  // rho' (re, im), |r'|^2
+    if (nblk) {
+      for (long long r0 = gtid; r0 < n; r0 += UX * nthr) {
+        cx<double> xv[UX], pv[UX];
+#pragma unroll
+        for (int u = 0; u < UX; u++) {
+          const long long r = r0 + u * nthr < n ? r0 + u * nthr : r0;
+          xv[u] = p.X[r];
+          pv[u] = p.P[r];
+        }
+#pragma unroll
+        for (int u = 0; u < UX; u++) {
+          const long long r = r0 + u * nthr;
+          if (r < n) p.X[r] = cfma(alpha, pv[u], xv[u]);
+        }
+      }
+      nblk_pass(R, p.Q, malpha, Rn, acc);
+    } else
     for (long long r0 = gtid; r0 < n; r0 += UX * nthr) {
       cx<double> xv[UX], pv[UX], qv[UX], rv[UX], dv[UX];
 #pragma unroll
@@ -1911,7 +1985,9 @@ int max_coop_grid_cocg1(int lpr) {
 }
 
 hipError_t launch_cocg1(const CocgParams<double> &p, int lpr, int grid, hipStream_t s) {
-  if (p.k != 1 || p.nsys > 1 || (p.pc != AFX_PC_DIAG && p.pc != AFX_PC_NONE) || !p.part4) return hipErrorInvalidValue;
+  if (p.k != 1 || p.nsys > 1 || (p.pc != AFX_PC_DIAG && p.pc != AFX_PC_NONE && p.pc != AFX_PC_NODEBLK) || !p.part4 ||
+      (p.pc == AFX_PC_NODEBLK && (!p.Zb || !p.bi || !p.tptr || !p.bm)))
+    return hipErrorInvalidValue;
   void *args[] = {(void *)&p};
   const void *f = p.dbg ? cocg1_kernel<true>(lpr) : cocg1_kernel<false>(lpr);
   return hipLaunchCooperativeKernel(f, dim3(grid), dim3(256), args, 0, s);
@@ -2434,6 +2510,99 @@ __global__ void k_grad_edges(int n, const int *en, const cx<double> *zn, cx<doub
   const cx<double> a = zn[en[2 * e]], b = zn[en[2 * e + 1]];
   z[e].r += b.r - a.r;
   z[e].i += b.i - a.i;
+}
+
+// Node-block Jacobi setup (AFX_PC_NODEBLK): the inverse of A's diagonal block over each node block
+// (bm[e] = i | (size - 1) << 4, rows e - i .. e - i + size - 1), A = K + i w M from the full CSR
+// (K, M) pairs. One thread per block (the thread of its first row): the block in registers,
+// in-place Gauss-Jordan without pivoting in FP64 (every principal submatrix of K + i w M with K
+// positive semidefinite and w M positive definite is nonsingular), then (X + X^T) / 2 rounded to
+// complex FP32, so the stored operator is exactly symmetric. Row i of the block is stored as
+// AFX_NB_W coefficients at bi[(e0 + i) * AFX_NB_W], zero beyond the block's size.
+__global__ void k_nodeblk_setup(int n, const unsigned char *__restrict__ bm, const int *__restrict__ rowptr,
+                                const int *__restrict__ col, const cx<double> *__restrict__ val, double w,
+                                cx<float> *__restrict__ bi) {
+  constexpr int W = AFX_NB_W;
+  const int e0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e0 >= n || (bm[e0] & 15) != 0) return;
+  const int sz = (bm[e0] >> 4) + 1;
+  cx<double> a[W][W];
+#pragma unroll
+  for (int i = 0; i < W; i++)
+#pragma unroll
+    for (int j = 0; j < W; j++) a[i][j] = {0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < W; i++) {
+    if (i >= sz) break;
+    for (int q = rowptr[e0 + i]; q < rowptr[e0 + i + 1]; q++) {
+      const int c = col[q] - e0;
+      const cx<double> v = mval(val[q], w);
+#pragma unroll
+      for (int j = 0; j < W; j++)
+        if (c == j && j < sz) a[i][j] = v;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < W; k++) {
+    if (k >= sz) break;
+    const cx<double> piv = cdivd(cx<double>{1.0, 0.0}, a[k][k]);
+    a[k][k] = {1.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < W; j++) a[k][j] = cmul(a[k][j], piv);
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+      if (i == k || i >= sz) continue;
+      const cx<double> f = a[i][k];
+      a[i][k] = {0.0, 0.0};
+#pragma unroll
+      for (int j = 0; j < W; j++) a[i][j] = cfma(cx<double>{-f.r, -f.i}, a[k][j], a[i][j]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < W; i++) {
+    if (i >= sz) break;
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      const cx<double> v = (j < sz) ? cx<double>{0.5 * (a[i][j].r + a[j][i].r), 0.5 * (a[i][j].i + a[j][i].i)}
+                                    : cx<double>{0.0, 0.0};
+      bi[(size_t)(e0 + i) * W + j] = cx<float>{(float)v.r, (float)v.i};
+    }
+  }
+}
+
+// z = Bi r for the parity hook: one block per tile of whole node blocks, r staged in LDS.
+__global__ void __launch_bounds__(256) k_nodeblk_apply(const int *__restrict__ tptr, const unsigned char *__restrict__ bm,
+                                                       const cx<float> *__restrict__ bi, const cx<double> *__restrict__ r,
+                                                       cx<double> *__restrict__ z) {
+  __shared__ cx<double> tr[256];
+  const int t = threadIdx.x;
+  const int rs = tptr[blockIdx.x], re = tptr[blockIdx.x + 1];
+  const int row = rs + t;
+  const bool ok = row < re;
+  tr[t] = ok ? r[row] : cx<double>{0.0, 0.0};
+  __syncthreads();
+  if (!ok) return;
+  const int m = bm[row], i = m & 15, sz = (m >> 4) + 1;
+  cx<double> acc = {0.0, 0.0};
+#pragma unroll
+  for (int j = 0; j < AFX_NB_W; j++) {
+    const cx<float> c = bi[(size_t)row * AFX_NB_W + j];
+    acc = cfma(cx<double>{(double)c.r, (double)c.i}, tr[t - i + min(j, sz - 1)], acc);
+  }
+  z[row] = acc;
+}
+
+hipError_t launch_nodeblk_setup(int n, const unsigned char *bm, const int *rowptr, const int *col,
+                                const cx<double> *val, double w, cx<float> *bi, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_nodeblk_setup, dim3((n + 127) / 128), dim3(128), 0, s, n, bm, rowptr, col, val, w, bi);
+  return hipGetLastError();
+}
+hipError_t launch_nodeblk_apply(int ntile, const int *tptr, const unsigned char *bm, const cx<float> *bi,
+                                const cx<double> *r, cx<double> *z, hipStream_t s) {
+  if (ntile <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_nodeblk_apply, dim3(ntile), dim3(256), 0, s, tptr, bm, bi, r, z);
+  return hipGetLastError();
 }
 
 hipError_t launch_nodal_diag(int nn, const int *nptr, const int *nadj, const int *en, const int *rowptr,

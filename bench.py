@@ -127,18 +127,23 @@ def bytes_per_iteration_grad(n, nnz, nn, nadj, edge_pc, nU=0, nL=0):
     return spmv + edge + nodal + pupd + xr
 
 
-def bytes_per_iteration_fused(n, nnz, nn, nadj, edge_pc):
-    """Compulsory HBM bytes of one iteration of the fused single-column kernel (k_cocg1: DIAG or
-    NONE edge preconditioner with the nodal correction), every array touched once per pass:
-    p pass: r, D (DIAG), p read, p written, the edges' end nodes (8 B per edge), zn read;
+def bytes_per_iteration_fused(n, nnz, nn, nadj, edge_pc, ntile=0):
+    """Compulsory HBM bytes of one iteration of the fused single-column kernel (k_cocg1: DIAG,
+    NONE or NODEBLK edge preconditioner with the nodal correction), every array touched once per
+    pass:
+    p pass: r (or, NODEBLK, Bi r), D (DIAG), p read, p written, the edges' end nodes (8 B per
+    edge), zn read;
     SpMV: (K, M) values 16 B + column 4 B per entry, row pointers, p read, q written;
-    x, r, z pass: x, p, q, r, D (DIAG) read, x and the next r written; on the nodes the incidence
-    list (4 B per entry), the row pointers, diag(G^T M G) read and zn written (r and q gathered
-    there are the arrays the edge part reads in the same pass)."""
+    x, r, z pass: x, p, q, r, D (DIAG) read, x and the next r written; NODEBLK: 8 complex-FP32
+    coefficients (64 B) and 1 B of block place per row, the tile bounds, Bi r written; on the
+    nodes the incidence list (4 B per entry), the row pointers, diag(G^T M G) read and zn written
+    (r and q gathered there are the arrays the edge part reads in the same pass)."""
     d = 16 * n if edge_pc == "DIAG" else 0
     ppass = 3 * 16 * n + d + 8 * n + 16 * nn
     spmv = nnz * 20 + 4 * (n + 1) + 2 * 16 * n
     xn = 6 * 16 * n + d + 4 * nadj + 4 * (nn + 1) + 8 * nn + 16 * nn
+    if edge_pc == "NODEBLK":
+        xn += 64 * n + n + 4 * (ntile + 1) + 16 * n
     return ppass + spmv + xn
 
 
@@ -257,8 +262,9 @@ def main():
         bnd[ln[bc].ravel()] = True
         nadj = int(np.count_nonzero(~bnd[ln]))  # (edge, interior end node) incidences
         nn = int(np.count_nonzero(~bnd))
-        if base in ("DIAG", "NONE"):
-            by = bytes_per_iteration_fused(n, nnz, nn, nadj, base)
+        if base in ("DIAG", "NONE", "NODEBLK"):
+            ntile = -(-n // 249) if base == "NODEBLK" else 0  # tiles hold at least 249 rows
+            by = bytes_per_iteration_fused(n, nnz, nn, nadj, base, ntile)
         else:
             by = bytes_per_iteration_grad(n, nnz, nn, nadj, base, s.npu, s.npl)
     else:
@@ -269,7 +275,7 @@ def main():
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                 kernel=(f"k_cocg1 (fused persistent COCG without sweeps, FP64, 1 column, {pc})"
-                        if base in ("DIAG", "NONE") else f"k_cocg (persistent COCG, FP64, 1 column, {pc})"),
+                        if base in ("DIAG", "NONE", "NODEBLK") else f"k_cocg (persistent COCG, FP64, 1 column, {pc})"),
                 bytes_per_iteration=by, avg_launch_ms=round(kms / max(launches, 1), 3),
                 iterations_per_launch=round(ipl, 1), us_per_iteration=round(1e3 * kms / max(iters, 1), 1),
                 traffic_source=tsrc)
@@ -303,7 +309,7 @@ def main():
             f"{variants['speedup_vs_plain']}x; site fields differ by {ferr:.1e}")
         del dXp
     # the other forms of the nodal correction, one solve each
-    for other in ("SSOR+GRAD", "SSOR_BJ+GRAD", "DIAG+GRAD"):
+    for other in ("SSOR+GRAD", "SSOR_BJ+GRAD", "DIAG+GRAD", "NODEBLK+GRAD"):
         if other == pc or time.time() - t_start > 300:
             continue
         s.precond_setup(other)

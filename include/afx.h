@@ -38,6 +38,13 @@ extern "C" {
 #define AFX_PC_DIAG 3 /* 'DIAG' */
 #define AFX_PC_SSOR_BJ 4 /* SSOR on the diagonal blocks of the device row partition (no reference
                             counterpart: changes the preconditioner, keeps the COCG iteration) */
+#define AFX_PC_NODEBLK 5 /* node-block Jacobi (no reference counterpart): the inverse of each
+                            diagonal block of A over a run of consecutive rows whose edges start at
+                            the same node (the edge numbering of MKLINE puts a node's edges towards
+                            higher-numbered nodes next to each other; runs of more than 8 rows are
+                            cut), symmetrised and kept in complex FP32. Sweep-free like DIAG; needs
+                            afx_set_nodes; one right-hand side per launch (several are solved one
+                            at a time), one frequency, not for stacked systems */
 /* Flag OR-ed into AFX_PC_SSOR, AFX_PC_SSOR_BJ or AFX_PC_DIAG (no reference counterpart): adds
    the nodal gradient-space correction G diag(G^T A G)^-1 G^T to the edge preconditioner, G the
    discrete gradient of the interior nodes (afx_set_nodes). The curl-curl part annihilates

@@ -411,6 +411,37 @@ def grad_correction(A, G, omega_eff=1.0):
     return lambda r: G @ (dn * (G.T @ r)), mnd
 
 
+def node_blocks(line, width=8):
+    """Node blocks of AFX_PC_NODEBLK (no reference counterpart): maximal runs of consecutive edges
+    with the same first node (line(1, e)), cut into pieces of at most `width` edges from the
+    start of the run. Returns the list of 0-based edge index arrays."""
+    first = np.asarray(line).reshape(-1, 2)[:, 0]
+    n = first.size
+    starts = np.r_[0, np.nonzero(np.diff(first))[0] + 1, n]
+    out = []
+    for a, c in zip(starts[:-1], starts[1:]):
+        for s0 in range(a, c, width):
+            out.append(np.arange(s0, min(c, s0 + width)))
+    return out
+
+
+def node_block_inverse(A, blocks):
+    """The node-block Jacobi operator: each diagonal block of A inverted (numpy LU with pivoting),
+    symmetrised as (X + X^T) / 2 and rounded to complex FP32, as a scipy CSR matrix."""
+    import scipy.sparse as sp
+    A = sp.csr_matrix(A)
+    n = A.shape[0]
+    rows, cols, vals = [], [], []
+    for idx in blocks:
+        X = np.linalg.inv(A[idx][:, idx].toarray())
+        X = (0.5 * (X + X.T)).astype(np.complex64).astype(np.complex128)
+        ii, jj = np.meshgrid(idx, idx, indexing="ij")
+        rows.append(ii.ravel())
+        cols.append(jj.ravel())
+        vals.append(X.ravel())
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+
+
 def cocg_precond(A, b, M, tol=1e-8, maxit=100000, x0=None):
     """COCG of COCG13_single.f90:181-323 (same recurrence, unconjugated forms, stop on the
     recurrence residual over ||b||) with an arbitrary complex-symmetric preconditioner M (a

@@ -107,3 +107,20 @@ def test_fullsize_solve_true_residual(full):
     print(f"[fullsize] solve: {st[0].iter} iterations, true residual {tr:.3e}", flush=True)
     assert rc == 0 and st[0].error == 0
     assert tr <= 1e-8
+
+
+def test_fullsize_nodeblk_solve_true_residual(full):
+    """The measured form's preconditioner at full size: NODEBLK+GRAD in the fused kernel, x
+    recomputed into the FP64 true residual with scipy."""
+    import torch
+    c, s, b, bc = full
+    s.precond_setup("NODEBLK+GRAD")
+    dB = torch.from_numpy(b).cuda()
+    dX = torch.zeros_like(dB)
+    st, rc = s.solve_device(dB, dX, precond="NODEBLK+GRAD", tol=1e-8, maxit=60000)
+    x = dX.cpu().numpy()
+    A, _ = _scipy_matrix(s)
+    tr = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+    print(f"[fullsize] NODEBLK+GRAD solve: {st[0].iter} iterations, true residual {tr:.3e}", flush=True)
+    assert rc == 0 and st[0].error == 0
+    assert tr <= 1.05e-8

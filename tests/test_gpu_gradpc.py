@@ -52,7 +52,27 @@ def test_grad_precond_apply_matches_oracle(small):
         assert err <= 1e-12
 
 
-@pytest.mark.parametrize("kind", ["SSOR+GRAD", "SSOR_BJ+GRAD", "DIAG+GRAD"])
+@pytest.mark.parametrize("kind", ["NODEBLK", "NODEBLK+GRAD"])
+def test_nodeblk_apply_matches_oracle(small, kind):
+    """The node-block Jacobi apply (AFX_PC_NODEBLK: block inverses formed on the device by
+    Gauss-Jordan, symmetrised, kept in complex FP32) against the oracle's (numpy LU inverses of the
+    same blocks, symmetrised, FP32), with and without the nodal correction: the two FP32 roundings
+    of the same FP64 inverse may differ by one unit, so to 1e-6."""
+    from oracle import oracle as O
+    s, b, bc, o, A, corr = small
+    rng = np.random.default_rng(5)
+    r = rng.standard_normal(s.n) + 1j * rng.standard_normal(s.n)
+    r[o["bc"]] = 0.0
+    s.precond_setup(kind)
+    z = s.precond_apply(r)
+    Bi = O.node_block_inverse(A, O.node_blocks(s_case(small).line))
+    zr = Bi @ r + (corr(r) if kind.endswith("+GRAD") else 0.0)
+    err = np.abs(z - zr).max() / np.abs(zr).max()
+    print(f"{kind} apply: rel err {err:.2e}", flush=True)
+    assert err <= 1e-6
+
+
+@pytest.mark.parametrize("kind", ["SSOR+GRAD", "SSOR_BJ+GRAD", "DIAG+GRAD", "NODEBLK+GRAD"])
 def test_grad_solve_small(small, case_small, kind):
     """Every form reaches the 1e-8 FP64 true residual, its site fields agree with the direct solve to
     1e-6, and it needs far fewer iterations than SSOR alone; SSOR+GRAD's count matches the oracle's
@@ -72,11 +92,15 @@ def test_grad_solve_small(small, case_small, kind):
         f, fd = O.obs_apply(item, val, x), O.obs_apply(item, val, xd)
         assert np.abs(f - fd).max() <= 1e-6 * np.abs(fd).max()
     assert st[0].iter * 5 <= 819  # SSOR alone: 819 iterations on this system
-    if kind == "SSOR+GRAD":
-        DD = O.precond_setup("SSOR", o["D"], o["INU"], o["AU"], o["INL"], o["IAL"], o["AL"])
-        M = lambda r: O.precond_apply("SSOR", o["INU"], o["IAU"], o["AU"], o["INL"], o["IAL"], o["AL"], DD, r) + corr(r)
+    if kind in ("SSOR+GRAD", "NODEBLK+GRAD"):
+        if kind == "SSOR+GRAD":
+            DD = O.precond_setup("SSOR", o["D"], o["INU"], o["AU"], o["INL"], o["IAL"], o["AL"])
+            M = lambda r: O.precond_apply("SSOR", o["INU"], o["IAU"], o["AU"], o["INL"], o["IAL"], o["AL"], DD, r) + corr(r)
+        else:
+            Bi = O.node_block_inverse(A, O.node_blocks(case_small.line))
+            M = lambda r: Bi @ r + corr(r)
         _, it_o, _ = O.cocg_precond(A, bo, M)
-        print(f"oracle SSOR+GRAD: {it_o} iterations", flush=True)
+        print(f"oracle {kind}: {it_o} iterations", flush=True)
         assert abs(st[0].iter - it_o) <= max(2, 0.1 * it_o)
 
 
@@ -94,7 +118,7 @@ def test_grad_solve_cfg1_fields_against_direct():
     item, ve, vb, _ = O.prepobscoeff(c, seafloor_sites(c, 16, 9))
     s.precond_setup("SSOR")
     _, st0 = s.solve(b, precond="SSOR", tol=1e-8, maxit=40000)
-    for kind in ("SSOR+GRAD", "DIAG+GRAD"):
+    for kind in ("SSOR+GRAD", "DIAG+GRAD", "NODEBLK+GRAD"):
         s.precond_setup(kind)
         x, st = s.solve(b, precond=kind, tol=1e-8, maxit=40000)
         tr = np.linalg.norm(bo - A @ x) / np.linalg.norm(bo)
@@ -173,4 +197,33 @@ def test_fused_single_column_kernel_is_deterministic(small, monkeypatch, kind):
         assert abs(i1 - i0) <= max(3, 0.05 * i0)
         for val in list(vb) + list(ve):
             f0, f1 = O.obs_apply(item, val, x0), O.obs_apply(item, val, x1)
+            assert np.abs(f1 - f0).max() <= 1e-6 * np.abs(f0).max()
+
+
+def test_nodeblk_solves_are_deterministic_and_match_diag(small):
+    """NODEBLK+GRAD in the fused kernel for each of the 5 constituents: every solve reaches a 1e-8
+    true residual, a repeated solve reproduces x bit for bit, and the site fields agree with the
+    DIAG+GRAD solution to 1e-6; the node blocks need fewer iterations than the diagonal."""
+    import torch
+    from actfemtide_amd import mesh
+    from oracle import oracle as O
+    s, b, bc, o, A, corr = small
+    item, ve, vb, _ = O.prepobscoeff(s_case(small), seafloor_sites(s_case(small), 8, 5))
+    dB = torch.from_numpy(np.ascontiguousarray(b)).cuda()
+    for c in ("M2", "S2", "N2", "K1", "O1"):
+        sc = np.array([2 * math.pi / mesh.PERIODS[c] / OMEGA_M2])
+        runs = []
+        for kind in ("DIAG+GRAD", "NODEBLK+GRAD", "NODEBLK+GRAD"):
+            s.precond_setup(kind)
+            dX = torch.zeros_like(dB)
+            st, rc = s.solve_device(dB, dX, omega_scale=sc, precond=kind, tol=1e-8, maxit=40000)
+            tr = float(s.true_residual_device(dB, dX, omega_scale=sc)[0])
+            assert rc == 0 and tr <= 1.05e-8, (c, kind, st[0].iter, tr)
+            runs.append((dX.cpu().numpy(), st[0].iter))
+        (xd, idg), (x1, i1), (x2, i2) = runs
+        print(f"{c}: DIAG+GRAD {idg} iterations, NODEBLK+GRAD {i1} and {i2}", flush=True)
+        assert i1 == i2 and np.array_equal(x1, x2)
+        assert i1 < idg
+        for val in list(vb) + list(ve):
+            f0, f1 = O.obs_apply(item, val, xd), O.obs_apply(item, val, x1)
             assert np.abs(f1 - f0).max() <= 1e-6 * np.abs(f0).max()

@@ -73,3 +73,34 @@ def test_corrected_preconditioner_is_symmetric_and_cocg_reaches_direct(small):
     for val in list(vb) + list(ve):
         f, fd = O.obs_apply(item, val, x), O.obs_apply(item, val, xd)
         assert np.abs(f - fd).max() <= 1e-6 * np.abs(fd).max()
+
+
+def test_node_block_jacobi_oracle():
+    """The oracle's node blocks and their inverses (AFX_PC_NODEBLK): the blocks partition the edges
+    into runs of at most 8 consecutive edges with one first node; each stored inverse times its
+    block is the identity to FP32 rounding, the operator is exactly symmetric, and with the nodal
+    correction COCG needs fewer iterations than with the diagonal (the measured reason for it)."""
+    from actfemtide_amd import mesh
+    from oracle import oracle as O
+    c = mesh.tide_case(8, 8, 6, seed=7)
+    o = O.forward_system(c, OMEGA_M2)
+    A = O.to_scipy(o["INU"], o["IAU"], o["INL"], o["IAL"], o["D"], o["AU"], o["AL"]).tocsr()
+    blocks = O.node_blocks(c.line)
+    cat = np.concatenate(blocks)
+    assert np.array_equal(cat, np.arange(A.shape[0]))
+    first = np.asarray(c.line)[:, 0]
+    for idx in blocks:
+        assert 1 <= idx.size <= 8 and np.all(np.diff(idx) == 1) and np.all(first[idx] == first[idx[0]])
+    Bi = O.node_block_inverse(A, blocks)
+    assert (Bi - Bi.T).count_nonzero() == 0
+    for idx in blocks[::7]:
+        prod = Bi[idx][:, idx].toarray() @ A[idx][:, idx].toarray()
+        assert np.abs(prod - np.eye(idx.size)).max() <= 1e-5
+    G, _ = O.discrete_gradient(c.line, c.node, o["bc"])
+    corr, _ = O.grad_correction(A, G)
+    b = o["b"]
+    Dd = 1.0 / A.diagonal()
+    _, it_d, _ = O.cocg_precond(A, b, lambda r: Dd * r + corr(r))
+    _, it_b, _ = O.cocg_precond(A, b, lambda r: Bi @ r + corr(r))
+    print(f"DIAG+GRAD {it_d} iterations, NODEBLK+GRAD {it_b}")
+    assert it_b < 0.8 * it_d

@@ -8,14 +8,15 @@ timed region; inputs are resident in HBM.
 
 A step is one solve from x0 = 0 to an FP64 true residual ||b - A x||/||b|| <= 1e-8 (recomputed
 on the device at the end) with FP64 COCG (the reference's recurrence, COCG13_single.f90:181-323)
-preconditioned by --precond, by default DIAG+GRAD: the diagonal edge preconditioner plus the nodal
-gradient-space correction G diag(G^T A G)^-1 G^T (afx.h, AFX_PC_GRAD). The whole iteration runs in
-one persistent launch per 2000 iterations. The timed loop runs --steps solves or stops after the
+preconditioned by --precond, by default NODEBLK+GRAD: node-block Jacobi (the inverse of A's
+diagonal block over each node's consecutive edges, afx.h AFX_PC_NODEBLK) plus the nodal
+gradient-space correction G diag(G^T A G)^-1 G^T (AFX_PC_GRAD). The whole iteration runs in one
+persistent launch per 2000 iterations. The timed loop runs --steps solves or stops after the
 step that passes --time-budget seconds; `steps` reports the solves timed.
 
 Beside it, the comparator: plain FP64 SSOR-COCG in the reference's row order (the preconditioner
 the reference's solveCG passes; its 'IC' stagnates on this system), one solve, and the other
-corrected forms (SSOR+GRAD, SSOR_BJ+GRAD), one solve each; "variants" holds their solves/s, the
+corrected forms (SSOR+GRAD, SSOR_BJ+GRAD, DIAG+GRAD), one solve each; "variants" holds their solves/s, the
 speedup of the measured form over plain SSOR, and how far the two solutions' seafloor site fields
 differ.
 
@@ -178,7 +179,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", default="cfg2", choices=sorted(SIZES))
-    ap.add_argument("--precond", default="DIAG+GRAD",
+    ap.add_argument("--precond", default="NODEBLK+GRAD",
                     help="preconditioner of the measured solve (afx.PC); the plain FP64 SSOR-COCG of the "
                          "reference's order is always timed beside it as the comparator")
     ap.add_argument("--plain-steps", type=int, default=1, help="timed solves of the plain SSOR comparator (0: skip)")
@@ -367,13 +368,15 @@ def main():
         variants[f"jacobian_{args.jacobian}_columns"] = jacobian_variant(s, case, w0, args.jacobian)
     if args.adjoint_batch:
         variants[f"adjoint_{3 * args.adjoint_batch}_columns"] = adjoint_batch_variant(s, case, opts, args.adjoint_batch)
+    # the batched kernels (k columns per launch, stacked systems) run the diagonal edge preconditioner
+    bopts = dict(opts, precond="DIAG+GRAD")
     if not args.no_variants and time.time() - t_start < 300:
         # the batched forms (a failure here is recorded, it does not end the run):
         # the 5 constituents of this mesh; solvePARDISOinv's 48 adjoint columns (16 seafloor sites
         # x 3 electric components) on the 32x32x24-cell mesh; the 16-mesh uneven batch
         for name, fn in (("constituents_5", lambda: constituents_variant(s, b, omegas, w0, opts)),
-                         ("adjoint_48_columns_32x32x24", lambda: adjoint32(pc, opts, w0)),
-                         ("mesh_batch_16", lambda: mesh_batch_variant(opts, w0, grad_only=True)),
+                         ("adjoint_48_columns_32x32x24", lambda: adjoint32("DIAG+GRAD", bopts, w0)),
+                         ("mesh_batch_16", lambda: mesh_batch_variant(bopts, w0, grad_only=True)),
                          ("gauss_newton_warm_starts_32x32x24", lambda: gn_variant())):
             try:
                 variants[name] = fn()
@@ -418,7 +421,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(case, w0, plain["iters"] if plain else its, bc=bc, grad_iters=its if pc == "DIAG+GRAD" else 0)
+        cpu = cpu_baseline(case, w0, plain["iters"] if plain else its, bc=bc,
+                           grad_iters=its if pc in ("DIAG+GRAD", "NODEBLK+GRAD") else 0, grad_edge=pc.split("+")[0])
 
     out = {
         "metric": "edge-FEM solves/s to 1e-8 rel. residual, warm-start mixed vs plain COCG, 1 GPU",
@@ -609,15 +613,18 @@ def constituents_variant(s, b, omegas, w0, opts):
     out = {}
     for width in ("columns", "multi"):
         os.environ["AFX_BATCH"] = width
+        # the k-column kernel exists for the diagonal edge preconditioner (node blocks are solved
+        # one column at a time)
+        wopts = dict(opts, precond="DIAG+GRAD") if width == "multi" and "NODEBLK" in str(opts["precond"]) else opts
         dX = torch.zeros_like(dB)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        st, rc = s.solve_device(dB, dX, omega_scale=sc, **opts)
+        st, rc = s.solve_device(dB, dX, omega_scale=sc, **wopts)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         tr = float(s.true_residual_device(dB, dX, omega_scale=sc).max())
         out[width] = dict(solves_per_s=round(k / dt, 4), seconds=round(dt, 4), iters=[x.iter for x in st],
-                          true_resid_max=tr, rc=rc)
+                          true_resid_max=tr, rc=rc, precond=wopts["precond"])
         log(f"5 constituents, {width}: {dt:.3f}s, iterations {[x.iter for x in st]}, worst {tr:.2e}")
     os.environ.pop("AFX_BATCH", None)
     # warm chain: column c starts from column c - 1's solution
@@ -776,7 +783,7 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(case, omega, plain_iters, bc=None, grad_iters=0):
+def cpu_baseline(case, omega, plain_iters, bc=None, grad_iters=0, grad_edge="DIAG"):
     """The reference's CPU solver, restated: the oracle's serial COCG13_single (C, SSOR, the
     reference's row order) on the host cores of this machine.
     - value: one core on the same 1.4M-edge system; the per-iteration time is the median of 5
@@ -788,8 +795,9 @@ def cpu_baseline(case, omega, plain_iters, bc=None, grad_iters=0):
     - superlu_16x16x12: a direct complex LU solve (scipy SuperLU, symmetric-mode minimum degree
       ordering; the PARDISO-class reference) on one core, median of 3;
     - same_algorithm (when grad_iters): the oracle's restatement of the measured form (COCG with
-      the diagonal edge preconditioner and the nodal correction, scipy, one core) on the same
-      system, the median of 3 samples of 5 iterations scaled to grad_iters -- extrapolated."""
+      the grad_edge edge preconditioner -- the diagonal or the node blocks -- and the nodal
+      correction, scipy, one core) on the same system, the median of 3 samples of 5 iterations
+      scaled to grad_iters -- extrapolated."""
     import scipy.sparse.linalg as sla
 
     from actfemtide_amd import mesh
@@ -843,20 +851,26 @@ def cpu_baseline(case, omega, plain_iters, bc=None, grad_iters=0):
         A = O.to_scipy(o["INU"], o["IAU"], o["INL"], o["IAL"], o["D"], o["AU"], o["AL"]).tocsr()
         G, _ = O.discrete_gradient(case.line, case.node, bc)
         corr, _ = O.grad_correction(A, G)
-        DDg = O.precond_setup("DIAG", o["D"], o["INU"], o["AU"], o["INL"], o["IAL"], o["AL"])
+        if grad_edge == "NODEBLK":
+            Bi = O.node_block_inverse(A, O.node_blocks(case.line))
+            M = lambda r: Bi @ r + corr(r)  # noqa: E731
+        else:
+            DDg = O.precond_setup("DIAG", o["D"], o["INU"], o["AU"], o["INL"], o["IAL"], o["AL"])
+            M = lambda r: DDg * r + corr(r)  # noqa: E731
         bb = np.asarray(o["b"])
-        O.cocg_precond(A, bb, lambda r: DDg * r + corr(r), tol=0.0, maxit=1)
+        O.cocg_precond(A, bb, M, tol=0.0, maxit=1)
         ts = []
         for _ in range(3):
             t0 = time.perf_counter()
-            O.cocg_precond(A, bb, lambda r: DDg * r + corr(r), tol=0.0, maxit=5)
+            O.cocg_precond(A, bb, M, tol=0.0, maxit=5)
             ts.append((time.perf_counter() - t0) / 5)
         t_g = float(np.median(ts))
         same = dict(value=round(1.0 / (t_g * grad_iters), 6), unit="solves/s", cores=1, kind="port",
                     extrapolated=True, ms_per_iteration=round(1e3 * t_g, 2), iters=grad_iters,
-                    note="the oracle's scipy restatement of the measured form (COCG, diagonal edge "
-                         "preconditioner + nodal gradient-space correction) on the same system, median of "
-                         "3 samples of 5 iterations scaled to the GPU solve's iteration count")
+                    note=f"the oracle's scipy restatement of the measured form (COCG, "
+                         f"{'node-block' if grad_edge == 'NODEBLK' else 'diagonal'} edge preconditioner + nodal "
+                         f"gradient-space correction) on the same system, median of 3 samples of 5 iterations "
+                         f"scaled to the GPU solve's iteration count")
         log(f"cpu baseline, same algorithm: {1e3 * t_g:.1f} ms per iteration, {same['value']} solves/s")
     log(f"cpu baseline: {1e3 * t_it:.1f} ms per iteration on 1 core ({1e3 * t_sweep:.1f} ms sweeps, parallel "
         f"phases {1e3 * t_par1:.1f} ms on 1 core, {1e3 * t_parn:.1f} ms on {ncores}); 16x16x12 full solve "

@@ -426,20 +426,38 @@ def node_blocks(line, width=8):
 
 
 def node_block_inverse(A, blocks):
-    """The node-block Jacobi operator: each diagonal block of A inverted (numpy LU with pivoting),
-    symmetrised as (X + X^T) / 2 and rounded to complex FP32, as a scipy CSR matrix."""
+    """The node-block Jacobi operator: each diagonal block of A inverted (numpy LU with pivoting,
+    batched over blocks padded to 8 x 8 with the identity), symmetrised as (X + X^T) / 2 and
+    rounded to complex FP32, as a scipy CSR matrix. The blocks' entries are scattered from A's
+    CSR entries whose row and column lie in the same block."""
     import scipy.sparse as sp
     A = sp.csr_matrix(A)
     n = A.shape[0]
-    rows, cols, vals = [], [], []
-    for idx in blocks:
-        X = np.linalg.inv(A[idx][:, idx].toarray())
-        X = (0.5 * (X + X.T)).astype(np.complex64).astype(np.complex128)
-        ii, jj = np.meshgrid(idx, idx, indexing="ij")
-        rows.append(ii.ravel())
-        cols.append(jj.ravel())
-        vals.append(X.ravel())
-    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+    W = 8
+    nb = len(blocks)
+    blk = np.empty(n, np.int64)
+    loc = np.empty(n, np.int64)
+    size = np.array([len(idx) for idx in blocks])
+    if size.max() > W:
+        raise ValueError("node blocks of more than 8 rows")
+    starts = np.array([idx[0] for idx in blocks])
+    blk[:] = np.repeat(np.arange(nb), size)
+    loc[:] = np.arange(n) - np.repeat(starts, size)
+    rows = np.repeat(np.arange(n), np.diff(A.indptr))
+    cols = A.indices
+    m = blk[rows] == blk[cols]
+    D = np.zeros((nb, W, W), np.complex128)
+    # the padding of a block smaller than 8 x 8 is the identity (it keeps the batch invertible)
+    D[:, np.arange(W), np.arange(W)] = np.where(np.arange(W)[None, :] < size[:, None], 0.0, 1.0)
+    D[blk[rows[m]], loc[rows[m]], loc[cols[m]]] = A.data[m]
+    X = np.linalg.inv(D)
+    X = (0.5 * (X + np.transpose(X, (0, 2, 1)))).astype(np.complex64).astype(np.complex128)
+    # row r's stored entries: its block's columns start .. start + size - 1
+    ii = np.repeat(np.arange(n), size[blk])
+    off = np.arange(W)[None, :]
+    jj = (starts[blk][:, None] + off)[off < size[blk][:, None]]
+    vals = X[blk[ii], loc[ii], jj - starts[blk[ii]]]
+    return sp.csr_matrix((vals, (ii, jj)), shape=(n, n))
 
 
 def cocg_precond(A, b, M, tol=1e-8, maxit=100000, x0=None):

@@ -18,7 +18,7 @@ module afx
   integer(c_int), parameter :: AFX_ERR_BREAKDOWN = -200, AFX_ERR_HIP = -300, AFX_ERR_ARG = -400
   integer(c_int), parameter :: AFX_PC_NONE = 0, AFX_PC_IC = 1, AFX_PC_SSOR = 2, AFX_PC_DIAG = 3
   integer(c_int), parameter :: AFX_FP64 = 0, AFX_MIXED = 1
-  integer(c_int), parameter :: AFX_PC_SSOR_BJ = 4, AFX_PC_GRAD = 256
+  integer(c_int), parameter :: AFX_PC_SSOR_BJ = 4, AFX_PC_NODEBLK = 5, AFX_PC_GRAD = 256
 
   type, bind(C) :: afx_opts
     integer(c_int) :: precond

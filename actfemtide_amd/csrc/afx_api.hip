@@ -163,11 +163,11 @@ struct afx_system_s {
   // node-block Jacobi (AFX_PC_NODEBLK, afx_set_nodes): blocks of consecutive rows whose edges
   // start at the same node (at most AFX_NB_W rows; longer runs are split), packed whole into
   // tiles of at most 256 rows; per row its place in its block (i | (size - 1) << 4), and the
-  // block inverses as AFX_NB_W complex-FP32 coefficients per row
+  // block inverses as AFX_NB_W complex-bfloat16 coefficients per row
   int ntile = 0;
   int *d_tptr = nullptr;
   unsigned char *d_bm = nullptr;
-  cx<float> *d_bi = nullptr;
+  afx::nbcoef *d_bi = nullptr;
 };
 
 namespace {

@@ -17,6 +17,8 @@ namespace afx {
 #define AFX_PIPE_W 16
 // rows of one node block of AFX_PC_NODEBLK (the coefficients kept per row)
 #define AFX_NB_W 8
+// a node-block coefficient: a complex number as two bfloat16 halves (real in the low 16 bits)
+typedef unsigned int nbcoef;
 // words of the grid barrier: top counter, generation, AFX_BAR_GROUPS group counters, each on
 // its own 128-byte line
 #define AFX_BAR_WORDS (32 * (2 + AFX_BAR_GROUPS))
@@ -125,11 +127,12 @@ struct CocgParams {
   cx<double> *Zn;
   double *part4;  // [G][5] per-block partials of the fused single-column kernel (c1: 2, rho and |r|^2: 3)
   // node-block Jacobi (pc == AFX_PC_NODEBLK, fused single-column kernel): tiles of whole blocks
-  // [ntile + 1], each row's place in its block, the coefficients, and Zb = Bi r of the residual
+  // [ntile + 1], each row's place in its block, the coefficients (AFX_NB_W bfloat16 complex per
+  // row), and Zb = Bi r of the residual
   int ntile;
   const int *tptr;
   const unsigned char *bm;
-  const cx<float> *bi;
+  const nbcoef *bi;
   cx<double> *Zb;
 };
 
@@ -163,8 +166,8 @@ hipError_t launch_cocg_diag(const CocgParams<double> &p, int lpr, int grid, hipS
 int max_coop_grid_cocg1(int lpr);
 // node-block Jacobi (AFX_PC_NODEBLK): block inverses from the CSR values at omega w, and z = Bi r
 hipError_t launch_nodeblk_setup(int n, const unsigned char *bm, const int *rowptr, const int *col,
-                                const cx<double> *val, double w, cx<float> *bi, hipStream_t s);
-hipError_t launch_nodeblk_apply(int ntile, const int *tptr, const unsigned char *bm, const cx<float> *bi,
+                                const cx<double> *val, double w, nbcoef *bi, hipStream_t s);
+hipError_t launch_nodeblk_apply(int ntile, const int *tptr, const unsigned char *bm, const nbcoef *bi,
                                 const cx<double> *r, cx<double> *z, hipStream_t s);
 hipError_t launch_cocg1(const CocgParams<double> &p, int lpr, int grid, hipStream_t s);
 int max_coop_grid_ic();

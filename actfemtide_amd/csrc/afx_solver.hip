@@ -68,6 +68,17 @@ __device__ __forceinline__ cx<T> down(cx<double> a) {
 // a / b as (a * conj(b)) / |b|^2 with b scaled by an exact power of two (C99 Annex G form,
 // the complex division the reference's compiled Fortran performs): the scaling avoids
 // overflow without changing the rounding of the textbook formula.
+// bfloat16 <-> float: round to nearest even (the coefficients are finite)
+__device__ __forceinline__ unsigned int f2bf16(float f) {
+  const unsigned int u = __float_as_uint(f);
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+// coefficient q (0..3) of a 16-byte group of node-block coefficients, as complex FP64
+__device__ __forceinline__ cx<double> nb_coef(uint4 g, int q) {
+  const unsigned int w = q == 0 ? g.x : q == 1 ? g.y : q == 2 ? g.z : g.w;
+  return {(double)__uint_as_float(w << 16), (double)__uint_as_float(w & 0xffff0000u)};
+}
+
 __device__ __forceinline__ cx<double> cdivd(cx<double> a, cx<double> b) {
   double c = b.r, d = b.i;
   const double m = fmax(fabs(c), fabs(d));
@@ -1769,7 +1780,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     constexpr int UT = 2;
     for (long long t0 = blockIdx.x; t0 < p.ntile; t0 += UT * (long long)gridDim.x) {
       cx<double> rv[UT], qv[UT];
-      cx<float> cv[UT][AFX_NB_W];
+      uint4 cv[UT][AFX_NB_W / 4];
       int row[UT], m[UT];
       bool ok[UT];
 #pragma unroll
@@ -1783,7 +1794,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         qv[u] = Qq ? Qq[rc] : cx<double>{0.0, 0.0};
         m[u] = ok[u] ? (int)p.bm[rc] : 0;
 #pragma unroll
-        for (int j = 0; j < AFX_NB_W; j++) cv[u][j] = p.bi[(size_t)rc * AFX_NB_W + j];
+        for (int j = 0; j < AFX_NB_W / 4; j++) cv[u][j] = ((const uint4 *)p.bi)[(size_t)rc * (AFX_NB_W / 4) + j];
       }
 #pragma unroll
       for (int u = 0; u < UT; u++) {
@@ -1802,7 +1813,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         cx<double> z = {0.0, 0.0};
 #pragma unroll
         for (int j = 0; j < AFX_NB_W; j++)
-          z = cfma(cx<double>{(double)cv[u][j].r, (double)cv[u][j].i}, tile_r[u][t - i + min(j, sz - 1)], z);
+          z = cfma(nb_coef(cv[u][j >> 2], j & 3), tile_r[u][t - i + min(j, sz - 1)], z);
         if (ok[u]) {
           p.Zb[row[u]] = z;
           const cx<double> rz = cmul(rv[u], z);
@@ -2517,11 +2528,12 @@ __global__ void k_grad_edges(int n, const int *en, const cx<double> *zn, cx<doub
 // (K, M) pairs. One thread per block (the thread of its first row): the block in registers,
 // in-place Gauss-Jordan without pivoting in FP64 (every principal submatrix of K + i w M with K
 // positive semidefinite and w M positive definite is nonsingular), then (X + X^T) / 2 rounded to
-// complex FP32, so the stored operator is exactly symmetric. Row i of the block is stored as
-// AFX_NB_W coefficients at bi[(e0 + i) * AFX_NB_W], zero beyond the block's size.
+// FP32 and then to bfloat16 (both to nearest even), so the stored operator is exactly symmetric.
+// Row i of the block is stored as AFX_NB_W coefficients at bi[(e0 + i) * AFX_NB_W], zero beyond
+// the block's size.
 __global__ void k_nodeblk_setup(int n, const unsigned char *__restrict__ bm, const int *__restrict__ rowptr,
                                 const int *__restrict__ col, const cx<double> *__restrict__ val, double w,
-                                cx<float> *__restrict__ bi) {
+                                nbcoef *__restrict__ bi) {
   constexpr int W = AFX_NB_W;
   const int e0 = blockIdx.x * blockDim.x + threadIdx.x;
   if (e0 >= n || (bm[e0] & 15) != 0) return;
@@ -2565,14 +2577,14 @@ __global__ void k_nodeblk_setup(int n, const unsigned char *__restrict__ bm, con
     for (int j = 0; j < W; j++) {
       const cx<double> v = (j < sz) ? cx<double>{0.5 * (a[i][j].r + a[j][i].r), 0.5 * (a[i][j].i + a[j][i].i)}
                                     : cx<double>{0.0, 0.0};
-      bi[(size_t)(e0 + i) * W + j] = cx<float>{(float)v.r, (float)v.i};
+      bi[(size_t)(e0 + i) * W + j] = f2bf16((float)v.r) | (f2bf16((float)v.i) << 16);
     }
   }
 }
 
 // z = Bi r for the parity hook: one block per tile of whole node blocks, r staged in LDS.
 __global__ void __launch_bounds__(256) k_nodeblk_apply(const int *__restrict__ tptr, const unsigned char *__restrict__ bm,
-                                                       const cx<float> *__restrict__ bi, const cx<double> *__restrict__ r,
+                                                       const nbcoef *__restrict__ bi, const cx<double> *__restrict__ r,
                                                        cx<double> *__restrict__ z) {
   __shared__ cx<double> tr[256];
   const int t = threadIdx.x;
@@ -2586,19 +2598,19 @@ __global__ void __launch_bounds__(256) k_nodeblk_apply(const int *__restrict__ t
   cx<double> acc = {0.0, 0.0};
 #pragma unroll
   for (int j = 0; j < AFX_NB_W; j++) {
-    const cx<float> c = bi[(size_t)row * AFX_NB_W + j];
-    acc = cfma(cx<double>{(double)c.r, (double)c.i}, tr[t - i + min(j, sz - 1)], acc);
+    const uint4 g = ((const uint4 *)bi)[(size_t)row * (AFX_NB_W / 4) + (j >> 2)];
+    acc = cfma(nb_coef(g, j & 3), tr[t - i + min(j, sz - 1)], acc);
   }
   z[row] = acc;
 }
 
 hipError_t launch_nodeblk_setup(int n, const unsigned char *bm, const int *rowptr, const int *col,
-                                const cx<double> *val, double w, cx<float> *bi, hipStream_t s) {
+                                const cx<double> *val, double w, nbcoef *bi, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_nodeblk_setup, dim3((n + 127) / 128), dim3(128), 0, s, n, bm, rowptr, col, val, w, bi);
   return hipGetLastError();
 }
-hipError_t launch_nodeblk_apply(int ntile, const int *tptr, const unsigned char *bm, const cx<float> *bi,
+hipError_t launch_nodeblk_apply(int ntile, const int *tptr, const unsigned char *bm, const nbcoef *bi,
                                 const cx<double> *r, cx<double> *z, hipStream_t s) {
   if (ntile <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_nodeblk_apply, dim3(ntile), dim3(256), 0, s, tptr, bm, bi, r, z);

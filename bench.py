@@ -135,8 +135,8 @@ def bytes_per_iteration_fused(n, nnz, nn, nadj, edge_pc, ntile=0):
     p pass: r (or, NODEBLK, Bi r), D (DIAG), p read, p written, the edges' end nodes (8 B per
     edge), zn read;
     SpMV: (K, M) values 16 B + column 4 B per entry, row pointers, p read, q written;
-    x, r, z pass: x, p, q, r, D (DIAG) read, x and the next r written; NODEBLK: 8 complex-FP32
-    coefficients (64 B) and 1 B of block place per row, the tile bounds, Bi r written; on the
+    x, r, z pass: x, p, q, r, D (DIAG) read, x and the next r written; NODEBLK: 8 complex-bfloat16
+    coefficients (32 B) and 1 B of block place per row, the tile bounds, Bi r written; on the
     nodes the incidence list (4 B per entry), the row pointers, diag(G^T M G) read and zn written
     (r and q gathered there are the arrays the edge part reads in the same pass)."""
     d = 16 * n if edge_pc == "DIAG" else 0
@@ -144,7 +144,7 @@ def bytes_per_iteration_fused(n, nnz, nn, nadj, edge_pc, ntile=0):
     spmv = nnz * 20 + 4 * (n + 1) + 2 * 16 * n
     xn = 6 * 16 * n + d + 4 * nadj + 4 * (nn + 1) + 8 * nn + 16 * nn
     if edge_pc == "NODEBLK":
-        xn += 64 * n + n + 4 * (ntile + 1) + 16 * n
+        xn += 32 * n + n + 4 * (ntile + 1) + 16 * n
     return ppass + spmv + xn
 
 

@@ -425,11 +425,19 @@ def node_blocks(line, width=8):
     return out
 
 
+def _bf16(x32):
+    """float32 array rounded to bfloat16 (nearest even on the float32 bits), returned as float32."""
+    u = np.ascontiguousarray(x32, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
 def node_block_inverse(A, blocks):
     """The node-block Jacobi operator: each diagonal block of A inverted (numpy LU with pivoting,
-    batched over blocks padded to 8 x 8 with the identity), symmetrised as (X + X^T) / 2 and
-    rounded to complex FP32, as a scipy CSR matrix. The blocks' entries are scattered from A's
-    CSR entries whose row and column lie in the same block."""
+    batched over blocks padded to 8 x 8 with the identity), symmetrised as (X + X^T) / 2, rounded
+    to complex FP32 and then each part to bfloat16 (nearest even), as a scipy CSR matrix. The
+    blocks' entries are scattered from A's CSR entries whose row and column lie in the same
+    block."""
     import scipy.sparse as sp
     A = sp.csr_matrix(A)
     n = A.shape[0]
@@ -451,7 +459,8 @@ def node_block_inverse(A, blocks):
     D[:, np.arange(W), np.arange(W)] = np.where(np.arange(W)[None, :] < size[:, None], 0.0, 1.0)
     D[blk[rows[m]], loc[rows[m]], loc[cols[m]]] = A.data[m]
     X = np.linalg.inv(D)
-    X = (0.5 * (X + np.transpose(X, (0, 2, 1)))).astype(np.complex64).astype(np.complex128)
+    X = (0.5 * (X + np.transpose(X, (0, 2, 1)))).astype(np.complex64)
+    X = _bf16(X.real).astype(np.float64) + 1j * _bf16(X.imag).astype(np.float64)
     # row r's stored entries: its block's columns start .. start + size - 1
     ii = np.repeat(np.arange(n), size[blk])
     off = np.arange(W)[None, :]

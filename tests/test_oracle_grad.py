@@ -78,7 +78,8 @@ def test_corrected_preconditioner_is_symmetric_and_cocg_reaches_direct(small):
 def test_node_block_jacobi_oracle():
     """The oracle's node blocks and their inverses (AFX_PC_NODEBLK): the blocks partition the edges
     into runs of at most 8 consecutive edges with one first node; each stored inverse times its
-    block is the identity to FP32 rounding, the operator is exactly symmetric, and with the nodal
+    block is the identity to the bfloat16 rounding of its coefficients, the operator is exactly
+    symmetric, the bfloat16 rounding is nearest-even on the float32 bits, and with the nodal
     correction COCG needs fewer iterations than with the diagonal (the measured reason for it)."""
     from actfemtide_amd import mesh
     from oracle import oracle as O
@@ -95,7 +96,11 @@ def test_node_block_jacobi_oracle():
     assert (Bi - Bi.T).count_nonzero() == 0
     for idx in blocks[::7]:
         prod = Bi[idx][:, idx].toarray() @ A[idx][:, idx].toarray()
-        assert np.abs(prod - np.eye(idx.size)).max() <= 1e-5
+        assert np.abs(prod - np.eye(idx.size)).max() <= 2e-2  # bfloat16 coefficients
+    import torch
+    xs = np.random.default_rng(3).standard_normal(100000).astype(np.float32) * 10.0 ** np.arange(-6, 4).repeat(10000)
+    xs = np.r_[xs, np.float32([1.00390625, 1.005859375, -1.01171875])]  # ties round to even
+    assert np.array_equal(O._bf16(xs), torch.from_numpy(xs).to(torch.bfloat16).to(torch.float32).numpy())
     G, _ = O.discrete_gradient(c.line, c.node, o["bc"])
     corr, _ = O.grad_correction(A, G)
     b = o["b"]

@@ -17,6 +17,10 @@ namespace afx {
 #define AFX_PIPE_W 16
 // rows of one node block of AFX_PC_NODEBLK (the coefficients kept per row)
 #define AFX_NB_W 8
+// tiles of the node-block pass in flight per round (each thread one row of each)
+#ifndef AFX_NB_UT
+#define AFX_NB_UT 2
+#endif
 // a node-block coefficient: a complex number as two bfloat16 halves (real in the low 16 bits)
 typedef unsigned int nbcoef;
 // words of the grid barrier: top counter, generation, AFX_BAR_GROUPS group counters, each on

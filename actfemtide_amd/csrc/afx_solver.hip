@@ -1731,7 +1731,7 @@ __device__ __forceinline__ cx<double> nodal_g_next(const int *__restrict__ nptr,
 template <int LPR, bool PROF>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) k_cocg1(CocgParams<double> p) {
   __shared__ double lds[4 * 4];
-  __shared__ cx<double> tile_r[2][256];  // node-block pass: r' of the round's tiles
+  __shared__ cx<double> tile_r[AFX_NB_UT][256];  // node-block pass: r' of the round's tiles
   constexpr int UP = 8, UX = 4;  // rows per thread in flight in the p pass and the x, r pass
   const int t = threadIdx.x;
   const long long gtid = (long long)blockIdx.x * 256 + t;
@@ -1774,12 +1774,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
   };
 
   // node-block pass: over tiles of whole blocks (UT tiles per round, every load of a round issued
-  // first), r' = r - alpha q (q null: r' = r) into Rn when Rn is given, |r'|^2 into acc[2],
-  // Zb = Bi r' with r' staged in LDS, and the edge part r'^T Zb of rho into acc[0..1]
-  auto nblk_pass = [&](const cx<double> *Rr, const cx<double> *Qq, cx<double> ma, cx<double> *Rn, double (&acc)[3]) {
-    constexpr int UT = 2;
+  // first), x += al p when Xx is given, r' = r - alpha q (q null: r' = r) into Rn when Rn is given,
+  // |r'|^2 into acc[2], Zb = Bi r' with r' staged in LDS, and the edge part r'^T Zb of rho into
+  // acc[0..1]
+  auto nblk_pass = [&](const cx<double> *Rr, const cx<double> *Qq, cx<double> ma, cx<double> *Rn, cx<double> *Xx,
+                       cx<double> al, double (&acc)[3]) {
+    constexpr int UT = AFX_NB_UT;
     for (long long t0 = blockIdx.x; t0 < p.ntile; t0 += UT * (long long)gridDim.x) {
-      cx<double> rv[UT], qv[UT];
+      cx<double> rv[UT], qv[UT], xv[UT], pv[UT];
       uint4 cv[UT][AFX_NB_W / 4];
       int row[UT], m[UT];
       bool ok[UT];
@@ -1792,6 +1794,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         const int rc = ok[u] ? row[u] : 0;
         rv[u] = Rr[rc];
         qv[u] = Qq ? Qq[rc] : cx<double>{0.0, 0.0};
+        xv[u] = Xx ? Xx[rc] : cx<double>{0.0, 0.0};
+        pv[u] = Xx ? p.P[rc] : cx<double>{0.0, 0.0};
         m[u] = ok[u] ? (int)p.bm[rc] : 0;
 #pragma unroll
         for (int j = 0; j < AFX_NB_W / 4; j++) cv[u][j] = ((const uint4 *)p.bi)[(size_t)rc * (AFX_NB_W / 4) + j];
@@ -1800,6 +1804,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
       for (int u = 0; u < UT; u++) {
         const cx<double> rn = ok[u] ? cfma(ma, qv[u], rv[u]) : cx<double>{0.0, 0.0};
         if (ok[u]) {
+          if (Xx) Xx[row[u]] = cfma(al, pv[u], xv[u]);
           if (Rn) Rn[row[u]] = rn;
           acc[2] += rn.r * rn.r + rn.i * rn.i;
         }
@@ -1829,7 +1834,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     double acc[3] = {0.0, 0.0, 0.0};
     const cx<double> *R = p.R;
     if (nblk)
-      nblk_pass(R, nullptr, cx<double>{0.0, 0.0}, nullptr, acc);
+      nblk_pass(R, nullptr, cx<double>{0.0, 0.0}, nullptr, nullptr, cx<double>{0.0, 0.0}, acc);
     else
       for (long long r0 = gtid; r0 < n; r0 += nthr) edge_rho(R[r0], diag ? DD[r0] : cx<double>{0.0, 0.0}, acc);
     if (grad)
@@ -1895,21 +1900,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     // ---- XN: x += alpha p, r' = r - alpha q, |r'|^2, and z, rho of r' ----
     double acc[3] = {0.0, 0.0, 0.0};  // rho' (re, im), |r'|^2
     if (nblk) {
-      for (long long r0 = gtid; r0 < n; r0 += UX * nthr) {
-        cx<double> xv[UX], pv[UX];
-#pragma unroll
-        for (int u = 0; u < UX; u++) {
-          const long long r = r0 + u * nthr < n ? r0 + u * nthr : r0;
-          xv[u] = p.X[r];
-          pv[u] = p.P[r];
-        }
-#pragma unroll
-        for (int u = 0; u < UX; u++) {
-          const long long r = r0 + u * nthr;
-          if (r < n) p.X[r] = cfma(alpha, pv[u], xv[u]);
-        }
-      }
-      nblk_pass(R, p.Q, malpha, Rn, acc);
+      nblk_pass(R, p.Q, malpha, Rn, p.X, alpha, acc);
     } else
     for (long long r0 = gtid; r0 < n; r0 += UX * nthr) {
       cx<double> xv[UX], pv[UX], qv[UX], rv[UX], dv[UX];

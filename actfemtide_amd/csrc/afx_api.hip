@@ -342,8 +342,8 @@ int ensure_precond(afx_system s, int kind_full, double sd, double sig, const std
   if (grad && (kind != AFX_PC_SSOR && kind != AFX_PC_SSOR_BJ && kind != AFX_PC_DIAG && kind != AFX_PC_NODEBLK))
     return fail(AFX_ERR_BADOPT, "AFX_PC_GRAD goes with SSOR, SSOR_BJ, DIAG or NODEBLK");
   if (grad && s->nn < 0) return fail(AFX_ERR_ARG, "AFX_PC_GRAD needs afx_set_nodes");
-  if (kind == AFX_PC_NODEBLK && (s->ntile <= 0 || wf.size() != 1 || s->nsys > 1))
-    return fail(AFX_ERR_ARG, "AFX_PC_NODEBLK needs afx_set_nodes, one frequency and one system");
+  if (kind == AFX_PC_NODEBLK && (s->ntile <= 0 || s->nsys > 1))
+    return fail(AFX_ERR_ARG, "AFX_PC_NODEBLK needs afx_set_nodes and one system");
   int rc = ensure_derived(s);
   if (rc) return rc;
   if (grad)
@@ -369,9 +369,12 @@ int ensure_precond(afx_system s, int kind_full, double sd, double sig, const std
                                            s->d_lvlL, s->d_rowL, s->d_Lptr, s->d_Lcol, s->d_Lval64, s->d_INU,
                                            s->d_canon + s->n, d_wf, s->d_DD, grid, s->stream);
   if (e == hipSuccess && kind == AFX_PC_NODEBLK) {
-    if (!s->d_bi && dalloc(&s->d_bi, (size_t)s->n * AFX_NB_W) != AFX_OK) e = hipErrorOutOfMemory;
-    if (e == hipSuccess)
-      e = afx::launch_nodeblk_setup(s->n, s->d_bm, s->d_rowptr, s->d_col, s->d_val64, wf[0], s->d_bi, s->stream);
+    // one slice of coefficients per distinct frequency
+    dfree(s->d_bi);
+    if (dalloc(&s->d_bi, (size_t)s->n * AFX_NB_W * nf) != AFX_OK) e = hipErrorOutOfMemory;
+    for (int f = 0; f < nf && e == hipSuccess; f++)
+      e = afx::launch_nodeblk_setup(s->n, s->d_bm, s->d_rowptr, s->d_col, s->d_val64, wf[f],
+                                    s->d_bi + (size_t)f * s->n * AFX_NB_W, s->stream);
   }
   if (e == hipSuccess) e = afx::launch_cast_vec((long long)s->n * nf, s->d_DD, s->d_DD32, s->stream);
   if (e == hipSuccess && nf == 1) e = afx::launch_dpos(s->n, s->fw.row, s->d_DD, s->fw.dpos64, s->fw.dpos32, s->stream);
@@ -855,8 +858,8 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   // the streaming kernel (no sweep: DIAG or NONE edge preconditioner, one column, FP64) runs
   // at several blocks per CU; every other case runs k_cocg on the sweep partition
   const bool nblk = (pc == AFX_PC_NODEBLK);
-  if (nblk && (multi || !fp64 || (stacked && s->nsys > 1)))
-    return fail(AFX_ERR_ARG, "AFX_PC_NODEBLK runs one FP64 column of one system per launch");
+  if (nblk && (k > 8 || !fp64 || (stacked && s->nsys > 1)))
+    return fail(AFX_ERR_ARG, "AFX_PC_NODEBLK runs FP64, at most 8 columns of one system per launch");
   const bool stream = fp64 && !(stacked && s->nsys > 1) && (pc == AFX_PC_DIAG || pc == AFX_PC_NONE || nblk) &&
                       (!multi || k <= 8) && (nblk || !getenv("AFX_NO_STREAM"));
   int per_cu = 1;
@@ -1435,15 +1438,18 @@ int solve_device_impl(afx_system s, int k, const cx<double> *dB, cx<double> *dX,
   const char *bm = getenv("AFX_BATCH");
   const int width = force_width >= 0 ? force_width
                     : bm ? (std::string(bm) == "multi" ? 1 : std::string(bm) == "stack" ? 2 : 0) : s->batch_multi[k];
+  // the node blocks, like the nodal correction, have no stacked-copies form: one column at a time
+  // unless the k-column kernel was chosen
   const bool grad = ((o->nset == 0 ? s->pc_kind : o->precond) & AFX_PC_GRAD) != 0 ||
                     ((o->nset == 0 ? s->pc_kind : o->precond) & 0xff) == AFX_PC_NODEBLK;
-  const bool multi = width == 1 && ((o->nset == 0 ? s->pc_kind : o->precond) & 0xff) != AFX_PC_NODEBLK;
+  const bool multi = width == 1;
   if (k > 1 && o->nset != 0 && width == 2 && !stacked && !grad)
     return solve_stacked_columns(s, k, dB, dX, omega_scale, o, stats);
   if (k > 1 && (o->nset != 0 || grad) && !multi) return solve_columns(s, k, dB, dX, omega_scale, o, stats);
   // the k-column kernel without sweeps takes at most 8 columns: larger blocks in groups of 8
   const int base_pc = (o->nset == 0 ? s->pc_kind : o->precond) & 0xff;
-  if (multi && k > 8 && o->precision == AFX_FP64 && (base_pc == AFX_PC_DIAG || base_pc == AFX_PC_NONE) &&
+  if (multi && k > 8 && o->precision == AFX_FP64 &&
+      (base_pc == AFX_PC_DIAG || base_pc == AFX_PC_NONE || base_pc == AFX_PC_NODEBLK) &&
       !getenv("AFX_NO_STREAM"))
     return solve_groups(s, k, 8, dB, dX, omega_scale, o, stats);
   std::vector<double> wcol, wf;
@@ -2412,7 +2418,7 @@ static int autotune_width(afx_system s, int k, int kind, int verbose) {
   const std::string keep_err = g_err;
   const int grp = stack_group(s, k);
   // stacked copies carry no node incidence: not a width for the nodal correction
-  const bool can_stack = grp >= 2 && !(kind & AFX_PC_GRAD) &&
+  const bool can_stack = grp >= 2 && !(kind & AFX_PC_GRAD) && (kind & 0xff) != AFX_PC_NODEBLK &&
                          ensure_stack(s, grp, std::vector<double>(wcol.begin(), wcol.begin() + grp)) == AFX_OK &&
                          lds_rows_for<double>(s->stack, false) > 0;
   g_err = keep_err;

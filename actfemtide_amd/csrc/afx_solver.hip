@@ -1368,6 +1368,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
   const int nthr = gridDim.x * 256;
   const int n = p.n, k = p.k;
   const bool diag = p.pc == AFX_PC_DIAG;
+  const bool nblk = p.pc == AFX_PC_NODEBLK;
   // element passes over the [n][k] vectors: element i = row * k + c, a stride that is a multiple
   // of k so every thread keeps one column (consecutive threads, consecutive 16-byte elements)
   const int estride = (nthr / k) * k;
@@ -1424,7 +1425,37 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         p.Zn[i] = zn;
         mine = addv(mine, cmul(g, zn));
       }
-    if (eact)
+    if (eact && nblk)
+      // node blocks: z of (row, column) from the row's 8 coefficients (the column's frequency
+      // slice) and the column's r on the rows of the row's block (at most 8 rows, stride k)
+      for (long long i0 = gtid; i0 < nk; i0 += 2LL * estride) {
+        cx<double> rb[2][AFX_NB_W], rv[2];
+        uint4 cv[2][AFX_NB_W / 4];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          const long long i = i0 + u * (long long)estride < nk ? i0 + u * (long long)estride : i0;
+          const long long row = i / k;
+          const int m = p.bm[row], loc = m & 15, sz = (m >> 4) + 1;
+          const uint4 *cf = (const uint4 *)p.bi + ((size_t)fe * n + row) * (AFX_NB_W / 4);
+#pragma unroll
+          for (int j = 0; j < AFX_NB_W / 4; j++) cv[u][j] = cf[j];
+#pragma unroll
+          for (int j = 0; j < AFX_NB_W; j++) rb[u][j] = p.R[(row - loc + min(j, sz - 1)) * k + ec];
+          rv[u] = p.R[i];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          const long long i = i0 + u * (long long)estride;
+          if (i < nk) {
+            cx<double> z = {0.0, 0.0};
+#pragma unroll
+            for (int j = 0; j < AFX_NB_W; j++) z = cfma(nb_coef(cv[u][j >> 2], j & 3), rb[u][j], z);
+            p.Z[i] = z;
+            mine = addv(mine, cmul(rv[u], z));
+          }
+        }
+      }
+    else if (eact)
       for (long long i0 = gtid; i0 < nk; i0 += 4LL * estride) {
         cx<double> rv[4], dv[4];
 #pragma unroll

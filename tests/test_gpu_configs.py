@@ -50,7 +50,8 @@ def cfg2():
 
 def test_cfg2_forms_against_a_tight_solve(cfg2):
     """Every measured form at a 1e-8 FP64 true residual has seafloor site fields within 1e-6 of a
-    1e-12 solve of the same system; AFX_MIXED (FP32 edge preconditioner) included."""
+    1e-12 solve of the same system: the bench's NODEBLK+GRAD (bfloat16 block coefficients) and the
+    other forms, AFX_MIXED (FP32 edge preconditioner) included."""
     import torch
     from actfemtide_amd import afx
     c, s, b = cfg2
@@ -62,8 +63,8 @@ def test_cfg2_forms_against_a_tight_solve(cfg2):
     tight = _fields(s, item, ve, vb, dT)
     print(f"tight SSOR+GRAD: {st[0].iter} iterations, true residual {st[0].true_resid:.2e}", flush=True)
     assert st[0].true_resid <= 1e-11
-    for kind, prec in (("DIAG+GRAD", afx.FP64), ("SSOR_BJ+GRAD", afx.FP64), ("SSOR+GRAD", afx.MIXED),
-                       ("DIAG+GRAD", afx.MIXED)):
+    for kind, prec in (("NODEBLK+GRAD", afx.FP64), ("DIAG+GRAD", afx.FP64), ("SSOR_BJ+GRAD", afx.FP64),
+                       ("SSOR+GRAD", afx.MIXED), ("DIAG+GRAD", afx.MIXED)):
         s.precond_setup(kind)
         dX = torch.zeros_like(dB)
         st, rc = s.solve_device(dB, dX, precond=kind, precision=prec, tol=1e-8, maxit=20000, max_outer=8)

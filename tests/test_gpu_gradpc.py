@@ -134,8 +134,9 @@ def test_grad_solve_cfg1_fields_against_direct():
 @pytest.mark.parametrize("k", [5, 12])
 def test_grad_k_columns_match_one_at_a_time(small, monkeypatch, k):
     """The 5 tidal constituents (per-column omega; 12 columns: the constituents repeated, solved
-    as blocks of 8 + 4) in k-column launches with DIAG+GRAD, SSOR+GRAD and DIAG alone give every
-    column about the iteration count of its own single-column solve, each to
+    as blocks of 8 + 4) in k-column launches with DIAG+GRAD, SSOR+GRAD, DIAG alone and NODEBLK+GRAD
+    (one slice of block coefficients per frequency) give every column about the iteration count
+    of its own single-column solve, each to
     a 1e-8 true residual, with site fields within 1e-6 of the single-column solve's (x itself is
     only pinned to about 1e-5 at this residual: its air gradient components barely act on b)."""
     import torch
@@ -145,7 +146,7 @@ def test_grad_k_columns_match_one_at_a_time(small, monkeypatch, k):
     item, ve, vb, _ = O.prepobscoeff(s_case(small), seafloor_sites(s_case(small), 8, 5))
     om = np.resize(np.array([2 * math.pi / mesh.PERIODS[c] for c in ("M2", "S2", "N2", "K1", "O1")]), k)
     dB = torch.from_numpy(np.ascontiguousarray(np.tile(b[:, None], (1, k)))).cuda()
-    for kind in ("DIAG+GRAD", "SSOR+GRAD", "DIAG"):
+    for kind in ("DIAG+GRAD", "SSOR+GRAD", "DIAG", "NODEBLK+GRAD"):
         s.precond_setup(kind)
         out = {}
         for width in ("columns", "multi"):

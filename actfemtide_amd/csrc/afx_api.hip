@@ -2650,7 +2650,8 @@ static std::mutex g_pds_mu;
 static afx_system g_pds_sys = nullptr;
 
 // Pattern, values, node incidence and preconditioner of the kept system; *pc receives the
-// preconditioner: DIAG with the nodal correction when the edges' end nodes are given, else SSOR.
+// preconditioner: node blocks with the nodal correction when the edges' end nodes are given,
+// else SSOR.
 static int pardiso_system(int doftot, const int *INU, const int *IAU, const int *INL, const int *IAL, const double *D,
                           const double *AU, const double *AL, int node, const int *line,
                           const unsigned char *line_bc, int *pc) {
@@ -2672,7 +2673,7 @@ static int pardiso_system(int doftot, const int *INU, const int *IAU, const int 
   *pc = AFX_PC_SSOR;
   if (line && node > 0) {
     if ((rc = afx_set_nodes(s, node, line, line_bc))) return rc;
-    *pc = AFX_PC_DIAG | AFX_PC_GRAD;
+    *pc = AFX_PC_NODEBLK | AFX_PC_GRAD;
   }
   if ((rc = afx_precond_setup(s, *pc, 1.0, 0.0))) return rc;
   if (fresh) rc = afx_autotune(s, 1, 0);

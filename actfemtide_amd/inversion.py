@@ -108,7 +108,7 @@ class GaussNewton:
     beta: the minimum-support stabiliser's beta; gamma: alpha's factor on the spectral radius."""
 
     def __init__(self, case, sites, omegas, m_ref, *, err_amp=0.01, err_pha=None, beta=10.0, gamma=0.1,
-                 precond="DIAG+GRAD", tol=1e-8, maxit=40000, warm=True, autotune=True):
+                 precond="NODEBLK+GRAD", tol=1e-8, maxit=40000, warm=True, autotune=True):
         import torch
         self.case = case
         self.omegas = np.asarray(omegas, np.float64)

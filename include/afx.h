@@ -264,7 +264,7 @@ int afx_cocg13_single_release(void);
    true residual tol (||b - A x|| / ||b|| per column, recomputed; PARDISO's own accuracy is about
    1e-12); with the edges' end nodes line(2, nline) (READLINE's g_line%line, 1-based) and node count,
    and the Dirichlet mask line_bc (GENBCCSEM's line_bc as one byte per edge), the preconditioner is
-   DIAG + AFX_PC_GRAD, otherwise SSOR (line = NULL). Returns 0, -100 (tol not reached) or an
+   AFX_PC_NODEBLK + AFX_PC_GRAD, otherwise SSOR (line = NULL). Returns 0, -100 (tol not reached) or an
    AFX_ERR_* where the reference stops. The system of the last call is kept while the pattern is
    the same (afx_solve_pardiso_release frees it); calls are serialised. ip is not needed. */
 int afx_solve_pardiso(int doftot, int nsr, const int *INU, const int *IAU, const int *INL, const int *IAL,

@@ -164,8 +164,8 @@ struct afx_system_s {
   // start at the same node (at most AFX_NB_W rows; longer runs are split), packed whole into
   // tiles of at most 256 rows; per row its place in its block (i | (size - 1) << 4), and the
   // block inverses as AFX_NB_W complex-bfloat16 coefficients per row
-  int ntile = 0;
-  int *d_tptr = nullptr;
+  int ntile = 0, nblk = 0;
+  int *d_tptr = nullptr, *d_bstart = nullptr;  // tile bounds [ntile + 1], first row of each block [nblk]
   unsigned char *d_bm = nullptr;
   afx::nbcoef *d_bi = nullptr;
 };
@@ -373,7 +373,7 @@ int ensure_precond(afx_system s, int kind_full, double sd, double sig, const std
     dfree(s->d_bi);
     if (dalloc(&s->d_bi, (size_t)s->n * AFX_NB_W * nf) != AFX_OK) e = hipErrorOutOfMemory;
     for (int f = 0; f < nf && e == hipSuccess; f++)
-      e = afx::launch_nodeblk_setup(s->n, s->d_bm, s->d_rowptr, s->d_col, s->d_val64, wf[f],
+      e = afx::launch_nodeblk_setup(s->nblk, s->d_bstart, s->d_bm, s->d_rowptr, s->d_col, s->d_val64, wf[f],
                                     s->d_bi + (size_t)f * s->n * AFX_NB_W, s->stream);
   }
   if (e == hipSuccess) e = afx::launch_cast_vec((long long)s->n * nf, s->d_DD, s->d_DD32, s->stream);
@@ -1152,7 +1152,7 @@ void release(afx_system s) {
   dfree(s->d_rstart); dfree(s->d_bar); dfree(s->d_err);
   dfree(s->d_sys_off); dfree(s->d_bsys); dfree(s->d_sblk);
   dfree(s->d_nptr); dfree(s->d_nadj); dfree(s->d_en); dfree(s->d_mnd); dfree(s->d_noff);
-  dfree(s->d_tptr); dfree(s->d_bm); dfree(s->d_bi);
+  dfree(s->d_tptr); dfree(s->d_bm); dfree(s->d_bi); dfree(s->d_bstart);
   if (s->stack) afx_destroy(s->stack);
   s->stack = nullptr;
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -1916,27 +1916,30 @@ int afx_set_nodes(afx_system s, int node, const int *line, const unsigned char *
   // by their first node, so a run is the node's edges towards higher-numbered nodes), cut into
   // pieces of at most AFX_NB_W rows; tiles of whole blocks, at most 256 rows each
   std::vector<unsigned char> bm(n);
-  std::vector<int> tptr(1, 0);
+  std::vector<int> tptr(1, 0), bstart;
   for (int e0 = 0; e0 < n;) {
     int e1 = e0 + 1;
     while (e1 < n && e1 - e0 < AFX_NB_W && line[2 * e1] == line[2 * e0]) e1++;
+    bstart.push_back(e0);
     if (e1 - tptr.back() > 256) tptr.push_back(e0);
     for (int e = e0; e < e1; e++) bm[e] = (unsigned char)((e - e0) | ((e1 - e0 - 1) << 4));
     e0 = e1;
   }
   tptr.push_back(n);
   dfree(s->d_nptr); dfree(s->d_nadj); dfree(s->d_en); dfree(s->d_mnd); dfree(s->d_noff);
-  dfree(s->d_tptr); dfree(s->d_bm); dfree(s->d_bi);
+  dfree(s->d_tptr); dfree(s->d_bm); dfree(s->d_bi); dfree(s->d_bstart);
   s->nn = -1;
-  s->ntile = 0;
+  s->ntile = s->nblk = 0;
   int rc;
   if (!noff.empty() && (rc = upload(&s->d_noff, noff.data(), noff.size()))) return rc;
   if ((rc = upload(&s->d_nptr, cnt.data(), cnt.size())) || (rc = upload(&s->d_nadj, adj.data(), adj.size())) ||
       (rc = upload(&s->d_en, en.data(), en.size())) || (rc = dalloc(&s->d_mnd, (size_t)nn + 1)) ||
-      (rc = upload(&s->d_tptr, tptr.data(), tptr.size())) || (rc = upload(&s->d_bm, bm.data(), bm.size())))
+      (rc = upload(&s->d_tptr, tptr.data(), tptr.size())) || (rc = upload(&s->d_bm, bm.data(), bm.size())) ||
+      (rc = upload(&s->d_bstart, bstart.data(), bstart.size())))
     return rc;
   s->nn = nn;
   s->ntile = (int)tptr.size() - 1;
+  s->nblk = (int)bstart.size();
   if (s->pc_kind & AFX_PC_GRAD || (s->pc_kind & 0xff) == AFX_PC_NODEBLK) s->pc_valid = false;
   return AFX_OK;
 }

@@ -169,8 +169,8 @@ hipError_t launch_cocg_diag(const CocgParams<double> &p, int lpr, int grid, hipS
 // one column, DIAG / NONE (+ nodal correction), FP64: the fused kernel, one block of 256 threads per CU
 int max_coop_grid_cocg1(int lpr);
 // node-block Jacobi (AFX_PC_NODEBLK): block inverses from the CSR values at omega w, and z = Bi r
-hipError_t launch_nodeblk_setup(int n, const unsigned char *bm, const int *rowptr, const int *col,
-                                const cx<double> *val, double w, nbcoef *bi, hipStream_t s);
+hipError_t launch_nodeblk_setup(int nblk, const int *bstart, const unsigned char *bm, const int *rowptr,
+                                const int *col, const cx<double> *val, double w, nbcoef *bi, hipStream_t s);
 hipError_t launch_nodeblk_apply(int ntile, const int *tptr, const unsigned char *bm, const nbcoef *bi,
                                 const cx<double> *r, cx<double> *z, hipStream_t s);
 hipError_t launch_cocg1(const CocgParams<double> &p, int lpr, int grid, hipStream_t s);

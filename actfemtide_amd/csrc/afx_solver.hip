@@ -2547,60 +2547,69 @@ __global__ void k_grad_edges(int n, const int *en, const cx<double> *zn, cx<doub
 
 // Node-block Jacobi setup (AFX_PC_NODEBLK): the inverse of A's diagonal block over each node block
 // (bm[e] = i | (size - 1) << 4, rows e - i .. e - i + size - 1), A = K + i w M from the full CSR
-// (K, M) pairs. One thread per block (the thread of its first row): the block in registers,
-// in-place Gauss-Jordan without pivoting in FP64 (every principal submatrix of K + i w M with K
-// positive semidefinite and w M positive definite is nonsingular), then (X + X^T) / 2 rounded to
-// FP32 and then to bfloat16 (both to nearest even), so the stored operator is exactly symmetric.
-// Row i of the block is stored as AFX_NB_W coefficients at bi[(e0 + i) * AFX_NB_W], zero beyond
-// the block's size.
-__global__ void k_nodeblk_setup(int n, const unsigned char *__restrict__ bm, const int *__restrict__ rowptr,
-                                const int *__restrict__ col, const cx<double> *__restrict__ val, double w,
-                                nbcoef *__restrict__ bi) {
+// (K, M) pairs. Eight lanes per block, lane i holding row i of the block padded to 8 x 8 with
+// identity rows; in-place Gauss-Jordan without pivoting in FP64 (every principal submatrix of
+// K + i w M with K positive semidefinite and w M positive definite is nonsingular; the identity
+// padding leaves the block's inverse unchanged), the pivot row taken from its lane by shuffles;
+// then (X + X^T) / 2 (the transpose through LDS) rounded to FP32 and then to bfloat16 (both to
+// nearest even), so the stored operator is exactly symmetric. Row i of the block is stored as
+// AFX_NB_W coefficients at bi[(e0 + i) * AFX_NB_W], zero beyond the block's size.
+__global__ void __launch_bounds__(256) k_nodeblk_setup(int nblk, const int *__restrict__ bstart,
+                                                       const unsigned char *__restrict__ bm,
+                                                       const int *__restrict__ rowptr, const int *__restrict__ col,
+                                                       const cx<double> *__restrict__ val, double w,
+                                                       nbcoef *__restrict__ bi) {
   constexpr int W = AFX_NB_W;
-  const int e0 = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e0 >= n || (bm[e0] & 15) != 0) return;
-  const int sz = (bm[e0] >> 4) + 1;
-  cx<double> a[W][W];
+  __shared__ cx<double> tr[256 / W][W][W];
+  const int li = threadIdx.x & (W - 1), grp = threadIdx.x / W;
+  const int g = blockIdx.x * (256 / W) + grp;
+  const bool okg = g < nblk;
+  const int e0 = okg ? bstart[g] : 0;
+  const int sz = okg ? (bm[e0] >> 4) + 1 : 1;
+  const bool live = okg && li < sz;
+  cx<double> a[W];
 #pragma unroll
-  for (int i = 0; i < W; i++)
+  for (int j = 0; j < W; j++) a[j] = {j == li ? 1.0 : 0.0, 0.0};
+  if (live) {
 #pragma unroll
-    for (int j = 0; j < W; j++) a[i][j] = {0.0, 0.0};
-#pragma unroll
-  for (int i = 0; i < W; i++) {
-    if (i >= sz) break;
-    for (int q = rowptr[e0 + i]; q < rowptr[e0 + i + 1]; q++) {
+    for (int j = 0; j < W; j++) a[j] = {0.0, 0.0};
+    for (int q = rowptr[e0 + li]; q < rowptr[e0 + li + 1]; q++) {
       const int c = col[q] - e0;
       const cx<double> v = mval(val[q], w);
 #pragma unroll
       for (int j = 0; j < W; j++)
-        if (c == j && j < sz) a[i][j] = v;
+        if (c == j && j < sz) a[j] = v;
     }
   }
+  const int base = threadIdx.x & ~(W - 1);
 #pragma unroll
   for (int k = 0; k < W; k++) {
-    if (k >= sz) break;
-    const cx<double> piv = cdivd(cx<double>{1.0, 0.0}, a[k][k]);
-    a[k][k] = {1.0, 0.0};
+    cx<double> rk[W];
 #pragma unroll
-    for (int j = 0; j < W; j++) a[k][j] = cmul(a[k][j], piv);
+    for (int j = 0; j < W; j++) {
+      rk[j].r = __shfl(a[j].r, base + k);
+      rk[j].i = __shfl(a[j].i, base + k);
+    }
+    const cx<double> piv = cdivd(cx<double>{1.0, 0.0}, rk[k]);
+    if (li == k) {
 #pragma unroll
-    for (int i = 0; i < W; i++) {
-      if (i == k || i >= sz) continue;
-      const cx<double> f = a[i][k];
-      a[i][k] = {0.0, 0.0};
+      for (int j = 0; j < W; j++) a[j] = (j == k) ? piv : cmul(rk[j], piv);
+    } else {
+      const cx<double> f = a[k];
 #pragma unroll
-      for (int j = 0; j < W; j++) a[i][j] = cfma(cx<double>{-f.r, -f.i}, a[k][j], a[i][j]);
+      for (int j = 0; j < W; j++)
+        a[j] = (j == k) ? cmul(cx<double>{-f.r, -f.i}, piv) : cfma(cx<double>{-f.r, -f.i}, cmul(rk[j], piv), a[j]);
     }
   }
 #pragma unroll
-  for (int i = 0; i < W; i++) {
-    if (i >= sz) break;
+  for (int j = 0; j < W; j++) tr[grp][li][j] = a[j];
+  __syncthreads();
+  if (!live) return;
 #pragma unroll
-    for (int j = 0; j < W; j++) {
-      const cx<double> v = (j < sz) ? cx<double>{0.5 * (a[i][j].r + a[j][i].r), 0.5 * (a[i][j].i + a[j][i].i)}
-                                    : cx<double>{0.0, 0.0};
-      bi[(size_t)(e0 + i) * W + j] = f2bf16((float)v.r) | (f2bf16((float)v.i) << 16);
-    }
+  for (int j = 0; j < W; j++) {
+    const cx<double> t = tr[grp][j][li];
+    const cx<double> v = (j < sz) ? cx<double>{0.5 * (a[j].r + t.r), 0.5 * (a[j].i + t.i)} : cx<double>{0.0, 0.0};
+    bi[(size_t)(e0 + li) * W + j] = f2bf16((float)v.r) | (f2bf16((float)v.i) << 16);
   }
 }
 
@@ -2626,10 +2635,12 @@ __global__ void __launch_bounds__(256) k_nodeblk_apply(const int *__restrict__ t
   z[row] = acc;
 }
 
-hipError_t launch_nodeblk_setup(int n, const unsigned char *bm, const int *rowptr, const int *col,
-                                const cx<double> *val, double w, nbcoef *bi, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_nodeblk_setup, dim3((n + 127) / 128), dim3(128), 0, s, n, bm, rowptr, col, val, w, bi);
+hipError_t launch_nodeblk_setup(int nblk, const int *bstart, const unsigned char *bm, const int *rowptr,
+                                const int *col, const cx<double> *val, double w, nbcoef *bi, hipStream_t s) {
+  if (nblk <= 0) return hipSuccess;
+  constexpr int per = 256 / AFX_NB_W;
+  hipLaunchKernelGGL(k_nodeblk_setup, dim3((nblk + per - 1) / per), dim3(256), 0, s, nblk, bstart, bm, rowptr, col,
+                     val, w, bi);
   return hipGetLastError();
 }
 hipError_t launch_nodeblk_apply(int ntile, const int *tptr, const unsigned char *bm, const nbcoef *bi,

@@ -342,8 +342,7 @@ int ensure_precond(afx_system s, int kind_full, double sd, double sig, const std
   if (grad && (kind != AFX_PC_SSOR && kind != AFX_PC_SSOR_BJ && kind != AFX_PC_DIAG && kind != AFX_PC_NODEBLK))
     return fail(AFX_ERR_BADOPT, "AFX_PC_GRAD goes with SSOR, SSOR_BJ, DIAG or NODEBLK");
   if (grad && s->nn < 0) return fail(AFX_ERR_ARG, "AFX_PC_GRAD needs afx_set_nodes");
-  if (kind == AFX_PC_NODEBLK && (s->ntile <= 0 || s->nsys > 1))
-    return fail(AFX_ERR_ARG, "AFX_PC_NODEBLK needs afx_set_nodes and one system");
+  if (kind == AFX_PC_NODEBLK && s->ntile <= 0) return fail(AFX_ERR_ARG, "AFX_PC_NODEBLK needs afx_set_nodes");
   int rc = ensure_derived(s);
   if (rc) return rc;
   if (grad)
@@ -858,8 +857,8 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   // the streaming kernel (no sweep: DIAG or NONE edge preconditioner, one column, FP64) runs
   // at several blocks per CU; every other case runs k_cocg on the sweep partition
   const bool nblk = (pc == AFX_PC_NODEBLK);
-  if (nblk && (k > 8 || !fp64 || (stacked && s->nsys > 1)))
-    return fail(AFX_ERR_ARG, "AFX_PC_NODEBLK runs FP64, at most 8 columns of one system per launch");
+  if (nblk && (k > 8 || !fp64))
+    return fail(AFX_ERR_ARG, "AFX_PC_NODEBLK runs FP64, at most 8 columns per launch");
   const bool stream = fp64 && !(stacked && s->nsys > 1) && (pc == AFX_PC_DIAG || pc == AFX_PC_NONE || nblk) &&
                       (!multi || k <= 8) && (nblk || !getenv("AFX_NO_STREAM"));
   int per_cu = 1;

@@ -999,7 +999,18 @@ __device__ __forceinline__ void cocg_body(const CocgParams<TP> &p) {
       } else {
         for (long long row = vlo; row < vend; row += vstep) {
           const cx<double> r = p.R[row];
-          const cx<TP> z = (p.pc == AFX_PC_DIAG) ? cmul(down<TP>(r), p.DD[row]) : down<TP>(r);
+          cx<TP> z;
+          if (p.pc == AFX_PC_NODEBLK) {  // node blocks: the row's 8 coefficients, r on its block's rows
+            const int m = p.bm[row], loc = m & 15, sz = (m >> 4) + 1;
+            const uint4 *cf = (const uint4 *)p.bi + (size_t)row * (AFX_NB_W / 4);
+            const uint4 g0 = cf[0], g1 = cf[1];
+            cx<double> acc = {0.0, 0.0};
+#pragma unroll
+            for (int j = 0; j < AFX_NB_W; j++) acc = cfma(nb_coef(j < 4 ? g0 : g1, j & 3), p.R[row - loc + min(j, sz - 1)], acc);
+            z = down<TP>(acc);
+          } else {
+            z = (p.pc == AFX_PC_DIAG) ? cmul(down<TP>(r), p.DD[row]) : down<TP>(r);
+          }
           p.Z[row] = z;
           rho_loc = addv(rho_loc, cmul(r, up(z)));
         }

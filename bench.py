@@ -368,16 +368,13 @@ def main():
         variants[f"jacobian_{args.jacobian}_columns"] = jacobian_variant(s, case, w0, args.jacobian)
     if args.adjoint_batch:
         variants[f"adjoint_{3 * args.adjoint_batch}_columns"] = adjoint_batch_variant(s, case, opts, args.adjoint_batch)
-    # stacked systems (the mesh batch) have no node-block form: they run the diagonal edge
-    # preconditioner
-    bopts = dict(opts, precond="DIAG+GRAD")
     if not args.no_variants and time.time() - t_start < 300:
         # the batched forms (a failure here is recorded, it does not end the run):
         # the 5 constituents of this mesh; solvePARDISOinv's 48 adjoint columns (16 seafloor sites
         # x 3 electric components) on the 32x32x24-cell mesh; the 16-mesh uneven batch
         for name, fn in (("constituents_5", lambda: constituents_variant(s, b, omegas, w0, opts)),
                          ("adjoint_48_columns_32x32x24", lambda: adjoint32(pc, opts, w0)),
-                         ("mesh_batch_16", lambda: mesh_batch_variant(bopts, w0, grad_only=True)),
+                         ("mesh_batch_16", lambda: mesh_batch_variant(opts, w0, grad_only=True)),
                          ("gauss_newton_warm_starts_32x32x24", lambda: gn_variant())):
             try:
                 variants[name] = fn()

@@ -43,16 +43,17 @@ extern "C" {
                             the same node (the edge numbering of MKLINE puts a node's edges towards
                             higher-numbered nodes next to each other; runs of more than 8 rows are
                             cut), symmetrised and kept in complex FP32. Sweep-free like DIAG; needs
-                            afx_set_nodes; one right-hand side per launch (several are solved one
-                            at a time), one frequency, not for stacked systems */
-/* Flag OR-ed into AFX_PC_SSOR, AFX_PC_SSOR_BJ or AFX_PC_DIAG (no reference counterpart): adds
+                            afx_set_nodes; FP64; one coefficient set per distinct column frequency;
+                            single columns, k-column launches (k <= 8) and stacked systems */
+/* Flag OR-ed into AFX_PC_SSOR, AFX_PC_SSOR_BJ, AFX_PC_DIAG or AFX_PC_NODEBLK (no reference counterpart): adds
    the nodal gradient-space correction G diag(G^T A G)^-1 G^T to the edge preconditioner, G the
    discrete gradient of the interior nodes (afx_set_nodes). The curl-curl part annihilates
    gradients (K G = 0), so A's gradient components are set by the tiny i*omega*mu0*sigma mass
    alone, and in the air they are what makes edge-only SSOR-COCG need tens of thousands of
    iterations; the correction solves for them on the nodes (diag(G^T A G) = i*omega*diag(G^T M G)).
-   The preconditioner stays complex symmetric, so COCG is unchanged. Single right-hand-side
-   launches only (several columns are solved one at a time; stacked systems are refused). */
+   The preconditioner stays complex symmetric, so COCG is unchanged. Single columns, k-column
+   launches and stacked systems (each system's interior nodes contiguous, afx_set_nodes on the
+   stack); the stacked-copies block width is not used with it. */
 #define AFX_PC_GRAD 0x100
 
 /* arithmetic of the Krylov loop */

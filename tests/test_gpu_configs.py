@@ -147,10 +147,12 @@ def test_cfg3_constituents_and_adjoint_columns(cfg3):
     print(f"240 adjoint solves: worst true residual {worst_tr:.2e}, worst |u^T b - P x| / |P x| {worst_id:.1e}")
 
 
-def test_cfg3_sixteen_meshes_one_launch():
+@pytest.mark.parametrize("pc", ["DIAG+GRAD", "NODEBLK+GRAD"])
+def test_cfg3_sixteen_meshes_one_launch(pc):
     """The 16 meshes of 8x8x6 to 40x40x30 cells (seeds 1004 + i, each its own crust) stacked and
-    solved in one per-system launch with DIAG+GRAD: every part at a 1e-8 true residual, its
-    iterations within a few of its own single solve."""
+    solved in one per-system launch with DIAG+GRAD and with NODEBLK+GRAD (the node blocks of the
+    stack are each part's own): every part at a 1e-8 true residual, its iterations within a few of
+    its own single solve."""
     import torch
     from actfemtide_amd import afx, mesh
     from actfemtide_amd.meshbatch import MeshBatch
@@ -165,16 +167,16 @@ def test_cfg3_sixteen_meshes_one_launch():
         bcs.append(bc)
     own = []
     for s, b in zip(parts, bs):
-        s.precond_setup("DIAG+GRAD")
-        x, st = s.solve(b, precond="DIAG+GRAD", tol=1e-8, maxit=40000)
+        s.precond_setup(pc)
+        x, st = s.solve(b, precond=pc, tol=1e-8, maxit=40000)
         own.append(st[0].iter)
     mb = MeshBatch(parts, per_system=True)
     mb.set_nodes([c.node for c in cases], [c.line for c in cases], bcs)
-    mb.system.precond_setup("DIAG+GRAD")
+    mb.system.precond_setup(pc)
     print(f"stack: {mb.m} systems, {mb.n} edges", flush=True)
-    xs, st, tr = mb.solve(bs, tol=1e-8, precond="DIAG+GRAD", maxit=40000)
+    xs, st, tr = mb.solve(bs, tol=1e-8, precond=pc, maxit=40000)
     its = [x.iter for x in st]
-    print(f"one launch: iterations {its}; own solves {own}; worst part residual {tr.max():.2e}", flush=True)
+    print(f"{pc} one launch: iterations {its}; own solves {own}; worst part residual {tr.max():.2e}", flush=True)
     assert np.all(tr <= 1.05e-8) and all(x.error == 0 for x in st)
     for a, bb in zip(its, own):
         assert abs(a - bb) <= max(3, 0.05 * bb)

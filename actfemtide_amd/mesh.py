@@ -245,6 +245,62 @@ def tide_case(nx: int, ny: int, nz: int, seed: int, h: float = 2.5, *, spheres: 
                     log10_sigma_cell=field)
 
 
+def refine_cells(case: TideCase, cells) -> TideCase:
+    """The case with each of the given crust cells split into 12 tets: a new node at the cell's
+    centre joined to the 12 triangles that cover the cell's faces (the faces its 6 tets do not
+    share with each other), so the neighbours' face triangulation is kept and the mesh stays
+    conforming. The new nodes take the highest ids and the new tets go last, keeping the crust
+    tets last; each new tet keeps its cell's conductivity. Refining the 8 cells around a grid
+    node gives that node 22 edges, and the corner nodes' runs of edges towards higher-numbered
+    nodes grow past 8, as on the reference's unstructured meshes."""
+    cells = np.unique(np.asarray(cells, np.int64))
+    crust = case.group >= GROUP_CRUST
+    cot = case.cell_of_tet.astype(np.int64)
+    sel = np.isin(cot, cells)
+    if not np.all(crust[sel]) or np.count_nonzero(sel) != 6 * cells.size:
+        raise ValueError("refine_cells: crust cells of a mesh with 6 tets per cell")
+    sig_tet = case.sigma_tet()
+    xyz = [case.xyz]
+    new_n4, new_sig, new_cell = [], [], []
+    nid = case.node
+    for c in cells:
+        tets = case.n4[cot == c] - 1
+        faces = {}
+        for t in tets:
+            for drop in range(4):
+                f = tuple(sorted(np.delete(t, drop)))
+                faces[f] = faces.get(f, 0) + 1
+        outer = [f for f, k in faces.items() if k == 1]
+        corners = np.unique(tets)
+        if len(outer) != 12 or corners.size != 8:
+            raise ValueError("refine_cells: cell is not a 6-tet cube")
+        xyz.append(case.xyz[corners].mean(axis=0, keepdims=True))
+        s = sig_tet[np.nonzero(cot == c)[0][0]]
+        for f in outer:
+            q = [f[0], f[1], f[2], nid]
+            x = np.vstack([case.xyz[list(f)], xyz[-1]])
+            v = np.dot(np.cross(x[3] - x[2], x[0] - x[2]), x[1] - x[2])
+            if v < 0:  # the reference's signed volume must be positive
+                q[0], q[1] = q[1], q[0]
+            new_n4.append(q)
+            new_sig.append(s)
+            new_cell.append(c)
+        nid += 1
+    keep = ~sel
+    n4 = np.vstack([case.n4[keep], np.asarray(new_n4, np.int64) + 1]).astype(np.int32)
+    group = np.concatenate([case.group[keep], np.full(len(new_n4), GROUP_CRUST, np.int32)]).astype(np.int32)
+    sigma_crust = np.concatenate([sig_tet[keep & crust], np.asarray(new_sig)])
+    cell_of_tet = np.concatenate([case.cell_of_tet[keep], np.asarray(new_cell)]).astype(np.int32)
+    xyz = np.vstack(xyz)
+    add = xyz.shape[0] - case.node
+    vxyz = np.vstack([case.vxyz, np.zeros((add, 3), np.complex128)])
+    fxyz = np.vstack([case.fxyz, np.zeros((add, 3), np.float64)])
+    line, n6line = _lines.mkline_mkn6(xyz.shape[0], n4)
+    return dataclasses.replace(case, xyz=xyz, n4=n4, group=group, line=line, n6line=n6line,
+                               sigma_crust=sigma_crust, rho_crust=1.0 / sigma_crust, vxyz=vxyz, fxyz=fxyz,
+                               cell_of_tet=cell_of_tet)
+
+
 def gaussian_random_field(nz: int, ny: int, nx: int, seed: int, mean: float = -2.0,
                           std: float = 0.7, corr_cells: float = 4.0) -> np.ndarray:
     """Stationary Gaussian random field on the cell grid (spectral synthesis with a

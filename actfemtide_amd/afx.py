@@ -8,6 +8,7 @@ The reference entry points this module mirrors (same argument meaning and error 
   PREPOBSCOEFF         src/solver_mpi/n_ebfem_bxyz_mpi.f90:653-726 -> System.prepobscoeff()
   CALOBSEBCOMP/RESP    src/solver_mpi/n_ebfem_bxyz_mpi.f90:397-423,570-609 -> calobsebcomp()
   solvePARDISOinv      src/src_inv/solvePARDISOinv.f90:5-244     -> solve_pardiso_inv()
+  solvePARDISOjointinv src/src_inv_joint/solvePARDISOjointinv.f90:5-333 -> solve_pardiso_jointinv_host()
   genjacobian1         src/src_inv_ap/m_jacobian_ap.f90:174-386  -> System.jacobian_device(),
                                                                     jacobian_ampphase_device()
   getnewmodel          src/src_inv_ap/m_jacobian_ap.f90:623-787  -> gn_update()
@@ -104,6 +105,9 @@ SIGNATURES = {
     "afx_solve_pardiso": ([_I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, _D], _I),
     "afx_solve_pardiso_inv": ([_I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, _I, _VP, _VP, _VP,
                                _VP, _VP, _I, _VP, _VP, _VP, _VP, _D], _I),
+    "afx_solve_pardiso_jointinv": ([_I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, _VP,
+                                    _VP, _I, _VP, _VP, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP,
+                                    _I, _VP, _VP, _VP, _VP, _D], _I),
     "afx_solve_pardiso_release": ([], _I),
     "afx_prepobscoeff": ([_VP, _I, _VP, _VP, _VP, _VP, _VP], _I),
     "afx_obs_apply_device": ([_VP, _I, _VP, _VP, _I, _VP, _VP], _I),
@@ -542,36 +546,56 @@ def solve_pardiso(D, AU, AL, INU, IAU, INL, IAL, rhs, node=0, line=None, line_bc
     return (x[:, 0] if np.ndim(rhs) == 1 else x), rc
 
 
+def _pack_ptr(PTR, ncomp, flags=None):
+    """Observation matrices as the C ABI takes them: a list of up to ncomp real CRS matrices
+    (stack 0-based, item 1-based, val) or None, the flagged ones with the same row count.
+    Returns (flags, nobs, stacks (ncomp, nobs+1), base, item, val); unflagged components are
+    left out."""
+    Ps = list(PTR) + [None] * (ncomp - len(PTR))
+    fl = np.array([0 if (P is None or (flags is not None and not flags[c])) else 1 for c, P in enumerate(Ps)],
+                  np.int32)
+    Ps = [P if fl[c] else None for c, P in enumerate(Ps)]  # the row count is the flagged components'
+    nobs = max((len(P[0]) - 1 for P in Ps if P is not None), default=0)
+    stacks = np.zeros((ncomp, nobs + 1), np.int32)
+    base = np.zeros(ncomp, np.int32)
+    items, vals, off = [], [], 0
+    for c, P in enumerate(Ps):
+        if P is None:
+            continue
+        stacks[c] = _i32(P[0])
+        base[c] = off
+        items.append(_i32(P[1]))
+        vals.append(_f64(P[2]))
+        off += len(P[1])
+    item = _i32(np.concatenate(items)) if items else np.zeros(1, np.int32)
+    val = _f64(np.concatenate(vals)) if vals else np.zeros(1)
+    return fl, nobs, stacks, base, item, val
+
+
+def _unpack_ut(flags, nobs, n, cap, ust, ntot, uit, uva):
+    import scipy.sparse as sp
+    ut = []
+    for c in range(len(flags)):
+        if not flags[c]:
+            ut.append(0)
+            continue
+        k = int(ntot[c])
+        ut.append(sp.csr_matrix((uva[c * cap:c * cap + k], uit[c * cap:c * cap + k] - 1, ust[c]), shape=(nobs, n)))
+    return ut
+
+
 def solve_pardiso_inv_host(D, AU, AL, INU, IAU, INL, IAL, rhs, PTR, iflag_comp=None, node=0, line=None,
                            line_bc=None, tol=1e-12, ut_cap=None):
     """solvePARDISOinv (src/src_inv/solvePARDISOinv.f90:5-23) through afx_solve_pardiso_inv on host
     arrays: PTR a list of 5 observation matrices (stack 0-based, item 1-based, val) or None, all
     with the same row count. Returns (xout, ut, rc) with ut[c] a scipy CSR [nobs x n] (0 when not
     flagged)."""
-    import scipy.sparse as sp
     D, AU, AL = _c128(D), _c128(AU), _c128(AL)
     INU, IAU, INL, IAL = _i32(INU), _i32(IAU), _i32(INL), _i32(IAL)
     n = len(D)
     r = np.asfortranarray(np.asarray(rhs, np.complex128).reshape(n, -1, order="F"))
     x = np.zeros_like(r, order="F")
-    flags = np.array([0 if (P is None or (iflag_comp is not None and not iflag_comp[c])) else 1
-                      for c, P in enumerate(list(PTR) + [None] * (5 - len(PTR)))], np.int32)
-    Ps = list(PTR) + [None] * (5 - len(PTR))
-    nobs = max((len(P[0]) - 1 for P in Ps if P is not None), default=0)
-    stacks = np.zeros((5, nobs + 1), np.int32)
-    base = np.zeros(5, np.int32)
-    items, vals, off = [], [], 0
-    for c, P in enumerate(Ps):
-        if P is None:
-            continue
-        st, it, va = _i32(P[0]), _i32(P[1]), _f64(P[2])
-        stacks[c] = st
-        base[c] = off
-        items.append(it)
-        vals.append(va)
-        off += len(it)
-    item = _i32(np.concatenate(items)) if items else np.zeros(1, np.int32)
-    val = _f64(np.concatenate(vals)) if vals else np.zeros(1)
+    flags, nobs, stacks, base, item, val = _pack_ptr(PTR, 5, iflag_comp)
     ln = None if line is None else _i32(line).reshape(-1, 2)
     bc = None if line_bc is None else np.ascontiguousarray(line_bc, dtype=np.uint8)
     cap = int(ut_cap) if ut_cap is not None else nobs * n
@@ -585,14 +609,45 @@ def solve_pardiso_inv_host(D, AU, AL, INU, IAU, INL, IAL, rhs, PTR, iflag_comp=N
                                      _p(uit), _p(uva), float(tol))
     if rc not in (AFX_OK, AFX_ERR_MAXIT):
         _err(rc)
-    ut = []
-    for c in range(5):
-        if not flags[c]:
-            ut.append(0)
-            continue
-        k = int(ntot[c])
-        ut.append(sp.csr_matrix((uva[c * cap:c * cap + k], uit[c * cap:c * cap + k] - 1, ust[c]), shape=(nobs, n)))
-    return x, ut, rc
+    return x, _unpack_ut(flags, nobs, n, cap, ust, ntot, uit, uva), rc
+
+
+def solve_pardiso_jointinv_host(D, AU, AL, INU, IAU, INL, IAL, rhs, rhs_mt, PTR, PTR_mt, iflag_comp=None,
+                                act=True, mt=True, node=0, line=None, line_bc=None, tol=1e-12):
+    """solvePARDISOjointinv (src/src_inv_joint/solvePARDISOjointinv.f90:5-26) through
+    afx_solve_pardiso_jointinv: rhs (n, nsr) active-source columns, rhs_mt (n, 2) MT columns,
+    PTR up to 5 and PTR_mt 4 observation matrices as in solve_pardiso_inv_host. Returns
+    (xout, xout_mt, ut, ut_mt, rc); xout_mt and ut_mt stay zero / 0 when mt is off, xout and ut
+    when act is off, and everything when no flagged PTR component has rows (the reference's early
+    exit)."""
+    D, AU, AL = _c128(D), _c128(AU), _c128(AL)
+    INU, IAU, INL, IAL = _i32(INU), _i32(IAU), _i32(INL), _i32(IAL)
+    n = len(D)
+    r = np.asfortranarray(np.asarray(rhs, np.complex128).reshape(n, -1, order="F"))
+    rm = np.asfortranarray(np.asarray(rhs_mt, np.complex128).reshape(n, 2, order="F"))
+    x = np.zeros_like(r, order="F")
+    xm = np.zeros_like(rm, order="F")
+    flags, nobs, stacks, base, item, val = _pack_ptr(PTR, 5, iflag_comp)
+    fl_mt, nobs_mt, st_mt, base_mt, item_mt, val_mt = _pack_ptr(PTR_mt, 4)
+    ln = None if line is None else _i32(line).reshape(-1, 2)
+    bc = None if line_bc is None else np.ascontiguousarray(line_bc, dtype=np.uint8)
+    cap, cap_mt = nobs * n, nobs_mt * n
+    ust, ntot = np.zeros((5, nobs + 1), np.int32), np.zeros(5, np.int32)
+    uit, uva = np.zeros(5 * max(cap, 1), np.int32), np.zeros(5 * max(cap, 1), np.complex128)
+    ust_mt, ntot_mt = np.zeros((4, nobs_mt + 1), np.int32), np.zeros(4, np.int32)
+    uit_mt, uva_mt = np.zeros(4 * max(cap_mt, 1), np.int32), np.zeros(4 * max(cap_mt, 1), np.complex128)
+    rc = lib().afx_solve_pardiso_jointinv(
+        1 if act else 0, 1 if mt else 0, n, r.shape[1], _p(INU), _p(IAU), _p(INL), _p(IAL), _p(D), _p(AU), _p(AL),
+        int(node), _p(ln), _p(bc), C.c_void_p(r.ctypes.data), C.c_void_p(rm.ctypes.data), C.c_void_p(x.ctypes.data),
+        C.c_void_p(xm.ctypes.data), nobs, _p(flags), _p(stacks), _p(base), _p(item), _p(val), nobs_mt, _p(st_mt),
+        _p(base_mt), _p(item_mt), _p(val_mt), cap, _p(ust), _p(ntot), _p(uit), _p(uva), cap_mt, _p(ust_mt),
+        _p(ntot_mt), _p(uit_mt), _p(uva_mt), float(tol))
+    if rc not in (AFX_OK, AFX_ERR_MAXIT):
+        _err(rc)
+    ut = _unpack_ut(flags if act else np.zeros(5, np.int32), nobs, n, cap, ust, ntot, uit, uva)
+    ut_mt = _unpack_ut(fl_mt if mt and nobs else np.zeros(4, np.int32), nobs_mt, n, cap_mt, ust_mt, ntot_mt, uit_mt,
+                       uva_mt)
+    return x, xm, ut, ut_mt, rc
 
 
 def solve_pardiso_release():

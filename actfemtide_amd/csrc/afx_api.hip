@@ -2651,6 +2651,16 @@ int afx_cocg13_single(int NP, int NPL, int NPU, double *D, double *AL, const int
 static std::mutex g_pds_mu;
 static afx_system g_pds_sys = nullptr;
 
+// The solvePARDISO-shaped entries promise the recomputed FP64 true residual tol: a solve whose
+// recurrence converged but whose true residual stayed above tol after the restarts (a right-hand
+// side beyond what FP64 can resolve) reports AFX_ERR_MAXIT, as one that ran out of iterations.
+static int pardiso_status(int rc, const afx_stats *st, int k, double tol) {
+  if (rc != AFX_OK) return rc;
+  for (int c = 0; c < k; c++)
+    if (!(st[c].true_resid <= tol)) return AFX_ERR_MAXIT;
+  return AFX_OK;
+}
+
 // Pattern, values, node incidence and preconditioner of the kept system; *pc receives the
 // preconditioner: node blocks with the nodal correction when the edges' end nodes are given,
 // else SSOR.
@@ -2706,56 +2716,34 @@ int afx_solve_pardiso(int doftot, int nsr, const int *INU, const int *IAU, const
   o.max_outer = 4;
   std::fill(xout, xout + 2 * (size_t)doftot * nsr, 0.0);
   std::vector<afx_stats> st(nsr);
-  return afx_solve(g_pds_sys, nsr, rhs, doftot, xout, doftot, &o, st.data());
+  return pardiso_status(afx_solve(g_pds_sys, nsr, rhs, doftot, xout, doftot, &o, st.data()), st.data(), nsr, tol);
 }
 
-int afx_solve_pardiso_inv(int doftot, int nsr, const int *INU, const int *IAU, const int *INL, const int *IAL,
-                          const double *D, const double *AU, const double *AL, int node, const int *line,
-                          const unsigned char *line_bc, const double *rhs, double *xout, int nobs,
-                          const int *iflag_comp, const int *ptr_stack, const int *ptr_base, const int *ptr_item,
-                          const double *ptr_val, int ut_cap, int *ut_stack, int *ut_ntot, int *ut_item, double *ut_val,
-                          double tol) {
-  if (nsr < 0 || nobs < 0 || !iflag_comp || !ut_ntot || !(tol > 0.0) || ut_cap < 0)
+// Adjoint solutions of the observation rows (solvePARDISOinv.f90:176-196): for every component
+// c < ncomp with flags[c] != 0 (all when flags is null), u_o = A^-1 P_o^T for the nobs rows of
+// PTR(c) in blocks of at most 64 columns, kept where |u| > 1e-10 as the CRS of u^T
+// (convcomp_full2crs and its transposition; m_matrix.f90:405-451). *ret becomes AFX_ERR_MAXIT when
+// some column missed the tolerance; a component needing more than ut_cap entries is reported
+// after all are counted.
+static int adjoint_components(afx_system s, const afx_opts &o, int doftot, int ncomp, int nobs, const int *flags,
+                              const int *ptr_stack, const int *ptr_base, const int *ptr_item, const double *ptr_val,
+                              int ut_cap, int *ut_stack, int *ut_ntot, int *ut_item, double *ut_val, int *ret) {
+  for (int c = 0; c < ncomp; c++) ut_ntot[c] = 0;
+  if (nobs == 0) return AFX_OK;
+  if (!ptr_stack || !ptr_base || !ptr_item || !ptr_val || !ut_stack || (ut_cap > 0 && (!ut_item || !ut_val)))
     return fail(AFX_ERR_ARG, "bad arguments");
-  std::lock_guard<std::mutex> lk(g_pds_mu);
-  int pc = 0;
-  int rc = pardiso_system(doftot, INU, IAU, INL, IAL, D, AU, AL, node, line, line_bc, &pc);
-  if (rc) return rc;
-  afx_system s = g_pds_sys;
-  afx_opts o;
-  afx_default_opts(&o);
-  o.precond = pc;
-  o.tol = tol;
-  o.maxit = 400000;
-  o.check_true = 1;
-  o.max_outer = 4;
-  int ret = AFX_OK;
-  // the forward solve (solvePARDISOinv.f90:157-164)
-  if (nsr > 0) {
-    if (!rhs || !xout) return fail(AFX_ERR_ARG, "bad arguments");
-    std::fill(xout, xout + 2 * (size_t)doftot * nsr, 0.0);
-    std::vector<afx_stats> st(nsr);
-    rc = afx_solve(s, nsr, rhs, doftot, xout, doftot, &o, st.data());
-    if (rc != AFX_OK && rc != AFX_ERR_MAXIT) return rc;
-    ret = rc;
-  }
-  for (int c = 0; c < 5; c++) ut_ntot[c] = 0;
-  if (nobs == 0) return ret;
-  if (!ptr_stack || !ptr_base || !ptr_item || !ptr_val || !ut_stack) return fail(AFX_ERR_ARG, "bad arguments");
-  // per flagged component: u_o = A^-1 P_o^T for the nobs rows of PTR(c) (:176-191) in blocks of
-  // at most 64 columns, kept where |u| > 1e-10 as the CRS of u^T (convcomp_full2crs and its
-  // transposition, :194-196; m_matrix.f90:405-451)
   const double threshold = 1.0e-10;
   const int blk = std::min(nobs, 64);
   cx<double> *dB = nullptr, *dX = nullptr;
+  int rc;
   if ((rc = dalloc(&dB, (size_t)doftot * blk)) || (rc = dalloc(&dX, (size_t)doftot * blk))) {
     dfree(dB);
     return rc;
   }
   std::vector<cx<double>> U((size_t)doftot * blk);
   bool short_cap = false;
-  for (int c = 0; c < 5 && rc == AFX_OK; c++) {
-    if (!iflag_comp[c]) continue;
+  for (int c = 0; c < ncomp && rc == AFX_OK; c++) {
+    if (flags && !flags[c]) continue;
     const int *stk = ptr_stack + (size_t)c * (nobs + 1);
     int *ust = ut_stack + (size_t)c * (nobs + 1);
     int *uit = ut_item + (size_t)c * ut_cap;
@@ -2775,12 +2763,12 @@ int afx_solve_pardiso_inv(int doftot, int nsr, const int *INU, const int *IAU, c
         break;
       }
       std::vector<afx_stats> st(k);
-      const int r = afx_solve_device(s, k, dB, dX, nullptr, &o, st.data());
+      const int r = pardiso_status(afx_solve_device(s, k, dB, dX, nullptr, &o, st.data()), st.data(), k, o.tol);
       if (r != AFX_OK && r != AFX_ERR_MAXIT) {
         rc = r;
         break;
       }
-      if (r == AFX_ERR_MAXIT) ret = AFX_ERR_MAXIT;
+      if (r == AFX_ERR_MAXIT) *ret = AFX_ERR_MAXIT;
       if (hipMemcpy(U.data(), dX, (size_t)doftot * k * sizeof(cx<double>), hipMemcpyDeviceToHost) != hipSuccess) {
         rc = fail(AFX_ERR_HIP, "hipMemcpy");
         break;
@@ -2808,6 +2796,93 @@ int afx_solve_pardiso_inv(int doftot, int nsr, const int *INU, const int *IAU, c
   dfree(dX);
   if (rc) return rc;
   if (short_cap) return fail(AFX_ERR_ARG, "ut_cap too small: ut_ntot holds the entries each component needs");
+  return AFX_OK;
+}
+
+// options of the solvePARDISO-shaped entries: FP64 COCG to the recomputed true residual tol
+static afx_opts pardiso_opts(int pc, double tol) {
+  afx_opts o;
+  afx_default_opts(&o);
+  o.precond = pc;
+  o.tol = tol;
+  o.maxit = 400000;
+  o.check_true = 1;
+  o.max_outer = 4;
+  return o;
+}
+
+int afx_solve_pardiso_inv(int doftot, int nsr, const int *INU, const int *IAU, const int *INL, const int *IAL,
+                          const double *D, const double *AU, const double *AL, int node, const int *line,
+                          const unsigned char *line_bc, const double *rhs, double *xout, int nobs,
+                          const int *iflag_comp, const int *ptr_stack, const int *ptr_base, const int *ptr_item,
+                          const double *ptr_val, int ut_cap, int *ut_stack, int *ut_ntot, int *ut_item, double *ut_val,
+                          double tol) {
+  if (nsr < 0 || nobs < 0 || !iflag_comp || !ut_ntot || !(tol > 0.0) || ut_cap < 0)
+    return fail(AFX_ERR_ARG, "bad arguments");
+  std::lock_guard<std::mutex> lk(g_pds_mu);
+  int pc = 0;
+  int rc = pardiso_system(doftot, INU, IAU, INL, IAL, D, AU, AL, node, line, line_bc, &pc);
+  if (rc) return rc;
+  afx_system s = g_pds_sys;
+  const afx_opts o = pardiso_opts(pc, tol);
+  int ret = AFX_OK;
+  // the forward solve (solvePARDISOinv.f90:157-164)
+  if (nsr > 0) {
+    if (!rhs || !xout) return fail(AFX_ERR_ARG, "bad arguments");
+    std::fill(xout, xout + 2 * (size_t)doftot * nsr, 0.0);
+    std::vector<afx_stats> st(nsr);
+    rc = pardiso_status(afx_solve(s, nsr, rhs, doftot, xout, doftot, &o, st.data()), st.data(), nsr, tol);
+    if (rc != AFX_OK && rc != AFX_ERR_MAXIT) return rc;
+    ret = rc;
+  }
+  rc = adjoint_components(s, o, doftot, 5, nobs, iflag_comp, ptr_stack, ptr_base, ptr_item, ptr_val, ut_cap, ut_stack,
+                          ut_ntot, ut_item, ut_val, &ret);
+  return rc ? rc : ret;
+}
+
+int afx_solve_pardiso_jointinv(int act, int mt, int doftot, int nsr, const int *INU, const int *IAU, const int *INL,
+                               const int *IAL, const double *D, const double *AU, const double *AL, int node,
+                               const int *line, const unsigned char *line_bc, const double *rhs, const double *rhs_mt,
+                               double *xout, double *xout_mt, int nobs, const int *iflag_comp, const int *ptr_stack,
+                               const int *ptr_base, const int *ptr_item, const double *ptr_val, int nobs_mt,
+                               const int *ptr_mt_stack, const int *ptr_mt_base, const int *ptr_mt_item,
+                               const double *ptr_mt_val, int ut_cap, int *ut_stack, int *ut_ntot, int *ut_item,
+                               double *ut_val, int ut_mt_cap, int *ut_mt_stack, int *ut_mt_ntot, int *ut_mt_item,
+                               double *ut_mt_val, double tol) {
+  if (nsr < 0 || nobs < 0 || nobs_mt < 0 || !iflag_comp || !ut_ntot || !ut_mt_ntot || !(tol > 0.0) || ut_cap < 0 ||
+      ut_mt_cap < 0 || (act && nsr > 0 && (!rhs || !xout)) || (mt && (!rhs_mt || !xout_mt)))
+    return fail(AFX_ERR_ARG, "bad arguments");
+  for (int c = 0; c < 5; c++) ut_ntot[c] = 0;
+  for (int c = 0; c < 4; c++) ut_mt_ntot[c] = 0;
+  // no flagged component with rows: the reference leaves before the factorisation (:56-64)
+  if (nobs == 0) return AFX_OK;
+  std::lock_guard<std::mutex> lk(g_pds_mu);
+  int pc = 0;
+  int rc = pardiso_system(doftot, INU, IAU, INL, IAL, D, AU, AL, node, line, line_bc, &pc);
+  if (rc) return rc;
+  afx_system s = g_pds_sys;
+  const afx_opts o = pardiso_opts(pc, tol);
+  int ret = AFX_OK;
+  // the active-source and the two MT forward solves (:173-197) share one launch: nsr + 2 columns
+  const int na = act ? nsr : 0, nm = mt ? 2 : 0, nc = na + nm;
+  if (nc > 0) {
+    std::vector<double> B(2 * (size_t)doftot * nc), X(2 * (size_t)doftot * nc, 0.0);
+    if (na) std::copy(rhs, rhs + 2 * (size_t)doftot * na, B.begin());
+    if (nm) std::copy(rhs_mt, rhs_mt + 2 * (size_t)doftot * nm, B.begin() + 2 * (size_t)doftot * na);
+    std::vector<afx_stats> st(nc);
+    rc = pardiso_status(afx_solve(s, nc, B.data(), doftot, X.data(), doftot, &o, st.data()), st.data(), nc, tol);
+    if (rc != AFX_OK && rc != AFX_ERR_MAXIT) return rc;
+    ret = rc;
+    if (na) std::copy(X.begin(), X.begin() + 2 * (size_t)doftot * na, xout);
+    if (nm) std::copy(X.begin() + 2 * (size_t)doftot * na, X.end(), xout_mt);
+  }
+  // the active-source components (:212-236), then the 4 MT components bx, by, ex, ey (:246-290)
+  if (act && (rc = adjoint_components(s, o, doftot, 5, nobs, iflag_comp, ptr_stack, ptr_base, ptr_item, ptr_val,
+                                      ut_cap, ut_stack, ut_ntot, ut_item, ut_val, &ret)))
+    return rc;
+  if (mt && (rc = adjoint_components(s, o, doftot, 4, nobs_mt, nullptr, ptr_mt_stack, ptr_mt_base, ptr_mt_item,
+                                     ptr_mt_val, ut_mt_cap, ut_mt_stack, ut_mt_ntot, ut_mt_item, ut_mt_val, &ret)))
+    return rc;
   return ret;
 }
 

@@ -42,7 +42,8 @@ extern "C" {
                             diagonal block of A over a run of consecutive rows whose edges start at
                             the same node (the edge numbering of MKLINE puts a node's edges towards
                             higher-numbered nodes next to each other; runs of more than 8 rows are
-                            cut), symmetrised and kept in complex FP32. Sweep-free like DIAG; needs
+                            cut), symmetrised and kept as bfloat16 complex (8 per row, 32 B), applied
+                            in FP64. Sweep-free like DIAG; needs
                             afx_set_nodes; FP64; one coefficient set per distinct column frequency;
                             single columns, k-column launches (k <= 8) and stacked systems */
 /* Flag OR-ed into AFX_PC_SSOR, AFX_PC_SSOR_BJ, AFX_PC_DIAG or AFX_PC_NODEBLK (no reference counterpart): adds
@@ -265,8 +266,9 @@ int afx_cocg13_single_release(void);
    true residual tol (||b - A x|| / ||b|| per column, recomputed; PARDISO's own accuracy is about
    1e-12); with the edges' end nodes line(2, nline) (READLINE's g_line%line, 1-based) and node count,
    and the Dirichlet mask line_bc (GENBCCSEM's line_bc as one byte per edge), the preconditioner is
-   AFX_PC_NODEBLK + AFX_PC_GRAD, otherwise SSOR (line = NULL). Returns 0, -100 (tol not reached) or an
-   AFX_ERR_* where the reference stops. The system of the last call is kept while the pattern is
+   AFX_PC_NODEBLK + AFX_PC_GRAD, otherwise SSOR (line = NULL). Returns 0, -100 (tol not reached: the
+   iterations ran out, or the recomputed true residual is still above tol after 4 restarts, as for a
+   right-hand side whose solution FP64 cannot resolve) or an AFX_ERR_* where the reference stops. The system of the last call is kept while the pattern is
    the same (afx_solve_pardiso_release frees it); calls are serialised. ip is not needed. */
 int afx_solve_pardiso(int doftot, int nsr, const int *INU, const int *IAU, const int *INL, const int *IAL,
                       const double *D, const double *AU, const double *AL, int node, const int *line,
@@ -289,6 +291,27 @@ int afx_solve_pardiso_inv(int doftot, int nsr, const int *INU, const int *IAU, c
                           const int *iflag_comp, const int *ptr_stack, const int *ptr_base, const int *ptr_item,
                           const double *ptr_val, int ut_cap, int *ut_stack, int *ut_ntot, int *ut_item, double *ut_val,
                           double tol);
+
+/* solvePARDISOjointinv(ACT, MT, nline, nsr, A, rhs, rhs_mt, xout, xout_mt, PTR, PTR_mt, ut, ut_mt,
+   iflag_comp, ip, np) (src/src_inv_joint/solvePARDISOjointinv.f90:5-26), the joint tidal + MT
+   caller's twin of afx_solve_pardiso_inv on one kept system (forward_joint.f90:93):
+   - act != 0: the nsr active-source columns rhs -> xout and, for every flagged component of PTR(5)
+     (nobs rows each), ut(c), all laid out as in afx_solve_pardiso_inv;
+   - mt != 0: the two MT columns rhs_mt(doftot, 2) -> xout_mt(doftot, 2) and the adjoint solutions of
+     all 4 MT components of PTR_mt (bx, by, ex, ey; nobs_mt rows each), returned as ut_mt(c) with
+     ptr_mt_* / ut_mt_* in the layout of ptr_* / ut_* (stride nobs_mt + 1, capacity ut_mt_cap).
+   The forward columns of both run in one launch. As the reference (:56-64), nothing is solved when
+   nobs = 0 (no flagged active component with rows), whatever mt says. The PTR_mt_*.dat files the
+   reference writes when ip = 0 (:248-261) are not written. Returns as afx_solve_pardiso_inv. */
+int afx_solve_pardiso_jointinv(int act, int mt, int doftot, int nsr, const int *INU, const int *IAU, const int *INL,
+                               const int *IAL, const double *D, const double *AU, const double *AL, int node,
+                               const int *line, const unsigned char *line_bc, const double *rhs, const double *rhs_mt,
+                               double *xout, double *xout_mt, int nobs, const int *iflag_comp, const int *ptr_stack,
+                               const int *ptr_base, const int *ptr_item, const double *ptr_val, int nobs_mt,
+                               const int *ptr_mt_stack, const int *ptr_mt_base, const int *ptr_mt_item,
+                               const double *ptr_mt_val, int ut_cap, int *ut_stack, int *ut_ntot, int *ut_item,
+                               double *ut_val, int ut_mt_cap, int *ut_mt_stack, int *ut_mt_ntot, int *ut_mt_item,
+                               double *ut_mt_val, double tol);
 
 /* Frees the system afx_solve_pardiso / afx_solve_pardiso_inv keep between calls (no reference
    counterpart: PARDISO frees its factors at the end of each call, solvePARDISO.f90:165-166). */

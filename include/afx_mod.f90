@@ -8,6 +8,7 @@
 !   call afx_solve_cg_f(...)        replaces   call solveCG(...)         (forward_bxyz_mpi.f90:70-94)
 !   call afx_solvepardiso_f(...)    replaces   call solvePARDISO(...)    (solvePARDISO.f90:6-16)
 !   afx_solve_pardiso_inv(...)      replaces   call solvePARDISOinv(...) (solvePARDISOinv.f90:5-23)
+!   afx_solve_pardiso_jointinv(...) replaces   call solvePARDISOjointinv(...) (solvePARDISOjointinv.f90:5-26)
 !   afx_solve(...) on a handle      keeps the system on the device across calls
 !   afx_ixyh(...)                   computes   IXYHOUT's seafloor currents (n_ebfem_bxyz_mpi.f90:192-268)
 module afx
@@ -164,6 +165,27 @@ module afx
       integer(c_signed_char), intent(in) :: line_bc(*)
       complex(c_double_complex), intent(out) :: xout(*), ut_val(*)
       integer(c_int), intent(out) :: ut_stack(*), ut_ntot(*), ut_item(*)
+      real(c_double), value :: tol
+    end function
+
+    ! solvePARDISOjointinv(ACT, MT, nline, nsr, A, rhs, rhs_mt, xout, xout_mt, PTR, PTR_mt, ut, ut_mt,
+    ! iflag_comp, ip, np): act / mt as 0 or 1, PTR_mt(4) and ut_mt(4) in the layout of PTR and ut
+    integer(c_int) function afx_solve_pardiso_jointinv(act, mt, doftot, nsr, INU, IAU, INL, IAL, D, AU, AL, node, &
+                                                       line, line_bc, rhs, rhs_mt, xout, xout_mt, nobs, iflag_comp, &
+                                                       ptr_stack, ptr_base, ptr_item, ptr_val, nobs_mt, ptr_mt_stack, &
+                                                       ptr_mt_base, ptr_mt_item, ptr_mt_val, ut_cap, ut_stack, ut_ntot, &
+                                                       ut_item, ut_val, ut_mt_cap, ut_mt_stack, ut_mt_ntot, ut_mt_item, &
+                                                       ut_mt_val, tol) bind(C, name="afx_solve_pardiso_jointinv")
+      import :: c_int, c_double, c_double_complex, c_signed_char
+      integer(c_int), value :: act, mt, doftot, nsr, node, nobs, nobs_mt, ut_cap, ut_mt_cap
+      integer(c_int), intent(in) :: INU(*), IAU(*), INL(*), IAL(*), line(*), iflag_comp(*)
+      integer(c_int), intent(in) :: ptr_stack(*), ptr_base(*), ptr_item(*)
+      integer(c_int), intent(in) :: ptr_mt_stack(*), ptr_mt_base(*), ptr_mt_item(*)
+      real(c_double), intent(in) :: ptr_val(*), ptr_mt_val(*)
+      complex(c_double_complex), intent(in) :: D(*), AU(*), AL(*), rhs(*), rhs_mt(*)
+      integer(c_signed_char), intent(in) :: line_bc(*)
+      complex(c_double_complex), intent(out) :: xout(*), xout_mt(*), ut_val(*), ut_mt_val(*)
+      integer(c_int), intent(out) :: ut_stack(*), ut_ntot(*), ut_item(*), ut_mt_stack(*), ut_mt_ntot(*), ut_mt_item(*)
       real(c_double), value :: tol
     end function
 

@@ -161,3 +161,85 @@ def test_host_array_solvepardisoinv_entry(case_small):
                                    line_bc=o["bc"], tol=1e-11, ut_cap=10)
     assert e.value.code == afx.AFX_ERR_ARG
     afx.solve_pardiso_release()
+
+
+def _dense_rows(PTR, n):
+    out = []
+    for st, it, val in PTR:
+        P = np.zeros((len(st) - 1, n))
+        for r in range(len(st) - 1):
+            P[r, it[st[r]:st[r + 1]] - 1] = val[st[r]:st[r + 1]]
+        out.append(P)
+    return out
+
+
+def test_host_array_solvepardisojointinv_entry(case_small):
+    """afx_solve_pardiso_jointinv, the solvePARDISOjointinv-shaped entry (joint tidal + MT): with
+    ACT and MT, the 2 active-source and 2 MT forward columns reach a 1e-11 true residual, and the
+    flagged PTR components' ut and all 4 PTR_mt components' ut_mt give the direct solve's observed
+    adjoint fields (P A^-1 P^T) to 1e-6, the active part equal to afx_solve_pardiso_inv's on the
+    same input to the same tolerance; with MT only, xout and ut stay empty; a right-hand side whose
+    solution FP64 cannot resolve is reported as -100 (tol not reached); with no flagged PTR
+    component, nothing is solved (the reference's early exit, solvePARDISOjointinv.f90:56-64)."""
+    import scipy.sparse.linalg as spla
+    from actfemtide_amd import afx
+    from oracle import oracle as O
+    from tests.conftest import seafloor_sites
+    c = case_small
+    o = O.forward_system(c, OMEGA_M2)
+    item, ve, vb, _ = O.prepobscoeff(c, seafloor_sites(c, 6, 23))
+    PTR = _ptr_list(item, ve, vb)
+    item2, ve2, vb2, _ = O.prepobscoeff(c, seafloor_sites(c, 4, 31))
+    st2 = np.arange(0, 6 * item2.shape[0] + 1, 6, dtype=np.int32)
+    it2 = item2.reshape(-1).astype(np.int32)
+    PTR_mt = [(st2, it2, vb2[0].reshape(-1)), (st2, it2, vb2[1].reshape(-1)),
+              (st2, it2, ve2[0].reshape(-1)), (st2, it2, ve2[1].reshape(-1))]  # bx, by, ex, ey
+    args = (o["D"], o["AU"], o["AL"], o["INU"], o["IAU"], o["INL"], o["IAL"])
+    b = np.asarray(o["b"])
+    rhs = np.stack([b, (1 - 0.5j) * b], axis=1)
+    A = O.to_scipy(o["INU"], o["IAU"], o["INL"], o["IAL"], o["D"], o["AU"], o["AL"]).tocsc()
+    rng = np.random.default_rng(8)
+    xr = rng.standard_normal(c.nline) + 1j * rng.standard_normal(c.nline)
+    xr[o["bc"]] = 0.0
+    rmt = np.stack([(0.3 + 1j) * b, A @ xr], axis=1)  # MT columns with resolvable solutions
+    flags = [1, 0, 1, 1, 1]
+    kw = dict(node=c.node, line=c.line, line_bc=o["bc"], tol=1e-11)
+    x, xm, ut, ut_mt, rc = afx.solve_pardiso_jointinv_host(*args, rhs, rmt, PTR, PTR_mt, iflag_comp=flags, **kw)
+    assert rc == 0 and ut[1] == 0
+    lu = spla.splu(A)
+    for xx, rr in ((x, rhs), (xm, rmt)):
+        for k in range(2):
+            assert np.linalg.norm(rr[:, k] - A @ xx[:, k]) <= 1.05e-11 * np.linalg.norm(rr[:, k])
+    x1, ut1, _ = afx.solve_pardiso_inv_host(*args, rhs, PTR, iflag_comp=flags, **kw)
+    Ps, Pm = _dense_rows(PTR, c.nline), _dense_rows(PTR_mt, c.nline)
+    Pall = np.vstack(Ps + Pm)
+    for name, comps, P, U in (("PTR", (0, 2, 3, 4), Ps, ut), ("PTR_mt", (0, 1, 2, 3), Pm, ut_mt)):
+        for comp in comps:
+            U_ref = np.array([lu.solve(p.astype(complex)) for p in P[comp]])
+            assert np.all(np.abs(U[comp].data) > 1e-10)
+            G, G_ref = Pall @ U[comp].toarray().T, Pall @ U_ref.T
+            err = np.abs(G - G_ref).max() / np.abs(G_ref).max()
+            print(f"{name} component {comp}: nnz {U[comp].nnz}, observed adjoint fields vs direct {err:.1e}", flush=True)
+            assert err <= 1e-6
+            if name == "PTR":
+                G1 = Pall @ ut1[comp].toarray().T
+                assert np.abs(G - G1).max() <= 1e-6 * np.abs(G1).max()
+    # MT only: the active outputs stay empty
+    x, xm, ut, ut_mt, rc = afx.solve_pardiso_jointinv_host(*args, rhs, rmt, PTR, PTR_mt, iflag_comp=flags, act=False,
+                                                           **kw)
+    assert rc == 0 and not x.any() and all(isinstance(u, int) for u in ut) and not any(isinstance(u, int) for u in ut_mt)
+    assert np.linalg.norm(rmt[:, 0] - A @ xm[:, 0]) <= 1.05e-11 * np.linalg.norm(rmt[:, 0])
+    # a right-hand side with random gradient components in the air: its solution is about 1e13 and
+    # no FP64 solve, direct or iterative, gets its residual below about 1e-4, so the entry reports
+    # -100 (tol not reached) and still returns the iterate
+    rr = rng.standard_normal((c.nline, 2)) + 1j * rng.standard_normal((c.nline, 2))
+    rr[o["bc"]] = 0.0
+    x, xm, ut, ut_mt, rc = afx.solve_pardiso_jointinv_host(*args, rhs, rr, PTR, PTR_mt, iflag_comp=flags, **kw)
+    tr = np.linalg.norm(rr[:, 0] - A @ xm[:, 0]) / np.linalg.norm(rr[:, 0])
+    td = np.linalg.norm(rr[:, 0] - A @ lu.solve(rr[:, 0])) / np.linalg.norm(rr[:, 0])
+    print(f"random MT column: rc {rc}, true residual {tr:.1e} (direct solve {td:.1e})", flush=True)
+    assert rc == afx.AFX_ERR_MAXIT and td > 1e-8 and tr < 1e-2
+    # no flagged active component: nothing is solved
+    x, xm, ut, ut_mt, rc = afx.solve_pardiso_jointinv_host(*args, rhs, rmt, PTR, PTR_mt, iflag_comp=[0] * 5, **kw)
+    assert rc == 0 and not x.any() and not xm.any() and all(isinstance(u, int) for u in ut + ut_mt)
+    afx.solve_pardiso_release()

@@ -7,8 +7,9 @@ tests reach neither (14 edges per node at most, runs of 7).
 
 Checked against the oracle on the same system: the NODEBLK+GRAD and DIAG+GRAD applies; the
 single-column, k-column and stacked solves at a 1e-8 true residual, with site fields against the
-direct solve and iteration counts against the oracle's preconditioned COCG; and the reference's
-SSOR, whose forward sweep meets rows wider than the single-column sweep layout here."""
+direct solve and iteration counts against the oracle's preconditioned COCG; the reference's
+SSOR, whose forward sweep meets rows wider than the single-column sweep layout here; point
+location in the cone tets; and the Jacobian accumulation over 12-tet parameters."""
 import math
 
 import numpy as np
@@ -182,3 +183,61 @@ def test_refined_sweep_preconditioners(refined, kind):
     print(f"refined {kind}: {st[0].iter} iterations (oracle {it_o}), true residual {tr:.2e}", flush=True)
     assert st[0].error == 0 and tr <= 1.05e-8
     assert abs(st[0].iter - it_o) <= max(3, 0.08 * it_o)
+
+
+def test_refined_point_location(refined):
+    """Point location and the observation rows (PREPOBSCOEFF) on the refined mesh against the
+    oracle's FINDELEMENT0 scan: random points inside the refined cells (so inside the cone tets),
+    the 16 centre nodes (each shared by 12 tets, where the scan's lowest-id rule decides), every
+    node, and face centroids of cone tets."""
+    from oracle import oracle as O
+    case, s, b, bc, o, A, corr = refined
+    rng = np.random.default_rng(21)
+    new = np.arange(case.ntet - 16 * 12, case.ntet)
+    n4 = np.asarray(case.n4) - 1
+    pts = []
+    for cell in np.unique(case.cell_of_tet[new]):
+        v = case.xyz[np.unique(n4[case.cell_of_tet == cell])]
+        lo, hi = v.min(axis=0), v.max(axis=0)
+        pts.append(lo + (hi - lo) * rng.uniform(0.02, 0.98, (6, 3)))
+    t = rng.choice(new, 24, replace=False)
+    face = case.xyz[n4[t][:, [0, 1, 3]]].mean(axis=1)
+    pts = np.concatenate(pts + [case.xyz[case.node - 16:], case.xyz, face])
+    item, ve, vb, iele = s.prepobscoeff(pts)
+    item_r, ve_r, vb_r, iele_r = O.prepobscoeff(case, pts)
+    bad = np.nonzero(iele != iele_r)[0]
+    print(f"refined point location: {len(pts)} points, {int(np.isin(iele, new).sum())} in cone tets, "
+          f"{bad.size} differ", flush=True)
+    assert bad.size == 0, [(int(i), int(iele[i]), int(iele_r[i])) for i in bad[:10]]
+    assert np.isin(iele, new).sum() >= 16 * 6
+    assert np.array_equal(item, item_r)
+    assert np.allclose(ve, ve_r, rtol=1e-12, atol=1e-12) and np.allclose(vb, vb_r, rtol=1e-12, atol=1e-12)
+
+
+def test_refined_jacobian_accumulation(refined):
+    """genjacobian1's accumulation (afx_jacobian_device) over every crust cell of the refined mesh,
+    16 of them 12-tet cells (72 edge-weight pairs, more than the 64 the kernel merges in LDS, so
+    those parameters take its direct loop), against the oracle on the same U and x."""
+    import torch
+    from oracle import oracle as O
+    from tests.conftest import crust_params
+    case, s, b, bc, o, A, corr = refined
+    par_ptr, par_tet, cells = crust_params(case)
+    sizes = np.diff(par_ptr)
+    assert (sizes == 12).sum() == 16 and (sizes == 6).sum() == len(sizes) - 16
+    rng = np.random.default_rng(13)
+    nobs = 10
+    U = (rng.standard_normal((nobs, s.n)) + 1j * rng.standard_normal((nobs, s.n))) * 1e-3
+    x = rng.standard_normal(s.n) + 1j * rng.standard_normal(s.n)
+    dU = torch.from_numpy(np.ascontiguousarray(U.T)).cuda()
+    dx = torch.from_numpy(x).cuda()
+    dJ = torch.zeros((len(par_ptr) - 1, nobs), dtype=torch.complex128, device="cuda")
+    s.jacobian_device(dU, dx, par_ptr, par_tet, case.sigma_tet(), OMEGA_M2, np.log(10.0), dJ)
+    J = dJ.cpu().numpy().T
+    J_ref = O.jacobian(U, x, par_ptr, par_tet, case, OMEGA_M2, np.log(10.0))
+    err = np.abs(J - J_ref).max() / np.abs(J_ref).max()
+    big = sizes == 12
+    err12 = np.abs(J[:, big] - J_ref[:, big]).max() / np.abs(J_ref[:, big]).max()
+    print(f"refined Jacobian: {len(sizes)} parameters x {nobs} columns, rel err {err:.1e} (12-tet cells {err12:.1e})",
+          flush=True)
+    assert err <= 1e-12 and err12 <= 1e-12

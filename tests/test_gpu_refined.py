@@ -50,6 +50,23 @@ def refined():
     s.close()
 
 
+def test_refined_assembly_matches_oracle(refined):
+    """GENMAT + COPY_UL + GENBCCSEM + SET_BC_ICCG on the device for the refined mesh (rows of up
+    to 25 entries, cone tets of two shapes) against the oracle's: the pattern exactly, the values
+    within 1e-12 of the largest diagonal entry, the Dirichlet mask and the right-hand side."""
+    case, s, b, bc, o, A, corr = refined
+    INU, IAU, INL, IAL = s.pattern()
+    for got, ref in ((INU, o["INU"]), (IAU, o["IAU"]), (INL, o["INL"]), (IAL, o["IAL"])):
+        assert np.array_equal(got, ref)
+    D, AU, AL = s.get_values()
+    scale = np.max(np.abs(o["D"]))
+    for got, ref in ((D, o["D"]), (AU, o["AU"]), (AL, o["AL"])):
+        assert np.max(np.abs(got - ref)) <= 1e-12 * scale
+    assert np.array_equal(bc, o["bc"])
+    bo = np.asarray(o["b"])
+    assert np.linalg.norm(b - bo) <= 1e-12 * np.linalg.norm(bo)
+
+
 @pytest.mark.parametrize("kind", ["DIAG+GRAD", "NODEBLK", "NODEBLK+GRAD"])
 def test_refined_apply_matches_oracle(refined, kind):
     """The preconditioner applies against the oracle's: DIAG+GRAD to 1e-12; the node blocks alone

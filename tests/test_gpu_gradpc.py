@@ -55,9 +55,9 @@ def test_grad_precond_apply_matches_oracle(small):
 @pytest.mark.parametrize("kind", ["NODEBLK", "NODEBLK+GRAD"])
 def test_nodeblk_apply_matches_oracle(small, kind):
     """The node-block Jacobi apply (AFX_PC_NODEBLK: block inverses formed on the device by
-    Gauss-Jordan, symmetrised, kept in complex FP32) against the oracle's (numpy LU inverses of the
-    same blocks, symmetrised, FP32), with and without the nodal correction: the two FP32 roundings
-    of the same FP64 inverse may differ by one unit, so to 1e-6."""
+    Gauss-Jordan, symmetrised, kept as bfloat16 complex) against the oracle's (numpy LU inverses of
+    the same blocks, symmetrised, rounded to bfloat16 the same way), with and without the nodal
+    correction: the two roundings of the same FP64 inverse may differ by one unit, so to 1e-6."""
     from oracle import oracle as O
     s, b, bc, o, A, corr = small
     rng = np.random.default_rng(5)

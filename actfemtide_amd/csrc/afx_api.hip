@@ -158,6 +158,7 @@ struct afx_system_s {
   // end nodes of each edge, and diag(G^T M G) of the current values
   int nn = -1;
   int *d_nptr = nullptr, *d_nadj = nullptr, *d_en = nullptr;
+  int *d_nadj16 = nullptr;  // the incidence lists at 16 slots per node (the first 16 of each)
   int *d_noff = nullptr;  // stacked systems: first interior node of each system [nsys + 1]
   double *d_mnd = nullptr;
   // node-block Jacobi (AFX_PC_NODEBLK, afx_set_nodes): blocks of consecutive rows whose edges
@@ -935,6 +936,7 @@ int krylov(afx_system s, int k, const cx<double> *B, cx<double> *X, const std::v
   p.nn = grad ? s->nn : 0;
   p.nptr = s->d_nptr;
   p.nadj = s->d_nadj;
+  p.nadj16 = s->d_nadj16;
   p.en = s->d_en;
   p.noff = s->d_noff;
   p.mnd = s->d_mnd;
@@ -1150,7 +1152,7 @@ void release(afx_system s) {
   free_side(s->bjbw);
   dfree(s->d_rstart); dfree(s->d_bar); dfree(s->d_err);
   dfree(s->d_sys_off); dfree(s->d_bsys); dfree(s->d_sblk);
-  dfree(s->d_nptr); dfree(s->d_nadj); dfree(s->d_en); dfree(s->d_mnd); dfree(s->d_noff);
+  dfree(s->d_nptr); dfree(s->d_nadj); dfree(s->d_nadj16); dfree(s->d_en); dfree(s->d_mnd); dfree(s->d_noff);
   dfree(s->d_tptr); dfree(s->d_bm); dfree(s->d_bi); dfree(s->d_bstart);
   if (s->stack) afx_destroy(s->stack);
   s->stack = nullptr;
@@ -1925,13 +1927,18 @@ int afx_set_nodes(afx_system s, int node, const int *line, const unsigned char *
     e0 = e1;
   }
   tptr.push_back(n);
-  dfree(s->d_nptr); dfree(s->d_nadj); dfree(s->d_en); dfree(s->d_mnd); dfree(s->d_noff);
+  dfree(s->d_nptr); dfree(s->d_nadj); dfree(s->d_nadj16); dfree(s->d_en); dfree(s->d_mnd); dfree(s->d_noff);
   dfree(s->d_tptr); dfree(s->d_bm); dfree(s->d_bi); dfree(s->d_bstart);
   s->nn = -1;
   s->ntile = s->nblk = 0;
   int rc;
   if (!noff.empty() && (rc = upload(&s->d_noff, noff.data(), noff.size()))) return rc;
+  // the lists again at a fixed width of 16 slots (0 = unused), loaded as four 16-byte vectors
+  std::vector<int> adj16((size_t)nn * 16, 0);
+  for (int a = 0; a < nn; a++)
+    for (int q = cnt[a]; q < cnt[a + 1] && q - cnt[a] < 16; q++) adj16[(size_t)a * 16 + (q - cnt[a])] = adj[q];
   if ((rc = upload(&s->d_nptr, cnt.data(), cnt.size())) || (rc = upload(&s->d_nadj, adj.data(), adj.size())) ||
+      (nn > 0 && (rc = upload(&s->d_nadj16, adj16.data(), adj16.size()))) ||
       (rc = upload(&s->d_en, en.data(), en.size())) || (rc = dalloc(&s->d_mnd, (size_t)nn + 1)) ||
       (rc = upload(&s->d_tptr, tptr.data(), tptr.size())) || (rc = upload(&s->d_bm, bm.data(), bm.size())) ||
       (rc = upload(&s->d_bstart, bstart.data(), bstart.size())))

@@ -126,6 +126,9 @@ struct CocgParams {
   int grad;
   int nn;
   const int *nptr, *nadj, *en;
+  // the same lists at a fixed width of 16 slots per node (0 = unused slot); a node with more
+  // incidences holds its first 16 here and the rest only in nadj
+  const int *nadj16;
   const int *noff;  // stacked systems: system q's interior nodes are [noff[q], noff[q+1])
   const double *mnd;
   cx<double> *Zn;

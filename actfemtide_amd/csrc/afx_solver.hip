@@ -1967,7 +1967,55 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
         }
       }
     }
-    if (grad)
+    if (grad && nblk && p.nadj16) {
+      // r' is in Rn once every block's tiles are done. Each node's 16 incidence slots load as four
+      // 16-byte vectors one node ahead, and its r' values are gathered (16 B per incidence, not r
+      // and q at 32 B): the same values added in the same order as nodal_g_next.
+      abort |= grid_barrier(p.bar, phase);
+      const int4 *A16 = (const int4 *)p.nadj16;
+      int4 cur[4];
+      if (gtid < p.nn)
+#pragma unroll
+        for (int j = 0; j < 4; j++) cur[j] = A16[gtid * 4 + j];
+      for (long long a = gtid; a < p.nn; a += nthr) {
+        const long long an = a + nthr < p.nn ? a + nthr : a;
+        int4 nxt[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) nxt[j] = A16[an * 4 + j];
+        const int q0 = p.nptr[a], q1 = p.nptr[a + 1];
+        int v[16];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          v[4 * j] = cur[j].x;
+          v[4 * j + 1] = cur[j].y;
+          v[4 * j + 2] = cur[j].z;
+          v[4 * j + 3] = cur[j].w;
+        }
+        cx<double> rr[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) rr[j] = Rn[v[j] > 0 ? v[j] - 1 : (v[j] < 0 ? -v[j] - 1 : 0)];
+        cx<double> g = {0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+          if (v[j] > 0) {
+            g.r += rr[j].r;
+            g.i += rr[j].i;
+          } else if (v[j] < 0) {
+            g.r -= rr[j].r;
+            g.i -= rr[j].i;
+          }
+        }
+        for (int q = q0 + 16; q < q1; q++) {  // incidences beyond the 16 slots
+          const int vq = p.nadj[q];
+          const cx<double> rn = Rn[(vq > 0 ? vq : -vq) - 1];
+          g.r += vq > 0 ? rn.r : -rn.r;
+          g.i += vq > 0 ? rn.i : -rn.i;
+        }
+        nodal_put(a, g, acc);
+#pragma unroll
+        for (int j = 0; j < 4; j++) cur[j] = nxt[j];
+      }
+    } else if (grad)
       for (long long a = gtid; a < p.nn; a += nthr) nodal_put(a, nodal_g_next<16>(p.nptr, p.nadj, R, p.Q, malpha, a), acc);
     grid_sum_n<3>(acc, part_xn, lds, p.bar, phase, abort);
     const double rr = acc[2];

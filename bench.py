@@ -137,12 +137,14 @@ def bytes_per_iteration_fused(n, nnz, nn, nadj, edge_pc, ntile=0):
     SpMV: (K, M) values 16 B + column 4 B per entry, row pointers, p read, q written;
     x, r, z pass: x, p, q, r, D (DIAG) read, x and the next r written; NODEBLK: 8 complex-bfloat16
     coefficients (32 B) and 1 B of block place per row, the tile bounds, Bi r written; on the
-    nodes the incidence list (4 B per entry), the row pointers, diag(G^T M G) read and zn written
-    (r and q gathered there are the arrays the edge part reads in the same pass)."""
+    nodes the incidence list (4 B per entry; NODEBLK: its 16-slot copy, 64 B per node), the row
+    pointers, diag(G^T M G) read and zn written (the r and q, or NODEBLK's r', gathered there are
+    arrays the edge part reads or writes in the same pass)."""
     d = 16 * n if edge_pc == "DIAG" else 0
     ppass = 3 * 16 * n + d + 8 * n + 16 * nn
     spmv = nnz * 20 + 4 * (n + 1) + 2 * 16 * n
-    xn = 6 * 16 * n + d + 4 * nadj + 4 * (nn + 1) + 8 * nn + 16 * nn
+    lists = 64 * nn if edge_pc == "NODEBLK" else 4 * nadj
+    xn = 6 * 16 * n + d + lists + 4 * (nn + 1) + 8 * nn + 16 * nn
     if edge_pc == "NODEBLK":
         xn += 32 * n + n + 4 * (ntile + 1) + 16 * n
     return ppass + spmv + xn

@@ -1662,8 +1662,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
 // ---------------------------------------------------------------------------------------
 // One column without sweeps (edge preconditioner DIAG or NONE, with or without the nodal
 // correction), FP64: one block of 256 threads per CU, three grid synchronisations per iteration
-// instead of four. The recurrence is COCG13_single's (:181-323); the preconditioner apply of
-// iteration it+1 moves into the x, r pass of iteration it:
+// instead of four (with node blocks and the nodal correction a fourth, after the tiles, lets the
+// nodal loop gather the r' they wrote). The recurrence is COCG13_single's (:181-323); the
+// preconditioner apply of iteration it+1 moves into the x, r pass of iteration it:
 //   P   p = z + beta p (:258-268), z = D r + G zn formed on the fly (z is never stored);
 //   S   q = A p (:273-288) and c1 = p^T q (:293-296)                      -> reduction of c1;
 //   XN  x += alpha p, r' = r - alpha q (:304-307) into the other residual buffer, |r'|^2
